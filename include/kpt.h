@@ -1,0 +1,366 @@
+/*
+ * kpt.h — C ABI of the MI355X (gfx950) Parker / focused-transport SDE particle push.
+ *
+ * Every entry point is a plain-C replacement for one function of the Kokkos driver
+ * path in LiuYihanSDU/Kokkos-ParticleTransport (src/main.cpp, namespace
+ * Reconnection2DCalibration) or of its generic solver / ParticleSystem API. The
+ * reference function each one replaces is cited beside its declaration.
+ *
+ * Conventions
+ *  - All functions return kpt_status: 0 on success, a negative KPT_ERR_* code on
+ *    failure. kpt_last_error(ctx) returns the message of the last failure on that
+ *    context (the reference throws std::runtime_error with the same text where one
+ *    exists).
+ *  - Device buffers are owned by the context. Host pointers passed in are borrowed
+ *    for the duration of the call only.
+ *  - One context per host thread; a context is not thread-safe. Calls are
+ *    synchronous: they return after the device work they enqueue has finished
+ *    (the reference fences after every kernel).
+ *  - Numerical failures inside the push never abort: the particle gets status 2
+ *    and is counted in kpt_push_stats.failures (src/main.cpp:1167-1171,
+ *    :1189-1193, :1274-1277, :1301-1305).
+ */
+#ifndef KPT_H
+#define KPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int kpt_status;
+
+enum {
+    KPT_OK = 0,
+    KPT_ERR_ARG = -1,      /* invalid argument or settings                      */
+    KPT_ERR_IO = -2,       /* file open/read/write or format failure            */
+    KPT_ERR_DEVICE = -3,   /* HIP runtime failure or no gfx950 device           */
+    KPT_ERR_CAPACITY = -4, /* particle capacity exceeded                        */
+    KPT_ERR_STATE = -5     /* call made in the wrong state (e.g. no field)      */
+};
+
+/* Packed coefficient layouts (component counts of the reference's packed fields). */
+typedef enum {
+    KPT_MODEL_PARKER16 = 16,  /* src/main.cpp:28, :62-79   */
+    KPT_MODEL_FOCUSED21 = 21  /* src/main.cpp:29, :38-60   */
+} kpt_model;
+
+typedef enum { KPT_SLOT_LOWER = 0, KPT_SLOT_UPPER = 1 } kpt_slot;
+
+/* Arithmetic of the push. FP64: everything in double, as the reference.
+ * FP32: coefficient fields stored and interpolated in float, transport
+ * coefficients in float, positions / momentum / time accumulated in double. */
+typedef enum { KPT_PREC_FP64 = 0, KPT_PREC_FP32 = 1 } kpt_precision;
+
+/* Noise source of the push. */
+typedef enum {
+    KPT_NOISE_PHILOX = 0,  /* counter-based Philox4x32-10 keyed by seed and particle id */
+    KPT_NOISE_INJECTED = 1 /* fixed host-provided draw table, see kpt_set_noise        */
+} kpt_noise_mode;
+
+/*
+ * Device parameter block. Mirrors Reconnection2DDeviceSettings field by field
+ * (src/main.cpp:246-283); the trailing fields are extensions of this library.
+ * Fill it with kpt_make_device_settings (src/main.cpp:595-636) or by hand.
+ */
+typedef struct kpt_device_settings {
+    int32_t max_particle_steps_per_interval;
+    int32_t momentum_dependency;
+    int32_t magnetic_dependency;
+    int32_t split_particles;
+    int32_t time_interpolation;
+    int32_t focused_transport;
+
+    double x_lower;
+    double x_upper;
+    double y_lower;
+    double y_upper;
+    double x_width;
+    double y_width;
+    double dx;
+    double dy;
+    double interpolation_shift_x;
+    double interpolation_shift_y;
+
+    double mhd_dt;
+    double p0;
+    double pmax;
+    double gamma_turb;
+    double pindex;
+    double kpara0;
+    double kperp_over_kpara;
+    double dt_min;
+    double dt_max;
+    double drift_param1;
+    double drift_param2;
+    double charge;
+    double particle_v0;
+    double duu0;
+    double pitch_angle_scattering_h0;
+    double split_ratio;
+    double pmin_split;
+    double mu_max;
+
+    /* extensions */
+    int32_t precision;  /* kpt_precision */
+    int32_t reserved0;
+} kpt_device_settings;
+
+/* Host-side controls that kpt_make_device_settings reads (the physics subset of
+ * Reconnection2DSettings, src/main.cpp:114-154, with the same defaults). */
+typedef struct kpt_host_settings {
+    int32_t max_particle_steps_per_interval; /* 200000 */
+    int32_t momentum_dependency;             /* 1 */
+    int32_t magnetic_dependency;             /* 1 */
+    int32_t split_particles;                 /* 1 */
+    int32_t time_interpolation;              /* 1 */
+    int32_t focused_transport;               /* 0 */
+    double mhd_dt;                           /* 0.1 */
+    double p0;                               /* 0.1 */
+    double pmin;                             /* 1e-2 (histogram lower edge) */
+    double pmax;                             /* 1e1  */
+    double gamma_turb;                       /* 1.6666667 */
+    double kpara0;                           /* 0.00743592 */
+    double kperp_over_kpara;                 /* 0.01 */
+    double dt_min_rel;                       /* 1e-7 */
+    double dt_max_rel;                       /* 1e-2 */
+    double drift_param1;                     /* 850964.408 */
+    double drift_param2;                     /* 13575468.975 */
+    double charge;                           /* -1 */
+    double particle_v0;                      /* 17.20195 */
+    double duu0;                             /* 5578.445 */
+    double split_ratio;                      /* 2 */
+    double pmin_split_over_p0;               /* 2 */
+    double mu_max;                           /* 0.99 */
+} kpt_host_settings;
+
+/* Particle arrays on the host, structure of arrays (the fields of
+ * ReconnectionParticleStorage, src/main.cpp:288-333, plus the particle id that
+ * keys the counter-based RNG). Any pointer may be NULL on download to skip it. */
+typedef struct kpt_particles_host {
+    double* x;
+    double* y;
+    double* p;
+    double* t;
+    double* step_dt;
+    double* weight;
+    double* mu;
+    int32_t* status;
+    int32_t* split_level;
+    uint64_t* id;
+} kpt_particles_host;
+
+typedef struct kpt_push_stats {
+    uint64_t particle_steps; /* executed Euler–Maruyama sub-steps (src/main.cpp:1283-1305) */
+    uint64_t failures;       /* particles that went from status 0 to status 2 in this call   */
+    uint64_t active_in;      /* particles with status 0 when the call started                */
+    double kernel_ms;        /* device time of the push kernel(s), from HIP events            */
+    uint32_t launches;       /* push kernel launches in this call                            */
+    uint32_t reserved;
+} kpt_push_stats;
+
+/* Mirrors ParticleSummary (src/main.cpp:338-350). */
+typedef struct kpt_summary {
+    uint64_t stored;
+    uint64_t active;
+    uint64_t inactive;
+    double weight_sum;
+    double p_min;
+    double p_max;
+    double p_avg;
+    double dt_min;
+    double dt_max;
+    double dt_avg;
+    int32_t max_split_level;
+    int32_t reserved;
+} kpt_summary;
+
+/* Injection region and timing. The driver's inject_uniform_particles
+ * (src/main.cpp:995-1048) is the default: whole domain, t ~ U(t0, t0 + dt_out).
+ * The axis-aligned region of inject_particles_uniform_if
+ * (include/ParticleGenerator.hpp:17-62, :884-958) is selected with use_region. */
+typedef struct kpt_inject_opts {
+    int32_t use_region;   /* 0: whole domain; 1: [region_x_lo, region_x_hi) x [region_y_lo, region_y_hi) */
+    int32_t fixed_time;   /* 0: t ~ U(t0, t0 + mhd_dt) as the driver; 1: t = t0 */
+    double region_x_lo;
+    double region_x_hi;
+    double region_y_lo;
+    double region_y_hi;
+    double p_init;        /* momentum of new particles; <= 0 selects settings.p0 */
+} kpt_inject_opts;
+
+typedef struct kpt_ctx kpt_ctx;
+
+/* ---- context ---------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with particle capacity `capacity` and
+ * RNG seed `seed` (the reference's RandomManager(seed), include/RandomManager.hpp:28;
+ * ReconnectionParticleStorage(cap), src/main.cpp:318-332). */
+kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t seed);
+kpt_status kpt_destroy(kpt_ctx* ctx);
+const char* kpt_last_error(const kpt_ctx* ctx);
+/* Message of the last failure of kpt_create (when no context exists yet). */
+const char* kpt_last_create_error(void);
+
+/* Library version string and the device architecture string of the context. */
+const char* kpt_version(void);
+kpt_status kpt_device_name(kpt_ctx* ctx, char* buf, size_t len);
+
+/* Default host settings (the member initialisers of Reconnection2DSettings,
+ * src/main.cpp:129-154). */
+kpt_status kpt_default_host_settings(kpt_host_settings* out);
+
+/* make_device_settings (src/main.cpp:595-636): derive the device block from host
+ * controls and the uniform grid given by its face coordinates. */
+kpt_status kpt_make_device_settings(const kpt_host_settings* host, int32_t nx, int32_t ny,
+                                    const double* x_edges, const double* y_edges,
+                                    kpt_device_settings* out);
+
+kpt_status kpt_set_seed(kpt_ctx* ctx, uint64_t seed);
+
+/* ---- fields ----------------------------------------------------------------- */
+
+/* load_parker_coefficient_frame / load_coefficient_frame (src/main.cpp:962-990):
+ * read one compact-field binary (include/AthenaFieldReader.hpp:274-360) and build
+ * the packed coefficient field of `model` in `slot`. */
+kpt_status kpt_field_load_compact(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model);
+
+/* Same as kpt_field_load_compact, from host arrays in the compact-field order
+ * (Bx, By, Bz, Vx, Vy, Vz each [ny][nx], x fastest). */
+kpt_status kpt_field_upload(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, int32_t ny,
+                            const double* x_edges, const double* y_edges,
+                            const double* bx, const double* by, const double* bz,
+                            const double* vx, const double* vy, const double* vz);
+
+/* lower = upper (src/main.cpp:2311): the upper slot becomes the lower slot. */
+kpt_status kpt_field_advance(kpt_ctx* ctx);
+
+/* Download the packed coefficients of `slot` for nodes i in [-1, nx], j in [-1, ny]
+ * as double[(ny+2)][(nx+2)][C] (C = model). `out_len` is the element count of `out`. */
+kpt_status kpt_field_download_coefficients(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len);
+
+/* ---- compact-field files (host) ------------------------------------------- */
+
+/* Parse a compact-field file into caller buffers. Pass NULL buffers to query
+ * nx, ny only. Validation follows include/AthenaFieldReader.hpp:86-270, except
+ * that non-uniform edges are accepted when allow_nonuniform != 0. */
+kpt_status kpt_compact_field_read(const char* path, int32_t* nx, int32_t* ny,
+                                  double* x_edges, double* y_edges, double* bx, double* by,
+                                  double* bz, double* vx, double* vy, double* vz,
+                                  int32_t allow_nonuniform, char* err, size_t err_len);
+kpt_status kpt_compact_field_write(const char* path, int32_t nx, int32_t ny,
+                                   const double* x_edges, const double* y_edges,
+                                   const double* bx, const double* by, const double* bz,
+                                   const double* vx, const double* vy, const double* vz);
+
+/* ---- particles -------------------------------------------------------------- */
+
+kpt_status kpt_particles_count(kpt_ctx* ctx, uint64_t* count);
+kpt_status kpt_particles_capacity(kpt_ctx* ctx, uint64_t* capacity);
+/* Replace the particle set with `count` host particles (status, split level and
+ * weight are taken from the arrays; a NULL id array assigns id = index). */
+kpt_status kpt_particles_upload(kpt_ctx* ctx, const kpt_particles_host* h, uint64_t count);
+/* Copy the first `count` stored particles to the host. */
+kpt_status kpt_particles_download(kpt_ctx* ctx, kpt_particles_host* h, uint64_t count);
+
+/* ---- the hot path ------------------------------------------------------------ */
+
+/* inject_uniform_particles (src/main.cpp:995-1048): append up to n particles. */
+kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inject_opts* opts,
+                      uint64_t n, double t0, uint32_t frame, uint64_t* injected);
+
+/* move_parker_particles_one_interval (src/main.cpp:1082-1327) when
+ * s->focused_transport == 0 and the field model is PARKER16, and
+ * move_particles_one_interval (src/main.cpp:1332-1755) when the field model is
+ * FOCUSED21 (Parker or reduced focused branch per s->focused_transport).
+ * Advances every status-0 particle from its time to t0 + s->mhd_dt. `frame`
+ * keys the Philox stream of this interval. */
+kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double t0,
+                             uint32_t frame, kpt_push_stats* stats);
+
+/* split_particles (src/main.cpp:1760-1839). Children are appended in parent slot
+ * order (one admissible schedule of the reference's atomic counter). */
+kpt_status kpt_split(kpt_ctx* ctx, const kpt_device_settings* s, uint64_t* split,
+                     uint64_t* skipped_capacity);
+
+/* write_momentum_histogram binning (src/main.cpp:1906-1942): weight per log-p bin
+ * over [pmin, pmax], active particles only, out-of-range dropped. */
+kpt_status kpt_histogram_logp(kpt_ctx* ctx, int32_t bins, double pmin, double pmax, double* out);
+
+/* Spatial density of active particles: weight per cell of a bins_x x bins_y
+ * uniform grid over [x_lo, x_hi) x [y_lo, y_hi) (out is [bins_y][bins_x]). */
+kpt_status kpt_histogram_xy(kpt_ctx* ctx, int32_t bins_x, int32_t bins_y, double x_lo, double x_hi,
+                            double y_lo, double y_hi, double* out);
+
+/* summarize_particles (src/main.cpp:1844-1901), computed on the device. */
+kpt_status kpt_summary_compute(kpt_ctx* ctx, kpt_summary* out);
+
+/* ---- noise ------------------------------------------------------------------ */
+
+/* Select the fixed-noise mode: the push of particle slot i at its k-th sub-step of
+ * an interval uses draws[(i * steps_per_particle + k) * 4 + d], d = 0..3, as the
+ * four U[-sqrt3, sqrt3] values of that step (src/main.cpp:1280-1291 for Parker;
+ * :1672-1698 for focused: ran1, ran2, ignored, pitch). The table is copied.
+ * s->max_particle_steps_per_interval must not exceed steps_per_particle. */
+kpt_status kpt_set_noise(kpt_ctx* ctx, const double* draws, uint64_t n_particles,
+                         uint32_t steps_per_particle);
+/* Return to Philox noise. */
+kpt_status kpt_clear_noise(kpt_ctx* ctx);
+
+/* Philox4x32-10 block (Salmon et al. 2011), exposed for known-answer tests. */
+void kpt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- v4 particle snapshots -------------------------------------------------- */
+
+/* write_reconnection_particle_snapshot (src/main.cpp:2131-2202): byte-identical
+ * version-4 KPTPRT file of the stored particles. */
+kpt_status kpt_snapshot_write(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
+                              double p0);
+/* load_reconnection_particle_snapshot (src/main.cpp:2009-2126): replace the
+ * particle set; time = restart_time, step_dt = dt_min. */
+kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
+                             double restart_time, double dt_min);
+
+/* Host-only halves of the two calls above (no device involved): write `count`
+ * host particles as a v4 file with the given header capacity, and read a v4
+ * file into host arrays (pass NULL arrays to query the count only; `count` is
+ * in/out: capacity of the arrays in, stored particles out). The reader applies
+ * the header checks of src/main.cpp:2018-2073; `err` receives the message. */
+kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h, uint64_t count,
+                                   uint64_t capacity, double charge, double split_ratio, double p0);
+kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint64_t* count,
+                                  double charge, double split_ratio, char* err, size_t err_len);
+
+/* ---- CSV outputs (host only) ------------------------------------------------ */
+
+/* Per-frame timing and split counters: FrameTiming (src/main.cpp:355-363). */
+typedef struct kpt_frame_timing {
+    double read_seconds;
+    double inject_seconds;
+    double move_seconds;
+    double split_seconds;
+    uint64_t injected;
+    uint64_t split;
+    uint64_t skipped_split_capacity;
+} kpt_frame_timing;
+
+/* write_momentum_histogram's CSV (src/main.cpp:1944-1963): header
+ * "bin,p_left,p_right,weight", edges exp(ln pmin + i dlnp), 16 significant digits. */
+kpt_status kpt_write_histogram_csv(const char* path, int32_t bins, double pmin, double pmax,
+                                   const double* weights);
+
+/* write_summary_header / write_summary_row (src/main.cpp:2207-2235) into a
+ * caller buffer (NUL-terminated, newline included). Returns KPT_ERR_ARG when the
+ * buffer is too small. */
+kpt_status kpt_format_summary_header(char* buf, size_t len);
+kpt_status kpt_format_summary_row(char* buf, size_t len, int32_t frame, double time,
+                                  const kpt_summary* summary, const kpt_frame_timing* timing,
+                                  double total_frame_seconds, double total_elapsed_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KPT_H */

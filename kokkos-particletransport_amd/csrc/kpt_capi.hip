@@ -1,0 +1,984 @@
+// kpt_capi.hip — host side of the C ABI declared in include/kpt.h.
+//
+// Owns the device buffers (particles in SoA, two coefficient frames in FP64 and
+// FP32, noise table, scratch for split / histogram / summary), validates
+// arguments the way the reference driver does (src/main.cpp:187-240,
+// include/AthenaFieldReader.hpp:86-270) and launches the kernels of
+// kpt_kernels.hip on the context's stream. Every call is synchronous.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "kpt_internal.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct FieldSlot {
+    bool valid = false;
+    int model = 0;
+    kpt::GridDesc g{};
+    double* raw = nullptr;  // 6 planes of nx*ny (Bx, By, Bz, Vx, Vy, Vz)
+    double* c64 = nullptr;
+    float* c32 = nullptr;
+    size_t nodes_cap = 0;
+    size_t raw_cap = 0;
+};
+
+}  // namespace
+
+struct kpt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t capacity = 0;
+    uint64_t count = 0;
+    uint64_t seed = 0;
+    uint64_t next_root = 0;
+    kpt::ParticleArrays P{};
+    FieldSlot slot[2];
+    double* noise = nullptr;
+    uint64_t noise_particles = 0;
+    uint32_t noise_steps = 0;
+    unsigned long long* counters = nullptr;  // 8 x u64
+    uint32_t* split_scratch = nullptr;
+    size_t split_scratch_cap = 0;
+    double* hist_partials = nullptr;
+    double* hist_out = nullptr;
+    void* summary_partials = nullptr;
+    kpt_summary* summary_out = nullptr;
+    std::string err;
+    char device_name[64] = {0};
+};
+
+namespace {
+
+kpt_status fail(kpt_ctx* ctx, kpt_status code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+kpt_status hip_fail(kpt_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, KPT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define KPT_HIP(ctx, call)                                  \
+    do {                                                    \
+        const hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t elems) {
+    *p = nullptr;
+    if (elems == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(p), elems * sizeof(T));
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+kpt::GridDesc make_grid_desc(int32_t nx, int32_t ny, const double* xe, const double* ye) {
+    kpt::GridDesc g{};
+    g.nx = nx;
+    g.ny = ny;
+    const double* e[2] = {xe, ye};
+    const int32_t n[2] = {nx, ny};
+    for (int d = 0; d < 2; ++d) {
+        g.start[d] = e[d][0];
+        g.step[d] = (e[d][n[d]] - e[d][0]) / static_cast<double>(n[d]);
+        g.inv_step[d] = 1.0 / g.step[d];
+        g.phys_lo[d] = g.start[d] + 0.0 * g.step[d];
+        g.phys_hi[d] = g.start[d] + static_cast<double>(n[d]) * g.step[d];
+        g.width[d] = g.phys_hi[d] - g.phys_lo[d];
+        g.inv_width[d] = g.width[d] != 0.0 ? 1.0 / g.width[d] : 0.0;
+    }
+    return g;
+}
+
+// validate_face_edges / validate_uniform_face_edges
+// (include/AthenaFieldReader.hpp:147-193).
+bool check_edges(const double* edges, size_t count, const char* axis, bool require_uniform,
+                 std::string& err) {
+    if (count < 2) {
+        err = std::string("AthenaFieldReader: ") + axis + " requires at least two face coordinates.";
+        return false;
+    }
+    for (size_t i = 0; i < count; ++i) {
+        if (!std::isfinite(edges[i])) {
+            err = std::string("AthenaFieldReader: non-finite ") + axis + " face coordinate.";
+            return false;
+        }
+        if (i > 0 && !(edges[i] > edges[i - 1])) {
+            err = std::string("AthenaFieldReader: ") + axis + " face coordinates must be strictly ascending.";
+            return false;
+        }
+    }
+    if (!require_uniform) return true;
+    const double intervals = static_cast<double>(count - 1);
+    const double span = edges[count - 1] - edges[0];
+    const double step = span / intervals;
+    if (!std::isfinite(step) || step <= 0.0) {
+        err = std::string("AthenaFieldReader: ") + axis + " face spacing must be positive and finite.";
+        return false;
+    }
+    for (size_t i = 1; i < count; ++i) {
+        const double cur = edges[i] - edges[i - 1];
+        const double mag = std::max({std::fabs(span), std::fabs(step), std::fabs(cur), 1.0});
+        const double tol = 1024.0 * std::numeric_limits<double>::epsilon() * mag;
+        if (std::fabs(cur - step) > tol) {
+            err = std::string("AthenaFieldReader: ") + axis +
+                  " face coordinates must be uniformly spaced for analytic grid loading.";
+            return false;
+        }
+    }
+    return true;
+}
+
+kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
+    if (!s) return fail(ctx, KPT_ERR_ARG, "settings pointer is null.");
+    const bool ok = std::isfinite(s->mhd_dt) && s->mhd_dt > 0.0 && std::isfinite(s->p0) && s->p0 > 0.0 &&
+                    std::isfinite(s->dt_min) && s->dt_min > 0.0 && std::isfinite(s->dt_max) &&
+                    s->dt_max >= s->dt_min && std::isfinite(s->x_width) && s->x_width > 0.0 &&
+                    std::isfinite(s->y_width) && s->y_width > 0.0 && std::isfinite(s->dx) && s->dx > 0.0 &&
+                    std::isfinite(s->dy) && s->dy > 0.0 && std::isfinite(s->charge) && s->charge != 0.0 &&
+                    std::isfinite(s->kpara0) && s->kpara0 > 0.0 && std::isfinite(s->kperp_over_kpara) &&
+                    s->kperp_over_kpara >= 0.0 && s->max_particle_steps_per_interval > 0 &&
+                    (s->precision == KPT_PREC_FP64 || s->precision == KPT_PREC_FP32);
+    if (!ok) return fail(ctx, KPT_ERR_ARG, "Reconnection2DSettings: invalid physical parameters.");
+    return KPT_OK;
+}
+
+kpt_status ensure_field_buffers(kpt_ctx* ctx, FieldSlot& f, int32_t nx, int32_t ny, int model) {
+    const size_t raw = static_cast<size_t>(nx) * ny * 6;
+    const size_t nodes = static_cast<size_t>(nx + 2) * (ny + 2) * kpt::coefficient_stride(model);
+    if (raw > f.raw_cap) {
+        dev_free(f.raw);
+        f.raw = nullptr;
+        f.raw_cap = 0;
+        KPT_HIP(ctx, dev_alloc(&f.raw, raw));
+        f.raw_cap = raw;
+    }
+    if (nodes > f.nodes_cap) {
+        dev_free(f.c64);
+        dev_free(f.c32);
+        f.c64 = nullptr;
+        f.c32 = nullptr;
+        f.nodes_cap = 0;
+        KPT_HIP(ctx, dev_alloc(&f.c64, nodes));
+        KPT_HIP(ctx, dev_alloc(&f.c32, nodes));
+        f.nodes_cap = nodes;
+    }
+    return KPT_OK;
+}
+
+// Compact-field parser shared by kpt_compact_field_read and kpt_field_load_compact.
+bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double>& xe,
+                  std::vector<double>& ye, std::vector<double>& comps, bool allow_nonuniform,
+                  std::string& err) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
+        err = "AthenaFieldReader: failed to open Athena binary file.";
+        return false;
+    }
+    in.read(reinterpret_cast<char*>(&nx), 4);
+    if (!in) {
+        err = "AthenaFieldReader: failed to read nx.";
+        return false;
+    }
+    in.read(reinterpret_cast<char*>(&ny), 4);
+    if (!in) {
+        err = "AthenaFieldReader: failed to read ny.";
+        return false;
+    }
+    if (nx <= 0 || ny <= 0) {
+        err = "AthenaFieldReader: Athena binary dimensions must be positive.";
+        return false;
+    }
+    const size_t cells = static_cast<size_t>(nx) * static_cast<size_t>(ny);
+    xe.resize(static_cast<size_t>(nx) + 1);
+    ye.resize(static_cast<size_t>(ny) + 1);
+    in.read(reinterpret_cast<char*>(xe.data()), static_cast<std::streamsize>(xe.size() * 8));
+    if (!in) {
+        err = "AthenaFieldReader: failed to read x_edges.";
+        return false;
+    }
+    in.read(reinterpret_cast<char*>(ye.data()), static_cast<std::streamsize>(ye.size() * 8));
+    if (!in) {
+        err = "AthenaFieldReader: failed to read y_edges.";
+        return false;
+    }
+    if (!check_edges(xe.data(), xe.size(), "x", !allow_nonuniform, err)) return false;
+    if (!check_edges(ye.data(), ye.size(), "y", !allow_nonuniform, err)) return false;
+    comps.resize(cells * 6);
+    static const char* names[6] = {"Bx", "By", "Bz", "Vx", "Vy", "Vz"};
+    for (int c = 0; c < 6; ++c) {
+        double* dst = comps.data() + static_cast<size_t>(c) * cells;
+        in.read(reinterpret_cast<char*>(dst), static_cast<std::streamsize>(cells * 8));
+        if (!in) {
+            err = std::string("AthenaFieldReader: failed to read ") + names[c] + ".";
+            return false;
+        }
+        for (size_t k = 0; k < cells; ++k) {
+            if (!std::isfinite(dst[k])) {
+                err = std::string("AthenaFieldReader: non-finite ") + names[c] + " value.";
+                return false;
+            }
+        }
+    }
+    if (in.peek() != std::char_traits<char>::eof()) {
+        err = "AthenaFieldReader: Athena binary file has trailing payload.";
+        return false;
+    }
+    return true;
+}
+
+kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, int32_t ny,
+                        const double* xe, const double* ye, const double* const planes[6]) {
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    if (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21)
+        return fail(ctx, KPT_ERR_ARG, "invalid coefficient model (16 or 21).");
+    if (nx <= 0 || ny <= 0) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: Athena binary dimensions must be positive.");
+    std::string err;
+    if (!check_edges(xe, static_cast<size_t>(nx) + 1, "x", true, err) ||
+        !check_edges(ye, static_cast<size_t>(ny) + 1, "y", true, err))
+        return fail(ctx, KPT_ERR_ARG, err);
+    FieldSlot& f = ctx->slot[slot];
+    f.valid = false;
+    kpt_status st = ensure_field_buffers(ctx, f, nx, ny, model);
+    if (st != KPT_OK) return st;
+    const size_t cells = static_cast<size_t>(nx) * ny;
+    for (int c = 0; c < 6; ++c) {
+        KPT_HIP(ctx, hipMemcpyAsync(f.raw + c * cells, planes[c], cells * sizeof(double), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
+    f.g = make_grid_desc(nx, ny, xe, ye);
+    f.model = model;
+    KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    f.valid = true;
+    return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kpt_version(void) { return "kpt-mi355x 0.1 (gfx950)"; }
+
+const char* kpt_last_create_error(void) { return g_create_error.c_str(); }
+
+kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t seed) {
+    g_create_error.clear();
+    if (!out) {
+        g_create_error = "output pointer is null.";
+        return KPT_ERR_ARG;
+    }
+    *out = nullptr;
+    if (capacity == 0) {
+        g_create_error = "Reconnection2DSettings: particle counts and capacity must be positive.";
+        return KPT_ERR_ARG;
+    }
+    if (capacity >= (1ull << 32)) {
+        g_create_error = "capacity must be below 2^32 particles.";
+        return KPT_ERR_ARG;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_create_error = "no HIP device is available (this library runs only on gfx950).";
+        return KPT_ERR_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "device index out of range.";
+        return KPT_ERR_ARG;
+    }
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) {
+        g_create_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+        return KPT_ERR_DEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_error = std::string("device architecture ") + prop.gcnArchName + " is not gfx950.";
+        return KPT_ERR_DEVICE;
+    }
+    if ((e = hipSetDevice(device)) != hipSuccess) {
+        g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return KPT_ERR_DEVICE;
+    }
+    kpt_ctx* ctx = new kpt_ctx();
+    ctx->device = device;
+    ctx->capacity = capacity;
+    ctx->seed = seed;
+    std::snprintf(ctx->device_name, sizeof(ctx->device_name), "%s", prop.gcnArchName);
+    auto bail = [&](hipError_t err, const char* what) {
+        g_create_error = std::string(what) + ": " + hipGetErrorString(err);
+        kpt_destroy(ctx);
+        return KPT_ERR_DEVICE;
+    };
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    kpt::ParticleArrays& P = ctx->P;
+    if ((e = dev_alloc(&P.x, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.y, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.p, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.t, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.step_dt, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.weight, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.mu, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.status, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.split_level, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.id, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
+    ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
+    if ((e = dev_alloc(&ctx->split_scratch, ctx->split_scratch_cap)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&ctx->hist_partials, kpt::hist_partial_elems(4096))) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&ctx->hist_out, 4096)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMalloc(&ctx->summary_partials, kpt::summary_partial_bytes())) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&ctx->summary_out, 1)) != hipSuccess) return bail(e, "hipMalloc");
+    // ReconnectionParticleStorage: status = 2, split level = 0, weight = 0
+    // (src/main.cpp:329-331).
+    std::vector<int32_t> two(static_cast<size_t>(std::min<uint64_t>(capacity, 1u << 20)), 2);
+    for (uint64_t off = 0; off < capacity; off += two.size()) {
+        const size_t n2 = static_cast<size_t>(std::min<uint64_t>(two.size(), capacity - off));
+        if ((e = hipMemcpy(P.status + off, two.data(), n2 * 4, hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy");
+    }
+    if ((e = hipMemset(P.split_level, 0, capacity * 4)) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(P.weight, 0, capacity * 8)) != hipSuccess) return bail(e, "hipMemset");
+    *out = ctx;
+    return KPT_OK;
+}
+
+kpt_status kpt_destroy(kpt_ctx* ctx) {
+    if (!ctx) return KPT_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    kpt::ParticleArrays& P = ctx->P;
+    for (void* p : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p),
+                    static_cast<void*>(P.t), static_cast<void*>(P.step_dt), static_cast<void*>(P.weight),
+                    static_cast<void*>(P.mu), static_cast<void*>(P.status), static_cast<void*>(P.split_level),
+                    static_cast<void*>(P.id), static_cast<void*>(ctx->counters),
+                    static_cast<void*>(ctx->split_scratch), static_cast<void*>(ctx->hist_partials),
+                    static_cast<void*>(ctx->hist_out), ctx->summary_partials,
+                    static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
+        dev_free(p);
+    for (FieldSlot& f : ctx->slot) {
+        dev_free(f.raw);
+        dev_free(f.c64);
+        dev_free(f.c32);
+    }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KPT_OK;
+}
+
+const char* kpt_last_error(const kpt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+kpt_status kpt_device_name(kpt_ctx* ctx, char* buf, size_t len) {
+    if (!ctx || !buf || len == 0) return KPT_ERR_ARG;
+    std::snprintf(buf, len, "%s", ctx->device_name);
+    return KPT_OK;
+}
+
+kpt_status kpt_default_host_settings(kpt_host_settings* o) {
+    if (!o) return KPT_ERR_ARG;
+    o->max_particle_steps_per_interval = 200000;
+    o->momentum_dependency = 1;
+    o->magnetic_dependency = 1;
+    o->split_particles = 1;
+    o->time_interpolation = 1;
+    o->focused_transport = 0;
+    o->mhd_dt = 0.1;
+    o->p0 = 0.1;
+    o->pmin = 1.0e-2;
+    o->pmax = 1.0e1;
+    o->gamma_turb = 1.6666667;
+    o->kpara0 = 0.00743592;
+    o->kperp_over_kpara = 0.01;
+    o->dt_min_rel = 1.0e-7;
+    o->dt_max_rel = 1.0e-2;
+    o->drift_param1 = 850964.408;
+    o->drift_param2 = 13575468.975;
+    o->charge = -1.0;
+    o->particle_v0 = 17.20195;
+    o->duu0 = 5578.445;
+    o->split_ratio = 2.0;
+    o->pmin_split_over_p0 = 2.0;
+    o->mu_max = 0.99;
+    return KPT_OK;
+}
+
+kpt_status kpt_make_device_settings(const kpt_host_settings* h, int32_t nx, int32_t ny, const double* xe,
+                                    const double* ye, kpt_device_settings* o) {
+    if (!h || !o || !xe || !ye || nx <= 0 || ny <= 0) return KPT_ERR_ARG;
+    const kpt::GridDesc g = make_grid_desc(nx, ny, xe, ye);
+    std::memset(o, 0, sizeof(*o));
+    o->max_particle_steps_per_interval = h->max_particle_steps_per_interval;
+    o->momentum_dependency = h->momentum_dependency;
+    o->magnetic_dependency = h->magnetic_dependency;
+    o->split_particles = h->split_particles ? 1 : 0;
+    o->time_interpolation = h->time_interpolation ? 1 : 0;
+    o->focused_transport = h->focused_transport ? 1 : 0;
+    o->x_lower = g.phys_lo[0];
+    o->x_upper = g.phys_hi[0];
+    o->y_lower = g.phys_lo[1];
+    o->y_upper = g.phys_hi[1];
+    o->x_width = o->x_upper - o->x_lower;
+    o->y_width = o->y_upper - o->y_lower;
+    o->dx = o->x_width / static_cast<double>(nx);
+    o->dy = o->y_width / static_cast<double>(ny);
+    o->interpolation_shift_x = 0.5 * o->dx;
+    o->interpolation_shift_y = 0.5 * o->dy;
+    o->mhd_dt = h->mhd_dt;
+    o->p0 = h->p0;
+    o->pmax = h->pmax;
+    o->gamma_turb = h->gamma_turb;
+    o->pindex = 3.0 - h->gamma_turb;
+    o->kpara0 = h->kpara0;
+    o->kperp_over_kpara = h->kperp_over_kpara;
+    o->dt_min = h->dt_min_rel * h->mhd_dt;
+    o->dt_max = h->dt_max_rel * h->mhd_dt;
+    o->drift_param1 = h->drift_param1;
+    o->drift_param2 = h->drift_param2;
+    o->charge = h->charge;
+    o->particle_v0 = h->particle_v0;
+    o->duu0 = h->duu0;
+    o->pitch_angle_scattering_h0 = 0.2;  // member default, src/main.cpp:279
+    o->split_ratio = h->split_ratio;
+    o->pmin_split = h->pmin_split_over_p0 * h->p0;
+    o->mu_max = h->mu_max;
+    o->precision = KPT_PREC_FP64;
+    return KPT_OK;
+}
+
+kpt_status kpt_set_seed(kpt_ctx* ctx, uint64_t seed) {
+    if (!ctx) return KPT_ERR_ARG;
+    ctx->seed = seed;
+    return KPT_OK;
+}
+
+// ---- fields ----------------------------------------------------------------
+
+kpt_status kpt_field_upload(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, int32_t ny,
+                            const double* xe, const double* ye, const double* bx, const double* by,
+                            const double* bz, const double* vx, const double* vy, const double* vz) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (!xe || !ye || !bx || !by || !bz || !vx || !vy || !vz)
+        return fail(ctx, KPT_ERR_ARG, "field array pointer is null.");
+    for (const double* plane : {bx, by, bz, vx, vy, vz}) {
+        const size_t cells = static_cast<size_t>(std::max(nx, 0)) * std::max(ny, 0);
+        for (size_t k = 0; k < cells; ++k)
+            if (!std::isfinite(plane[k])) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: non-finite field value.");
+    }
+    const double* planes[6] = {bx, by, bz, vx, vy, vz};
+    (void)hipSetDevice(ctx->device);
+    return upload_field(ctx, slot, model, nx, ny, xe, ye, planes);
+}
+
+kpt_status kpt_field_load_compact(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    int32_t nx = 0, ny = 0;
+    std::vector<double> xe, ye, comps;
+    std::string err;
+    if (!read_compact(path, nx, ny, xe, ye, comps, false, err)) return fail(ctx, KPT_ERR_IO, err);
+    const size_t cells = static_cast<size_t>(nx) * ny;
+    const double* planes[6];
+    for (int c = 0; c < 6; ++c) planes[c] = comps.data() + c * cells;
+    (void)hipSetDevice(ctx->device);
+    return upload_field(ctx, slot, model, nx, ny, xe.data(), ye.data(), planes);
+}
+
+kpt_status kpt_field_advance(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (!ctx->slot[KPT_SLOT_UPPER].valid) return fail(ctx, KPT_ERR_STATE, "upper field slot is empty.");
+    std::swap(ctx->slot[KPT_SLOT_LOWER], ctx->slot[KPT_SLOT_UPPER]);
+    ctx->slot[KPT_SLOT_UPPER].valid = false;
+    return KPT_OK;
+}
+
+kpt_status kpt_field_download_coefficients(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len) {
+    if (!ctx || !out || (slot != 0 && slot != 1)) return KPT_ERR_ARG;
+    const FieldSlot& f = ctx->slot[slot];
+    if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    const size_t nodes = static_cast<size_t>(f.g.nx + 2) * (f.g.ny + 2);
+    const int stride = kpt::coefficient_stride(f.model);
+    if (out_len < nodes * f.model) return fail(ctx, KPT_ERR_ARG, "output buffer too small.");
+    std::vector<double> tmp(nodes * stride);
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemcpy(tmp.data(), f.c64, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t n = 0; n < nodes; ++n)
+        for (int c = 0; c < f.model; ++c) out[n * f.model + c] = tmp[n * stride + c];
+    return KPT_OK;
+}
+
+kpt_status kpt_compact_field_read(const char* path, int32_t* nx, int32_t* ny, double* x_edges, double* y_edges,
+                                  double* bx, double* by, double* bz, double* vx, double* vy, double* vz,
+                                  int32_t allow_nonuniform, char* err, size_t err_len) {
+    int32_t n0 = 0, n1 = 0;
+    std::vector<double> xe, ye, comps;
+    std::string e;
+    if (!path || !read_compact(path, n0, n1, xe, ye, comps, allow_nonuniform != 0, e)) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", path ? e.c_str() : "path is null.");
+        return path ? KPT_ERR_IO : KPT_ERR_ARG;
+    }
+    if (nx) *nx = n0;
+    if (ny) *ny = n1;
+    const size_t cells = static_cast<size_t>(n0) * n1;
+    if (x_edges) std::memcpy(x_edges, xe.data(), xe.size() * 8);
+    if (y_edges) std::memcpy(y_edges, ye.data(), ye.size() * 8);
+    double* outs[6] = {bx, by, bz, vx, vy, vz};
+    for (int c = 0; c < 6; ++c)
+        if (outs[c]) std::memcpy(outs[c], comps.data() + c * cells, cells * 8);
+    return KPT_OK;
+}
+
+kpt_status kpt_compact_field_write(const char* path, int32_t nx, int32_t ny, const double* xe, const double* ye,
+                                   const double* bx, const double* by, const double* bz, const double* vx,
+                                   const double* vy, const double* vz) {
+    if (!path || nx <= 0 || ny <= 0 || !xe || !ye || !bx || !by || !bz || !vx || !vy || !vz) return KPT_ERR_ARG;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return KPT_ERR_IO;
+    const size_t cells = static_cast<size_t>(nx) * ny;
+    out.write(reinterpret_cast<const char*>(&nx), 4);
+    out.write(reinterpret_cast<const char*>(&ny), 4);
+    out.write(reinterpret_cast<const char*>(xe), static_cast<std::streamsize>((static_cast<size_t>(nx) + 1) * 8));
+    out.write(reinterpret_cast<const char*>(ye), static_cast<std::streamsize>((static_cast<size_t>(ny) + 1) * 8));
+    for (const double* plane : {bx, by, bz, vx, vy, vz})
+        out.write(reinterpret_cast<const char*>(plane), static_cast<std::streamsize>(cells * 8));
+    return out ? KPT_OK : KPT_ERR_IO;
+}
+
+// ---- particles -------------------------------------------------------------
+
+kpt_status kpt_particles_count(kpt_ctx* ctx, uint64_t* count) {
+    if (!ctx || !count) return KPT_ERR_ARG;
+    *count = ctx->count;
+    return KPT_OK;
+}
+
+kpt_status kpt_particles_capacity(kpt_ctx* ctx, uint64_t* capacity) {
+    if (!ctx || !capacity) return KPT_ERR_ARG;
+    *capacity = ctx->capacity;
+    return KPT_OK;
+}
+
+kpt_status kpt_particles_upload(kpt_ctx* ctx, const kpt_particles_host* h, uint64_t count) {
+    if (!ctx || !h) return KPT_ERR_ARG;
+    if (count > ctx->capacity) return fail(ctx, KPT_ERR_CAPACITY, "particle count exceeds capacity.");
+    if (count > 0 && (!h->x || !h->y || !h->p || !h->t || !h->step_dt || !h->weight || !h->mu || !h->status ||
+                      !h->split_level))
+        return fail(ctx, KPT_ERR_ARG, "particle array pointer is null.");
+    (void)hipSetDevice(ctx->device);
+    kpt::ParticleArrays& P = ctx->P;
+    if (count > 0) {
+        KPT_HIP(ctx, hipMemcpy(P.x, h->x, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.y, h->y, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.p, h->p, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.t, h->t, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.step_dt, h->step_dt, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.weight, h->weight, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.mu, h->mu, count * 8, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.status, h->status, count * 4, hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(P.split_level, h->split_level, count * 4, hipMemcpyHostToDevice));
+    }
+    uint64_t max_root = 0;
+    if (h->id) {
+        if (count > 0) KPT_HIP(ctx, hipMemcpy(P.id, h->id, count * 8, hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < count; ++i) max_root = std::max<uint64_t>(max_root, (h->id[i] & 0xFFFFFFFFull) + 1);
+    } else if (count > 0) {
+        std::vector<uint64_t> ids(count);
+        for (uint64_t i = 0; i < count; ++i) ids[i] = i;
+        KPT_HIP(ctx, hipMemcpy(P.id, ids.data(), count * 8, hipMemcpyHostToDevice));
+        max_root = count;
+    }
+    // Slots past the stored range return to the storage defaults.
+    if (ctx->count > count) {
+        std::vector<int32_t> two(ctx->count - count, 2);
+        KPT_HIP(ctx, hipMemcpy(P.status + count, two.data(), two.size() * 4, hipMemcpyHostToDevice));
+    }
+    ctx->count = count;
+    ctx->next_root = max_root;
+    return KPT_OK;
+}
+
+kpt_status kpt_particles_download(kpt_ctx* ctx, kpt_particles_host* h, uint64_t count) {
+    if (!ctx || !h) return KPT_ERR_ARG;
+    if (count > ctx->count) return fail(ctx, KPT_ERR_ARG, "download count exceeds stored particles.");
+    if (count == 0) return KPT_OK;
+    (void)hipSetDevice(ctx->device);
+    const kpt::ParticleArrays& P = ctx->P;
+    if (h->x) KPT_HIP(ctx, hipMemcpy(h->x, P.x, count * 8, hipMemcpyDeviceToHost));
+    if (h->y) KPT_HIP(ctx, hipMemcpy(h->y, P.y, count * 8, hipMemcpyDeviceToHost));
+    if (h->p) KPT_HIP(ctx, hipMemcpy(h->p, P.p, count * 8, hipMemcpyDeviceToHost));
+    if (h->t) KPT_HIP(ctx, hipMemcpy(h->t, P.t, count * 8, hipMemcpyDeviceToHost));
+    if (h->step_dt) KPT_HIP(ctx, hipMemcpy(h->step_dt, P.step_dt, count * 8, hipMemcpyDeviceToHost));
+    if (h->weight) KPT_HIP(ctx, hipMemcpy(h->weight, P.weight, count * 8, hipMemcpyDeviceToHost));
+    if (h->mu) KPT_HIP(ctx, hipMemcpy(h->mu, P.mu, count * 8, hipMemcpyDeviceToHost));
+    if (h->status) KPT_HIP(ctx, hipMemcpy(h->status, P.status, count * 4, hipMemcpyDeviceToHost));
+    if (h->split_level) KPT_HIP(ctx, hipMemcpy(h->split_level, P.split_level, count * 4, hipMemcpyDeviceToHost));
+    if (h->id) KPT_HIP(ctx, hipMemcpy(h->id, P.id, count * 8, hipMemcpyDeviceToHost));
+    return KPT_OK;
+}
+
+// ---- hot path ----------------------------------------------------------------
+
+kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inject_opts* opts, uint64_t n,
+                      double t0, uint32_t frame, uint64_t* injected) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_settings(ctx, s);
+    if (st != KPT_OK) return st;
+    kpt_inject_opts o{};
+    if (opts) o = *opts;
+    if (o.use_region && !(o.region_x_hi > o.region_x_lo && o.region_y_hi > o.region_y_lo))
+        return fail(ctx, KPT_ERR_ARG, "injection region must have positive extent.");
+    if (injected) *injected = 0;
+    if (ctx->count >= ctx->capacity) return KPT_OK;
+    const uint64_t m = std::min(n, ctx->capacity - ctx->count);
+    (void)hipSetDevice(ctx->device);
+    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
+    KPT_HIP(ctx, kpt::launch_inject(ctx->P, ctx->count, m, ctx->next_root, *s, o, t0, frame, key, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->count += m;
+    ctx->next_root += m;
+    if (injected) *injected = m;
+    return KPT_OK;
+}
+
+kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame,
+                             kpt_push_stats* stats) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_settings(ctx, s);
+    if (st != KPT_OK) return st;
+    const FieldSlot& lo = ctx->slot[KPT_SLOT_LOWER];
+    const FieldSlot& hi = ctx->slot[KPT_SLOT_UPPER];
+    if (!lo.valid) return fail(ctx, KPT_ERR_STATE, "lower coefficient frame is not loaded.");
+    if (s->time_interpolation != 0) {
+        if (!hi.valid) return fail(ctx, KPT_ERR_STATE, "upper coefficient frame is not loaded.");
+        if (hi.model != lo.model || hi.g.nx != lo.g.nx || hi.g.ny != lo.g.ny || hi.g.start[0] != lo.g.start[0] ||
+            hi.g.start[1] != lo.g.start[1] || hi.g.step[0] != lo.g.step[0] || hi.g.step[1] != lo.g.step[1])
+            return fail(ctx, KPT_ERR_STATE, "lower and upper coefficient frames have different grids or models.");
+    }
+    if (ctx->noise) {
+        if (ctx->noise_particles < ctx->count)
+            return fail(ctx, KPT_ERR_STATE, "noise table covers fewer particles than are stored.");
+        if (static_cast<uint64_t>(s->max_particle_steps_per_interval) > ctx->noise_steps)
+            return fail(ctx, KPT_ERR_STATE, "max_particle_steps_per_interval exceeds the noise table's steps.");
+    }
+    kpt::PushArgs a{};
+    a.P = ctx->P;
+    a.count = ctx->count;
+    const bool fp32 = s->precision == KPT_PREC_FP32;
+    a.field_lo = fp32 ? static_cast<const void*>(lo.c32) : static_cast<const void*>(lo.c64);
+    a.field_hi = s->time_interpolation != 0 ? (fp32 ? static_cast<const void*>(hi.c32) : static_cast<const void*>(hi.c64))
+                                            : a.field_lo;
+    a.stride = kpt::coefficient_stride(lo.model);
+    a.g = lo.g;
+    a.s = *s;
+    a.t0 = t0;
+    a.t_end = t0 + s->mhd_dt;
+    a.key[0] = static_cast<uint32_t>(ctx->seed);
+    a.key[1] = static_cast<uint32_t>(ctx->seed >> 32);
+    a.frame = frame;
+    a.noise = ctx->noise;
+    a.noise_steps = ctx->noise_steps;
+    a.counters = ctx->counters;
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
+    KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, ctx->stream));
+    KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long c[8];
+    KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    KPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) {
+        stats->particle_steps = c[0];
+        stats->failures = c[1];
+        stats->active_in = c[2];
+        stats->kernel_ms = ms;
+        stats->launches = ctx->count > 0 ? 1u : 0u;
+        stats->reserved = 0;
+    }
+    return KPT_OK;
+}
+
+kpt_status kpt_split(kpt_ctx* ctx, const kpt_device_settings* s, uint64_t* split, uint64_t* skipped) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_settings(ctx, s);
+    if (st != KPT_OK) return st;
+    if (split) *split = 0;
+    if (skipped) *skipped = 0;
+    if (ctx->count >= ctx->capacity || s->split_particles == 0 || ctx->count == 0) return KPT_OK;
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_split(ctx->P, ctx->count, ctx->capacity, *s, ctx->split_scratch, ctx->counters,
+                                   ctx->stream));
+    unsigned long long c[2];
+    uint32_t total = 0;
+    const unsigned tiles = static_cast<unsigned>(kpt::split_scratch_elems(ctx->count) - 1);
+    KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(&total, ctx->split_scratch + tiles, 4, hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->count = std::min<uint64_t>(ctx->count + total, ctx->capacity);
+    if (split) *split = c[0];
+    if (skipped) *skipped = c[1];
+    return KPT_OK;
+}
+
+kpt_status kpt_histogram_logp(kpt_ctx* ctx, int32_t bins, double pmin, double pmax, double* out) {
+    if (!ctx || !out) return KPT_ERR_ARG;
+    if (bins <= 0 || bins > 4096) return fail(ctx, KPT_ERR_ARG, "histogram bins must be in [1, 4096].");
+    if (!(pmin > 0.0) || !(pmax > pmin) || !std::isfinite(pmax))
+        return fail(ctx, KPT_ERR_ARG, "histogram range must satisfy 0 < pmin < pmax.");
+    const double log_pmin = std::log(pmin);
+    const double inv_dlogp = static_cast<double>(bins) / (std::log(pmax) - log_pmin);
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, kpt::launch_hist_logp(ctx->P, ctx->count, bins, log_pmin, inv_dlogp, pmin, pmax, ctx->hist_partials,
+                                       ctx->hist_out, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(out, ctx->hist_out, bins * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+kpt_status kpt_histogram_xy(kpt_ctx* ctx, int32_t bx, int32_t by, double x_lo, double x_hi, double y_lo,
+                            double y_hi, double* out) {
+    if (!ctx || !out) return KPT_ERR_ARG;
+    if (bx <= 0 || by <= 0 || static_cast<int64_t>(bx) * by > 4096)
+        return fail(ctx, KPT_ERR_ARG, "spatial histogram must have 1..4096 bins.");
+    if (!(x_hi > x_lo) || !(y_hi > y_lo)) return fail(ctx, KPT_ERR_ARG, "spatial histogram range is empty.");
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, kpt::launch_hist_xy(ctx->P, ctx->count, bx, by, x_lo, x_hi, y_lo, y_hi, ctx->hist_partials,
+                                     ctx->hist_out, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(out, ctx->hist_out, static_cast<size_t>(bx) * by * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+kpt_status kpt_summary_compute(kpt_ctx* ctx, kpt_summary* out) {
+    if (!ctx || !out) return KPT_ERR_ARG;
+    if (ctx->count == 0) {
+        std::memset(out, 0, sizeof(*out));
+        return KPT_OK;
+    }
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, kpt::launch_summary(ctx->P, ctx->count, ctx->summary_partials, ctx->summary_out, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(out, ctx->summary_out, sizeof(kpt_summary), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+// ---- noise -------------------------------------------------------------------
+
+kpt_status kpt_set_noise(kpt_ctx* ctx, const double* draws, uint64_t n_particles, uint32_t steps) {
+    if (!ctx || !draws || n_particles == 0 || steps == 0)
+        return fail(ctx, KPT_ERR_ARG, "noise table must be non-empty.");
+    (void)hipSetDevice(ctx->device);
+    dev_free(ctx->noise);
+    ctx->noise = nullptr;
+    const size_t elems = static_cast<size_t>(n_particles) * steps * 4;
+    KPT_HIP(ctx, dev_alloc(&ctx->noise, elems));
+    KPT_HIP(ctx, hipMemcpy(ctx->noise, draws, elems * sizeof(double), hipMemcpyHostToDevice));
+    ctx->noise_particles = n_particles;
+    ctx->noise_steps = steps;
+    return KPT_OK;
+}
+
+kpt_status kpt_clear_noise(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    dev_free(ctx->noise);
+    ctx->noise = nullptr;
+    ctx->noise_particles = 0;
+    ctx->noise_steps = 0;
+    return KPT_OK;
+}
+
+void kpt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t m0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t m1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(m1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(m0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = static_cast<uint32_t>(m1);
+        c2 = n2;
+        c3 = static_cast<uint32_t>(m0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// ---- v4 snapshots --------------------------------------------------------------
+
+kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h, uint64_t count,
+                                   uint64_t capacity, double charge, double split_ratio, double p0) {
+    if (!path || !h) return KPT_ERR_ARG;
+    if (count > 0 && (!h->x || !h->y || !h->p || !h->mu || !h->weight || !h->status || !h->split_level))
+        return KPT_ERR_ARG;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return KPT_ERR_IO;
+    // Header (src/main.cpp:2155-2181): 88 bytes. Records (:2183-2201): 100 bytes.
+    std::vector<char> buf;
+    buf.reserve(88 + static_cast<size_t>(count) * 100);
+    auto put = [&buf](const void* v, size_t bytes) {
+        const char* c = static_cast<const char*>(v);
+        buf.insert(buf.end(), c, c + bytes);
+    };
+    const char magic[8] = {'K', 'P', 'T', 'P', 'R', 'T', '\0', '\0'};
+    const uint32_t version = 4u, endian = 0x01020304u;
+    const int32_t dim = 2, species = 3;
+    const double rest_mass = 1.0, c_light = 1.0, energy_scale = 1.0, min_child_weight = 0.0;
+    put(magic, 8);
+    put(&version, 4);
+    put(&endian, 4);
+    put(&dim, 4);
+    put(&species, 4);
+    put(&count, 8);
+    put(&capacity, 8);
+    put(&rest_mass, 8);
+    put(&charge, 8);
+    put(&c_light, 8);
+    put(&energy_scale, 8);
+    put(&split_ratio, 8);
+    put(&min_child_weight, 8);
+    const double e0 = std::sqrt(1.0 + p0 * p0) - 1.0;
+    const int32_t sort_key = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        put(&i, 8);
+        put(&h->status[i], 4);
+        put(&h->split_level[i], 4);
+        put(&sort_key, 4);
+        for (int rep = 0; rep < 3; ++rep) {  // position, previous, previous-step
+            put(&h->x[i], 8);
+            put(&h->y[i], 8);
+        }
+        put(&h->p[i], 8);
+        put(&h->mu[i], 8);
+        put(&h->weight[i], 8);
+        put(&e0, 8);
+    }
+    out.write(buf.data(), static_cast<std::streamsize>(buf.size()));
+    return out ? KPT_OK : KPT_ERR_IO;
+}
+
+kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint64_t* count_io, double charge,
+                                  double split_ratio, char* err, size_t err_len) {
+    auto bad = [&](kpt_status code, const std::string& m) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", m.c_str());
+        return code;
+    };
+    if (!path || !count_io) return bad(KPT_ERR_ARG, "path or count pointer is null.");
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return bad(KPT_ERR_IO, "Failed to open restart particle snapshot.");
+    char magic[8];
+    in.read(magic, 8);
+    const char expect[8] = {'K', 'P', 'T', 'P', 'R', 'T', '\0', '\0'};
+    if (!in || std::memcmp(magic, expect, 8) != 0)
+        return bad(KPT_ERR_IO, "Restart particle snapshot is not a repository KPTPRT file.");
+    auto get = [&in](void* v, size_t bytes) {
+        in.read(static_cast<char*>(v), static_cast<std::streamsize>(bytes));
+        return static_cast<bool>(in);
+    };
+    uint32_t version = 0, endian = 0;
+    int32_t dim = 0, species = 0;
+    uint64_t count = 0, capacity = 0;
+    double hdr[6];
+    if (!get(&version, 4) || !get(&endian, 4)) return bad(KPT_ERR_IO, "Failed to read particle binary scalar.");
+    if (version != 4u) return bad(KPT_ERR_IO, "Restart particle snapshot must use particle binary version 4.");
+    if (endian != 0x01020304u) return bad(KPT_ERR_IO, "Restart particle snapshot endian does not match this host.");
+    if (!get(&dim, 4) || !get(&species, 4) || !get(&count, 8) || !get(&capacity, 8) || !get(hdr, 48))
+        return bad(KPT_ERR_IO, "Failed to read particle binary scalar.");
+    if (dim != 2)
+        return bad(KPT_ERR_IO, "Restart particle snapshot dimensionality does not match reconnection storage.");
+    if (h == nullptr) {
+        *count_io = count;
+        return KPT_OK;
+    }
+    if (count > *count_io) {
+        return bad(KPT_ERR_CAPACITY, "Restart particle snapshot stores " + std::to_string(count) +
+                                         " particles, but configured capacity is " + std::to_string(*count_io) +
+                                         ". Snapshot capacity was " + std::to_string(capacity) + ".");
+    }
+    auto matches = [](double a, double b) {
+        if (!std::isfinite(a) || !std::isfinite(b)) return false;
+        const double scale = std::max({1.0, std::fabs(a), std::fabs(b)});
+        return std::fabs(a - b) <= 1.0e-12 * scale;
+    };
+    if (!matches(hdr[1], charge))
+        return bad(KPT_ERR_ARG, "Restart particle snapshot charge does not match current settings.");
+    if (!matches(hdr[4], split_ratio))
+        return bad(KPT_ERR_ARG, "Restart particle snapshot split ratio does not match current settings.");
+    std::vector<char> rec(static_cast<size_t>(count) * 100);
+    if (count > 0 && !get(rec.data(), rec.size())) return bad(KPT_ERR_IO, "Failed to read particle binary scalar.");
+    for (uint64_t i = 0; i < count; ++i) {
+        const char* r = rec.data() + i * 100;
+        if (h->id) std::memcpy(&h->id[i], r + 0, 8);
+        if (h->status) std::memcpy(&h->status[i], r + 8, 4);
+        if (h->split_level) std::memcpy(&h->split_level[i], r + 12, 4);
+        if (h->x) std::memcpy(&h->x[i], r + 20, 8);
+        if (h->y) std::memcpy(&h->y[i], r + 28, 8);
+        if (h->p) std::memcpy(&h->p[i], r + 68, 8);
+        if (h->mu) std::memcpy(&h->mu[i], r + 76, 8);
+        if (h->weight) std::memcpy(&h->weight[i], r + 84, 8);
+    }
+    *count_io = count;
+    return KPT_OK;
+}
+
+kpt_status kpt_snapshot_write(kpt_ctx* ctx, const char* path, double charge, double split_ratio, double p0) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    const uint64_t n = ctx->count;
+    std::vector<double> x(n), y(n), p(n), mu(n), w(n);
+    std::vector<int32_t> status(n), level(n);
+    kpt_particles_host h{x.data(), y.data(), p.data(), nullptr, nullptr, w.data(), mu.data(),
+                         status.data(), level.data(), nullptr};
+    kpt_status st = kpt_particles_download(ctx, &h, n);
+    if (st != KPT_OK) return st;
+    st = kpt_snapshot_write_host(path, &h, n, ctx->capacity, charge, split_ratio, p0);
+    if (st != KPT_OK) return fail(ctx, st, "Failed to write particle snapshot.");
+    return KPT_OK;
+}
+
+kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
+                             double restart_time, double dt_min) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    char err[512] = {0};
+    uint64_t count = 0;
+    kpt_status st = kpt_snapshot_read_host(path, nullptr, &count, charge, split_ratio, err, sizeof(err));
+    if (st != KPT_OK) return fail(ctx, st, err);
+    if (count > ctx->capacity) {
+        return fail(ctx, KPT_ERR_CAPACITY, "Restart particle snapshot stores " + std::to_string(count) +
+                                               " particles, but configured capacity is " +
+                                               std::to_string(ctx->capacity) + ".");
+    }
+    std::vector<double> x(count), y(count), p(count), t(count, restart_time), dt(count, dt_min), w(count), mu(count);
+    std::vector<int32_t> status(count), level(count);
+    kpt_particles_host h{x.data(), y.data(), p.data(), t.data(), dt.data(), w.data(), mu.data(),
+                         status.data(), level.data(), nullptr};
+    st = kpt_snapshot_read_host(path, &h, &count, charge, split_ratio, err, sizeof(err));
+    if (st != KPT_OK) return fail(ctx, st, err);
+    return kpt_particles_upload(ctx, &h, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// kpt_internal.h — launch interface between the C-ABI host code and the gfx950
+// kernels. Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/kpt.h"
+
+namespace kpt {
+
+// Coefficient components are stored node-major: all components of one node are
+// contiguous, nodes i in [-1, nx], j in [-1, ny] (the stencil of the mover never
+// leaves that range), node (i, j) at ((j + 1) * (nx + 2) + (i + 1)) * stride.
+// The stride is the component count rounded up to a multiple of 8 so every node
+// starts on a 64-byte boundary (FP64 Parker: 128 B = one L2 line per node).
+inline int coefficient_stride(int model) { return (model + 7) / 8 * 8; }
+
+// Uniform cell-centred grid of a loaded frame (AnalyticCoordinateGrid with
+// UniformSpacing, include/AnalyticCoordinateGrid.hpp:159-210, and the bounds
+// cache of include/MappedCoordinateGrid.hpp:117-145).
+struct GridDesc {
+    int32_t nx, ny;
+    double start[2];
+    double step[2];
+    double inv_step[2];
+    double phys_lo[2], phys_hi[2], width[2], inv_width[2];
+};
+
+struct ParticleArrays {
+    double* x;
+    double* y;
+    double* p;
+    double* t;
+    double* step_dt;
+    double* weight;
+    double* mu;
+    int32_t* status;
+    int32_t* split_level;
+    uint64_t* id;
+};
+
+struct PushArgs {
+    ParticleArrays P;
+    uint64_t count;
+    const void* field_lo;  // double or float, node-major, `stride` components per node
+    const void* field_hi;
+    int32_t stride;
+    GridDesc g;
+    kpt_device_settings s;
+    double t0;
+    double t_end;
+    uint32_t key[2];
+    uint32_t frame;
+    const double* noise;  // injected draw table or nullptr
+    uint32_t noise_steps;
+    unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
+};
+
+// Device-side entry points (kpt_kernels.hip). All launch on `stream`.
+hipError_t launch_pack(const double* raw /* 6 planes nx*ny */, const GridDesc& g, int model,
+                       double* out64, float* out32, hipStream_t stream);
+hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, hipStream_t stream);
+hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
+                         const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
+                         uint32_t frame, const uint32_t key[2], hipStream_t stream);
+// Deterministic split. `scratch` must hold split_scratch_elems(count) uint32.
+size_t split_scratch_elems(uint64_t count);
+hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capacity,
+                        const kpt_device_settings& s, uint32_t* scratch,
+                        unsigned long long* counters /* [0] split, [1] skipped */,
+                        hipStream_t stream);
+// Histograms: `partials` holds hist_partial_elems(bins) doubles.
+size_t hist_partial_elems(int bins);
+hipError_t launch_hist_logp(const ParticleArrays& P, uint64_t count, int bins, double log_pmin,
+                            double inv_dlogp, double pmin, double pmax, double* partials,
+                            double* out, hipStream_t stream);
+hipError_t launch_hist_xy(const ParticleArrays& P, uint64_t count, int bx, int by, double x_lo,
+                          double x_hi, double y_lo, double y_hi, double* partials, double* out,
+                          hipStream_t stream);
+// Summary: `partials` holds summary_partial_bytes() bytes.
+size_t summary_partial_bytes();
+hipError_t launch_summary(const ParticleArrays& P, uint64_t count, void* partials,
+                          kpt_summary* out_device, hipStream_t stream);
+
+}  // namespace kpt

@@ -1,0 +1,455 @@
+"""ctypes binding of libkpt.so (the C ABI of include/kpt.h) for tests and bench.
+
+The structures mirror include/kpt.h field by field. The library is loaded
+lazily; `load()` raises if libkpt.so is missing, so nothing here can fall back
+to a CPU path: every compute call goes to the gfx950 kernels.
+
+Particle sets are dicts of numpy arrays with the keys of PARTICLE_FIELDS.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkpt.so")
+
+KPT_OK = 0
+MODEL_PARKER16 = 16
+MODEL_FOCUSED21 = 21
+SLOT_LOWER = 0
+SLOT_UPPER = 1
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+PARTICLE_FIELDS = (
+    ("x", np.float64), ("y", np.float64), ("p", np.float64), ("t", np.float64),
+    ("step_dt", np.float64), ("weight", np.float64), ("mu", np.float64),
+    ("status", np.int32), ("split_level", np.int32), ("id", np.uint64),
+)
+
+
+class DeviceSettings(C.Structure):
+    """kpt_device_settings: Reconnection2DDeviceSettings (src/main.cpp:246-283)."""
+    _fields_ = [
+        ("max_particle_steps_per_interval", C.c_int32),
+        ("momentum_dependency", C.c_int32),
+        ("magnetic_dependency", C.c_int32),
+        ("split_particles", C.c_int32),
+        ("time_interpolation", C.c_int32),
+        ("focused_transport", C.c_int32),
+        ("x_lower", C.c_double), ("x_upper", C.c_double),
+        ("y_lower", C.c_double), ("y_upper", C.c_double),
+        ("x_width", C.c_double), ("y_width", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("interpolation_shift_x", C.c_double), ("interpolation_shift_y", C.c_double),
+        ("mhd_dt", C.c_double), ("p0", C.c_double), ("pmax", C.c_double),
+        ("gamma_turb", C.c_double), ("pindex", C.c_double), ("kpara0", C.c_double),
+        ("kperp_over_kpara", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("drift_param1", C.c_double), ("drift_param2", C.c_double), ("charge", C.c_double),
+        ("particle_v0", C.c_double), ("duu0", C.c_double),
+        ("pitch_angle_scattering_h0", C.c_double), ("split_ratio", C.c_double),
+        ("pmin_split", C.c_double), ("mu_max", C.c_double),
+        ("precision", C.c_int32), ("reserved0", C.c_int32),
+    ]
+
+    def copy(self) -> "DeviceSettings":
+        out = DeviceSettings()
+        C.pointer(out)[0] = self
+        return out
+
+
+class HostSettings(C.Structure):
+    """kpt_host_settings: physics subset of Reconnection2DSettings (src/main.cpp:114-154)."""
+    _fields_ = [
+        ("max_particle_steps_per_interval", C.c_int32),
+        ("momentum_dependency", C.c_int32),
+        ("magnetic_dependency", C.c_int32),
+        ("split_particles", C.c_int32),
+        ("time_interpolation", C.c_int32),
+        ("focused_transport", C.c_int32),
+        ("mhd_dt", C.c_double), ("p0", C.c_double), ("pmin", C.c_double), ("pmax", C.c_double),
+        ("gamma_turb", C.c_double), ("kpara0", C.c_double), ("kperp_over_kpara", C.c_double),
+        ("dt_min_rel", C.c_double), ("dt_max_rel", C.c_double),
+        ("drift_param1", C.c_double), ("drift_param2", C.c_double), ("charge", C.c_double),
+        ("particle_v0", C.c_double), ("duu0", C.c_double), ("split_ratio", C.c_double),
+        ("pmin_split_over_p0", C.c_double), ("mu_max", C.c_double),
+    ]
+
+
+class ParticlesHost(C.Structure):
+    _fields_ = [
+        ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+        ("p", C.POINTER(C.c_double)), ("t", C.POINTER(C.c_double)),
+        ("step_dt", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
+        ("mu", C.POINTER(C.c_double)), ("status", C.POINTER(C.c_int32)),
+        ("split_level", C.POINTER(C.c_int32)), ("id", C.POINTER(C.c_uint64)),
+    ]
+
+
+class PushStats(C.Structure):
+    _fields_ = [
+        ("particle_steps", C.c_uint64), ("failures", C.c_uint64), ("active_in", C.c_uint64),
+        ("kernel_ms", C.c_double), ("launches", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class Summary(C.Structure):
+    """kpt_summary: ParticleSummary (src/main.cpp:338-350)."""
+    _fields_ = [
+        ("stored", C.c_uint64), ("active", C.c_uint64), ("inactive", C.c_uint64),
+        ("weight_sum", C.c_double), ("p_min", C.c_double), ("p_max", C.c_double),
+        ("p_avg", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("dt_avg", C.c_double), ("max_split_level", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class FrameTiming(C.Structure):
+    """kpt_frame_timing: FrameTiming (src/main.cpp:355-363)."""
+    _fields_ = [
+        ("read_seconds", C.c_double), ("inject_seconds", C.c_double),
+        ("move_seconds", C.c_double), ("split_seconds", C.c_double),
+        ("injected", C.c_uint64), ("split", C.c_uint64), ("skipped_split_capacity", C.c_uint64),
+    ]
+
+
+class InjectOpts(C.Structure):
+    _fields_ = [
+        ("use_region", C.c_int32), ("fixed_time", C.c_int32),
+        ("region_x_lo", C.c_double), ("region_x_hi", C.c_double),
+        ("region_y_lo", C.c_double), ("region_y_hi", C.c_double),
+        ("p_init", C.c_double),
+    ]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def particles_struct(parts: dict) -> ParticlesHost:
+    """ParticlesHost pointing into a dict of contiguous numpy arrays (None -> NULL)."""
+    h = ParticlesHost()
+    for name, dt in PARTICLE_FIELDS:
+        a = parts.get(name)
+        if a is None:
+            continue
+        assert a.dtype == dt and a.flags.c_contiguous, name
+        ctype = {np.float64: C.c_double, np.int32: C.c_int32, np.uint64: C.c_uint64}[dt]
+        setattr(h, name, a.ctypes.data_as(C.POINTER(ctype)))
+    return h
+
+
+def empty_particles(n: int) -> dict:
+    return {name: np.zeros(n, dtype=dt) for name, dt in PARTICLE_FIELDS}
+
+
+# ---------------------------------------------------------------------------
+# library loading and signatures
+# ---------------------------------------------------------------------------
+_lib = None
+
+EXPORTS = {
+    "kpt_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint64, C.c_uint64]),
+    "kpt_destroy": (C.c_int, [C.c_void_p]),
+    "kpt_last_error": (C.c_char_p, [C.c_void_p]),
+    "kpt_last_create_error": (C.c_char_p, []),
+    "kpt_version": (C.c_char_p, []),
+    "kpt_device_name": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "kpt_default_host_settings": (C.c_int, [C.POINTER(HostSettings)]),
+    "kpt_make_device_settings": (C.c_int, [C.POINTER(HostSettings), C.c_int32, C.c_int32,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(DeviceSettings)]),
+    "kpt_set_seed": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "kpt_field_load_compact": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
+    "kpt_field_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+                         + [C.POINTER(C.c_double)] * 8),
+    "kpt_field_advance": (C.c_int, [C.c_void_p]),
+    "kpt_field_download_coefficients": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                                  C.c_size_t]),
+    "kpt_compact_field_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+                               + [C.POINTER(C.c_double)] * 8 + [C.c_int32, C.c_char_p, C.c_size_t]),
+    "kpt_compact_field_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32]
+                                + [C.POINTER(C.c_double)] * 8),
+    "kpt_particles_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kpt_particles_capacity": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kpt_particles_upload": (C.c_int, [C.c_void_p, C.POINTER(ParticlesHost), C.c_uint64]),
+    "kpt_particles_download": (C.c_int, [C.c_void_p, C.POINTER(ParticlesHost), C.c_uint64]),
+    "kpt_inject": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.POINTER(InjectOpts),
+                             C.c_uint64, C.c_double, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "kpt_push_interval": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.c_double,
+                                    C.c_uint32, C.POINTER(PushStats)]),
+    "kpt_split": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.POINTER(C.c_uint64),
+                            C.POINTER(C.c_uint64)]),
+    "kpt_histogram_logp": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double,
+                                     C.POINTER(C.c_double)]),
+    "kpt_histogram_xy": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    "kpt_summary_compute": (C.c_int, [C.c_void_p, C.POINTER(Summary)]),
+    "kpt_set_noise": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.c_uint32]),
+    "kpt_clear_noise": (C.c_int, [C.c_void_p]),
+    "kpt_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "kpt_snapshot_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double]),
+    "kpt_snapshot_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double,
+                                    C.c_double]),
+    "kpt_write_histogram_csv": (C.c_int, [C.c_char_p, C.c_int32, C.c_double, C.c_double,
+                                          C.POINTER(C.c_double)]),
+    "kpt_format_summary_header": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "kpt_format_summary_row": (C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_double,
+                                         C.POINTER(Summary), C.POINTER(FrameTiming), C.c_double,
+                                         C.c_double]),
+    "kpt_snapshot_write_host": (C.c_int, [C.c_char_p, C.POINTER(ParticlesHost), C.c_uint64,
+                                          C.c_uint64, C.c_double, C.c_double, C.c_double]),
+    "kpt_snapshot_read_host": (C.c_int, [C.c_char_p, C.POINTER(ParticlesHost),
+                                         C.POINTER(C.c_uint64), C.c_double, C.c_double,
+                                         C.c_char_p, C.c_size_t]),
+}
+
+
+def load():
+    """Load libkpt.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C kokkos-particletransport_amd`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class KptError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"kpt error {code}: {msg}")
+        self.code = code
+
+
+def default_host_settings() -> HostSettings:
+    h = HostSettings()
+    _check_static(load().kpt_default_host_settings(C.byref(h)))
+    return h
+
+
+def make_device_settings(host: HostSettings, x_edges, y_edges) -> DeviceSettings:
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    out = DeviceSettings()
+    _check_static(load().kpt_make_device_settings(C.byref(host), len(xe) - 1, len(ye) - 1,
+                                                  _dp(xe), _dp(ye), C.byref(out)))
+    return out
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().kpt_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def compact_field_read(path, allow_nonuniform=False):
+    lib = load()
+    nx, ny = C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(512)
+    st = lib.kpt_compact_field_read(path.encode(), C.byref(nx), C.byref(ny), *([None] * 8),
+                                    int(allow_nonuniform), err, 512)
+    if st != KPT_OK:
+        raise KptError(st, err.value.decode())
+    n0, n1 = nx.value, ny.value
+    xe = np.empty(n0 + 1)
+    ye = np.empty(n1 + 1)
+    planes = [np.empty((n1, n0)) for _ in range(6)]
+    st = lib.kpt_compact_field_read(path.encode(), C.byref(nx), C.byref(ny), _dp(xe), _dp(ye),
+                                    *[_dp(a) for a in planes], int(allow_nonuniform), err, 512)
+    if st != KPT_OK:
+        raise KptError(st, err.value.decode())
+    return xe, ye, planes
+
+
+def compact_field_write(path, x_edges, y_edges, planes):
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    ps = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+    st = load().kpt_compact_field_write(path.encode(), len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye),
+                                        *[_dp(a) for a in ps])
+    _check_static(st)
+
+
+def write_histogram_csv(path, pmin, pmax, weights):
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    _check_static(load().kpt_write_histogram_csv(path.encode(), len(w), pmin, pmax, _dp(w)))
+
+
+def summary_csv_header() -> str:
+    buf = C.create_string_buffer(1024)
+    _check_static(load().kpt_format_summary_header(buf, 1024))
+    return buf.value.decode()
+
+
+def summary_csv_row(frame, time, summary: Summary, timing: FrameTiming, total_frame_s,
+                    total_elapsed_s) -> str:
+    buf = C.create_string_buffer(2048)
+    _check_static(load().kpt_format_summary_row(buf, 2048, frame, time, C.byref(summary),
+                                                C.byref(timing), total_frame_s, total_elapsed_s))
+    return buf.value.decode()
+
+
+def snapshot_write_host(path, parts: dict, count, capacity, charge, split_ratio, p0):
+    h = particles_struct(parts)
+    _check_static(load().kpt_snapshot_write_host(path.encode(), C.byref(h), count, capacity,
+                                                 charge, split_ratio, p0))
+
+
+def snapshot_read_host(path, charge, split_ratio) -> dict:
+    """Read a v4 snapshot into a particle dict (t and step_dt are left zero)."""
+    lib = load()
+    n = C.c_uint64(0)
+    err = C.create_string_buffer(512)
+    st = lib.kpt_snapshot_read_host(path.encode(), None, C.byref(n), charge, split_ratio, err, 512)
+    if st != KPT_OK:
+        raise KptError(st, err.value.decode())
+    parts = empty_particles(n.value)
+    h = particles_struct(parts)
+    st = lib.kpt_snapshot_read_host(path.encode(), C.byref(h), C.byref(n), charge, split_ratio,
+                                    err, 512)
+    if st != KPT_OK:
+        raise KptError(st, err.value.decode())
+    return parts
+
+
+def _check_static(st):
+    if st != KPT_OK:
+        raise KptError(st, "call failed")
+
+
+class Context:
+    """One kpt_ctx: particle storage, two coefficient frames and the RNG seed."""
+
+    def __init__(self, capacity: int, seed: int = 114514, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.kpt_create(C.byref(h), device, capacity, seed)
+        if st != KPT_OK:
+            raise KptError(st, self.lib.kpt_last_create_error().decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kpt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != KPT_OK:
+            raise KptError(st, self.lib.kpt_last_error(self.h).decode())
+
+    @property
+    def count(self) -> int:
+        n = C.c_uint64()
+        self._check(self.lib.kpt_particles_count(self.h, C.byref(n)))
+        return n.value
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(64)
+        self._check(self.lib.kpt_device_name(self.h, buf, 64))
+        return buf.value.decode()
+
+    def set_seed(self, seed: int):
+        self._check(self.lib.kpt_set_seed(self.h, seed))
+
+    # fields
+    def field_upload(self, slot, model, x_edges, y_edges, planes):
+        xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+        ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+        ps = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+        self._check(self.lib.kpt_field_upload(self.h, slot, model, len(xe) - 1, len(ye) - 1,
+                                              _dp(xe), _dp(ye), *[_dp(a) for a in ps]))
+
+    def field_load_compact(self, path, slot, model):
+        self._check(self.lib.kpt_field_load_compact(self.h, path.encode(), slot, model))
+
+    def field_advance(self):
+        self._check(self.lib.kpt_field_advance(self.h))
+
+    def field_coefficients(self, slot, nx, ny, model):
+        out = np.empty((ny + 2) * (nx + 2) * model)
+        self._check(self.lib.kpt_field_download_coefficients(self.h, slot, _dp(out), out.size))
+        return out.reshape(ny + 2, nx + 2, model)
+
+    # particles
+    def upload(self, parts: dict, count=None):
+        n = len(parts["x"]) if count is None else count
+        h = particles_struct(parts)
+        self._check(self.lib.kpt_particles_upload(self.h, C.byref(h), n))
+
+    def download(self, count=None) -> dict:
+        n = self.count if count is None else count
+        parts = empty_particles(n)
+        h = particles_struct(parts)
+        self._check(self.lib.kpt_particles_download(self.h, C.byref(h), n))
+        return parts
+
+    def inject(self, s: DeviceSettings, n, t0, frame, opts: InjectOpts | None = None) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.kpt_inject(self.h, C.byref(s), C.byref(opts) if opts else None, n,
+                                        t0, frame, C.byref(out)))
+        return out.value
+
+    def push_interval(self, s: DeviceSettings, t0, frame=0) -> PushStats:
+        st = PushStats()
+        self._check(self.lib.kpt_push_interval(self.h, C.byref(s), t0, frame, C.byref(st)))
+        return st
+
+    def split(self, s: DeviceSettings):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.kpt_split(self.h, C.byref(s), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def histogram_logp(self, bins, pmin, pmax):
+        out = np.empty(bins)
+        self._check(self.lib.kpt_histogram_logp(self.h, bins, pmin, pmax, _dp(out)))
+        return out
+
+    def histogram_xy(self, bx, by, x_lo, x_hi, y_lo, y_hi):
+        out = np.empty(bx * by)
+        self._check(self.lib.kpt_histogram_xy(self.h, bx, by, x_lo, x_hi, y_lo, y_hi, _dp(out)))
+        return out.reshape(by, bx)
+
+    def summary(self) -> dict:
+        s = Summary()
+        self._check(self.lib.kpt_summary_compute(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def set_noise(self, draws: np.ndarray):
+        d = np.ascontiguousarray(draws, dtype=np.float64)
+        assert d.ndim == 3 and d.shape[2] == 4
+        self._check(self.lib.kpt_set_noise(self.h, _dp(d), d.shape[0], d.shape[1]))
+
+    def clear_noise(self):
+        self._check(self.lib.kpt_clear_noise(self.h))
+
+    def snapshot_write(self, path, charge, split_ratio, p0):
+        self._check(self.lib.kpt_snapshot_write(self.h, path.encode(), charge, split_ratio, p0))
+
+    def snapshot_read(self, path, charge, split_ratio, restart_time, dt_min):
+        self._check(self.lib.kpt_snapshot_read(self.h, path.encode(), charge, split_ratio,
+                                               restart_time, dt_min))

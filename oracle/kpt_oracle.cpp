@@ -1,0 +1,848 @@
+// kpt_oracle.cpp — CPU FP64 restatement of the reference particle push.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the oracle that the HIP library is
+// checked against, and the timed CPU baseline ("restatement of the Kokkos OpenMP
+// path"). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it, and only as the checker or the CPU baseline: the product path
+// never calls it.
+//
+// It follows the reference code literally rather than efficiently: fields are
+// stored on the ghosted domain (2 ghost layers, include/Grid.hpp:65) with the
+// x-fastest flattening of include/Field.hpp:184-192, ghosts are filled by
+// periodic copy (include/Field.hpp:274-316), the interpolator is the generic
+// multilinear corner loop of include/FieldInterpolator.hpp:346-380 with the
+// stencil of :509-560, and the movers restate src/main.cpp:1082-1755 line by line.
+//
+// Parity status per function (how each is pinned; the tests name the checks):
+//   philox4x32_10      Random123 known-answer vectors (Salmon et al. 2011).
+//   field packing      closed forms on linear / quadratic / trigonometric fields,
+//                      periodic-ghost identities.
+//   interpolation      exact on bilinear fields, node values, periodicity.
+//   parker push        closed forms: pure diffusion variance, V=0 -> p const,
+//                      constant div V -> exponential p, dt limiter cases.
+//   focused push       closed forms: streaming along uniform B, D_mumu formula,
+//                      mu clamp; otherwise shares the Parker branch structure.
+//   split / histogram / summary / inject   invariants (weight conservation,
+//                      totals) and the reference's tracked histogram CSV edges.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../include/kpt.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): 10 rounds of the 4x32 S-box with Weyl key schedule.
+// ---------------------------------------------------------------------------
+inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+    const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+    const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+    const uint32_t n0 = hi1 ^ c[1] ^ k[0];
+    const uint32_t n2 = hi0 ^ c[3] ^ k[1];
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+}
+
+inline void philox10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k[0] += 0x9E3779B9u;
+            k[1] += 0xBB67AE85u;
+        }
+        philox_round(c, k);
+    }
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// U(0,1) from 32 random bits, open at both ends: (r + 1/2) / 2^32.
+inline double u01(uint32_t r) {
+    return (static_cast<double>(r) + 0.5) * (1.0 / 4294967296.0);
+}
+
+// StochasticSampler::uniform_sqrt3 (include/RandomManager.hpp:110-114):
+// drand(-sqrt3, sqrt3) = lower + (upper - lower) * U.
+inline double uniform_sqrt3(uint32_t r) {
+    constexpr double sqrt3 = 1.7320508075688772935;
+    return -sqrt3 + (sqrt3 - -sqrt3) * u01(r);
+}
+
+// ---------------------------------------------------------------------------
+// Ghosted, periodic, uniform 2-D grid and field (include/Grid.hpp,
+// include/AnalyticCoordinateGrid.hpp, include/MappedCoordinateGrid.hpp).
+// ---------------------------------------------------------------------------
+constexpr int kGhost = 2;
+
+struct OGrid {
+    int n[2];
+    double start[2];  // x0 = first face coordinate
+    double step[2];   // uniform face spacing (AthenaFieldReader.hpp:167-193)
+    double phys_lo[2], phys_hi[2], width[2], inv_width[2];
+
+    // AnalyticCoordinateGrid face / cell coordinate (:159-176).
+    double face(int d, double i) const { return start[d] + i * step[d]; }
+    double cell(int d, double i) const { return 0.5 * (face(d, i) + face(d, i + 1.0)); }
+    // Cell-centred inverse map, uniform spacing (:205-210).
+    double cell_grid_index(int d, double x) const { return (x - start[d]) / step[d] - 0.5; }
+    int lower_ghosted(int) const { return -kGhost; }
+    int upper_ghosted(int d) const { return n[d] + kGhost - 1; }
+    // MappedCoordinateGrid::wrap_coordinate (:295-315).
+    double wrap(int d, double x) const {
+        double w = x - std::floor((x - phys_lo[d]) * inv_width[d]) * width[d];
+        if (w >= phys_hi[d]) w -= width[d];
+        if (w < phys_lo[d]) w += width[d];
+        return w;
+    }
+};
+
+struct OField {
+    OGrid g;
+    int comps = 0;
+    int ext[2];                  // ghosted extents n + 4
+    std::vector<double> data;    // [point][comp], point = (j+2)*ext0 + (i+2)
+
+    size_t flat(int i, int j) const {
+        return static_cast<size_t>(j + kGhost) * ext[0] + static_cast<size_t>(i + kGhost);
+    }
+    double at(int i, int j, int c) const { return data[flat(i, j) * comps + c]; }
+    double& at(int i, int j, int c) { return data[flat(i, j) * comps + c]; }
+};
+
+void make_grid(OGrid& g, int nx, int ny, const double* xe, const double* ye) {
+    g.n[0] = nx;
+    g.n[1] = ny;
+    const double* e[2] = {xe, ye};
+    for (int d = 0; d < 2; ++d) {
+        g.start[d] = e[d][0];
+        g.step[d] = (e[d][g.n[d]] - e[d][0]) / static_cast<double>(g.n[d]);
+        // refresh_bounds_cache_on_host (MappedCoordinateGrid.hpp:117-145)
+        g.phys_lo[d] = g.face(d, 0.0);
+        g.phys_hi[d] = g.face(d, static_cast<double>(g.n[d]));
+        g.width[d] = g.phys_hi[d] - g.phys_lo[d];
+        g.inv_width[d] = g.width[d] != 0.0 ? 1.0 / g.width[d] : 0.0;
+    }
+}
+
+// Field::fill_ghost_cells, periodic in both axes (Field.hpp:274-316).
+void fill_ghosts_periodic(OField& f) {
+    const int nx = f.g.n[0], ny = f.g.n[1];
+    for (int j = -kGhost; j < ny + kGhost; ++j) {
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            if (i >= 0 && i < nx && j >= 0 && j < ny) continue;
+            int si = i % nx;
+            if (si < 0) si += nx;
+            int sj = j % ny;
+            if (sj < 0) sj += ny;
+            for (int c = 0; c < f.comps; ++c) f.at(i, j, c) = f.at(si, sj, c);
+        }
+    }
+}
+
+// derivative_component (src/main.cpp:714-770): central difference on the ghosted
+// domain, one-sided second order at the outermost ghost layer.
+double derivative(const OField& f, int i, int j, int dim, int comp) {
+    if (f.g.n[dim] <= 1) return 0.0;
+    const int lower = f.g.lower_ghosted(dim);
+    const int upper = f.g.upper_ghosted(dim);
+    int idx[2] = {i, j};
+    auto val = [&](int k) {
+        int a[2] = {i, j};
+        a[dim] = k;
+        return f.at(a[0], a[1], comp);
+    };
+    if (idx[dim] <= lower) {
+        const double x0 = f.g.cell(dim, lower);
+        const double x2 = f.g.cell(dim, lower + 2);
+        return (-3.0 * val(lower) + 4.0 * val(lower + 1) - val(lower + 2)) / (x2 - x0);
+    }
+    if (idx[dim] >= upper) {
+        const double x0 = f.g.cell(dim, upper - 2);
+        const double x2 = f.g.cell(dim, upper);
+        return (3.0 * val(upper) - 4.0 * val(upper - 1) + val(upper - 2)) / (x2 - x0);
+    }
+    const double xl = f.g.cell(dim, idx[dim] - 1);
+    const double xr = f.g.cell(dim, idx[dim] + 1);
+    return (val(idx[dim] + 1) - val(idx[dim] - 1)) / (xr - xl);
+}
+
+double bmag_at(const OField& b, int i, int j) {
+    const double bx = b.at(i, j, 0), by = b.at(i, j, 1), bz = b.at(i, j, 2);
+    return std::sqrt(bx * bx + by * by + bz * bz);
+}
+
+// derivative_magnetic_magnitude (src/main.cpp:775-834).
+double derivative_bmag(const OField& b, int i, int j, int dim) {
+    if (b.g.n[dim] <= 1) return 0.0;
+    const int lower = b.g.lower_ghosted(dim);
+    const int upper = b.g.upper_ghosted(dim);
+    int idx[2] = {i, j};
+    auto val = [&](int k) {
+        int a[2] = {i, j};
+        a[dim] = k;
+        return bmag_at(b, a[0], a[1]);
+    };
+    if (idx[dim] <= lower) {
+        const double x0 = b.g.cell(dim, lower);
+        const double x2 = b.g.cell(dim, lower + 2);
+        return (-3.0 * val(lower) + 4.0 * val(lower + 1) - val(lower + 2)) / (x2 - x0);
+    }
+    if (idx[dim] >= upper) {
+        const double x0 = b.g.cell(dim, upper - 2);
+        const double x2 = b.g.cell(dim, upper);
+        return (3.0 * val(upper) - 4.0 * val(upper - 1) + val(upper - 2)) / (x2 - x0);
+    }
+    const double xl = b.g.cell(dim, idx[dim] - 1);
+    const double xr = b.g.cell(dim, idx[dim] + 1);
+    return (val(idx[dim] + 1) - val(idx[dim] - 1)) / (xr - xl);
+}
+
+// make_parker_reconnection_coefficients (src/main.cpp:839-892) and
+// make_reconnection_coefficients (src/main.cpp:897-957), over every ghosted point.
+void pack_coefficients(const OField& B, const OField& V, OField& out, int model) {
+    out.g = B.g;
+    out.comps = model;
+    out.ext[0] = B.ext[0];
+    out.ext[1] = B.ext[1];
+    out.data.assign(static_cast<size_t>(out.ext[0]) * out.ext[1] * model, 0.0);
+    const int nx = B.g.n[0], ny = B.g.n[1];
+    for (int j = -kGhost; j < ny + kGhost; ++j) {
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            const double bx = B.at(i, j, 0), by = B.at(i, j, 1), bz = B.at(i, j, 2);
+            const double bm = std::sqrt(bx * bx + by * by + bz * bz);
+            if (model == KPT_MODEL_PARKER16) {
+                out.at(i, j, 0) = V.at(i, j, 0);
+                out.at(i, j, 1) = V.at(i, j, 1);
+                out.at(i, j, 2) = bx;
+                out.at(i, j, 3) = by;
+                out.at(i, j, 4) = bz;
+                out.at(i, j, 5) = bm;
+                out.at(i, j, 6) = derivative(V, i, j, 0, 0);
+                out.at(i, j, 7) = derivative(V, i, j, 1, 1);
+                out.at(i, j, 8) = derivative(B, i, j, 0, 0);
+                out.at(i, j, 9) = derivative(B, i, j, 1, 0);
+                out.at(i, j, 10) = derivative(B, i, j, 0, 1);
+                out.at(i, j, 11) = derivative(B, i, j, 1, 1);
+                out.at(i, j, 12) = derivative(B, i, j, 0, 2);
+                out.at(i, j, 13) = derivative(B, i, j, 1, 2);
+                out.at(i, j, 14) = derivative_bmag(B, i, j, 0);
+                out.at(i, j, 15) = derivative_bmag(B, i, j, 1);
+            } else {
+                out.at(i, j, 0) = V.at(i, j, 0);
+                out.at(i, j, 1) = V.at(i, j, 1);
+                out.at(i, j, 2) = V.at(i, j, 2);
+                out.at(i, j, 3) = bx;
+                out.at(i, j, 4) = by;
+                out.at(i, j, 5) = bz;
+                out.at(i, j, 6) = bm;
+                out.at(i, j, 7) = derivative(V, i, j, 0, 0);
+                out.at(i, j, 8) = derivative(V, i, j, 1, 0);
+                out.at(i, j, 9) = derivative(V, i, j, 0, 1);
+                out.at(i, j, 10) = derivative(V, i, j, 1, 1);
+                out.at(i, j, 11) = derivative(V, i, j, 0, 2);
+                out.at(i, j, 12) = derivative(V, i, j, 1, 2);
+                out.at(i, j, 13) = derivative(B, i, j, 0, 0);
+                out.at(i, j, 14) = derivative(B, i, j, 1, 0);
+                out.at(i, j, 15) = derivative(B, i, j, 0, 1);
+                out.at(i, j, 16) = derivative(B, i, j, 1, 1);
+                out.at(i, j, 17) = derivative(B, i, j, 0, 2);
+                out.at(i, j, 18) = derivative(B, i, j, 1, 2);
+                out.at(i, j, 19) = derivative_bmag(B, i, j, 0);
+                out.at(i, j, 20) = derivative_bmag(B, i, j, 1);
+            }
+        }
+    }
+}
+
+// LinearPositionInterpolator::sample with WrapPeriodic on the ghosted domain
+// (include/FieldInterpolator.hpp:346-380, :465-480, :509-560).
+void sample(const OField& f, const double q_in[2], double* result) {
+    double q[2];
+    for (int d = 0; d < 2; ++d) q[d] = f.g.wrap(d, q_in[d]);
+    int lo[2], hi[2];
+    double w[2];
+    for (int d = 0; d < 2; ++d) {
+        const int lb = f.g.lower_ghosted(d);
+        const int ub = f.g.upper_ghosted(d);
+        const double logical = f.g.cell_grid_index(d, q[d]);
+        if (logical <= static_cast<double>(lb)) {
+            lo[d] = hi[d] = lb;
+            w[d] = 0.0;
+            continue;
+        }
+        if (logical >= static_cast<double>(ub)) {
+            lo[d] = hi[d] = ub;
+            w[d] = 0.0;
+            continue;
+        }
+        const int cand = static_cast<int>(std::floor(logical));
+        lo[d] = cand;
+        hi[d] = cand + 1;
+        w[d] = logical - static_cast<double>(cand);
+        if (lo[d] < lb) {
+            lo[d] = hi[d] = lb;
+            w[d] = 0.0;
+        } else if (hi[d] > ub) {
+            lo[d] = hi[d] = ub;
+            w[d] = 0.0;
+        }
+    }
+    for (int c = 0; c < f.comps; ++c) result[c] = 0.0;
+    for (int corner = 0; corner < 4; ++corner) {
+        int idx[2];
+        double weight = 1.0;
+        for (int d = 0; d < 2; ++d) {
+            const bool up = ((corner >> d) & 1) != 0;
+            idx[d] = up ? hi[d] : lo[d];
+            weight *= up ? w[d] : (1.0 - w[d]);
+        }
+        for (int c = 0; c < f.comps; ++c) result[c] += weight * f.at(idx[0], idx[1], c);
+    }
+}
+
+// sample_time_interpolated_coefficients (src/main.cpp:1053-1077).
+void sample_time(const OField& lo, const OField& hi, const double q[2], double rt, int use_time,
+                 double* out) {
+    double a[32], b[32];
+    sample(lo, q, a);
+    if (use_time == 0) {
+        for (int c = 0; c < lo.comps; ++c) out[c] = a[c];
+        return;
+    }
+    sample(hi, q, b);
+    const double uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
+    const double lw = 1.0 - uw;
+    for (int c = 0; c < lo.comps; ++c) out[c] = lw * a[c] + uw * b[c];
+}
+
+// wrap_periodic_coordinate (src/main.cpp:641-665).
+double wrap_periodic_coordinate(double value, double lower, double upper, double width) {
+    double wrapped = value;
+    if (wrapped >= upper) {
+        wrapped -= width;
+        if (wrapped >= upper) wrapped -= std::floor((wrapped - lower) / width) * width;
+    } else if (wrapped < lower) {
+        wrapped += width;
+        if (wrapped < lower) wrapped -= std::floor((wrapped - lower) / width) * width;
+    }
+    if (wrapped >= upper) wrapped -= width;
+    if (wrapped < lower) wrapped += width;
+    return wrapped;
+}
+
+inline double sq(double v) { return v * v; }
+inline double min_positive(double a, double b) { return a < b ? a : b; }
+inline double clamp_mu(double v, double m) { return v > m ? m : (v < -m ? -m : v); }
+
+struct NoiseSource {
+    const double* table;  // injected draws or nullptr
+    uint32_t steps;
+    uint32_t key[2];
+    uint32_t frame;
+
+    // The four U[-sqrt3, sqrt3] values of sub-step k of the particle in slot i
+    // with id `id`.
+    void draws(uint64_t i, uint64_t id, uint32_t k, double r[4]) const {
+        if (table) {
+            const double* row = table + (static_cast<size_t>(i) * steps + k) * 4;
+            for (int d = 0; d < 4; ++d) r[d] = row[d];
+            return;
+        }
+        const uint32_t ctr[4] = {static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), frame,
+                                 k};
+        uint32_t out[4];
+        philox10(ctr, key, out);
+        for (int d = 0; d < 4; ++d) r[d] = uniform_sqrt3(out[d]);
+    }
+};
+
+// One particle through one interval: move_parker_particles_one_interval
+// (src/main.cpp:1117-1325) when model == 16, move_particles_one_interval
+// (src/main.cpp:1368-1753) when model == 21. Returns the executed sub-steps.
+uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_settings& S,
+                  double interval_start, const NoiseSource& noise, uint64_t index,
+                  kpt_particles_host& P, bool* failed) {
+    *failed = false;
+    if (P.status[index] != 0) return 0;
+    const int model = lower.comps;
+    const bool focused_transport = model == KPT_MODEL_FOCUSED21 && S.focused_transport != 0;
+    const double interval_end = interval_start + S.mhd_dt;
+    double x = P.x[index], y = P.y[index], p = P.p[index], t = P.t[index];
+    double pitch_mu = model == KPT_MODEL_FOCUSED21 ? clamp_mu(P.mu[index], S.mu_max) : 0.0;
+    double dt = P.step_dt[index];
+    double diagnostic_dt = dt;
+    int step_guard = 0;
+    int status = 0;
+    uint64_t steps = 0;
+    const uint64_t id = P.id ? P.id[index] : index;
+    double c[32];
+
+    while (t < interval_end && step_guard < S.max_particle_steps_per_interval && status == 0) {
+        ++step_guard;
+        x = wrap_periodic_coordinate(x, S.x_lower, S.x_upper, S.x_width);
+        y = wrap_periodic_coordinate(y, S.y_lower, S.y_upper, S.y_width);
+        const double q[2] = {x + S.interpolation_shift_x, y + S.interpolation_shift_y};
+        const double rt = (t - interval_start) / S.mhd_dt;
+        sample_time(lower, upper, q, rt, S.time_interpolation, c);
+
+        double vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = 0, dvy_dx = 0, dvy_dy, dvz_dx = 0,
+                                          dvz_dy = 0, dbx_dx, dbx_dy, dby_dx, dby_dy, dbz_dx,
+                                          dbz_dy, db_dx, db_dy;
+        if (model == KPT_MODEL_PARKER16) {
+            vx = c[0]; vy = c[1]; bx = c[2]; by = c[3]; bz = c[4]; b = c[5];
+            dvx_dx = c[6]; dvy_dy = c[7]; dbx_dx = c[8]; dbx_dy = c[9]; dby_dx = c[10];
+            dby_dy = c[11]; dbz_dx = c[12]; dbz_dy = c[13]; db_dx = c[14]; db_dy = c[15];
+        } else {
+            vx = c[0]; vy = c[1]; bx = c[3]; by = c[4]; bz = c[5]; b = c[6];
+            dvx_dx = c[7]; dvx_dy = c[8]; dvy_dx = c[9]; dvy_dy = c[10]; dvz_dx = c[11];
+            dvz_dy = c[12]; dbx_dx = c[13]; dbx_dy = c[14]; dby_dx = c[15]; dby_dy = c[16];
+            dbz_dx = c[17]; dbz_dy = c[18]; db_dx = c[19]; db_dy = c[20];
+        }
+
+        if (!std::isfinite(p) || p <= 0.0 || !std::isfinite(b) || b <= 0.0) {
+            status = 2;
+            break;
+        }
+        const double ib = 1.0 / b;
+        const double ib2 = ib * ib;
+        const double ib3 = ib2 * ib;
+
+        double knorm_para = 1.0;
+        if (S.magnetic_dependency == 1) knorm_para *= std::pow(b, S.gamma_turb - 2.0);
+        double knorm = knorm_para;
+        if (S.momentum_dependency == 1) knorm *= std::pow(p / S.p0, S.pindex);
+        const double kpara = S.kpara0 * knorm;
+        const double kperp = kpara * S.kperp_over_kpara;
+        if (!std::isfinite(kpara) || kpara <= 0.0 || !std::isfinite(kperp) || kperp < 0.0) {
+            status = 2;
+            break;
+        }
+        const double skpara = std::sqrt(2.0 * kpara);
+        const double skperp = std::sqrt(2.0 * kperp);
+        // Parker-only mover: unguarded sqrt (src/main.cpp:1197-1198); shared mover
+        // guards it (src/main.cpp:1456-1457).
+        const double skpara_perp = model == KPT_MODEL_PARKER16
+                                       ? std::sqrt(2.0 * (kpara - kperp))
+                                       : (kpara > kperp ? std::sqrt(2.0 * (kpara - kperp)) : 0.0);
+        const double kpp = focused_transport ? -kperp : kpara - kperp;
+
+        double dkdx = 0.0, dkdy = 0.0;
+        if (S.magnetic_dependency == 1) {
+            const double me = S.gamma_turb - 2.0;
+            dkdx = db_dx * ib * me;
+            dkdy = db_dy * ib * me;
+        }
+        const double dkxx_dx =
+            kperp * dkdx + kpp * dkdx * bx * bx * ib2 + 2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+        const double dkyy_dy =
+            kperp * dkdy + kpp * dkdy * by * by * ib2 + 2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+        const double dkxy_dx = kpp * dkdx * bx * by * ib2 +
+                               kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+        const double dkxy_dy = kpp * dkdy * bx * by * ib2 +
+                               kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+        const double drift_denominator =
+            std::sqrt(sq(S.drift_param1 * S.p0 / p) + sq(S.drift_param2 * S.p0 * S.p0 / (p * p)));
+
+        double dx_dt = 0.0, dy_dt = 0.0, dp_dt = 0.0, dmu_dt = 0.0, duu = 0.0;
+        if (focused_transport) {
+            // src/main.cpp:1499-1589
+            pitch_mu = clamp_mu(pitch_mu, S.mu_max);
+            const double mu2 = pitch_mu * pitch_mu;
+            const double muf1 = 0.5 * (1.0 - mu2);
+            const double muf2 = 0.5 * (3.0 * mu2 - 1.0);
+            const double kx = bx * dbx_dx + by * dbx_dy;
+            const double ky = bx * dby_dx + by * dby_dy;
+            const double kz = bx * dbz_dx + by * dbz_dy;
+            const double bdot_curvb = bx * dbz_dy - by * dbz_dx + bz * (dby_dx - dbx_dy);
+            const double vdp = 1.0 / S.charge / drift_denominator;
+            const double vdx = vdp * (muf1 * (-bz * db_dy) * ib2 + mu2 * (by * kz - bz * ky) * ib3 +
+                                      muf1 * bx * bdot_curvb * ib3);
+            const double vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
+                                      muf1 * by * bdot_curvb * ib3);
+            const double speed = S.particle_v0 * p / S.p0;
+            if (!std::isfinite(speed) || speed <= 0.0) {
+                status = 2;
+                break;
+            }
+            const double vpb = speed * pitch_mu * ib;
+            const double vbx = vpb * bx;
+            const double vby = vpb * by;
+            dx_dt = vx + vdx + vbx + dkxx_dx + dkxy_dy;
+            dy_dt = vy + vdy + vby + dkxy_dx + dkyy_dy;
+            const double divv = dvx_dx + dvy_dy;
+            const double bb_gradv = (bx * (bx * dvx_dx + by * dvx_dy) + by * (bx * dvy_dx + by * dvy_dy) +
+                                     bz * (bx * dvz_dx + by * dvz_dy)) * ib2;
+            const double bv_gradv = (bx * (vx * dvx_dx + vy * dvx_dy) + by * (vx * dvy_dx + vy * dvy_dy) +
+                                     bz * (vx * dvz_dx + vy * dvz_dy)) * ib;
+            const double acc_rate = -(muf1 * divv + muf2 * bb_gradv + pitch_mu * bv_gradv / speed);
+            dp_dt = p * acc_rate;
+            const double div_bnorm = -(bx * db_dx + by * db_dy) * ib2;
+            dmu_dt = speed * div_bnorm + pitch_mu * divv - 3.0 * pitch_mu * bb_gradv -
+                     2.0 * bv_gradv / speed;
+            dmu_dt *= 0.5 * (1.0 - mu2);
+            const double abs_mu = std::fabs(pitch_mu);
+            const double dtmp = std::pow(abs_mu, S.gamma_turb - 1.0) + S.pitch_angle_scattering_h0;
+            duu = S.duu0 * (1.0 - mu2) * dtmp;
+            double duu_du = 0.0;
+            if (pitch_mu > 0.0) {
+                duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp + (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
+            } else if (pitch_mu < 0.0) {
+                duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp - (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
+            }
+            double duu_norm = 1.0;
+            if (S.magnetic_dependency == 1) duu_norm *= std::pow(b, 2.0 - S.gamma_turb);
+            if (S.momentum_dependency == 1) duu_norm *= std::pow(p / S.p0, S.gamma_turb - 1.0);
+            duu *= duu_norm;
+            duu_du *= duu_norm;
+            dmu_dt += duu_du;
+            if (!std::isfinite(dmu_dt) || !std::isfinite(duu) || duu < 0.0) {
+                status = 2;
+                break;
+            }
+        } else {
+            // src/main.cpp:1225-1241 (Parker mover), :1590-1601 (shared mover)
+            const double vdp = 1.0 / (3.0 * S.charge) / drift_denominator;
+            const double vdx = vdp * (dbz_dy * ib2 - 2.0 * bz * db_dy * ib3);
+            const double vdy = vdp * (-dbz_dx * ib2 + 2.0 * bz * db_dx * ib3);
+            dx_dt = vx + vdx + dkxx_dx + dkxy_dy;
+            dy_dt = vy + vdy + dkxy_dx + dkyy_dy;
+            const double divv = dvx_dx + dvy_dy;
+            dp_dt = -p * divv / 3.0;
+        }
+        (void)dvz_dx;
+
+        // dt limiter (src/main.cpp:1243-1277, :1603-1665)
+        if (focused_transport) {
+            const double ds = skperp > 0.0 ? skperp : skpara;
+            if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0 && dmu_dt != 0.0 && ds > 0.0) {
+                dt = min_positive(sq(0.5 * S.dx / ds), sq(0.5 * S.dy / ds));
+                dt = min_positive(dt, sq(ds / dx_dt));
+                dt = min_positive(dt, sq(ds / dy_dt));
+                dt = min_positive(dt, 0.1 * p / std::fabs(dp_dt));
+                dt = min_positive(dt, 0.1 / std::fabs(dmu_dt));
+                dt = min_positive(dt, 2.0 * duu / sq(dmu_dt));
+            } else {
+                dt = S.dt_min;
+            }
+        } else if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
+            const double as = skperp > 0.0 ? skperp : skpara;
+            dt = min_positive(sq(0.5 * S.dx / skpara), sq(0.5 * S.dy / skpara));
+            dt = min_positive(dt, sq(as / dx_dt));
+            dt = min_positive(dt, sq(as / dy_dt));
+            dt = min_positive(dt, 0.1 * p / std::fabs(dp_dt));
+        } else {
+            dt = S.dt_min;
+        }
+        if (dt < S.dt_min) dt = S.dt_min;
+        if (dt > S.dt_max) dt = S.dt_max;
+        diagnostic_dt = dt;
+        const double remaining = interval_end - t;
+        if (dt > remaining) dt = remaining;
+        if (!std::isfinite(dt) || dt <= 0.0) {
+            status = 2;
+            break;
+        }
+
+        const double sdt = std::sqrt(dt);
+        double r[4];
+        noise.draws(index, id, static_cast<uint32_t>(step_guard - 1), r);
+        double delta_x, delta_y;
+        const double delta_p = dp_dt * dt;
+        if (focused_transport) {
+            // src/main.cpp:1671-1702
+            const double ran1 = r[0], ran2 = r[1], pitch_random = r[3];
+            const double bxn = bx * ib, byn = by * ib, bzn = bz * ib;
+            const double bxy_norm = std::sqrt(bxn * bxn + byn * byn);
+            double sx = 0.0, sy = 0.0;
+            if (bxy_norm > 0.0) {
+                const double ibxy = 1.0 / bxy_norm;
+                sx = skperp * ibxy * sdt * (-bxn * bzn * ran1 - by * ran2);
+                sy = skperp * ibxy * sdt * (-byn * bzn * ran1 + bx * ran2);
+            }
+            delta_x = dx_dt * dt + sx;
+            delta_y = dy_dt * dt + sy;
+            pitch_mu = clamp_mu(pitch_mu + dmu_dt * dt + pitch_random * std::sqrt(2.0 * duu) * sdt, S.mu_max);
+        } else {
+            // src/main.cpp:1279-1292
+            const double ran1 = r[0], ran2 = r[1], ran3 = r[2];
+            delta_x = dx_dt * dt + ran1 * skperp * sdt + ran3 * skpara_perp * sdt * bx * ib;
+            delta_y = dy_dt * dt + ran2 * skperp * sdt + ran3 * skpara_perp * sdt * by * ib;
+        }
+        x += delta_x;
+        y += delta_y;
+        p += delta_p;
+        t += dt;
+        ++steps;
+        if (p < 0.25 * S.p0) p = 0.25 * S.p0;
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(p) || !std::isfinite(t)) {
+            status = 2;
+            break;
+        }
+    }
+    if (step_guard >= S.max_particle_steps_per_interval && t < interval_end) status = 2;
+
+    P.x[index] = wrap_periodic_coordinate(x, S.x_lower, S.x_upper, S.x_width);
+    P.y[index] = wrap_periodic_coordinate(y, S.y_lower, S.y_upper, S.y_width);
+    P.p[index] = p;
+    if (model == KPT_MODEL_FOCUSED21) P.mu[index] = pitch_mu;
+    P.t[index] = t;
+    P.step_dt[index] = diagnostic_dt;
+    P.status[index] = status;
+    *failed = status != 0;
+    return steps;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C interface of the oracle.
+// ===========================================================================
+extern "C" {
+
+void kpto_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox10(ctr, key, out);
+}
+
+double kpto_uniform_sqrt3(uint32_t r) { return uniform_sqrt3(r); }
+
+// Build one packed coefficient frame from compact-field arrays.
+void* kpto_field_create(int32_t nx, int32_t ny, const double* xe, const double* ye, const double* bx,
+                        const double* by, const double* bz, const double* vx, const double* vy,
+                        const double* vz, int32_t model) {
+    if (nx <= 0 || ny <= 0 || (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21)) {
+        return nullptr;
+    }
+    OField B, V;
+    for (OField* f : {&B, &V}) {
+        make_grid(f->g, nx, ny, xe, ye);
+        f->comps = 3;
+        f->ext[0] = nx + 2 * kGhost;
+        f->ext[1] = ny + 2 * kGhost;
+        f->data.assign(static_cast<size_t>(f->ext[0]) * f->ext[1] * 3, 0.0);
+    }
+    const double* bs[3] = {bx, by, bz};
+    const double* vs[3] = {vx, vy, vz};
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            const size_t s = static_cast<size_t>(j) * nx + i;
+            for (int c = 0; c < 3; ++c) {
+                B.at(i, j, c) = bs[c][s];
+                V.at(i, j, c) = vs[c][s];
+            }
+        }
+    }
+    fill_ghosts_periodic(B);
+    fill_ghosts_periodic(V);
+    OField* out = new OField();
+    pack_coefficients(B, V, *out, model);
+    return out;
+}
+
+void kpto_field_destroy(void* f) { delete static_cast<OField*>(f); }
+
+// Packed coefficients of nodes i in [-1, nx], j in [-1, ny], as
+// [(ny+2)][(nx+2)][C]. Returns the element count written, or -1.
+int64_t kpto_field_coefficients(const void* handle, double* out, size_t len) {
+    const OField& f = *static_cast<const OField*>(handle);
+    const int nx = f.g.n[0], ny = f.g.n[1];
+    const size_t need = static_cast<size_t>(nx + 2) * (ny + 2) * f.comps;
+    if (len < need) return -1;
+    size_t k = 0;
+    for (int j = -1; j <= ny; ++j)
+        for (int i = -1; i <= nx; ++i)
+            for (int c = 0; c < f.comps; ++c) out[k++] = f.at(i, j, c);
+    return static_cast<int64_t>(need);
+}
+
+// Interpolate all components of `handle` at one position (q is the sampling
+// coordinate, already shifted). For interpolator tests.
+void kpto_field_sample(const void* handle, double qx, double qy, double* out) {
+    const double q[2] = {qx, qy};
+    sample(*static_cast<const OField*>(handle), q, out);
+}
+
+// move_*_particles_one_interval over `count` particles with OpenMP threads.
+// noise == nullptr selects Philox keyed by (seed, id, frame, sub-step).
+void kpto_push_interval(const void* lower, const void* upper, const kpt_device_settings* s,
+                        double t0, uint32_t frame, uint64_t seed, kpt_particles_host* P,
+                        uint64_t count, const double* noise, uint32_t noise_steps, int threads,
+                        uint64_t* steps_out, uint64_t* failures_out) {
+    const OField& lo = *static_cast<const OField*>(lower);
+    const OField& hi = *static_cast<const OField*>(upper ? upper : lower);
+    NoiseSource ns{noise, noise_steps,
+                   {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)}, frame};
+    uint64_t steps = 0, failures = 0;
+    const int64_t n = static_cast<int64_t>(count);
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : steps, failures) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        bool failed = false;
+        const bool was_active = P->status[i] == 0;
+        steps += move_one(lo, hi, *s, t0, ns, static_cast<uint64_t>(i), *P, &failed);
+        if (was_active && failed) ++failures;
+    }
+    if (steps_out) *steps_out = steps;
+    if (failures_out) *failures_out = failures;
+}
+
+// inject_uniform_particles (src/main.cpp:995-1048) with the axis-aligned region
+// option of include/ParticleGenerator.hpp:17-62. Draws: Philox with counter
+// (root id, 0xFFFFFFFF, frame, 0) -> x, y, t, mu. Returns the inserted count.
+uint64_t kpto_inject(const kpt_device_settings* s, const kpt_inject_opts* o, uint64_t requested,
+                     double t0, uint32_t frame, uint64_t seed, uint64_t first_root,
+                     kpt_particles_host* P, uint64_t count, uint64_t capacity) {
+    if (count >= capacity) return 0;
+    const uint64_t n = requested < capacity - count ? requested : capacity - count;
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    const double xl = o && o->use_region ? o->region_x_lo : s->x_lower;
+    const double xu = o && o->use_region ? o->region_x_hi : s->x_upper;
+    const double yl = o && o->use_region ? o->region_y_lo : s->y_lower;
+    const double yu = o && o->use_region ? o->region_y_hi : s->y_upper;
+    const double pinit = o && o->p_init > 0.0 ? o->p_init : s->p0;
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint64_t i = count + k;
+        const uint64_t root = first_root + k;
+        const uint32_t ctr[4] = {static_cast<uint32_t>(root), 0xFFFFFFFFu, frame,
+                                 static_cast<uint32_t>(root >> 32)};
+        uint32_t r[4];
+        philox10(ctr, key, r);
+        P->x[i] = xl + (xu - xl) * u01(r[0]);
+        P->y[i] = yl + (yu - yl) * u01(r[1]);
+        P->p[i] = pinit;
+        P->t[i] = (o && o->fixed_time) ? t0 : t0 + (s->mhd_dt - 0.0) * u01(r[2]);
+        P->step_dt[i] = s->dt_min;
+        P->weight[i] = 1.0;
+        P->mu[i] = s->mu_max * (2.0 * u01(r[3]) - 1.0);
+        P->status[i] = 0;
+        P->split_level[i] = 0;
+        if (P->id) P->id[i] = root;
+    }
+    return n;
+}
+
+// split_particles (src/main.cpp:1760-1839), children appended in parent slot
+// order. The child of a parent at level L with id (root | path << 32) gets
+// path | (1 << L). Returns the new count; split / skipped through pointers.
+uint64_t kpto_split(const kpt_device_settings* s, kpt_particles_host* P, uint64_t count,
+                    uint64_t capacity, uint64_t* split_out, uint64_t* skipped_out) {
+    uint64_t split = 0, skipped = 0;
+    if (count >= capacity || s->split_particles == 0) {
+        if (split_out) *split_out = 0;
+        if (skipped_out) *skipped_out = 0;
+        return count;
+    }
+    uint64_t counter = count;
+    for (uint64_t parent = 0; parent < count; ++parent) {
+        if (P->status[parent] != 0) continue;
+        const int old_level = P->split_level[parent];
+        const double threshold = s->pmin_split * std::pow(s->split_ratio, static_cast<double>(old_level));
+        const double p = P->p[parent];
+        if (!(p > threshold && p <= s->pmax)) continue;
+        const uint64_t child = counter++;
+        if (child >= capacity) {
+            ++skipped;
+            continue;
+        }
+        const int new_level = old_level + 1;
+        const double w = std::pow(0.5, static_cast<double>(new_level));
+        P->x[child] = P->x[parent];
+        P->y[child] = P->y[parent];
+        P->p[child] = P->p[parent];
+        P->t[child] = P->t[parent];
+        P->step_dt[child] = P->step_dt[parent];
+        P->weight[parent] = w;
+        P->weight[child] = w;
+        P->mu[child] = P->mu[parent];
+        P->status[child] = 0;
+        P->split_level[parent] = new_level;
+        P->split_level[child] = new_level;
+        if (P->id) {
+            const uint64_t pid = P->id[parent];
+            const uint64_t path = (pid >> 32) | (static_cast<uint64_t>(1) << old_level);
+            P->id[child] = (pid & 0xFFFFFFFFull) | (path << 32);
+        }
+        ++split;
+    }
+    if (split_out) *split_out = split;
+    if (skipped_out) *skipped_out = skipped;
+    return counter < capacity ? counter : capacity;
+}
+
+// write_momentum_histogram binning (src/main.cpp:1910-1941) with `bins` bins.
+void kpto_histogram_logp(const kpt_particles_host* P, uint64_t count, int32_t bins, double pmin,
+                         double pmax, double* out) {
+    for (int b = 0; b < bins; ++b) out[b] = 0.0;
+    const double log_pmin = std::log(pmin);
+    const double inv_dlogp = static_cast<double>(bins) / (std::log(pmax) - log_pmin);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double p = P->p[i];
+        if (p < pmin || p > pmax) continue;
+        int bin = static_cast<int>((std::log(p) - log_pmin) * inv_dlogp);
+        if (bin < 0) bin = 0;
+        if (bin >= bins) bin = bins - 1;
+        out[bin] += P->weight[i];
+    }
+}
+
+// Spatial density: weight per cell of a uniform bins_x x bins_y grid.
+void kpto_histogram_xy(const kpt_particles_host* P, uint64_t count, int32_t bx, int32_t by,
+                       double x_lo, double x_hi, double y_lo, double y_hi, double* out) {
+    for (int64_t k = 0; k < static_cast<int64_t>(bx) * by; ++k) out[k] = 0.0;
+    const double sx = static_cast<double>(bx) / (x_hi - x_lo);
+    const double sy = static_cast<double>(by) / (y_hi - y_lo);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double x = P->x[i], y = P->y[i];
+        if (!(x >= x_lo && x < x_hi && y >= y_lo && y < y_hi)) continue;
+        int ix = static_cast<int>((x - x_lo) * sx);
+        int iy = static_cast<int>((y - y_lo) * sy);
+        if (ix >= bx) ix = bx - 1;
+        if (iy >= by) iy = by - 1;
+        out[static_cast<int64_t>(iy) * bx + ix] += P->weight[i];
+    }
+}
+
+// summarize_particles (src/main.cpp:1844-1901).
+void kpto_summary(const kpt_particles_host* P, uint64_t count, kpt_summary* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->stored = count;
+    if (count == 0) return;
+    double p_sum = 0.0, dt_sum = 0.0;
+    s->p_min = std::numeric_limits<double>::infinity();
+    s->dt_min = std::numeric_limits<double>::infinity();
+    s->p_max = 0.0;
+    s->dt_max = 0.0;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] == 0) {
+            ++s->active;
+            const double p = P->p[i], dt = P->step_dt[i];
+            s->weight_sum += P->weight[i];
+            p_sum += p;
+            dt_sum += dt;
+            s->p_min = p < s->p_min ? p : s->p_min;
+            s->p_max = p > s->p_max ? p : s->p_max;
+            s->dt_min = dt < s->dt_min ? dt : s->dt_min;
+            s->dt_max = dt > s->dt_max ? dt : s->dt_max;
+            if (P->split_level[i] > s->max_split_level) s->max_split_level = P->split_level[i];
+        } else {
+            ++s->inactive;
+        }
+    }
+    if (s->active > 0) {
+        s->p_avg = p_sum / static_cast<double>(s->active);
+        s->dt_avg = dt_sum / static_cast<double>(s->active);
+    } else {
+        s->p_min = 0.0;
+        s->dt_min = 0.0;
+    }
+}
+
+}  // extern "C"

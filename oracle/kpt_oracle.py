@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/kpt_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker or the timed CPU
+baseline. The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "kokkos-particletransport_amd"))
+import kpt  # noqa: E402  (struct layouts of include/kpt.h only)
+
+_lib = None
+
+
+def _has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " avx512f " in f" {line.strip()} " and " avx512vl " in f" {line.strip()} "
+    except OSError:
+        pass
+    return False
+
+
+def lib_path() -> str:
+    v4 = os.path.join(HERE, "libkpt_oracle_v4.so")
+    if _has_avx512() and os.path.exists(v4):
+        return v4
+    return os.path.join(HERE, "libkpt_oracle.so")
+
+
+def load():
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise OSError(f"{path} is missing: build it with `make -C oracle`")
+        lib = C.CDLL(path)
+        P = C.POINTER
+        dp = P(C.c_double)
+        sig = {
+            "kpto_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+            "kpto_uniform_sqrt3": (C.c_double, [C.c_uint32]),
+            "kpto_field_create": (C.c_void_p, [C.c_int32, C.c_int32] + [dp] * 8 + [C.c_int32]),
+            "kpto_field_destroy": (None, [C.c_void_p]),
+            "kpto_field_coefficients": (C.c_int64, [C.c_void_p, dp, C.c_size_t]),
+            "kpto_field_sample": (None, [C.c_void_p, C.c_double, C.c_double, dp]),
+            "kpto_push_interval": (None, [C.c_void_p, C.c_void_p, P(kpt.DeviceSettings), C.c_double,
+                                          C.c_uint32, C.c_uint64, P(kpt.ParticlesHost), C.c_uint64,
+                                          dp, C.c_uint32, C.c_int, P(C.c_uint64), P(C.c_uint64)]),
+            "kpto_inject": (C.c_uint64, [P(kpt.DeviceSettings), P(kpt.InjectOpts), C.c_uint64,
+                                         C.c_double, C.c_uint32, C.c_uint64, C.c_uint64,
+                                         P(kpt.ParticlesHost), C.c_uint64, C.c_uint64]),
+            "kpto_split": (C.c_uint64, [P(kpt.DeviceSettings), P(kpt.ParticlesHost), C.c_uint64,
+                                        C.c_uint64, P(C.c_uint64), P(C.c_uint64)]),
+            "kpto_histogram_logp": (None, [P(kpt.ParticlesHost), C.c_uint64, C.c_int32, C.c_double,
+                                           C.c_double, dp]),
+            "kpto_histogram_xy": (None, [P(kpt.ParticlesHost), C.c_uint64, C.c_int32, C.c_int32,
+                                         C.c_double, C.c_double, C.c_double, C.c_double, dp]),
+            "kpto_summary": (None, [P(kpt.ParticlesHost), C.c_uint64, P(kpt.Summary)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().kpto_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform_sqrt3(r: int) -> float:
+    return load().kpto_uniform_sqrt3(r)
+
+
+class Field:
+    """One packed coefficient frame on the ghosted grid."""
+
+    def __init__(self, x_edges, y_edges, planes, model):
+        lib = load()
+        self.xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+        self.ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+        ps = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+        self.nx, self.ny, self.model = len(self.xe) - 1, len(self.ye) - 1, model
+        self.h = lib.kpto_field_create(self.nx, self.ny, _dp(self.xe), _dp(self.ye),
+                                       *[_dp(a) for a in ps], model)
+        if not self.h:
+            raise ValueError("oracle field creation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().kpto_field_destroy(self.h)
+            self.h = None
+
+    def coefficients(self):
+        out = np.empty((self.ny + 2) * (self.nx + 2) * self.model)
+        n = load().kpto_field_coefficients(self.h, _dp(out), out.size)
+        assert n == out.size
+        return out.reshape(self.ny + 2, self.nx + 2, self.model)
+
+    def sample(self, qx, qy):
+        out = np.zeros(32)
+        load().kpto_field_sample(self.h, qx, qy, _dp(out))
+        return out[: self.model]
+
+
+def push_interval(lower: Field, upper: Field | None, s, t0, frame, seed, parts, noise=None,
+                  threads=None):
+    """Advance parts (dict of numpy arrays, modified in place). Returns (steps, failures)."""
+    n = len(parts["x"])
+    h = kpt.particles_struct(parts)
+    steps, fails = C.c_uint64(), C.c_uint64()
+    nz = None
+    nsteps = 0
+    if noise is not None:
+        nz = np.ascontiguousarray(noise, dtype=np.float64)
+        assert nz.ndim == 3 and nz.shape[0] >= n and nz.shape[2] == 4
+        nsteps = nz.shape[1]
+        assert s.max_particle_steps_per_interval <= nsteps
+    threads = threads or os.cpu_count() or 1
+    load().kpto_push_interval(lower.h, upper.h if upper else None, C.byref(s), t0, frame, seed,
+                              C.byref(h), n, _dp(nz), nsteps, threads, C.byref(steps),
+                              C.byref(fails))
+    return steps.value, fails.value
+
+
+def inject(s, opts, requested, t0, frame, seed, first_root, parts, count, capacity):
+    h = kpt.particles_struct(parts)
+    return load().kpto_inject(C.byref(s), C.byref(opts) if opts else None, requested, t0, frame,
+                              seed, first_root, C.byref(h), count, capacity)
+
+
+def split(s, parts, count, capacity):
+    h = kpt.particles_struct(parts)
+    a, b = C.c_uint64(), C.c_uint64()
+    n = load().kpto_split(C.byref(s), C.byref(h), count, capacity, C.byref(a), C.byref(b))
+    return n, a.value, b.value
+
+
+def histogram_logp(parts, count, bins, pmin, pmax):
+    out = np.empty(bins)
+    load().kpto_histogram_logp(C.byref(kpt.particles_struct(parts)), count, bins, pmin, pmax,
+                               _dp(out))
+    return out
+
+
+def histogram_xy(parts, count, bx, by, x_lo, x_hi, y_lo, y_hi):
+    out = np.empty(bx * by)
+    load().kpto_histogram_xy(C.byref(kpt.particles_struct(parts)), count, bx, by, x_lo, x_hi,
+                             y_lo, y_hi, _dp(out))
+    return out.reshape(by, bx)
+
+
+def summary(parts, count) -> dict:
+    s = kpt.Summary()
+    load().kpto_summary(C.byref(kpt.particles_struct(parts)), count, C.byref(s))
+    return s.as_dict()

@@ -1,0 +1,304 @@
+"""GPU parity: every kernel behind the C ABI against the CPU oracle on the same
+seeded inputs. Integer / index work (injection draws, split, histogram counts)
+is compared bit for bit; FP64 push updates to 1e-12 relative (positions relative
+to the domain width), as the target states; full runs statistically (chi-square
+and KS p > 0.01)."""
+import numpy as np
+import pytest
+
+import kpt
+import kpt_oracle as O
+import workloads as W
+from helpers import SQRT3, chi2_two_sample, copy_parts, random_particles, uniform_edges
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12  # FP64 single-step / short-run parity bar
+
+
+def _ctx(capacity, seed=114514):
+    return kpt.Context(capacity, seed=seed)
+
+
+def _harris(nx=64, ny=32):
+    return W.harris_field(nx, ny)
+
+
+def _perturbed(planes, rng, amp=0.05):
+    return [p + amp * rng.standard_normal(p.shape) * (np.abs(p).max() + 0.1) for p in planes]
+
+
+def _assert_state_close(a, b, s, keys=("x", "y", "p", "t", "step_dt", "mu"), rel=REL):
+    scale = {"x": s.x_width, "y": s.y_width}
+    for k in keys:
+        x, y = a[k], b[k]
+        den = np.maximum(np.maximum(np.abs(x), np.abs(y)), scale.get(k, 0.0))
+        den = np.where(den == 0, 1.0, den)
+        err = np.abs(x - y) / den
+        assert err.max() <= rel, f"{k}: max rel err {err.max():.3e} at {int(err.argmax())}"
+    np.testing.assert_array_equal(a["status"], b["status"])
+
+
+def test_device_is_gfx950():
+    with _ctx(16) as c:
+        assert c.device_name().startswith("gfx950")
+
+
+@pytest.mark.parametrize("model", [16, 21])
+def test_pack_matches_oracle(model):
+    rng = np.random.default_rng(11)
+    xe, ye, planes = _harris(48, 40)
+    planes = _perturbed(planes, rng)
+    with _ctx(16) as c:
+        c.field_upload(kpt.SLOT_LOWER, model, xe, ye, planes)
+        got = c.field_coefficients(kpt.SLOT_LOWER, 48, 40, model)
+    want = O.Field(xe, ye, planes, model).coefficients()
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+def test_inject_bitexact():
+    xe, ye, planes = _harris()
+    s = kpt.make_device_settings(kpt.default_host_settings(), xe, ye)
+    o = kpt.InjectOpts()
+    o.use_region, o.region_x_lo, o.region_x_hi, o.region_y_lo, o.region_y_hi = 1, -0.7, 0.9, -0.1, 0.1
+    cap = 50000
+    with _ctx(cap, seed=987654321987) as c:
+        n1 = c.inject(s, 30000, 2.5, 11, o)
+        n2 = c.inject(s, 30000, 2.6, 12)  # clamps at capacity, whole domain
+        got = c.download()
+    want = kpt.empty_particles(cap)
+    m1 = O.inject(s, o, 30000, 2.5, 11, 987654321987, 0, want, 0, cap)
+    m2 = O.inject(s, None, 30000, 2.6, 12, 987654321987, m1, want, m1, cap)
+    assert (n1, n2) == (m1, m2) == (30000, 20000)
+    for k in ("x", "y", "p", "t", "step_dt", "weight", "mu", "status", "split_level", "id"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
+
+def _noise_case(model, focused, time_interp, physics, steps, n=4096, seed=5):
+    rng = np.random.default_rng(seed)
+    xe, ye, planes = _harris()
+    upper = _perturbed(planes, rng, 0.02)
+    h = kpt.default_host_settings()
+    if physics == "c1":
+        h = W.c1_host_settings()
+    h.focused_transport = int(focused)
+    h.time_interpolation = int(time_interp)
+    h.max_particle_steps_per_interval = steps
+    if physics == "reference":
+        h.p0 = 1.0
+        h.mhd_dt = 1e-3
+        h.dt_min_rel = 1e-3
+        h.dt_max_rel = 1e-1
+        h.duu0 = 5.0
+        h.particle_v0 = 0.5
+    s = kpt.make_device_settings(h, xe, ye)
+    parts = random_particles(n, s, rng, p_range=(0.3, 3.0) if physics == "reference" else (1, 1))
+    parts["t"][:] = rng.uniform(0, 0.5 * s.mhd_dt, n)
+    noise = rng.uniform(-SQRT3, SQRT3, size=(n, steps, 4))
+    return xe, ye, planes, upper, s, parts, noise
+
+
+@pytest.mark.parametrize("model,focused", [(16, False), (21, False), (21, True)])
+@pytest.mark.parametrize("time_interp", [1, 0])
+@pytest.mark.parametrize("physics", ["reference", "c1"])
+@pytest.mark.parametrize("steps", [1, 10])
+def test_push_injected_noise_matches_oracle(model, focused, time_interp, physics, steps):
+    xe, ye, planes, upper, s, parts, noise = _noise_case(model, focused, time_interp, physics, steps)
+    n = len(parts["x"])
+    want = copy_parts(parts)
+    lo, hi = O.Field(xe, ye, planes, model), O.Field(xe, ye, upper, model)
+    o_steps, o_fail = O.push_interval(lo, hi, s, 0.0, 3, 1, want, noise=noise)
+    with _ctx(n) as c:
+        c.field_upload(kpt.SLOT_LOWER, model, xe, ye, planes)
+        c.field_upload(kpt.SLOT_UPPER, model, xe, ye, upper)
+        c.upload(parts)
+        c.set_noise(noise)
+        st = c.push_interval(s, 0.0, 3)
+        got = c.download()
+    assert st.particle_steps == o_steps and st.failures == o_fail and st.active_in == n
+    _assert_state_close(got, want, s)
+
+
+def test_push_philox_trajectories_match_oracle():
+    """Same Philox stream on both sides: 200 adaptive sub-steps with the reference
+    physics; positions agree per particle (dt-limiter ties may fork a few)."""
+    xe, ye, planes, upper, s, parts, _ = _noise_case(16, False, 1, "reference", 1, n=8192, seed=9)
+    s.max_particle_steps_per_interval = 200000
+    want = copy_parts(parts)
+    lo, hi = O.Field(xe, ye, planes, 16), O.Field(xe, ye, upper, 16)
+    o_steps, _ = O.push_interval(lo, hi, s, 0.0, 17, 4242, want)
+    with _ctx(len(parts["x"]), seed=4242) as c:
+        c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, planes)
+        c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, upper)
+        c.upload(parts)
+        st = c.push_interval(s, 0.0, 17)
+        got = c.download()
+    assert abs(st.particle_steps / o_steps - 1) < 1e-3
+    close = np.abs(got["x"] - want["x"]) <= 1e-9 * s.x_width
+    close &= np.abs(got["p"] - want["p"]) <= 1e-9 * np.abs(want["p"])
+    assert close.mean() > 0.99, close.mean()
+
+
+def test_c1_statistics_against_oracle():
+    """Parker, plain FP64, small: the full 256 x 128 / 65 536-particle / 1000-step case.
+    Same seed -> per-particle agreement; independent seeds -> the spatial density
+    (64 x 32) and momentum spectrum agree within Monte Carlo error."""
+    cfg = W.C1
+    xe, ye, planes = W.harris_field(cfg["nx"], cfg["ny"])
+    s = kpt.make_device_settings(W.c1_host_settings(), xe, ye)
+    o = W.c1_inject_opts()
+    n = cfg["particles"]
+    lo = O.Field(xe, ye, planes, 16)
+    results = {}
+    for seed in (114514, 2718):
+        with _ctx(n, seed=seed) as c:
+            c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, planes)
+            c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, planes)
+            assert c.inject(s, n, 0.0, 0, o) == n
+            st = c.push_interval(s, 0.0, 0)
+            assert st.failures == 0
+            results[seed] = (c.download(), c.histogram_xy(64, 32, -1, 1, -0.5, 0.5),
+                             c.histogram_logp(cfg["hist_bins"], cfg["hist_pmin"], cfg["hist_pmax"]))
+    want = kpt.empty_particles(n)
+    O.inject(s, o, n, 0.0, 0, 114514, 0, want, 0, n)
+    O.push_interval(lo, lo, s, 0.0, 0, 114514, want)
+    got, hxy, hp = results[114514]
+    assert np.max(np.abs(got["x"] - want["x"])) < 1e-10
+    assert np.max(np.abs(got["p"] / want["p"] - 1)) < 1e-10
+    np.testing.assert_array_equal(hp, O.histogram_logp(want, n, 64, 0.1, 1e3))
+    # independent seeds: statistical agreement
+    other, hxy2, hp2 = results[2718]
+    _, _, pval = chi2_two_sample(hxy, hxy2)
+    assert pval > 0.01, pval
+    _, _, pval = chi2_two_sample(hp, hp2)
+    assert pval > 0.01, pval
+    from scipy import stats
+
+    assert stats.ks_2samp(got["y"], other["y"]).pvalue > 0.01
+    assert stats.ks_2samp(got["p"], other["p"]).pvalue > 0.01
+
+
+def test_split_bitexact():
+    h = kpt.default_host_settings()
+    h.p0 = 1.0
+    h.split_ratio = 1.2
+    h.pmax = 10.0
+    xe, ye = uniform_edges(8, -1, 1), uniform_edges(8, -1, 1)
+    s = kpt.make_device_settings(h, xe, ye)
+    rng = np.random.default_rng(8)
+    n, cap = 70001, 80000  # ~19k candidates for 9999 free slots: capacity is reached
+    parts = random_particles(n, s, rng, p_range=(0.5, 20.0))
+    parts["split_level"][:] = rng.integers(0, 6, n)
+    parts["weight"][:] = 0.5 ** parts["split_level"]
+    parts["status"][rng.uniform(size=n) < 0.1] = 2
+    parts["id"][:] = rng.integers(0, 2**31, n).astype(np.uint64)
+    want = kpt.empty_particles(cap)
+    want["status"][:] = 2
+    for k in parts:
+        want[k][:n] = parts[k]
+    new_n, o_split, o_skip = O.split(s, want, n, cap)
+    with _ctx(cap) as c:
+        c.upload(parts)
+        split, skipped = c.split(s)
+        assert c.count == new_n
+        got = c.download()
+    assert (split, skipped) == (o_split, o_skip) and o_skip > 0  # capacity reached
+    for k in ("x", "y", "p", "t", "step_dt", "weight", "mu", "status", "split_level", "id"):
+        np.testing.assert_array_equal(got[k], want[k][:new_n], err_msg=k)
+
+
+def test_histograms_and_summary_match_oracle():
+    h = kpt.default_host_settings()
+    xe, ye = uniform_edges(8, -1, 1), uniform_edges(8, -0.5, 0.5)
+    s = kpt.make_device_settings(h, xe, ye)
+    rng = np.random.default_rng(21)
+    n = 300001
+    parts = random_particles(n, s, rng, p_range=(0.005, 20.0))
+    parts["weight"][:] = 0.5 ** rng.integers(0, 12, n)
+    parts["status"][rng.uniform(size=n) < 0.07] = 2
+    parts["step_dt"][:] = rng.uniform(1e-7, 1e-3, n)
+    parts["split_level"][:] = rng.integers(0, 9, n)
+    with _ctx(n) as c:
+        c.upload(parts)
+        hp = c.histogram_logp(128, 0.01, 10.0)
+        hxy = c.histogram_xy(64, 32, -1, 1, -0.5, 0.5)
+        sm = c.summary()
+    np.testing.assert_array_equal(hp, O.histogram_logp(parts, n, 128, 0.01, 10.0))
+    np.testing.assert_array_equal(hxy, O.histogram_xy(parts, n, 64, 32, -1, 1, -0.5, 0.5))
+    want = O.summary(parts, n)
+    for k in ("stored", "active", "inactive", "max_split_level", "p_min", "p_max", "dt_min", "dt_max"):
+        assert sm[k] == want[k], k
+    for k in ("weight_sum", "p_avg", "dt_avg"):
+        assert sm[k] == pytest.approx(want[k], rel=1e-12), k
+
+
+def test_failure_paths():
+    xe, ye = uniform_edges(16, -1, 1), uniform_edges(16, -1, 1)
+    h = kpt.default_host_settings()
+    s = kpt.make_device_settings(h, xe, ye)
+    zero = [np.zeros((16, 16))] * 6
+    parts = random_particles(1000, s, np.random.default_rng(0))
+    parts["status"][::2] = 2
+    with _ctx(1000) as c:
+        c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, zero)
+        c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, zero)
+        c.upload(parts)
+        st = c.push_interval(s, 0.0)
+        assert (st.failures, st.active_in, st.particle_steps) == (500, 500, 0)
+        assert np.all(c.download()["status"] == 2)
+        # step guard
+        one = [np.ones((16, 16))] + [np.zeros((16, 16))] * 5
+        c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, one)
+        c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, one)
+        parts["status"][:] = 0
+        c.upload(parts)
+        s2 = s.copy()
+        s2.max_particle_steps_per_interval = 5
+        st = c.push_interval(s2, 0.0)
+        assert st.failures == 1000 and st.particle_steps == 5000
+        # noise table shorter than the guard is rejected on the host
+        c.set_noise(np.zeros((1000, 3, 4)))
+        with pytest.raises(kpt.KptError, match="noise table"):
+            c.push_interval(s2, 0.0)
+        c.clear_noise()
+        # missing upper frame with time interpolation
+        c.field_advance()
+        with pytest.raises(kpt.KptError, match="upper coefficient frame"):
+            c.push_interval(s, 0.0)
+
+
+def test_snapshot_roundtrip_through_device(tmp_path):
+    h = kpt.default_host_settings()
+    xe, ye = uniform_edges(8, 0, 2), uniform_edges(8, 0, 2)
+    s = kpt.make_device_settings(h, xe, ye)
+    parts = random_particles(5000, s, np.random.default_rng(3))
+    path = str(tmp_path / "particles_00007.bin")
+    with _ctx(8000) as c:
+        c.upload(parts)
+        c.snapshot_write(path, -1.0, 2.0, 0.1)
+    with _ctx(8000) as c:
+        c.snapshot_read(path, -1.0, 2.0, 0.7, s.dt_min)
+        got = c.download()
+    for k in ("x", "y", "p", "mu", "weight", "status", "split_level"):
+        np.testing.assert_array_equal(got[k], parts[k])
+    assert np.all(got["t"] == 0.7) and np.all(got["step_dt"] == s.dt_min)
+
+
+def test_fp32_bulk_close_to_fp64():
+    """FP32 coefficient storage / interpolation: one 10-step interval stays within
+    1e-5 relative of the FP64 push on the same noise."""
+    xe, ye, planes, upper, s, parts, noise = _noise_case(16, False, 1, "reference", 10, n=4096)
+    res = {}
+    for prec in (kpt.PREC_FP64, kpt.PREC_FP32):
+        s2 = s.copy()
+        s2.precision = prec
+        with _ctx(len(parts["x"])) as c:
+            c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, planes)
+            c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, upper)
+            c.upload(parts)
+            c.set_noise(noise)
+            c.push_interval(s2, 0.0)
+            res[prec] = c.download()
+    a, b = res[kpt.PREC_FP64], res[kpt.PREC_FP32]
+    assert np.max(np.abs(a["x"] - b["x"])) < 1e-5 * s.x_width
+    assert np.max(np.abs(a["p"] / b["p"] - 1)) < 1e-5
