@@ -726,8 +726,11 @@ kpt_status kpt_split(kpt_ctx* ctx, const kpt_device_settings* s, uint64_t* split
     if (ctx->count >= ctx->capacity || s->split_particles == 0 || ctx->count == 0) return KPT_OK;
     (void)hipSetDevice(ctx->device);
     KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
-    KPT_HIP(ctx, kpt::launch_split(ctx->P, ctx->count, ctx->capacity, *s, ctx->split_scratch, ctx->counters,
-                                   ctx->stream));
+    kpt::SplitTable tab;
+    for (int L = 0; L < kpt::SplitTable::kLevels; ++L)
+        tab.thr[L] = s->pmin_split * std::pow(s->split_ratio, static_cast<double>(L));
+    KPT_HIP(ctx, kpt::launch_split(ctx->P, ctx->count, ctx->capacity, *s, tab, ctx->split_scratch,
+                                   ctx->counters, ctx->stream));
     unsigned long long c[2];
     uint32_t total = 0;
     const unsigned tiles = static_cast<unsigned>(kpt::split_scratch_elems(ctx->count) - 1);

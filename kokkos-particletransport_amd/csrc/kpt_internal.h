@@ -65,10 +65,16 @@ hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, hi
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);
+// Split thresholds pmin_split * split_ratio^level for level < kLevels.
+struct SplitTable {
+    static constexpr int kLevels = 64;
+    double thr[kLevels];
+};
+
 // Deterministic split. `scratch` must hold split_scratch_elems(count) uint32.
 size_t split_scratch_elems(uint64_t count);
 hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capacity,
-                        const kpt_device_settings& s, uint32_t* scratch,
+                        const kpt_device_settings& s, const SplitTable& tab, uint32_t* scratch,
                         unsigned long long* counters /* [0] split, [1] skipped */,
                         hipStream_t stream);
 // Histograms: `partials` holds hist_partial_elems(bins) doubles.

@@ -586,20 +586,25 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(ParticleArrays P, uint64
 // inside the tile with wavefront ballots and appends the children in parent
 // slot order.
 // ---------------------------------------------------------------------------
+// The threshold pmin_split * ratio^level comes from a host-computed table so
+// the predicate is the same double the host library (and the oracle) computes.
 __device__ __forceinline__ bool split_candidate(const ParticleArrays& P, uint64_t i,
-                                                const kpt_device_settings& s) {
+                                                const kpt_device_settings& s, const SplitTable& tab) {
     if (P.status[i] != 0) return false;
-    const double thr = s.pmin_split * pow(s.split_ratio, static_cast<double>(P.split_level[i]));
+    const int level = P.split_level[i];
+    const double thr = (level >= 0 && level < SplitTable::kLevels)
+                           ? tab.thr[level]
+                           : s.pmin_split * pow(s.split_ratio, static_cast<double>(level));
     const double p = P.p[i];
     return p > thr && p <= s.pmax;
 }
 
 __global__ void __launch_bounds__(kBlock) split_count_kernel(ParticleArrays P, uint64_t count,
-                                                             kpt_device_settings s,
+                                                             kpt_device_settings s, SplitTable tab,
                                                              uint32_t* __restrict__ tile_counts) {
     __shared__ uint32_t wave_counts[kBlock / 64];
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    const bool f = i < count && split_candidate(P, i, s);
+    const bool f = i < count && split_candidate(P, i, s, tab);
     const uint64_t m = __ballot(f);
     if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = __popcll(m);
     __syncthreads();
@@ -640,11 +645,12 @@ __global__ void __launch_bounds__(1024) split_scan_kernel(uint32_t* __restrict__
 
 __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, uint64_t count,
                                                              uint64_t capacity, kpt_device_settings s,
+                                                             SplitTable tab,
                                                              const uint32_t* __restrict__ tile_offsets,
                                                              unsigned long long* counters) {
     __shared__ uint32_t wave_counts[kBlock / 64];
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    const bool f = i < count && split_candidate(P, i, s);
+    const bool f = i < count && split_candidate(P, i, s, tab);
     const uint64_t m = __ballot(f);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -662,7 +668,7 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, u
         } else {
             const int old_level = P.split_level[i];
             const int new_level = old_level + 1;
-            const double w = pow(0.5, static_cast<double>(new_level));
+            const double w = ldexp(1.0, -new_level);  // 0.5^level, exact
             P.x[child] = P.x[i];
             P.y[child] = P.y[i];
             P.p[child] = P.p[i];
@@ -675,7 +681,7 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, u
             P.split_level[i] = new_level;
             P.split_level[child] = new_level;
             const uint64_t pid = P.id[i];
-            const uint64_t path = (pid >> 32) | (1ull << (old_level & 31));
+            const uint64_t path = (pid >> 32) | (old_level < 32 ? (1ull << old_level) : 0ull);
             P.id[child] = (pid & 0xFFFFFFFFull) | (path << 32);
             done = 1;
         }
@@ -903,13 +909,13 @@ hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, ui
 size_t split_scratch_elems(uint64_t count) { return static_cast<size_t>(grid_for(count)) + 1; }
 
 hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capacity,
-                        const kpt_device_settings& s, uint32_t* scratch, unsigned long long* counters,
-                        hipStream_t stream) {
+                        const kpt_device_settings& s, const SplitTable& tab, uint32_t* scratch,
+                        unsigned long long* counters, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const unsigned tiles = grid_for(count);
-    hipLaunchKernelGGL(split_count_kernel, dim3(tiles), dim3(kBlock), 0, stream, P, count, s, scratch);
+    hipLaunchKernelGGL(split_count_kernel, dim3(tiles), dim3(kBlock), 0, stream, P, count, s, tab, scratch);
     hipLaunchKernelGGL(split_scan_kernel, dim3(1), dim3(1024), 0, stream, scratch, tiles);
-    hipLaunchKernelGGL(split_apply_kernel, dim3(tiles), dim3(kBlock), 0, stream, P, count, capacity, s,
+    hipLaunchKernelGGL(split_apply_kernel, dim3(tiles), dim3(kBlock), 0, stream, P, count, capacity, s, tab,
                        scratch, counters);
     return hipGetLastError();
 }

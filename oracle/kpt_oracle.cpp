@@ -765,7 +765,8 @@ uint64_t kpto_split(const kpt_device_settings* s, kpt_particles_host* P, uint64_
         P->split_level[child] = new_level;
         if (P->id) {
             const uint64_t pid = P->id[parent];
-            const uint64_t path = (pid >> 32) | (static_cast<uint64_t>(1) << old_level);
+            const uint64_t path =
+                (pid >> 32) | (old_level < 32 ? (static_cast<uint64_t>(1) << old_level) : 0);
             P->id[child] = (pid & 0xFFFFFFFFull) | (path << 32);
         }
         ++split;
