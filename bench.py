@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sort", action="store_true",
+                    help="push in injection order instead of sorting by cell before each interval")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -136,9 +138,15 @@ def main():
         assert ctx.inject(s, n, 0.0, 0, opts) == n
         start = ctx.download()
 
+        def interval(frame):
+            # one bench step: cell sort (ParticleSystem::sort_by_key) + the push
+            if not args.no_sort:
+                ctx.sort(s)
+            return ctx.push_interval(s, frame * s.mhd_dt, frame)
+
         frame = 0
         for _ in range(args.warmup):
-            st = ctx.push_interval(s, frame * s.mhd_dt, frame)
+            st = interval(frame)
             log(f"warmup frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
             frame += 1
         steps = 0
@@ -147,7 +155,7 @@ def main():
         failures = 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            st = ctx.push_interval(s, frame * s.mhd_dt, frame)
+            st = interval(frame)
             steps += st.particle_steps
             failures += st.failures
             kernel_ms.append(st.kernel_ms)

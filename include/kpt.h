@@ -294,6 +294,15 @@ kpt_status kpt_histogram_logp(kpt_ctx* ctx, int32_t bins, double pmin, double pm
 kpt_status kpt_histogram_xy(kpt_ctx* ctx, int32_t bins_x, int32_t bins_y, double x_lo, double x_hi,
                             double y_lo, double y_hi, double* out);
 
+/* ParticleSystem::sort_by_key + compact_active_particles
+ * (include/ParticleSystem.hpp:715-793): reorder the stored particles by the
+ * lower-left node of their interpolation stencil on the lower frame's grid
+ * (stable), inactive particles last. With compact != 0 the inactive particles
+ * are dropped and the count shrinks to the active count. Slot order changes, so
+ * the injected-noise mode (which is keyed by slot) rejects this call. */
+kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_t compact,
+                              uint64_t* active);
+
 /* summarize_particles (src/main.cpp:1844-1901), computed on the device. */
 kpt_status kpt_summary_compute(kpt_ctx* ctx, kpt_summary* out);
 

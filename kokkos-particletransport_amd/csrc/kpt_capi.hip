@@ -55,6 +55,10 @@ struct kpt_ctx {
     double* hist_out = nullptr;
     void* summary_partials = nullptr;
     kpt_summary* summary_out = nullptr;
+    // cell sort: second particle set (allocated on first sort) and scratch
+    kpt::ParticleArrays P2{};
+    bool p2_allocated = false;
+    kpt::SortScratch sort{};
     std::string err;
     char device_name[64] = {0};
 };
@@ -378,6 +382,19 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.c64);
         dev_free(f.c32);
     }
+    if (ctx->p2_allocated) {
+        kpt::ParticleArrays& Q = ctx->P2;
+        for (void* p : {static_cast<void*>(Q.x), static_cast<void*>(Q.y), static_cast<void*>(Q.p),
+                        static_cast<void*>(Q.t), static_cast<void*>(Q.step_dt), static_cast<void*>(Q.weight),
+                        static_cast<void*>(Q.mu), static_cast<void*>(Q.status),
+                        static_cast<void*>(Q.split_level), static_cast<void*>(Q.id)})
+            dev_free(p);
+    }
+    for (int k = 0; k < 2; ++k) {
+        dev_free(ctx->sort.keys[k]);
+        dev_free(ctx->sort.vals[k]);
+    }
+    dev_free(ctx->sort.temp);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -770,6 +787,60 @@ kpt_status kpt_histogram_xy(kpt_ctx* ctx, int32_t bx, int32_t by, double x_lo, d
     KPT_HIP(ctx, hipMemcpyAsync(out, ctx->hist_out, static_cast<size_t>(bx) * by * sizeof(double),
                                 hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_t compact, uint64_t* active) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_settings(ctx, s);
+    if (st != KPT_OK) return st;
+    if (ctx->noise) return fail(ctx, KPT_ERR_STATE, "the injected-noise table is keyed by slot: clear it before sorting.");
+    const FieldSlot& lo = ctx->slot[KPT_SLOT_LOWER];
+    if (!lo.valid) return fail(ctx, KPT_ERR_STATE, "lower coefficient frame is not loaded.");
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->p2_allocated) {
+        kpt::ParticleArrays& Q = ctx->P2;
+        const uint64_t cap = ctx->capacity;
+        KPT_HIP(ctx, dev_alloc(&Q.x, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.y, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.p, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.t, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.step_dt, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.weight, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.mu, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.status, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.split_level, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.id, cap));
+        for (int k = 0; k < 2; ++k) {
+            KPT_HIP(ctx, dev_alloc(&ctx->sort.keys[k], cap));
+            KPT_HIP(ctx, dev_alloc(&ctx->sort.vals[k], cap));
+        }
+        ctx->sort.temp_bytes = kpt::sort_temp_bytes(cap);
+        KPT_HIP(ctx, hipMalloc(&ctx->sort.temp, std::max<size_t>(ctx->sort.temp_bytes, 1)));
+        // the slots past `count` keep the storage default status 2
+        std::vector<int32_t> two(static_cast<size_t>(std::min<uint64_t>(cap, 1u << 20)), 2);
+        for (uint64_t off = 0; off < cap; off += two.size()) {
+            const size_t n2 = static_cast<size_t>(std::min<uint64_t>(two.size(), cap - off));
+            KPT_HIP(ctx, hipMemcpy(Q.status + off, two.data(), n2 * 4, hipMemcpyHostToDevice));
+        }
+        ctx->p2_allocated = true;
+    }
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_sort(ctx->P, ctx->P2, ctx->count, lo.g, s->interpolation_shift_x,
+                                  s->interpolation_shift_y, ctx->sort, ctx->counters, ctx->stream));
+    unsigned long long act = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&act, ctx->counters, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::swap(ctx->P, ctx->P2);
+    if (compact) {
+        // dropped slots return to the storage default (status 2)
+        if (act < ctx->count) {
+            std::vector<int32_t> two(ctx->count - act, 2);
+            KPT_HIP(ctx, hipMemcpy(ctx->P.status + act, two.data(), two.size() * 4, hipMemcpyHostToDevice));
+        }
+        ctx->count = act;
+    }
+    if (active) *active = act;
     return KPT_OK;
 }
 

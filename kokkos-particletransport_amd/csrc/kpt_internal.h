@@ -85,6 +85,21 @@ hipError_t launch_hist_logp(const ParticleArrays& P, uint64_t count, int bins, d
 hipError_t launch_hist_xy(const ParticleArrays& P, uint64_t count, int bx, int by, double x_lo,
                           double x_hi, double y_lo, double y_hi, double* partials, double* out,
                           hipStream_t stream);
+// Cell sort (kpt_sort.hip): keys/values double buffers of `capacity` u32 each
+// plus radix-sort temporary storage of sort_temp_bytes(capacity).
+struct SortScratch {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    void* temp;
+    size_t temp_bytes;
+};
+size_t sort_temp_bytes(uint64_t capacity);
+// Writes the sorted particle set into `dst`; `active` (device) accumulates the
+// number of status-0 particles.
+hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,
+                       double shift_x, double shift_y, SortScratch& w, unsigned long long* active,
+                       hipStream_t stream);
+
 // Summary: `partials` holds summary_partial_bytes() bytes.
 size_t summary_partial_bytes();
 hipError_t launch_summary(const ParticleArrays& P, uint64_t count, void* partials,

@@ -1,0 +1,106 @@
+// kpt_sort.hip — cell sort and compaction of the particle set
+// (ParticleSystem::assign_sort_keys / sort_by_key / compact_active_particles,
+// include/ParticleSystem.hpp:715-793, which use Kokkos::BinSort).
+//
+// Key = node index of the lower-left stencil corner on the lower frame's grid,
+// the same node the push gathers first, so after the sort the 64 lanes of a
+// wavefront gather overlapping 128 B lines and adjacent workgroups touch adjacent
+// field rows. Inactive particles get the largest key and land at the end. The
+// stable LSD radix sort (rocPRIM through hipCUB) keeps equal keys in slot order,
+// so the result is deterministic.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kpt_internal.h"
+
+namespace kpt {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint64_t count, GridDesc g,
+                                                          double shift_x, double shift_y, uint32_t inactive_key,
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                          unsigned long long* __restrict__ active) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    bool act = false;
+    if (i < count) {
+        uint32_t key = inactive_key;
+        if (P.status[i] == 0) {
+            act = true;
+            const double q[2] = {P.x[i] + shift_x, P.y[i] + shift_y};
+            const int n[2] = {g.nx, g.ny};
+            int idx[2];
+            for (int d = 0; d < 2; ++d) {
+                double w = q[d] - floor((q[d] - g.phys_lo[d]) * g.inv_width[d]) * g.width[d];
+                if (w >= g.phys_hi[d]) w -= g.width[d];
+                if (w < g.phys_lo[d]) w += g.width[d];
+                double l = floor((w - g.start[d]) * g.inv_step[d] - 0.5);
+                l = l == l ? l : -1.0;  // NaN positions sort first; the push marks them
+                l = l < -1.0 ? -1.0 : (l > n[d] - 1 ? n[d] - 1 : l);
+                idx[d] = static_cast<int>(l);
+            }
+            key = static_cast<uint32_t>((idx[1] + 1) * (g.nx + 2) + (idx[0] + 1));
+        }
+        keys[i] = key;
+        vals[i] = static_cast<uint32_t>(i);
+    }
+    const uint64_t m = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(active, static_cast<unsigned long long>(__popcll(m)));
+}
+
+__global__ void __launch_bounds__(kBlock) gather_kernel(ParticleArrays src, ParticleArrays dst, uint64_t count,
+                                                        const uint32_t* __restrict__ perm) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t s = perm[i];
+    dst.x[i] = src.x[s];
+    dst.y[i] = src.y[s];
+    dst.p[i] = src.p[s];
+    dst.t[i] = src.t[s];
+    dst.step_dt[i] = src.step_dt[s];
+    dst.weight[i] = src.weight[s];
+    dst.mu[i] = src.mu[s];
+    dst.status[i] = src.status[s];
+    dst.split_level[i] = src.split_level[s];
+    dst.id[i] = src.id[s];
+}
+
+int key_bits(uint32_t max_key) {
+    int b = 1;
+    while (b < 32 && (static_cast<uint64_t>(1) << b) <= max_key) ++b;
+    return b;
+}
+
+}  // namespace
+
+size_t sort_temp_bytes(uint64_t capacity) {
+    size_t bytes = 0;
+    hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, static_cast<int>(capacity), 0, 32);
+    return bytes;
+}
+
+hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,
+                       double shift_x, double shift_y, SortScratch& w, unsigned long long* active,
+                       hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const uint32_t inactive_key = static_cast<uint32_t>((g.nx + 2) * (g.ny + 2));
+    const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(sort_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, count, g, shift_x, shift_y,
+                       inactive_key, w.keys[0], w.vals[0], active);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipcub::DoubleBuffer<uint32_t> k(w.keys[0], w.keys[1]), v(w.vals[0], w.vals[1]);
+    size_t bytes = w.temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, k, v, static_cast<int>(count), 0, key_bits(inactive_key),
+                                           stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, dst, count, v.Current());
+    return hipGetLastError();
+}
+
+}  // namespace kpt

@@ -190,6 +190,8 @@ EXPORTS = {
     "kpt_histogram_xy": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.POINTER(C.c_double)]),
     "kpt_summary_compute": (C.c_int, [C.c_void_p, C.POINTER(Summary)]),
+    "kpt_sort_particles": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.c_int32,
+                                     C.POINTER(C.c_uint64)]),
     "kpt_set_noise": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.c_uint32]),
     "kpt_clear_noise": (C.c_int, [C.c_void_p]),
     "kpt_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -433,6 +435,12 @@ class Context:
         out = np.empty(bx * by)
         self._check(self.lib.kpt_histogram_xy(self.h, bx, by, x_lo, x_hi, y_lo, y_hi, _dp(out)))
         return out.reshape(by, bx)
+
+    def sort(self, s: DeviceSettings, compact=False) -> int:
+        """Sort by stencil cell (inactive last); returns the active count."""
+        act = C.c_uint64()
+        self._check(self.lib.kpt_sort_particles(self.h, C.byref(s), int(compact), C.byref(act)))
+        return act.value
 
     def summary(self) -> dict:
         s = Summary()

@@ -186,6 +186,55 @@ def test_c1_statistics_against_oracle():
     assert stats.ks_2samp(got["p"], other["p"]).pvalue > 0.01
 
 
+def test_sort_is_a_permutation_and_does_not_change_physics():
+    """Cell sort: same particle multiset, keys non-decreasing, inactive last,
+    compaction drops them; the Philox stream is keyed by particle id, so the
+    push of the sorted set equals the push of the unsorted set particle by
+    particle (matched by id), bit for bit."""
+    xe, ye, planes, upper, s, parts, _ = _noise_case(16, False, 1, "reference", 1, n=50000, seed=4)
+    s.max_particle_steps_per_interval = 200000
+    parts["status"][::9] = 2
+    parts["id"][:] = np.random.default_rng(1).permutation(len(parts["x"])).astype(np.uint64)
+    n = len(parts["x"])
+    res = {}
+    for do_sort in (False, True):
+        with _ctx(n, seed=77) as c:
+            c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, planes)
+            c.field_upload(kpt.SLOT_UPPER, 16, xe, ye, upper)
+            c.upload(parts)
+            if do_sort:
+                act = c.sort(s)
+                assert act == int((parts["status"] == 0).sum()) and c.count == n
+                srt = c.download()
+                for k in parts:
+                    np.testing.assert_array_equal(np.sort(srt[k]), np.sort(parts[k]), err_msg=k)
+                assert np.all(srt["status"][:act] == 0) and np.all(srt["status"][act:] == 2)
+                qx = srt["x"][:act] + 0.5 * s.dx
+                qx = np.where(qx >= s.x_upper, qx - s.x_width, qx)
+                qy = srt["y"][:act] + 0.5 * s.dy
+                qy = np.where(qy >= s.y_upper, qy - s.y_width, qy)
+                ix = np.floor((qx - s.x_lower) / s.dx - 0.5).astype(int)
+                iy = np.floor((qy - s.y_lower) / s.dy - 0.5).astype(int)
+                key = (iy + 1) * 66 + (ix + 1)  # 64 x 32 grid: 66 stored nodes per row
+                assert np.mean(np.diff(key) >= 0) > 0.999
+            st = c.push_interval(s, 0.0, 5)
+            got = c.download()
+            res[do_sort] = (st.particle_steps, got)
+    assert res[False][0] == res[True][0]
+    a, b = res[False][1], res[True][1]
+    ia, ib = np.argsort(a["id"]), np.argsort(b["id"])
+    for k in ("x", "y", "p", "t", "step_dt", "status"):
+        np.testing.assert_array_equal(a[k][ia], b[k][ib], err_msg=k)
+    with _ctx(n, seed=77) as c:
+        c.field_upload(kpt.SLOT_LOWER, 16, xe, ye, planes)
+        c.upload(parts)
+        act = c.sort(s, compact=True)
+        assert c.count == act == int((parts["status"] == 0).sum())
+        c.set_noise(np.zeros((n, 1, 4)))
+        with pytest.raises(kpt.KptError, match="keyed by slot"):
+            c.sort(s)
+
+
 def test_split_bitexact():
     h = kpt.default_host_settings()
     h.p0 = 1.0
