@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sort", action="store_true",
                     help="push in injection order instead of sorting by cell before each interval")
+    ap.add_argument("--chunks", type=int, default=8,
+                    help="time chunks per interval, each preceded by a cell sort (push_chunks)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -138,9 +140,12 @@ def main():
         assert ctx.inject(s, n, 0.0, 0, opts) == n
         start = ctx.download()
 
+        s.push_chunks = 1 if args.no_sort else args.chunks
+
         def interval(frame):
-            # one bench step: cell sort (ParticleSystem::sort_by_key) + the push
-            if not args.no_sort:
+            # one bench step = one interval: with push_chunks > 1 the library
+            # re-sorts by cell before every chunk; with one chunk sort here
+            if not args.no_sort and s.push_chunks <= 1:
                 ctx.sort(s)
             return ctx.push_interval(s, frame * s.mhd_dt, frame)
 

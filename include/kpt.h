@@ -104,8 +104,14 @@ typedef struct kpt_device_settings {
     double mu_max;
 
     /* extensions */
-    int32_t precision;  /* kpt_precision */
-    int32_t reserved0;
+    int32_t precision;   /* kpt_precision */
+    /* kpt_push_interval splits the interval into this many equal time chunks and
+     * launches the push once per chunk, re-sorting particles by cell before each
+     * chunk (kpt_sort_particles; skipped in the injected-noise mode). Particles
+     * are not stopped at chunk ends: the sub-step counter, the step guard and the
+     * truncation of the last step to the interval end carry across chunks, so the
+     * result equals one launch. 0 or 1: one launch, no sort. */
+    int32_t push_chunks;
 } kpt_device_settings;
 
 /* Host-side controls that kpt_make_device_settings reads (the physics subset of

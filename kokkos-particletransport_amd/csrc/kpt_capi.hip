@@ -157,14 +157,18 @@ kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
                     std::isfinite(s->dy) && s->dy > 0.0 && std::isfinite(s->charge) && s->charge != 0.0 &&
                     std::isfinite(s->kpara0) && s->kpara0 > 0.0 && std::isfinite(s->kperp_over_kpara) &&
                     s->kperp_over_kpara >= 0.0 && s->max_particle_steps_per_interval > 0 &&
-                    (s->precision == KPT_PREC_FP64 || s->precision == KPT_PREC_FP32);
+                    (s->precision == KPT_PREC_FP64 || s->precision == KPT_PREC_FP32) &&
+                    s->push_chunks >= 0 && s->push_chunks <= 4096;
     if (!ok) return fail(ctx, KPT_ERR_ARG, "Reconnection2DSettings: invalid physical parameters.");
     return KPT_OK;
 }
 
+kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_done, bool compact,
+                         uint64_t* active);
+
 kpt_status ensure_field_buffers(kpt_ctx* ctx, FieldSlot& f, int32_t nx, int32_t ny, int model) {
     const size_t raw = static_cast<size_t>(nx) * ny * 6;
-    const size_t nodes = static_cast<size_t>(nx + 2) * (ny + 2) * kpt::coefficient_stride(model);
+    const size_t nodes = kpt::coefficient_plane(nx, ny) * static_cast<size_t>(model);
     if (raw > f.raw_cap) {
         dev_free(f.raw);
         f.raw = nullptr;
@@ -343,6 +347,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     if ((e = dev_alloc(&P.status, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&P.split_level, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&P.id, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.substep, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
     ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
     if ((e = dev_alloc(&ctx->split_scratch, ctx->split_scratch_cap)) != hipSuccess) return bail(e, "hipMalloc");
@@ -372,7 +377,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
     for (void* p : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p),
                     static_cast<void*>(P.t), static_cast<void*>(P.step_dt), static_cast<void*>(P.weight),
                     static_cast<void*>(P.mu), static_cast<void*>(P.status), static_cast<void*>(P.split_level),
-                    static_cast<void*>(P.id), static_cast<void*>(ctx->counters),
+                    static_cast<void*>(P.id), static_cast<void*>(P.substep), static_cast<void*>(ctx->counters),
                     static_cast<void*>(ctx->split_scratch), static_cast<void*>(ctx->hist_partials),
                     static_cast<void*>(ctx->hist_out), ctx->summary_partials,
                     static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
@@ -387,7 +392,8 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         for (void* p : {static_cast<void*>(Q.x), static_cast<void*>(Q.y), static_cast<void*>(Q.p),
                         static_cast<void*>(Q.t), static_cast<void*>(Q.step_dt), static_cast<void*>(Q.weight),
                         static_cast<void*>(Q.mu), static_cast<void*>(Q.status),
-                        static_cast<void*>(Q.split_level), static_cast<void*>(Q.id)})
+                        static_cast<void*>(Q.split_level), static_cast<void*>(Q.id),
+                        static_cast<void*>(Q.substep)})
             dev_free(p);
     }
     for (int k = 0; k < 2; ++k) {
@@ -530,14 +536,13 @@ kpt_status kpt_field_download_coefficients(kpt_ctx* ctx, int32_t slot, double* o
     if (!ctx || !out || (slot != 0 && slot != 1)) return KPT_ERR_ARG;
     const FieldSlot& f = ctx->slot[slot];
     if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
-    const size_t nodes = static_cast<size_t>(f.g.nx + 2) * (f.g.ny + 2);
-    const int stride = kpt::coefficient_stride(f.model);
+    const size_t nodes = kpt::coefficient_plane(f.g.nx, f.g.ny);
     if (out_len < nodes * f.model) return fail(ctx, KPT_ERR_ARG, "output buffer too small.");
-    std::vector<double> tmp(nodes * stride);
+    std::vector<double> tmp(nodes * f.model);
     (void)hipSetDevice(ctx->device);
     KPT_HIP(ctx, hipMemcpy(tmp.data(), f.c64, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t n = 0; n < nodes; ++n)
-        for (int c = 0; c < f.model; ++c) out[n * f.model + c] = tmp[n * stride + c];
+        for (int c = 0; c < f.model; ++c) out[n * f.model + c] = tmp[c * nodes + n];
     return KPT_OK;
 }
 
@@ -695,13 +700,12 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
             return fail(ctx, KPT_ERR_STATE, "max_particle_steps_per_interval exceeds the noise table's steps.");
     }
     kpt::PushArgs a{};
-    a.P = ctx->P;
     a.count = ctx->count;
     const bool fp32 = s->precision == KPT_PREC_FP32;
     a.field_lo = fp32 ? static_cast<const void*>(lo.c32) : static_cast<const void*>(lo.c64);
     a.field_hi = s->time_interpolation != 0 ? (fp32 ? static_cast<const void*>(hi.c32) : static_cast<const void*>(hi.c64))
                                             : a.field_lo;
-    a.stride = kpt::coefficient_stride(lo.model);
+    a.plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
     a.g = lo.g;
     a.s = *s;
     a.t0 = t0;
@@ -713,22 +717,39 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.noise_steps = ctx->noise_steps;
     a.counters = ctx->counters;
     (void)hipSetDevice(ctx->device);
-    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
-    KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
-    KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, ctx->stream));
-    KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    unsigned long long c[8];
-    KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0.0f;
-    KPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const int chunks = s->push_chunks > 1 ? s->push_chunks : 1;
+    const bool resort = chunks > 1 && ctx->noise == nullptr;
+    unsigned long long total[3] = {0, 0, 0};
+    double ms_total = 0.0;
+    for (int k = 0; k < chunks; ++k) {
+        if (resort) {
+            st = sort_in_place(ctx, *s, a.t_end, false, nullptr);
+            if (st != KPT_OK) return st;
+        }
+        a.P = ctx->P;
+        a.first_chunk = k == 0 ? 1 : 0;
+        a.t_stop = k == chunks - 1 ? a.t_end : t0 + s->mhd_dt * static_cast<double>(k + 1) / chunks;
+        KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, ctx->stream));
+        KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        unsigned long long c[4];
+        KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0.0f;
+        KPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms_total += ms;
+        total[0] += c[0];
+        total[1] += c[1];
+        if (k == 0) total[2] = c[2];
+    }
     if (stats) {
-        stats->particle_steps = c[0];
-        stats->failures = c[1];
-        stats->active_in = c[2];
-        stats->kernel_ms = ms;
-        stats->launches = ctx->count > 0 ? 1u : 0u;
+        stats->particle_steps = total[0];
+        stats->failures = total[1];
+        stats->active_in = total[2];
+        stats->kernel_ms = ms_total;
+        stats->launches = ctx->count > 0 ? static_cast<uint32_t>(chunks) : 0u;
         stats->reserved = 0;
     }
     return KPT_OK;
@@ -795,9 +816,20 @@ kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_
     kpt_status st = check_settings(ctx, s);
     if (st != KPT_OK) return st;
     if (ctx->noise) return fail(ctx, KPT_ERR_STATE, "the injected-noise table is keyed by slot: clear it before sorting.");
-    const FieldSlot& lo = ctx->slot[KPT_SLOT_LOWER];
-    if (!lo.valid) return fail(ctx, KPT_ERR_STATE, "lower coefficient frame is not loaded.");
+    if (!ctx->slot[KPT_SLOT_LOWER].valid) return fail(ctx, KPT_ERR_STATE, "lower coefficient frame is not loaded.");
     (void)hipSetDevice(ctx->device);
+    return sort_in_place(ctx, *s, std::numeric_limits<double>::infinity(), compact != 0, active);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Cell sort of the stored particles into the second particle set, then swap.
+// Particles with t >= t_done sort after the moving ones (see launch_sort).
+kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_done, bool compact,
+                         uint64_t* active) {
+    const FieldSlot& lo = ctx->slot[KPT_SLOT_LOWER];
     if (!ctx->p2_allocated) {
         kpt::ParticleArrays& Q = ctx->P2;
         const uint64_t cap = ctx->capacity;
@@ -811,6 +843,7 @@ kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_
         KPT_HIP(ctx, dev_alloc(&Q.status, cap));
         KPT_HIP(ctx, dev_alloc(&Q.split_level, cap));
         KPT_HIP(ctx, dev_alloc(&Q.id, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.substep, cap));
         for (int k = 0; k < 2; ++k) {
             KPT_HIP(ctx, dev_alloc(&ctx->sort.keys[k], cap));
             KPT_HIP(ctx, dev_alloc(&ctx->sort.vals[k], cap));
@@ -825,11 +858,12 @@ kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_
         }
         ctx->p2_allocated = true;
     }
-    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
-    KPT_HIP(ctx, kpt::launch_sort(ctx->P, ctx->P2, ctx->count, lo.g, s->interpolation_shift_x,
-                                  s->interpolation_shift_y, ctx->sort, ctx->counters, ctx->stream));
+    unsigned long long* act_dev = ctx->counters + 4;
+    KPT_HIP(ctx, hipMemsetAsync(act_dev, 0, sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_sort(ctx->P, ctx->P2, ctx->count, lo.g, s.interpolation_shift_x,
+                                  s.interpolation_shift_y, t_done, ctx->sort, act_dev, ctx->stream));
     unsigned long long act = 0;
-    KPT_HIP(ctx, hipMemcpyAsync(&act, ctx->counters, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(&act, act_dev, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::swap(ctx->P, ctx->P2);
     if (compact) {
@@ -843,6 +877,10 @@ kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_
     if (active) *active = act;
     return KPT_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 kpt_status kpt_summary_compute(kpt_ctx* ctx, kpt_summary* out) {
     if (!ctx || !out) return KPT_ERR_ARG;

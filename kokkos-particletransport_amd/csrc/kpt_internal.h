@@ -10,12 +10,15 @@
 
 namespace kpt {
 
-// Coefficient components are stored node-major: all components of one node are
-// contiguous, nodes i in [-1, nx], j in [-1, ny] (the stencil of the mover never
-// leaves that range), node (i, j) at ((j + 1) * (nx + 2) + (i + 1)) * stride.
-// The stride is the component count rounded up to a multiple of 8 so every node
-// starts on a 64-byte boundary (FP64 Parker: 128 B = one L2 line per node).
-inline int coefficient_stride(int model) { return (model + 7) / 8 * 8; }
+// Coefficient frames are component-planar: component k of node (i, j) sits at
+// k * plane + (j + 1) * (nx + 2) + (i + 1), with nodes i in [-1, nx], j in
+// [-1, ny] (the stencil of the mover never leaves that range) and
+// plane = (nx + 2) * (ny + 2). Cell-sorted lanes of a wavefront then read
+// neighbouring 8-byte (or 4-byte) values of one plane, so a wavefront-wide load
+// touches a few cache lines instead of one line per lane.
+inline uint64_t coefficient_plane(int nx, int ny) {
+    return static_cast<uint64_t>(nx + 2) * static_cast<uint64_t>(ny + 2);
+}
 
 // Uniform cell-centred grid of a loaded frame (AnalyticCoordinateGrid with
 // UniformSpacing, include/AnalyticCoordinateGrid.hpp:159-210, and the bounds
@@ -39,18 +42,21 @@ struct ParticleArrays {
     int32_t* status;
     int32_t* split_level;
     uint64_t* id;
+    uint32_t* substep;  // sub-steps taken in the current interval (push chunks only)
 };
 
 struct PushArgs {
     ParticleArrays P;
     uint64_t count;
-    const void* field_lo;  // double or float, node-major, `stride` components per node
+    const void* field_lo;  // double or float, component-planar (see coefficient_plane)
     const void* field_hi;
-    int32_t stride;
+    uint64_t plane;
     GridDesc g;
     kpt_device_settings s;
     double t0;
     double t_end;
+    double t_stop;       // particles stop when t >= t_stop (chunk end, <= t_end)
+    int32_t first_chunk; // 1: the sub-step counter starts at 0, else it is read
     uint32_t key[2];
     uint32_t frame;
     const double* noise;  // injected draw table or nullptr
@@ -96,9 +102,12 @@ struct SortScratch {
 size_t sort_temp_bytes(uint64_t capacity);
 // Writes the sorted particle set into `dst`; `active` (device) accumulates the
 // number of status-0 particles.
+// Particles with t >= t_done (finished their interval) sort after the active
+// ones and before the inactive ones; pass +inf to treat every active particle
+// alike.
 hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,
-                       double shift_x, double shift_y, SortScratch& w, unsigned long long* active,
-                       hipStream_t stream);
+                       double shift_x, double shift_y, double t_done, SortScratch& w,
+                       unsigned long long* active, hipStream_t stream);
 
 // Summary: `partials` holds summary_partial_bytes() bytes.
 size_t summary_partial_bytes();

@@ -98,7 +98,7 @@ __device__ __forceinline__ int wrap_index(int i, int n) {
 }
 
 __global__ void __launch_bounds__(kBlock) pack_kernel(const double* __restrict__ raw, GridDesc g,
-                                                      int model, int stride,
+                                                      int model, uint64_t cplane,
                                                       double* __restrict__ out64,
                                                       float* __restrict__ out32) {
     const int nxp = g.nx + 2, nyp = g.ny + 2;
@@ -174,11 +174,9 @@ __global__ void __launch_bounds__(kBlock) pack_kernel(const double* __restrict__
         c[19] = dbm_dx;
         c[20] = dbm_dy;
     }
-    double* o64 = out64 + node * stride;
-    float* o32 = out32 + node * stride;
-    for (int k = 0; k < stride; ++k) {
-        o64[k] = c[k];
-        o32[k] = static_cast<float>(c[k]);
+    for (int k = 0; k < model; ++k) {
+        out64[k * cplane + node] = c[k];
+        out32[k * cplane + node] = static_cast<float>(c[k]);
     }
 }
 
@@ -187,48 +185,18 @@ __global__ void __launch_bounds__(kBlock) pack_kernel(const double* __restrict__
 // Corner order and weights follow LinearPositionInterpolator::sample
 // (include/FieldInterpolator.hpp:363-378): (lo,lo), (hi,lo), (lo,hi), (hi,hi).
 // ---------------------------------------------------------------------------
+// Each corner value is its own load from component plane k; the two corners of a
+// row, (i, j) and (i + 1, j), are adjacent in the plane.
 template <int C, typename FT>
-struct Sampler;
-
-template <int C>
-struct Sampler<C, double> {
-    static_assert(C % 2 == 0, "even component count");
-    __device__ __forceinline__ static void corners(const double* __restrict__ f, int64_t n00,
-                                                   int nxp, int stride, double w00, double w10,
-                                                   double w01, double w11, double* out) {
-        const double2* p00 = reinterpret_cast<const double2*>(f + n00 * stride);
-        const double2* p10 = reinterpret_cast<const double2*>(f + (n00 + 1) * stride);
-        const double2* p01 = reinterpret_cast<const double2*>(f + (n00 + nxp) * stride);
-        const double2* p11 = reinterpret_cast<const double2*>(f + (n00 + nxp + 1) * stride);
+__device__ __forceinline__ void sample_corners(const FT* __restrict__ f, int64_t n00, int nxp,
+                                               uint64_t plane, FT w00, FT w10, FT w01, FT w11,
+                                               FT* out) {
 #pragma unroll
-        for (int k = 0; k < C / 2; ++k) {
-            const double2 a = p00[k], b = p10[k], c = p01[k], d = p11[k];
-            out[2 * k] = w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x;
-            out[2 * k + 1] = w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y;
-        }
+    for (int k = 0; k < C; ++k) {
+        const FT* q = f + k * plane + n00;
+        out[k] = w00 * q[0] + w10 * q[1] + w01 * q[nxp] + w11 * q[nxp + 1];
     }
-};
-
-template <int C>
-struct Sampler<C, float> {
-    static_assert(C % 4 == 0, "component count multiple of 4");
-    __device__ __forceinline__ static void corners(const float* __restrict__ f, int64_t n00,
-                                                   int nxp, int stride, float w00, float w10,
-                                                   float w01, float w11, float* out) {
-        const float4* p00 = reinterpret_cast<const float4*>(f + n00 * stride);
-        const float4* p10 = reinterpret_cast<const float4*>(f + (n00 + 1) * stride);
-        const float4* p01 = reinterpret_cast<const float4*>(f + (n00 + nxp) * stride);
-        const float4* p11 = reinterpret_cast<const float4*>(f + (n00 + nxp + 1) * stride);
-#pragma unroll
-        for (int k = 0; k < C / 4; ++k) {
-            const float4 a = p00[k], b = p10[k], c = p01[k], d = p11[k];
-            out[4 * k + 0] = w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x;
-            out[4 * k + 1] = w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y;
-            out[4 * k + 2] = w00 * a.z + w10 * b.z + w01 * c.z + w11 * d.z;
-            out[4 * k + 3] = w00 * a.w + w10 * b.w + w01 * c.w + w11 * d.w;
-        }
-    }
-};
+}
 
 // Node index and weights of the stencil along one axis for a sampling coordinate
 // q (already shifted by interpolation_shift): the cell-centred inverse map of
@@ -259,7 +227,7 @@ __device__ __forceinline__ bool axis_stencil(double q, const GridDesc& g, int d,
 // ---------------------------------------------------------------------------
 template <int MODEL, bool FOCUSED, typename FT>
 __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
-    constexpr int CP = MODEL == 16 ? 16 : 24;  // padded components read per node
+    constexpr int CP = MODEL;  // components read per corner
     const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     uint64_t my_steps = 0;
     uint32_t my_fail = 0, my_active = 0;
@@ -281,10 +249,13 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
             pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
         }
         const uint64_t id = a.P.id[idx];
-        int step_guard = 0;
+        // Sub-steps already taken in this interval (earlier chunks): the step
+        // guard and the Philox sub-step counter continue from there.
+        int step_guard = a.first_chunk ? 0 : static_cast<int>(a.P.substep[idx]);
         int status = 0;
+        const double t_stop = a.t_stop;
 
-        while (t < interval_end && step_guard < s.max_particle_steps_per_interval && status == 0) {
+        while (t < t_stop && step_guard < s.max_particle_steps_per_interval && status == 0) {
             ++step_guard;
             x = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
             y = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
@@ -305,10 +276,10 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
             double c[CP];
             {
                 FT lo[CP];
-                Sampler<CP, FT>::corners(flo, n00, nxp, a.stride, w00, w10, w01, w11, lo);
+                sample_corners<CP, FT>(flo, n00, nxp, a.plane, w00, w10, w01, w11, lo);
                 if (s.time_interpolation != 0) {
                     FT hi[CP];
-                    Sampler<CP, FT>::corners(fhi, n00, nxp, a.stride, w00, w10, w01, w11, hi);
+                    sample_corners<CP, FT>(fhi, n00, nxp, a.plane, w00, w10, w01, w11, hi);
                     const double rt = (t - interval_start) / s.mhd_dt;
                     const double uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
                     const double lw = 1.0 - uw;
@@ -324,7 +295,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
             double vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = 0.0, dvy_dx = 0.0, dvy_dy,
                                               dvz_dx = 0.0, dvz_dy = 0.0, dbx_dx, dbx_dy, dby_dx,
                                               dby_dy, dbz_dx, dbz_dy, db_dx, db_dy;
-            if (MODEL == 16) {
+            if constexpr (MODEL == 16) {
                 vx = c[0]; vy = c[1]; bx = c[2]; by = c[3]; bz = c[4]; b = c[5];
                 dvx_dx = c[6]; dvy_dy = c[7]; dbx_dx = c[8]; dbx_dy = c[9];
                 dby_dx = c[10]; dby_dy = c[11]; dbz_dx = c[12]; dbz_dy = c[13];
@@ -536,6 +507,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
         a.P.t[idx] = t;
         a.P.step_dt[idx] = diagnostic_dt;
         a.P.status[idx] = status;
+        a.P.substep[idx] = static_cast<uint32_t>(step_guard);
         my_fail = status != 0 ? 1u : 0u;
     }
 
@@ -871,7 +843,7 @@ hipError_t launch_pack(const double* raw, const GridDesc& g, int model, double* 
                        hipStream_t stream) {
     const uint64_t nodes = static_cast<uint64_t>(g.nx + 2) * (g.ny + 2);
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for(nodes)), dim3(kBlock), 0, stream, raw, g, model,
-                       coefficient_stride(model), out64, out32);
+                       coefficient_plane(g.nx, g.ny), out64, out32);
     return hipGetLastError();
 }
 

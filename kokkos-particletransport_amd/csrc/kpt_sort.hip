@@ -22,14 +22,18 @@ namespace {
 constexpr int kBlock = 256;
 
 __global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint64_t count, GridDesc g,
-                                                          double shift_x, double shift_y, uint32_t inactive_key,
+                                                          double shift_x, double shift_y, double t_done,
+                                                          uint32_t inactive_key,
                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                           unsigned long long* __restrict__ active) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
     bool act = false;
     if (i < count) {
         uint32_t key = inactive_key;
-        if (P.status[i] == 0) {
+        if (P.status[i] == 0 && P.t[i] >= t_done) {
+            act = true;
+            key = inactive_key - 1;  // finished this interval: after the moving ones
+        } else if (P.status[i] == 0) {
             act = true;
             const double q[2] = {P.x[i] + shift_x, P.y[i] + shift_y};
             const int n[2] = {g.nx, g.ny};
@@ -67,6 +71,7 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(ParticleArrays src, Part
     dst.status[i] = src.status[s];
     dst.split_level[i] = src.split_level[s];
     dst.id[i] = src.id[s];
+    dst.substep[i] = src.substep[s];
 }
 
 int key_bits(uint32_t max_key) {
@@ -85,13 +90,15 @@ size_t sort_temp_bytes(uint64_t capacity) {
 }
 
 hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,
-                       double shift_x, double shift_y, SortScratch& w, unsigned long long* active,
-                       hipStream_t stream) {
+                       double shift_x, double shift_y, double t_done, SortScratch& w,
+                       unsigned long long* active, hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    const uint32_t inactive_key = static_cast<uint32_t>((g.nx + 2) * (g.ny + 2));
+    // node keys are < (nx + 2)(ny + 2); the two classes after them are
+    // "finished this interval" and "inactive"
+    const uint32_t inactive_key = static_cast<uint32_t>((g.nx + 2) * (g.ny + 2) + 1);
     const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(sort_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, count, g, shift_x, shift_y,
-                       inactive_key, w.keys[0], w.vals[0], active);
+                       t_done, inactive_key, w.keys[0], w.vals[0], active);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipcub::DoubleBuffer<uint32_t> k(w.keys[0], w.keys[1]), v(w.vals[0], w.vals[1]);

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkpt.so")
+LIB_PATH = os.environ.get("KPT_LIB") or os.path.join(HERE, "libkpt.so")
 
 KPT_OK = 0
 MODEL_PARKER16 = 16
@@ -52,7 +52,7 @@ class DeviceSettings(C.Structure):
         ("particle_v0", C.c_double), ("duu0", C.c_double),
         ("pitch_angle_scattering_h0", C.c_double), ("split_ratio", C.c_double),
         ("pmin_split", C.c_double), ("mu_max", C.c_double),
-        ("precision", C.c_int32), ("reserved0", C.c_int32),
+        ("precision", C.c_int32), ("push_chunks", C.c_int32),
     ]
 
     def copy(self) -> "DeviceSettings":

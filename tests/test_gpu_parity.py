@@ -235,6 +235,48 @@ def test_sort_is_a_permutation_and_does_not_change_physics():
             c.sort(s)
 
 
+@pytest.mark.parametrize("model,focused", [(16, False), (21, True)])
+def test_chunked_push_equals_single_launch(model, focused):
+    """push_chunks = 7 (re-sort before every chunk) reproduces the one-launch push
+    particle by particle (matched by id); with the injected-noise table the chunks
+    run unsorted and still match the oracle to 1e-12."""
+    xe, ye, planes, upper, s, parts, noise = _noise_case(model, focused, 1, "reference", 12, n=20000, seed=6)
+    s.max_particle_steps_per_interval = 200000
+    parts["status"][::11] = 2
+    n = len(parts["x"])
+    res = {}
+    for chunks in (1, 7):
+        s2 = s.copy()
+        s2.push_chunks = chunks
+        with _ctx(n, seed=31) as c:
+            c.field_upload(kpt.SLOT_LOWER, model, xe, ye, planes)
+            c.field_upload(kpt.SLOT_UPPER, model, xe, ye, upper)
+            c.upload(parts)
+            st = c.push_interval(s2, 0.0, 2)
+            assert st.launches == chunks
+            res[chunks] = (st.particle_steps, st.failures, st.active_in, c.download())
+    assert res[1][:3] == res[7][:3]
+    a, b = res[1][3], res[7][3]
+    ia, ib = np.argsort(a["id"]), np.argsort(b["id"])
+    for k in ("x", "y", "p", "t", "step_dt", "mu", "status", "weight"):
+        np.testing.assert_array_equal(a[k][ia], b[k][ib], err_msg=k)
+    # noise table + chunks (no re-sort): still the oracle's result
+    s.max_particle_steps_per_interval = 12
+    s.push_chunks = 5
+    want = copy_parts(parts)
+    lo, hi = O.Field(xe, ye, planes, model), O.Field(xe, ye, upper, model)
+    o_steps, o_fail = O.push_interval(lo, hi, s, 0.0, 2, 1, want, noise=noise)
+    with _ctx(n) as c:
+        c.field_upload(kpt.SLOT_LOWER, model, xe, ye, planes)
+        c.field_upload(kpt.SLOT_UPPER, model, xe, ye, upper)
+        c.upload(parts)
+        c.set_noise(noise)
+        st = c.push_interval(s, 0.0, 2)
+        got = c.download()
+    assert (st.particle_steps, st.failures) == (o_steps, o_fail)
+    _assert_state_close(got, want, s)
+
+
 def test_split_bitexact():
     h = kpt.default_host_settings()
     h.p0 = 1.0
