@@ -112,7 +112,29 @@ typedef struct kpt_device_settings {
      * truncation of the last step to the interval end carry across chunks, so the
      * result equals one launch. 0 or 1: one launch, no sort. */
     int32_t push_chunks;
+
+    /* Pitch-angle closure of the focused branch (FOCUSED21 with focused_transport):
+     *  KPT_CLOSURE_REDUCED: the driver's D_mumu = duu0 (1-mu^2)(|mu|^(g-1) + h0)
+     *    B^(2-g) (p/p0)^(g-1), one Euler mu step, clamp to +-mu_max
+     *    (src/main.cpp:1551-1584, :1699-1702);
+     *  KPT_CLOSURE_QLT: slab-QLT D_mumu and dD_mumu/dmu of
+     *    LegencyModel::evaluate_pitch_angle_scattering (include/LegencyModel.hpp:
+     *    539-689) with Omega = |q| |B| / gamma, gamma = sqrt(1 + p^2),
+     *    speed = p / gamma (m = c = 1), and the mu update of
+     *    FocusTransportSolver::picard_pitch_angle_update (3 Picard iterations,
+     *    include/FocusSolver.hpp:700-719) reflecting at +-mu_max (:454-470);
+     *  KPT_CLOSURE_MLP: as QLT, with D_mumu / (Omega sigma2) and its mu
+     *    derivative taken from the loaded MLP inside its training hull
+     *    (kpt_mlp_set) and from the exact QLT outside it. */
+    int32_t closure;
+    int32_t reserved1;
+    double qlt_sigma2;          /* delta B^2 / B0^2 */
+    double qlt_correlation_length;
+    double qlt_spectral_index;  /* gamma_k, PitchAngleScatteringParameters default 5/3 */
+    double qlt_h0;              /* resonance regularisation, default 1e-6 */
 } kpt_device_settings;
+
+enum { KPT_CLOSURE_REDUCED = 0, KPT_CLOSURE_QLT = 1, KPT_CLOSURE_MLP = 2 };
 
 /* Host-side controls that kpt_make_device_settings reads (the physics subset of
  * Reconnection2DSettings, src/main.cpp:114-154, with the same defaults). */

@@ -160,6 +160,20 @@ kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
                     (s->precision == KPT_PREC_FP64 || s->precision == KPT_PREC_FP32) &&
                     s->push_chunks >= 0 && s->push_chunks <= 4096;
     if (!ok) return fail(ctx, KPT_ERR_ARG, "Reconnection2DSettings: invalid physical parameters.");
+    if (s->closure != KPT_CLOSURE_REDUCED) {
+        // PitchAngleScatteringParameters::validate (LegencyModel.hpp:207-222) plus
+        // the positive sigma2 and Lc that evaluate_pitch_angle_scattering requires
+        const bool qok = (s->closure == KPT_CLOSURE_QLT || s->closure == KPT_CLOSURE_MLP) &&
+                         std::isfinite(s->qlt_spectral_index) && s->qlt_spectral_index > 1.0 &&
+                         std::isfinite(s->qlt_h0) && s->qlt_h0 > 0.0 && std::isfinite(s->qlt_sigma2) &&
+                         s->qlt_sigma2 > 0.0 && std::isfinite(s->qlt_correlation_length) &&
+                         s->qlt_correlation_length > 0.0 && s->mu_max > 0.0 && s->mu_max <= 1.0;
+        if (!qok)
+            return fail(ctx, KPT_ERR_ARG,
+                        "LegencyModel::PitchAngleScatteringParameters: coefficients and floors must be finite, "
+                        "with spectral_index > 1, resonance_regularization_h0 > 0, sigma2 > 0, Lc > 0 and "
+                        "0 < mu_max <= 1.");
+    }
     return KPT_OK;
 }
 
@@ -484,6 +498,12 @@ kpt_status kpt_make_device_settings(const kpt_host_settings* h, int32_t nx, int3
     o->pmin_split = h->pmin_split_over_p0 * h->p0;
     o->mu_max = h->mu_max;
     o->precision = KPT_PREC_FP64;
+    o->push_chunks = 1;
+    o->closure = KPT_CLOSURE_REDUCED;
+    o->qlt_sigma2 = 0.0;
+    o->qlt_correlation_length = 0.0;
+    o->qlt_spectral_index = 5.0 / 3.0;  // PitchAngleScatteringParameters (LegencyModel.hpp:196-198)
+    o->qlt_h0 = 1.0e-6;
     return KPT_OK;
 }
 
@@ -716,6 +736,7 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.noise = ctx->noise;
     a.noise_steps = ctx->noise_steps;
     a.counters = ctx->counters;
+    a.qlt_norm = s->qlt_spectral_index * std::sin(M_PI / s->qlt_spectral_index) / M_PI;
     (void)hipSetDevice(ctx->device);
     const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
     const int chunks = s->push_chunks > 1 ? s->push_chunks : 1;
@@ -732,7 +753,7 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
         a.t_stop = k == chunks - 1 ? a.t_end : t0 + s->mhd_dt * static_cast<double>(k + 1) / chunks;
         KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
         KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, s->closure, ctx->stream));
         KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         unsigned long long c[4];
         KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));

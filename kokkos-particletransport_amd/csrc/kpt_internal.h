@@ -61,13 +61,14 @@ struct PushArgs {
     uint32_t frame;
     const double* noise;  // injected draw table or nullptr
     uint32_t noise_steps;
+    double qlt_norm;      // gamma_k sin(pi / gamma_k) / pi (pitch_angle_spectrum_normalization)
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
 };
 
 // Device-side entry points (kpt_kernels.hip). All launch on `stream`.
 hipError_t launch_pack(const double* raw /* 6 planes nx*ny */, const GridDesc& g, int model,
                        double* out64, float* out32, hipStream_t stream);
-hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, hipStream_t stream);
+hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream);
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);

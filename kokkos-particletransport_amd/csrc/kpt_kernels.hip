@@ -225,8 +225,48 @@ __device__ __forceinline__ bool axis_stencil(double q, const GridDesc& g, int d,
 // MODEL: 16 or 21 packed components. FOCUSED: reduced focused branch.
 // FT: storage / interpolation type of the coefficient frames.
 // ---------------------------------------------------------------------------
-template <int MODEL, bool FOCUSED, typename FT>
+// FocusTransportSolver::reflected_pitch_angle_cosine (include/FocusSolver.hpp:
+// 454-470): mirror at +-upper. The mirror loop is bounded (16 passes); a value
+// still outside is folded by the closed-form triangle wave of period 4 upper,
+// so every lane leaves this function after a fixed amount of work.
+__device__ __forceinline__ double reflect_mu(double mu, double upper) {
+    const double lower = -upper;
+    for (int k = 0; k < 16 && (mu > upper || mu < lower); ++k) {
+        if (mu > upper) mu = 2.0 * upper - mu;
+        if (mu < lower) mu = 2.0 * lower - mu;
+    }
+    if (mu > upper || mu < lower) {
+        const double period = 4.0 * upper;
+        double m = mu + upper;
+        m -= period * floor(m / period);
+        if (m > 2.0 * upper) m = period - m;
+        mu = m - upper;
+    }
+    return mu;
+}
+
+// Slab-QLT D_mumu and dD_mumu/dmu (include/LegencyModel.hpp:557-643) for one
+// trial mu, with omega, xi = omega Lc / v and xi^gamma_k fixed for the sub-step.
+// `pref` = (pi / 4) omega sigma2 gamma_k sin(pi / gamma_k) / pi.
+__device__ __forceinline__ void qlt_scattering(double mu, double xi, double xi_g, double pref, double gk,
+                                               double h0, double& D, double& dD) {
+    const double one_minus_mu2 = 1.0 - mu * mu;
+    D = 0.0;
+    dD = 0.0;
+    if (one_minus_mu2 <= 0.0) return;
+    const double mu_h = fabs(mu) + h0;
+    const double pm1 = pow(mu_h, gk - 1.0);  // mu_h^(gamma_k - 1)
+    const double den = pm1 * mu_h + xi_g;    // mu_h^gamma_k + xi^gamma_k
+    D = pref * one_minus_mu2 * xi * pm1 / den;
+    if (D == 0.0 || mu == 0.0) return;
+    const double sign_mu = mu > 0.0 ? 1.0 : -1.0;
+    const double shape = sign_mu * ((gk - 1.0) / mu_h - gk * pm1 / den);
+    dD = D * (-2.0 * mu / one_minus_mu2 + shape);
+}
+
+template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
 __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
+    constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr int CP = MODEL;  // components read per corner
     const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     uint64_t my_steps = 0;
@@ -246,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
         double pitch_mu = 0.0;
         if (MODEL == 21) {
             const double m = a.P.mu[idx];
-            pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
+            pitch_mu = QLT ? reflect_mu(m, s.mu_max) : (m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m));
         }
         const uint64_t id = a.P.id[idx];
         // Sub-steps already taken in this interval (earlier chunks): the step
@@ -351,6 +391,9 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                 sqrt(sqr(s.drift_param1 * s.p0 / p) + sqr(s.drift_param2 * s.p0 * s.p0 / (p * p)));
 
             double dx_dt, dy_dt, dp_dt, dmu_dt = 0.0, duu = 0.0;
+            // focusing-rate pieces and QLT constants reused by the Picard passes
+            double f_speed = 0.0, f_divv = 0.0, f_bb = 0.0, f_bv = 0.0, f_divb = 0.0;
+            double q_xi = 0.0, q_xig = 0.0, q_pref = 0.0;
             if (FOCUSED) {
                 // reduced focused branch (src/main.cpp:1499-1589)
                 const double mu2 = pitch_mu * pitch_mu;
@@ -365,7 +408,9 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                                           muf1 * bx * bdot_curvb * ib3);
                 const double vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
                                           muf1 * by * bdot_curvb * ib3);
-                const double speed = s.particle_v0 * p / s.p0;
+                // QLT: relativistic speed p / gamma (m = c = 1)
+                const double gamma_p = QLT ? sqrt(1.0 + p * p) : 1.0;
+                const double speed = QLT ? p / gamma_p : s.particle_v0 * p / s.p0;
                 if (!isfinite(speed) || speed <= 0.0) {
                     status = 2;
                     break;
@@ -383,19 +428,32 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                 dmu_dt = speed * div_bnorm + pitch_mu * divv - 3.0 * pitch_mu * bb_gradv -
                          2.0 * bv_gradv / speed;
                 dmu_dt *= 0.5 * (1.0 - mu2);
-                const double abs_mu = fabs(pitch_mu);
-                const double dtmp = pow(abs_mu, s.gamma_turb - 1.0) + s.pitch_angle_scattering_h0;
-                duu = s.duu0 * (1.0 - mu2) * dtmp;
                 double duu_du = 0.0;
-                if (pitch_mu != 0.0) {
-                    const double g2 = (1.0 - mu2) * pow(abs_mu, s.gamma_turb - 2.0);
-                    duu_du = s.duu0 * (-2.0 * pitch_mu * dtmp + (pitch_mu > 0.0 ? g2 : -g2));
+                if constexpr (QLT) {
+                    f_speed = speed;
+                    f_divv = divv;
+                    f_bb = bb_gradv;
+                    f_bv = bv_gradv;
+                    f_divb = div_bnorm;
+                    const double omega = fabs(s.charge) * b / gamma_p;
+                    q_xi = omega * s.qlt_correlation_length / speed;
+                    q_xig = pow(q_xi, s.qlt_spectral_index);
+                    q_pref = 0.25 * M_PI * omega * s.qlt_sigma2 * a.qlt_norm;
+                    qlt_scattering(pitch_mu, q_xi, q_xig, q_pref, s.qlt_spectral_index, s.qlt_h0, duu, duu_du);
+                } else {
+                    const double abs_mu = fabs(pitch_mu);
+                    const double dtmp = pow(abs_mu, s.gamma_turb - 1.0) + s.pitch_angle_scattering_h0;
+                    duu = s.duu0 * (1.0 - mu2) * dtmp;
+                    if (pitch_mu != 0.0) {
+                        const double g2 = (1.0 - mu2) * pow(abs_mu, s.gamma_turb - 2.0);
+                        duu_du = s.duu0 * (-2.0 * pitch_mu * dtmp + (pitch_mu > 0.0 ? g2 : -g2));
+                    }
+                    double duu_norm = 1.0;
+                    if (s.magnetic_dependency == 1) duu_norm *= pow(b, 2.0 - s.gamma_turb);
+                    if (s.momentum_dependency == 1) duu_norm *= pow(p / s.p0, s.gamma_turb - 1.0);
+                    duu *= duu_norm;
+                    duu_du *= duu_norm;
                 }
-                double duu_norm = 1.0;
-                if (s.magnetic_dependency == 1) duu_norm *= pow(b, 2.0 - s.gamma_turb);
-                if (s.momentum_dependency == 1) duu_norm *= pow(p / s.p0, s.gamma_turb - 1.0);
-                duu *= duu_norm;
-                duu_du *= duu_norm;
                 dmu_dt += duu_du;
                 if (!isfinite(dmu_dt) || !isfinite(duu) || duu < 0.0) {
                     status = 2;
@@ -479,8 +537,29 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                 }
                 delta_x = dx_dt * dt + sx;
                 delta_y = dy_dt * dt + sy;
-                const double m = pitch_mu + dmu_dt * dt + r3 * sqrt(2.0 * duu) * sdt;
-                pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
+                if constexpr (QLT) {
+                    // FocusTransportSolver::picard_pitch_angle_update (FocusSolver.hpp:700-719):
+                    // three passes, drift and diffusion at the trial mu, one noise value.
+                    const double mu0 = pitch_mu;
+                    double rate = dmu_dt, D = duu, trial = mu0;
+#pragma unroll 1
+                    for (int it = 0; it < 3; ++it) {
+                        if (it > 0) {
+                            const double m = reflect_mu(trial, s.mu_max);
+                            double dD;
+                            qlt_scattering(m, q_xi, q_xig, q_pref, s.qlt_spectral_index, s.qlt_h0, D, dD);
+                            rate = 0.5 * (1.0 - m * m) *
+                                       (f_speed * f_divb + m * f_divv - 3.0 * m * f_bb - 2.0 * f_bv / f_speed) +
+                                   dD;
+                        }
+                        const double scale = D > 0.0 ? sqrt(2.0 * D * dt) : 0.0;
+                        trial = reflect_mu(mu0 + dt * rate + scale * r3, s.mu_max);
+                    }
+                    pitch_mu = trial;
+                } else {
+                    const double m = pitch_mu + dmu_dt * dt + r3 * sqrt(2.0 * duu) * sdt;
+                    pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
+                }
             } else {
                 // src/main.cpp:1279-1292 (ran1, ran2, ran3, ignored)
                 delta_x = dx_dt * dt + r0 * skperp * sdt + r2 * skpara_perp * sdt * bx * ib;
@@ -847,24 +926,31 @@ hipError_t launch_pack(const double* raw, const GridDesc& g, int model, double* 
     return hipGetLastError();
 }
 
-hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, hipStream_t stream) {
+hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream) {
     if (a.count == 0) return hipSuccess;
     const dim3 grid(grid_for(a.count)), block(kBlock);
+    constexpr int R = KPT_CLOSURE_REDUCED, Q = KPT_CLOSURE_QLT;
     if (model == KPT_MODEL_PARKER16) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<16, false, float>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<16, false, float, R>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<16, false, double>), grid, block, 0, stream, a);
-    } else if (focused) {
+            hipLaunchKernelGGL((push_kernel<16, false, double, R>), grid, block, 0, stream, a);
+    } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, true, float>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, float, Q>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<21, true, double>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, double, Q>), grid, block, 0, stream, a);
+    } else if (focused) {
+        if (closure != R) return hipErrorInvalidValue;
+        if (fp32)
+            hipLaunchKernelGGL((push_kernel<21, true, float, R>), grid, block, 0, stream, a);
+        else
+            hipLaunchKernelGGL((push_kernel<21, true, double, R>), grid, block, 0, stream, a);
     } else {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, false, float>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, float, R>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<21, false, double>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, double, R>), grid, block, 0, stream, a);
     }
     return hipGetLastError();
 }

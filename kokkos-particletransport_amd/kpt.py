@@ -23,6 +23,9 @@ SLOT_LOWER = 0
 SLOT_UPPER = 1
 PREC_FP64 = 0
 PREC_FP32 = 1
+CLOSURE_REDUCED = 0
+CLOSURE_QLT = 1
+CLOSURE_MLP = 2
 
 PARTICLE_FIELDS = (
     ("x", np.float64), ("y", np.float64), ("p", np.float64), ("t", np.float64),
@@ -53,6 +56,9 @@ class DeviceSettings(C.Structure):
         ("pitch_angle_scattering_h0", C.c_double), ("split_ratio", C.c_double),
         ("pmin_split", C.c_double), ("mu_max", C.c_double),
         ("precision", C.c_int32), ("push_chunks", C.c_int32),
+        ("closure", C.c_int32), ("reserved1", C.c_int32),
+        ("qlt_sigma2", C.c_double), ("qlt_correlation_length", C.c_double),
+        ("qlt_spectral_index", C.c_double), ("qlt_h0", C.c_double),
     ]
 
     def copy(self) -> "DeviceSettings":

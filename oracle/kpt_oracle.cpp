@@ -342,6 +342,54 @@ inline double sq(double v) { return v * v; }
 inline double min_positive(double a, double b) { return a < b ? a : b; }
 inline double clamp_mu(double v, double m) { return v > m ? m : (v < -m ? -m : v); }
 
+// FocusTransportSolver::reflected_pitch_angle_cosine (include/FocusSolver.hpp:
+// 454-470): mirror at +-upper until inside. The mirror sequence is bounded
+// here; past it the same map is applied in closed form (a triangle wave of
+// period 4 upper), which only differs for |mu| in the thousands.
+double reflect_mu(double mu, double upper) {
+    const double lower = -upper;
+    for (int k = 0; k < 16 && (mu > upper || mu < lower); ++k) {
+        if (mu > upper) mu = 2.0 * upper - mu;
+        if (mu < lower) mu = 2.0 * lower - mu;
+    }
+    if (mu > upper || mu < lower) {
+        const double period = 4.0 * upper;
+        double m = mu + upper;
+        m -= period * std::floor(m / period);
+        if (m > 2.0 * upper) m = period - m;
+        mu = m - upper;
+    }
+    return mu;
+}
+
+// Slab-QLT pitch-angle scattering: LegencyModel::pitch_angle_spectrum_normalization,
+// pitch_angle_diffusion_coefficient and pitch_angle_diffusion_derivative
+// (include/LegencyModel.hpp:539-643), with omega = |q| |B| / gamma (m = c = 1).
+struct QltD {
+    double D, dD;
+};
+QltD qlt_scattering(double mu, double omega, double speed, double sigma2, double lc, double gk, double h0) {
+    QltD r{0.0, 0.0};
+    const double one_minus_mu2 = 1.0 - mu * mu;
+    if (one_minus_mu2 <= 0.0) return r;
+    const double pi = 3.14159265358979323846;
+    const double mu_h = std::fabs(mu) + h0;
+    const double xi = omega * lc / speed;
+    const double abs_mu_gamma = std::pow(mu_h, gk);
+    const double xi_gamma = std::pow(xi, gk);
+    const double den = abs_mu_gamma + xi_gamma;
+    const double norm = gk * std::sin(pi / gk) / pi;
+    const double prefactor = 0.25 * pi * omega * sigma2 * norm;
+    r.D = prefactor * one_minus_mu2 * xi * std::pow(mu_h, gk - 1.0) / den;
+    if (r.D == 0.0) return r;
+    const double sign_mu = mu > 0.0 ? 1.0 : (mu < 0.0 ? -1.0 : 0.0);
+    if (sign_mu == 0.0) return r;
+    const double shape = sign_mu * ((gk - 1.0) / mu_h - gk * std::pow(mu_h, gk - 1.0) / den);
+    const double endpoint = -2.0 * mu / one_minus_mu2;
+    r.dD = r.D * (endpoint + shape);
+    return r;
+}
+
 struct NoiseSource {
     const double* table;  // injected draws or nullptr
     uint32_t steps;
@@ -374,9 +422,12 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
     if (P.status[index] != 0) return 0;
     const int model = lower.comps;
     const bool focused_transport = model == KPT_MODEL_FOCUSED21 && S.focused_transport != 0;
+    const bool qlt = focused_transport && S.closure != KPT_CLOSURE_REDUCED;
+    // the pitch-angle cosine: clamped by the driver, reflected by the generic solver
+    auto bound_mu = [&](double m) { return qlt ? reflect_mu(m, S.mu_max) : clamp_mu(m, S.mu_max); };
     const double interval_end = interval_start + S.mhd_dt;
     double x = P.x[index], y = P.y[index], p = P.p[index], t = P.t[index];
-    double pitch_mu = model == KPT_MODEL_FOCUSED21 ? clamp_mu(P.mu[index], S.mu_max) : 0.0;
+    double pitch_mu = model == KPT_MODEL_FOCUSED21 ? bound_mu(P.mu[index]) : 0.0;
     double dt = P.step_dt[index];
     double diagnostic_dt = dt;
     int step_guard = 0;
@@ -452,9 +503,11 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
             std::sqrt(sq(S.drift_param1 * S.p0 / p) + sq(S.drift_param2 * S.p0 * S.p0 / (p * p)));
 
         double dx_dt = 0.0, dy_dt = 0.0, dp_dt = 0.0, dmu_dt = 0.0, duu = 0.0;
+        // pieces of the focusing rate, kept for the Picard iterations of the QLT closure
+        double f_speed = 0.0, f_divv = 0.0, f_bb = 0.0, f_bv = 0.0, f_divb = 0.0, f_omega = 0.0;
         if (focused_transport) {
             // src/main.cpp:1499-1589
-            pitch_mu = clamp_mu(pitch_mu, S.mu_max);
+            pitch_mu = bound_mu(pitch_mu);
             const double mu2 = pitch_mu * pitch_mu;
             const double muf1 = 0.5 * (1.0 - mu2);
             const double muf2 = 0.5 * (3.0 * mu2 - 1.0);
@@ -467,7 +520,10 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
                                       muf1 * bx * bdot_curvb * ib3);
             const double vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
                                       muf1 * by * bdot_curvb * ib3);
-            const double speed = S.particle_v0 * p / S.p0;
+            // QLT closure: relativistic speed p / gamma with m = c = 1
+            // (ParticleProperties::speed_from_momentum_magnitude, ParticleSystem.hpp:170-184)
+            const double gamma_p = qlt ? std::sqrt(1.0 + p * p) : 1.0;
+            const double speed = qlt ? p / gamma_p : S.particle_v0 * p / S.p0;
             if (!std::isfinite(speed) || speed <= 0.0) {
                 status = 2;
                 break;
@@ -488,20 +544,34 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
             dmu_dt = speed * div_bnorm + pitch_mu * divv - 3.0 * pitch_mu * bb_gradv -
                      2.0 * bv_gradv / speed;
             dmu_dt *= 0.5 * (1.0 - mu2);
-            const double abs_mu = std::fabs(pitch_mu);
-            const double dtmp = std::pow(abs_mu, S.gamma_turb - 1.0) + S.pitch_angle_scattering_h0;
-            duu = S.duu0 * (1.0 - mu2) * dtmp;
             double duu_du = 0.0;
-            if (pitch_mu > 0.0) {
-                duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp + (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
-            } else if (pitch_mu < 0.0) {
-                duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp - (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
+            if (qlt) {
+                // LegencyModel::evaluate_pitch_angle_scattering (LegencyModel.hpp:650-689)
+                f_speed = speed;
+                f_divv = divv;
+                f_bb = bb_gradv;
+                f_bv = bv_gradv;
+                f_divb = div_bnorm;
+                f_omega = std::fabs(S.charge) * b / gamma_p;
+                const QltD q = qlt_scattering(pitch_mu, f_omega, speed, S.qlt_sigma2, S.qlt_correlation_length,
+                                              S.qlt_spectral_index, S.qlt_h0);
+                duu = q.D;
+                duu_du = q.dD;
+            } else {
+                const double abs_mu = std::fabs(pitch_mu);
+                const double dtmp = std::pow(abs_mu, S.gamma_turb - 1.0) + S.pitch_angle_scattering_h0;
+                duu = S.duu0 * (1.0 - mu2) * dtmp;
+                if (pitch_mu > 0.0) {
+                    duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp + (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
+                } else if (pitch_mu < 0.0) {
+                    duu_du = S.duu0 * (-2.0 * pitch_mu * dtmp - (1.0 - mu2) * std::pow(abs_mu, S.gamma_turb - 2.0));
+                }
+                double duu_norm = 1.0;
+                if (S.magnetic_dependency == 1) duu_norm *= std::pow(b, 2.0 - S.gamma_turb);
+                if (S.momentum_dependency == 1) duu_norm *= std::pow(p / S.p0, S.gamma_turb - 1.0);
+                duu *= duu_norm;
+                duu_du *= duu_norm;
             }
-            double duu_norm = 1.0;
-            if (S.magnetic_dependency == 1) duu_norm *= std::pow(b, 2.0 - S.gamma_turb);
-            if (S.momentum_dependency == 1) duu_norm *= std::pow(p / S.p0, S.gamma_turb - 1.0);
-            duu *= duu_norm;
-            duu_du *= duu_norm;
             dmu_dt += duu_du;
             if (!std::isfinite(dmu_dt) || !std::isfinite(duu) || duu < 0.0) {
                 status = 2;
@@ -569,7 +639,30 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
             }
             delta_x = dx_dt * dt + sx;
             delta_y = dy_dt * dt + sy;
-            pitch_mu = clamp_mu(pitch_mu + dmu_dt * dt + pitch_random * std::sqrt(2.0 * duu) * sdt, S.mu_max);
+            if (qlt) {
+                // FocusTransportSolver::picard_pitch_angle_update (FocusSolver.hpp:700-719),
+                // the drift and diffusion re-evaluated at the trial mu
+                // (evaluate_pitch_angle_at_mu, :665-694), the same noise in every pass.
+                const double mu0 = pitch_mu;
+                double rate = dmu_dt, D = duu, trial = mu0;
+                for (int it = 0; it < 3; ++it) {
+                    if (it > 0) {
+                        const double m = bound_mu(trial);
+                        const double m2 = m * m;
+                        const QltD q = qlt_scattering(m, f_omega, f_speed, S.qlt_sigma2, S.qlt_correlation_length,
+                                                      S.qlt_spectral_index, S.qlt_h0);
+                        rate = 0.5 * (1.0 - m2) *
+                                   (f_speed * f_divb + m * f_divv - 3.0 * m * f_bb - 2.0 * f_bv / f_speed) +
+                               q.dD;
+                        D = q.D;
+                    }
+                    const double scale = D > 0.0 ? std::sqrt(2.0 * D * dt) : 0.0;
+                    trial = bound_mu(mu0 + dt * rate + scale * pitch_random);
+                }
+                pitch_mu = trial;
+            } else {
+                pitch_mu = clamp_mu(pitch_mu + dmu_dt * dt + pitch_random * std::sqrt(2.0 * duu) * sdt, S.mu_max);
+            }
         } else {
             // src/main.cpp:1279-1292
             const double ran1 = r[0], ran2 = r[1], ran3 = r[2];
@@ -612,6 +705,16 @@ void kpto_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
 }
 
 double kpto_uniform_sqrt3(uint32_t r) { return uniform_sqrt3(r); }
+
+// Slab-QLT D_mumu and dD_mumu/dmu at one state (out[0], out[1]).
+void kpto_qlt_scattering(double mu, double omega, double speed, double sigma2, double lc, double gk,
+                         double h0, double* out) {
+    const QltD q = qlt_scattering(mu, omega, speed, sigma2, lc, gk, h0);
+    out[0] = q.D;
+    out[1] = q.dD;
+}
+
+double kpto_reflect_mu(double mu, double upper) { return reflect_mu(mu, upper); }
 
 // Build one packed coefficient frame from compact-field arrays.
 void* kpto_field_create(int32_t nx, int32_t ny, const double* xe, const double* ye, const double* bx,

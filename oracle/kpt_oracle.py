@@ -66,6 +66,8 @@ def load():
             "kpto_histogram_xy": (None, [P(kpt.ParticlesHost), C.c_uint64, C.c_int32, C.c_int32,
                                          C.c_double, C.c_double, C.c_double, C.c_double, dp]),
             "kpto_summary": (None, [P(kpt.ParticlesHost), C.c_uint64, P(kpt.Summary)]),
+            "kpto_qlt_scattering": (None, [C.c_double] * 7 + [dp]),
+            "kpto_reflect_mu": (C.c_double, [C.c_double, C.c_double]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -89,6 +91,17 @@ def philox4x32_10(ctr, key):
 
 def uniform_sqrt3(r: int) -> float:
     return load().kpto_uniform_sqrt3(r)
+
+
+def qlt_scattering(mu, omega, speed, sigma2, lc, gk=5.0 / 3.0, h0=1e-6):
+    """Slab-QLT (D_mumu, dD_mumu/dmu) of the oracle."""
+    out = np.zeros(2)
+    load().kpto_qlt_scattering(mu, omega, speed, sigma2, lc, gk, h0, _dp(out))
+    return out[0], out[1]
+
+
+def reflect_mu(mu, upper=1.0):
+    return load().kpto_reflect_mu(mu, upper)
 
 
 class Field:

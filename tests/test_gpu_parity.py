@@ -127,6 +127,32 @@ def test_push_injected_noise_matches_oracle(model, focused, time_interp, physics
     _assert_state_close(got, want, s)
 
 
+@pytest.mark.parametrize("time_interp", [1, 0])
+@pytest.mark.parametrize("steps", [1, 10])
+@pytest.mark.parametrize("mu_max", [1.0, 0.99])
+def test_push_qlt_closure_matches_oracle(time_interp, steps, mu_max):
+    """Exact slab-QLT closure with Picard mu update and reflection, injected noise."""
+    xe, ye, planes, upper, s, parts, noise = _noise_case(21, True, time_interp, "reference", steps, seed=12)
+    s.closure = kpt.CLOSURE_QLT
+    s.qlt_sigma2 = 0.1
+    s.qlt_correlation_length = 0.02
+    s.mu_max = mu_max
+    parts["mu"][:] = np.random.default_rng(3).uniform(-1, 1, len(parts["x"]))
+    n = len(parts["x"])
+    want = copy_parts(parts)
+    lo, hi = O.Field(xe, ye, planes, 21), O.Field(xe, ye, upper, 21)
+    o_steps, o_fail = O.push_interval(lo, hi, s, 0.0, 3, 1, want, noise=noise)
+    with _ctx(n) as c:
+        c.field_upload(kpt.SLOT_LOWER, 21, xe, ye, planes)
+        c.field_upload(kpt.SLOT_UPPER, 21, xe, ye, upper)
+        c.upload(parts)
+        c.set_noise(noise)
+        st = c.push_interval(s, 0.0, 3)
+        got = c.download()
+    assert st.particle_steps == o_steps and st.failures == o_fail
+    _assert_state_close(got, want, s)
+
+
 def test_push_philox_trajectories_match_oracle():
     """Same Philox stream on both sides: 200 adaptive sub-steps with the reference
     physics; positions agree per particle (dt-limiter ties may fork a few)."""
