@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the Parker SDE push on one MI355X.
+"""Benchmark of the SDE particle push on one MI355X.
 
-Workload (one bench step = one MHD frame interval of the whole particle set):
-Parker transport at production shape, FP64 plain push: 1024 x 512 Harris sheet
-with a 4-mode island chain and 1% velocity noise, two time-blended frames, the
-driver's default kappa / drift physics (p0 = 0.1, adaptive dt), 4 194 304
-particles resident in HBM, one 0.1 interval per step.
+One bench step = one 0.1 MHD frame interval of the whole particle set, with the
+inputs resident in HBM.
+
+Workload `focused` (default; the headline): focused transport on the
+1024 x 512 island-chain Harris field (two time-blended frames), 8 388 608
+particles with mu ~ U[-1, 1] and p log-uniform on [1, 1e3], slab-QLT
+pitch-angle scattering (gamma_k = 5/3, sigma2 = 0.1, Lc = 0.02), fixed
+dt = 5e-5 (2000 sub-steps per interval). The measured path is the FP32 bulk
+push with the chosen closure (--closure); the same particles are then pushed by
+the plain FP64 exact-QLT push for the speed-up.
+
+Workload `parker`: Parker transport at production shape, 4 194 304 particles,
+the driver's default physics with adaptive dt, FP64 or FP32.
 
 Prints one JSON line. `value` is particle-steps/s over the timed steps (exact
 sub-step counts from the kernel). `roofline` prices the push kernel's algorithmic
-bytes (2 frames x 4 corners x 16 components x 8 B per sub-step, plus the
-particle state read and written once per launch) against 8 TB/s, with the
-kernel time measured by HIP events on the library's stream. `cpu_baseline` times
-the FP64 CPU restatement of the reference push on a bounded sample of the same
-particles on this host.
+bytes (frames x 4 corners x components x word size per sub-step, plus the
+particle state read and written once per launch) against 8 TB/s, with the kernel
+time from HIP events recorded on the library's stream around every launch.
+`cpu_baseline` times the FP64 CPU restatement of the same push on a bounded
+sample of the same particles on this host.
 """
 import argparse
 import json
@@ -27,29 +35,27 @@ sys.path.insert(0, os.path.join(REPO, "kokkos-particletransport_amd"))
 import numpy as np  # noqa: E402
 
 import kpt  # noqa: E402
-import workloads  # noqa: E402
+import workloads as W  # noqa: E402
 
 METRIC = "particle-steps/s, FP32+MLP-D_μμ vs FP64 plain push, at matched spectrum χ² error"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_STEP = 2 * 4 * 16 * 8  # frames x corners x components x sizeof(double)
-STATE_BYTES = 88                  # particle state read + written per launch
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_workload(nx, ny, n, seed):
-    xe, ye, f0 = workloads.island_field(nx, ny, seed=seed)
-    _, _, f1 = workloads.island_field(nx, ny, seed=seed + 1)
-    h = kpt.default_host_settings()
-    h.split_ratio = 1.2
-    h.split_particles = 0
-    s = kpt.make_device_settings(h, xe, ye)
-    return xe, ye, f0, f1, s
+def step_bytes(model, fp32, frames=2):
+    """Algorithmic bytes of one sub-step: the coefficient stencil the mover reads."""
+    return frames * 4 * model * (4 if fp32 else 8)
 
 
-def cpu_baseline(xe, ye, f0, f1, s, start, target_seconds=10.0):
+def state_bytes(model):
+    """Particle state read and written once per launch."""
+    return 88 if model == kpt.MODEL_PARKER16 else 104
+
+
+def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0):
     """Time the oracle (FP64 restatement of the reference push) on a bounded
     sample of the bench's own particles, using this host's cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -57,28 +63,35 @@ def cpu_baseline(xe, ye, f0, f1, s, start, target_seconds=10.0):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = max(1, min(threads, os.cpu_count() or 1))
-    lo = kpt_oracle.Field(xe, ye, f0, kpt.MODEL_PARKER16)
-    hi = kpt_oracle.Field(xe, ye, f1, kpt.MODEL_PARKER16)
+    lo = kpt_oracle.Field(xe, ye, f0, model)
+    hi = kpt_oracle.Field(xe, ye, f1, model)
+    s64 = s.copy()
+    s64.precision = kpt.PREC_FP64
+    if s64.closure == kpt.CLOSURE_MLP:
+        s64.closure = kpt.CLOSURE_QLT
 
     def run(m):
         parts = {k: v[:m].copy() for k, v in start.items()}
         t0 = time.perf_counter()
-        steps, _ = kpt_oracle.push_interval(lo, hi, s, 0.0, 0, 7, parts, threads=threads)
+        steps, _ = kpt_oracle.push_interval(lo, hi, s64, 0.0, 0, 7, parts, threads=threads)
         return steps, time.perf_counter() - t0
 
-    m = 2048
+    m = 1024
     steps, dt = run(m)
     rate = steps / max(dt, 1e-9)
     per_particle = steps / m
     m = int(min(len(start["x"]), max(m, target_seconds * rate / max(per_particle, 1.0))))
     steps, dt = run(m)
+    what = ("src/main.cpp:1082-1327" if model == kpt.MODEL_PARKER16 else
+            "src/main.cpp:1332-1755 with the slab-QLT closure of include/LegencyModel.hpp:539-689 "
+            "and the Picard mu update of include/FocusSolver.hpp:700-719")
     return {
         "value": steps / dt,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{m} of the bench particles, one 0.1 interval ({steps} sub-steps, {dt:.1f} s), "
-                  f"FP64 restatement of src/main.cpp:1082-1327 with OpenMP",
+                  f"FP64 restatement of {what}, OpenMP",
     }
 
 
@@ -102,20 +115,45 @@ def copy_probe(mib=512, repeats=3):
     print(json.dumps({"copy_probe_mib": mib, "repeats": repeats}), flush=True)
 
 
+def timed_push(ctx, s, warmup, steps, first_frame=0):
+    """Warm-up and timed intervals; returns (steps, failures, elapsed s, kernel ms per
+    interval, state bytes of the launches, active after)."""
+    frame = first_frame
+    for _ in range(warmup):
+        st = ctx.push_interval(s, frame * s.mhd_dt, frame)
+        log(f"  warmup frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
+        frame += 1
+    total, fails, kernel_ms, active = 0, 0, [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = ctx.push_interval(s, frame * s.mhd_dt, frame)
+        total += st.particle_steps
+        fails += st.failures
+        kernel_ms.append(st.kernel_ms)
+        active.append(st.active_in * st.launches)
+        log(f"  frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
+        frame += 1
+    elapsed = time.perf_counter() - t0
+    return total, fails, elapsed, kernel_ms, active
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run device-to-device copies of 512 MiB (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=4 * 1024 * 1024)
+    ap.add_argument("--workload", choices=["focused", "parker"], default="focused")
+    ap.add_argument("--particles", type=int, default=0, help="0: the workload's size")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default=None,
+                    help="default: fp32 for focused, fp64 for parker")
+    ap.add_argument("--closure", choices=["qlt", "mlp"], default="qlt")
+    ap.add_argument("--no-fp64-leg", action="store_true",
+                    help="focused: skip the plain FP64 push of the same particles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-sort", action="store_true",
-                    help="push in injection order instead of sorting by cell before each interval")
     ap.add_argument("--chunks", type=int, default=8,
                     help="time chunks per interval, each preceded by a cell sort (push_chunks)")
     args = ap.parse_args()
@@ -128,51 +166,74 @@ def main():
               "diagnostic interval; not built).", file=sys.stderr)
         sys.exit(2)
 
-    xe, ye, f0, f1, s = build_workload(args.nx, args.ny, args.particles, seed=7)
-    s.precision = kpt.PREC_FP32 if args.precision == "fp32" else kpt.PREC_FP64
-    n = args.particles
-    log(f"bench: {args.nx}x{args.ny} field, {n} particles, {args.precision}")
+    focused = args.workload == "focused"
+    prec = args.precision or ("fp32" if focused else "fp64")
+    fp32 = prec == "fp32"
+    model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
+    xe, ye, f0 = W.island_field(args.nx, args.ny, seed=7)
+    _, _, f1 = W.island_field(args.nx, args.ny, seed=8)
+    if focused:
+        n = args.particles or W.C4["particles"]
+        closure = kpt.CLOSURE_MLP if args.closure == "mlp" else kpt.CLOSURE_QLT
+        s = W.c3_device_settings(xe, ye, closure=closure,
+                                 precision=kpt.PREC_FP32 if fp32 else kpt.PREC_FP64)
+        start = W.c3_particles(n, s, seed=11)
+    else:
+        n = args.particles or 4 * 1024 * 1024
+        h = kpt.default_host_settings()
+        h.split_particles = 0
+        s = kpt.make_device_settings(h, xe, ye)
+        s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
+        start = None
+    s.push_chunks = args.chunks
+    log(f"bench: {args.workload} {args.nx}x{args.ny}, {n} particles, {prec}"
+        + (f", closure {args.closure}" if focused else ""))
+
+    fp64_leg = None
     with kpt.Context(n, seed=114514) as ctx:
-        ctx.field_upload(kpt.SLOT_LOWER, kpt.MODEL_PARKER16, xe, ye, f0)
-        ctx.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, xe, ye, f1)
-        opts = kpt.InjectOpts()
-        opts.fixed_time = 1
-        assert ctx.inject(s, n, 0.0, 0, opts) == n
-        start = ctx.download()
-
-        s.push_chunks = 1 if args.no_sort else args.chunks
-
-        def interval(frame):
-            # one bench step = one interval: with push_chunks > 1 the library
-            # re-sorts by cell before every chunk; with one chunk sort here
-            if not args.no_sort and s.push_chunks <= 1:
-                ctx.sort(s)
-            return ctx.push_interval(s, frame * s.mhd_dt, frame)
-
-        frame = 0
-        for _ in range(args.warmup):
-            st = interval(frame)
-            log(f"warmup frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
-            frame += 1
-        steps = 0
-        kernel_ms = []
-        launches_bytes = []
-        failures = 0
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = interval(frame)
-            steps += st.particle_steps
-            failures += st.failures
-            kernel_ms.append(st.kernel_ms)
-            launches_bytes.append(st.particle_steps * BYTES_PER_STEP + st.active_in * STATE_BYTES)
-            log(f"frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
-            frame += 1
-        elapsed = time.perf_counter() - t0
+        ctx.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
+        ctx.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
+        if focused:
+            ctx.upload(start)
+        else:
+            opts = kpt.InjectOpts()
+            opts.fixed_time = 1
+            assert ctx.inject(s, n, 0.0, 0, opts) == n
+            start = ctx.download()
+        steps, failures, elapsed, kernel_ms, active = timed_push(ctx, s, args.warmup, args.steps)
         summary = ctx.summary()
+        if focused and not args.no_fp64_leg:
+            log("plain FP64 push of the same particles ...")
+            s64 = s.copy()
+            s64.precision = kpt.PREC_FP64
+            s64.closure = kpt.CLOSURE_QLT
+            ctx.upload(start)
+            st64, f64, el64, km64, _ = timed_push(ctx, s64, args.warmup, args.steps)
+            fp64_leg = {"value": st64 / el64, "sub_steps": int(st64), "failures": int(f64),
+                        "kernel_ms_avg": float(np.mean(km64))}
 
     value = steps / elapsed
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = float(np.mean(launches_bytes)) / avg_kernel_s / 1e9
+    per_step = step_bytes(model, fp32)
+    launch_bytes = (steps / args.steps) * per_step + float(np.mean(active)) * state_bytes(model)
+    achieved = launch_bytes / avg_kernel_s / 1e9
+    cfg = {
+        "workload": (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}_island_{n}" if focused else
+                     f"parker_{prec}_{args.nx}x{args.ny}_island_{n}"),
+        "field": f"{args.nx}x{args.ny} uniform grid, 2 time-blended frames",
+        "particles": n,
+        "interval": s.mhd_dt,
+        "sub_steps": int(steps),
+        "failures": int(failures),
+        "active_after": int(summary["active"]),
+        "push_chunks": args.chunks,
+    }
+    if focused:
+        cfg["closure"] = args.closure
+        cfg["dt"] = s.dt_min
+        if fp64_leg:
+            cfg["fp64_plain"] = fp64_leg
+            cfg["speedup_vs_fp64_plain"] = value / fp64_leg["value"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -184,18 +245,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (seeded island-chain Harris field, uniform injection)",
-        "config": {
-            "workload": f"parker_{args.precision}_plain_push_{args.nx}x{args.ny}_island_{n}",
-            "field": f"{args.nx}x{args.ny} uniform grid, 2 time-blended frames",
-            "particles": n,
-            "interval": s.mhd_dt,
-            "sub_steps": int(steps),
-            "failures": int(failures),
-            "active_after": int(summary["active"]),
-            "path": "FP64 plain Parker push (the FP32+MLP focused path is not built yet)",
-        },
+        "dtype": "f32+f64" if fp32 else "f64",
+        "data": "synthetic (seeded island-chain Harris field; "
+                + ("uniform x, y, mu, log-uniform p)" if focused else "uniform injection)"),
+        "config": cfg,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -204,12 +257,12 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel_ms_avg": avg_kernel_s * 1e3,
-            "bytes_per_step": BYTES_PER_STEP,
+            "bytes_per_step": per_step,
         },
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(xe, ye, f0, f1, s, start)
+        out["cpu_baseline"] = cpu_baseline(xe, ye, f0, f1, model, s, start)
     print(json.dumps(out), flush=True)
 
 

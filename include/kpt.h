@@ -50,8 +50,13 @@ typedef enum {
 typedef enum { KPT_SLOT_LOWER = 0, KPT_SLOT_UPPER = 1 } kpt_slot;
 
 /* Arithmetic of the push. FP64: everything in double, as the reference.
- * FP32: coefficient fields stored and interpolated in float, transport
- * coefficients in float, positions / momentum / time accumulated in double. */
+ * FP32: coefficient frames stored and interpolated in float; transport
+ * coefficients, dt candidates, noise and increments in float; x, y, t, mu and p
+ * accumulated in double, and the adiabatic dp = p (rate) dt formed in double.
+ * A lane whose interpolated |B| is below fp64_null_b re-samples the FP64
+ * frames and takes that sub-step's coefficient arithmetic in double (the drift
+ * and the diffusion-tensor divergence scale as 1/|B|^2 and 1/|B|^3 near
+ * current-sheet nulls). */
 typedef enum { KPT_PREC_FP64 = 0, KPT_PREC_FP32 = 1 } kpt_precision;
 
 /* Noise source of the push. */
@@ -132,6 +137,8 @@ typedef struct kpt_device_settings {
     double qlt_correlation_length;
     double qlt_spectral_index;  /* gamma_k, PitchAngleScatteringParameters default 5/3 */
     double qlt_h0;              /* resonance regularisation, default 1e-6 */
+    double fp64_null_b;         /* KPT_PREC_FP32 only: |B| below which a sub-step
+                                   runs in FP64 (default 0.02; 0 = never) */
 } kpt_device_settings;
 
 enum { KPT_CLOSURE_REDUCED = 0, KPT_CLOSURE_QLT = 1, KPT_CLOSURE_MLP = 2 };

@@ -158,7 +158,8 @@ kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
                     std::isfinite(s->kpara0) && s->kpara0 > 0.0 && std::isfinite(s->kperp_over_kpara) &&
                     s->kperp_over_kpara >= 0.0 && s->max_particle_steps_per_interval > 0 &&
                     (s->precision == KPT_PREC_FP64 || s->precision == KPT_PREC_FP32) &&
-                    s->push_chunks >= 0 && s->push_chunks <= 4096;
+                    s->push_chunks >= 0 && s->push_chunks <= 4096 && std::isfinite(s->fp64_null_b) &&
+                    s->fp64_null_b >= 0.0;
     if (!ok) return fail(ctx, KPT_ERR_ARG, "Reconnection2DSettings: invalid physical parameters.");
     if (s->closure != KPT_CLOSURE_REDUCED) {
         // PitchAngleScatteringParameters::validate (LegencyModel.hpp:207-222) plus
@@ -504,6 +505,7 @@ kpt_status kpt_make_device_settings(const kpt_host_settings* h, int32_t nx, int3
     o->qlt_correlation_length = 0.0;
     o->qlt_spectral_index = 5.0 / 3.0;  // PitchAngleScatteringParameters (LegencyModel.hpp:196-198)
     o->qlt_h0 = 1.0e-6;
+    o->fp64_null_b = 0.02;
     return KPT_OK;
 }
 
@@ -725,6 +727,8 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.field_lo = fp32 ? static_cast<const void*>(lo.c32) : static_cast<const void*>(lo.c64);
     a.field_hi = s->time_interpolation != 0 ? (fp32 ? static_cast<const void*>(hi.c32) : static_cast<const void*>(hi.c64))
                                             : a.field_lo;
+    a.field_lo64 = lo.c64;
+    a.field_hi64 = s->time_interpolation != 0 ? hi.c64 : lo.c64;
     a.plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
     a.g = lo.g;
     a.s = *s;

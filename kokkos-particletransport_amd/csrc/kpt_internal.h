@@ -50,6 +50,8 @@ struct PushArgs {
     uint64_t count;
     const void* field_lo;  // double or float, component-planar (see coefficient_plane)
     const void* field_hi;
+    const double* field_lo64;  // FP64 frames, read by FP32 lanes below fp64_null_b
+    const double* field_hi64;
     uint64_t plane;
     GridDesc g;
     kpt_device_settings s;

@@ -53,8 +53,6 @@ __device__ __forceinline__ double u01(uint32_t r) {
 
 constexpr double kSqrt3 = 1.7320508075688772935;
 
-__device__ __forceinline__ double sqr(double v) { return v * v; }
-
 // wrap_periodic_coordinate (src/main.cpp:641-665).
 __device__ __forceinline__ double wrap_periodic(double v, double lo, double hi, double width) {
     double w = v;
@@ -219,12 +217,42 @@ __device__ __forceinline__ bool axis_stencil(double q, const GridDesc& g, int d,
 }
 
 // ---------------------------------------------------------------------------
-// The push: one lane advances one particle through the whole interval
-// (move_parker_particles_one_interval, src/main.cpp:1113-1325, and the shared
-// mover move_particles_one_interval, src/main.cpp:1364-1753).
-// MODEL: 16 or 21 packed components. FOCUSED: reduced focused branch.
-// FT: storage / interpolation type of the coefficient frames.
+// Arithmetic of one sub-step, templated on its type RT. The double overloads
+// are the plain library operations; the float ones use the native v_rcp_f32,
+// v_sqrt_f32, v_log_f32 and v_exp_f32 instructions (about 1 ulp each).
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T sqr(T v) { return v * v; }
+__device__ __forceinline__ double rabs(double v) { return fabs(v); }
+__device__ __forceinline__ float rabs(float v) { return fabsf(v); }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rsqrt_(double v) { return sqrt(v); }
+__device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ double rrcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float rrcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double rpow(double a, double e) { return pow(a, e); }
+__device__ __forceinline__ float rpow(float a, float e) {
+    // a^e = 2^(e log2 a) for normal positive a; anything else takes powf.
+    return a >= 1.17549435e-38f ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(a)) : powf(a, e);
+}
+template <typename RT>
+__device__ __forceinline__ bool rfinite(RT v) { return __builtin_isfinite(v); }
+
+// One U[-sqrt3, sqrt3] draw from 32 Philox bits. Double: (r + 1/2) 2^-32 as the
+// injected-noise oracle. Float: the top 23 bits, (2k + 1) 2^-24, exact in float.
+template <typename RT>
+__device__ __forceinline__ RT draw_sqrt3(uint32_t r);
+template <>
+__device__ __forceinline__ double draw_sqrt3<double>(uint32_t r) {
+    return -kSqrt3 + (kSqrt3 - -kSqrt3) * u01(r);
+}
+template <>
+__device__ __forceinline__ float draw_sqrt3<float>(uint32_t r) {
+    const float u = static_cast<float>(r >> 9) * 0x1.0p-23f + 0x1.0p-24f;
+    return fmaf(u, static_cast<float>(2.0 * kSqrt3), static_cast<float>(-kSqrt3));
+}
+
 // FocusTransportSolver::reflected_pitch_angle_cosine (include/FocusSolver.hpp:
 // 454-470): mirror at +-upper. The mirror loop is bounded (16 passes); a value
 // still outside is folded by the closed-form triangle wave of period 4 upper,
@@ -248,25 +276,276 @@ __device__ __forceinline__ double reflect_mu(double mu, double upper) {
 // Slab-QLT D_mumu and dD_mumu/dmu (include/LegencyModel.hpp:557-643) for one
 // trial mu, with omega, xi = omega Lc / v and xi^gamma_k fixed for the sub-step.
 // `pref` = (pi / 4) omega sigma2 gamma_k sin(pi / gamma_k) / pi.
-__device__ __forceinline__ void qlt_scattering(double mu, double xi, double xi_g, double pref, double gk,
-                                               double h0, double& D, double& dD) {
-    const double one_minus_mu2 = 1.0 - mu * mu;
-    D = 0.0;
-    dD = 0.0;
-    if (one_minus_mu2 <= 0.0) return;
-    const double mu_h = fabs(mu) + h0;
-    const double pm1 = pow(mu_h, gk - 1.0);  // mu_h^(gamma_k - 1)
-    const double den = pm1 * mu_h + xi_g;    // mu_h^gamma_k + xi^gamma_k
+template <typename RT>
+__device__ __forceinline__ void qlt_scattering(RT mu, RT xi, RT xi_g, RT pref, RT gk, RT h0, RT& D, RT& dD) {
+    const RT one_minus_mu2 = RT(1) - mu * mu;
+    D = RT(0);
+    dD = RT(0);
+    if (one_minus_mu2 <= RT(0)) return;
+    const RT mu_h = rabs(mu) + h0;
+    const RT pm1 = rpow(mu_h, gk - RT(1));  // mu_h^(gamma_k - 1)
+    const RT den = pm1 * mu_h + xi_g;       // mu_h^gamma_k + xi^gamma_k
     D = pref * one_minus_mu2 * xi * pm1 / den;
-    if (D == 0.0 || mu == 0.0) return;
-    const double sign_mu = mu > 0.0 ? 1.0 : -1.0;
-    const double shape = sign_mu * ((gk - 1.0) / mu_h - gk * pm1 / den);
-    dD = D * (-2.0 * mu / one_minus_mu2 + shape);
+    if (D == RT(0) || mu == RT(0)) return;
+    const RT sign_mu = mu > RT(0) ? RT(1) : RT(-1);
+    const RT shape = sign_mu * ((gk - RT(1)) / mu_h - gk * pm1 / den);
+    dD = D * (RT(-2) * mu / one_minus_mu2 + shape);
 }
 
+// Terms of one sub-step at the particle's sampled coefficients.
+template <typename RT>
+struct StepTerms {
+    RT dx_dt, dy_dt, dmu_dt, duu;
+    double dp_dt;  // adiabatic rate, formed in double in every precision
+    RT skpara, skperp, skpara_perp;
+    RT bx, by, bz, ib;
+    // focusing pieces and QLT constants reused by the Picard passes
+    RT f_speed, f_divv, f_bb, f_bv, f_divb, q_xi, q_xig, q_pref;
+};
+
+template <typename D, typename S>
+__device__ __forceinline__ void narrow_terms(const StepTerms<S>& a, StepTerms<D>& o) {
+    o.dx_dt = D(a.dx_dt); o.dy_dt = D(a.dy_dt); o.dmu_dt = D(a.dmu_dt); o.duu = D(a.duu);
+    o.dp_dt = a.dp_dt;
+    o.skpara = D(a.skpara); o.skperp = D(a.skperp); o.skpara_perp = D(a.skpara_perp);
+    o.bx = D(a.bx); o.by = D(a.by); o.bz = D(a.bz); o.ib = D(a.ib);
+    o.f_speed = D(a.f_speed); o.f_divv = D(a.f_divv); o.f_bb = D(a.f_bb); o.f_bv = D(a.f_bv);
+    o.f_divb = D(a.f_divb); o.q_xi = D(a.q_xi); o.q_xig = D(a.q_xig); o.q_pref = D(a.q_pref);
+}
+
+// Rates of the mover at one sample: kappa model (src/main.cpp:1177-1199),
+// divergence of the diffusion tensor (:1210-1223), Parker drift and rates
+// (:1225-1241) or the focused branch (:1499-1589) with the reduced or the
+// slab-QLT pitch-angle closure. Returns false where the reference marks the
+// particle failed (status 2).
+template <int MODEL, bool FOCUSED, bool QLT, typename RT>
+__device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d, const kpt_device_settings& s,
+                                           double qlt_norm, StepTerms<RT>& o) {
+    RT vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = RT(0), dvy_dx = RT(0), dvy_dy, dvz_dx = RT(0), dvz_dy = RT(0),
+                                  dbx_dx, dbx_dy, dby_dx, dby_dy, dbz_dx, dbz_dy, db_dx, db_dy;
+    if constexpr (MODEL == 16) {
+        vx = c[0]; vy = c[1]; bx = c[2]; by = c[3]; bz = c[4]; b = c[5];
+        dvx_dx = c[6]; dvy_dy = c[7]; dbx_dx = c[8]; dbx_dy = c[9];
+        dby_dx = c[10]; dby_dy = c[11]; dbz_dx = c[12]; dbz_dy = c[13];
+        db_dx = c[14]; db_dy = c[15];
+    } else {
+        vx = c[0]; vy = c[1]; bx = c[3]; by = c[4]; bz = c[5]; b = c[6];
+        dvx_dx = c[7]; dvx_dy = c[8]; dvy_dx = c[9]; dvy_dy = c[10];
+        dvz_dx = c[11]; dvz_dy = c[12]; dbx_dx = c[13]; dbx_dy = c[14];
+        dby_dx = c[15]; dby_dy = c[16]; dbz_dx = c[17]; dbz_dy = c[18];
+        db_dx = c[19]; db_dy = c[20];
+    }
+    (void)dvz_dx;
+    if (!rfinite(p_d) || p_d <= 0.0 || !rfinite(b) || b <= RT(0)) return false;
+    const RT p = static_cast<RT>(p_d);
+    const RT ib = rrcp(b);
+    const RT ib2 = ib * ib;
+    const RT ib3 = ib2 * ib;
+
+    RT knorm = RT(1);
+    if (s.magnetic_dependency == 1) knorm *= rpow(b, static_cast<RT>(s.gamma_turb - 2.0));
+    if (s.momentum_dependency == 1) knorm *= rpow(static_cast<RT>(p_d / s.p0), static_cast<RT>(s.pindex));
+    const RT kpara = static_cast<RT>(s.kpara0) * knorm;
+    const RT kperp = kpara * static_cast<RT>(s.kperp_over_kpara);
+    if (!rfinite(kpara) || kpara <= RT(0) || !rfinite(kperp) || kperp < RT(0)) return false;
+    o.skpara = rsqrt_(RT(2) * kpara);
+    o.skperp = rsqrt_(RT(2) * kperp);
+    o.skpara_perp = MODEL == 16 ? rsqrt_(RT(2) * (kpara - kperp))
+                                : (kpara > kperp ? rsqrt_(RT(2) * (kpara - kperp)) : RT(0));
+    const RT kpp = FOCUSED ? -kperp : kpara - kperp;
+
+    RT dkdx = RT(0), dkdy = RT(0);
+    if (s.magnetic_dependency == 1) {
+        const RT me = static_cast<RT>(s.gamma_turb - 2.0);
+        dkdx = db_dx * ib * me;
+        dkdy = db_dy * ib * me;
+    }
+    const RT dkxx_dx = kperp * dkdx + kpp * dkdx * bx * bx * ib2 +
+                       RT(2) * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+    const RT dkyy_dy = kperp * dkdy + kpp * dkdy * by * by * ib2 +
+                       RT(2) * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+    const RT dkxy_dx = kpp * dkdx * bx * by * ib2 +
+                       kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - RT(2) * bx * by * db_dx * ib3);
+    const RT dkxy_dy = kpp * dkdy * bx * by * ib2 +
+                       kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - RT(2) * bx * by * db_dy * ib3);
+    const RT drift_den = rsqrt_(sqr(static_cast<RT>(s.drift_param1 * s.p0) / p) +
+                                sqr(static_cast<RT>(s.drift_param2 * s.p0 * s.p0) / (p * p)));
+
+    o.bx = bx;
+    o.by = by;
+    o.bz = bz;
+    o.ib = ib;
+    o.dmu_dt = RT(0);
+    o.duu = RT(0);
+    o.f_speed = o.f_divv = o.f_bb = o.f_bv = o.f_divb = o.q_xi = o.q_xig = o.q_pref = RT(0);
+    if constexpr (FOCUSED) {
+        const RT mu = static_cast<RT>(mu_d);
+        const RT mu2 = mu * mu;
+        const RT muf1 = RT(0.5) * (RT(1) - mu2);
+        const RT muf2 = RT(0.5) * (RT(3) * mu2 - RT(1));
+        const RT kx = bx * dbx_dx + by * dbx_dy;
+        const RT ky = bx * dby_dx + by * dby_dy;
+        const RT kz = bx * dbz_dx + by * dbz_dy;
+        const RT bdot_curvb = bx * dbz_dy - by * dbz_dx + bz * (dby_dx - dbx_dy);
+        const RT vdp = static_cast<RT>(1.0 / s.charge) / drift_den;
+        const RT vdx = vdp * (muf1 * (-bz * db_dy) * ib2 + mu2 * (by * kz - bz * ky) * ib3 +
+                              muf1 * bx * bdot_curvb * ib3);
+        const RT vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
+                              muf1 * by * bdot_curvb * ib3);
+        // QLT: relativistic speed p / gamma (m = c = 1)
+        const RT gamma_p = QLT ? rsqrt_(RT(1) + p * p) : RT(1);
+        const RT speed = QLT ? p / gamma_p : static_cast<RT>(s.particle_v0) * p / static_cast<RT>(s.p0);
+        if (!rfinite(speed) || speed <= RT(0)) return false;
+        const RT vpb = speed * mu * ib;
+        o.dx_dt = vx + vdx + vpb * bx + dkxx_dx + dkxy_dy;
+        o.dy_dt = vy + vdy + vpb * by + dkxy_dx + dkyy_dy;
+        const RT divv = dvx_dx + dvy_dy;
+        const RT bb_gradv = (bx * (bx * dvx_dx + by * dvx_dy) + by * (bx * dvy_dx + by * dvy_dy) +
+                             bz * (bx * dvz_dx + by * dvz_dy)) * ib2;
+        const RT bv_gradv = (bx * (vx * dvx_dx + vy * dvx_dy) + by * (vx * dvy_dx + vy * dvy_dy) +
+                             bz * (vx * dvz_dx + vy * dvz_dy)) * ib;
+        o.dp_dt = p_d * -static_cast<double>(muf1 * divv + muf2 * bb_gradv + mu * bv_gradv / speed);
+        const RT div_bnorm = -(bx * db_dx + by * db_dy) * ib2;
+        RT dmu_dt = speed * div_bnorm + mu * divv - RT(3) * mu * bb_gradv - RT(2) * bv_gradv / speed;
+        dmu_dt *= RT(0.5) * (RT(1) - mu2);
+        RT duu, duu_du = RT(0);
+        if constexpr (QLT) {
+            o.f_speed = speed;
+            o.f_divv = divv;
+            o.f_bb = bb_gradv;
+            o.f_bv = bv_gradv;
+            o.f_divb = div_bnorm;
+            const RT omega = static_cast<RT>(fabs(s.charge)) * b / gamma_p;
+            o.q_xi = omega * static_cast<RT>(s.qlt_correlation_length) / speed;
+            o.q_xig = rpow(o.q_xi, static_cast<RT>(s.qlt_spectral_index));
+            o.q_pref = static_cast<RT>(0.25 * M_PI) * omega * static_cast<RT>(s.qlt_sigma2) *
+                       static_cast<RT>(qlt_norm);
+            qlt_scattering<RT>(mu, o.q_xi, o.q_xig, o.q_pref, static_cast<RT>(s.qlt_spectral_index),
+                               static_cast<RT>(s.qlt_h0), duu, duu_du);
+        } else {
+            const RT abs_mu = rabs(mu);
+            const RT dtmp = rpow(abs_mu, static_cast<RT>(s.gamma_turb - 1.0)) +
+                            static_cast<RT>(s.pitch_angle_scattering_h0);
+            duu = static_cast<RT>(s.duu0) * (RT(1) - mu2) * dtmp;
+            if (mu != RT(0)) {
+                const RT g2 = (RT(1) - mu2) * rpow(abs_mu, static_cast<RT>(s.gamma_turb - 2.0));
+                duu_du = static_cast<RT>(s.duu0) * (RT(-2) * mu * dtmp + (mu > RT(0) ? g2 : -g2));
+            }
+            RT duu_norm = RT(1);
+            if (s.magnetic_dependency == 1) duu_norm *= rpow(b, static_cast<RT>(2.0 - s.gamma_turb));
+            if (s.momentum_dependency == 1)
+                duu_norm *= rpow(static_cast<RT>(p_d / s.p0), static_cast<RT>(s.gamma_turb - 1.0));
+            duu *= duu_norm;
+            duu_du *= duu_norm;
+        }
+        dmu_dt += duu_du;
+        if (!rfinite(dmu_dt) || !rfinite(duu) || duu < RT(0)) return false;
+        o.dmu_dt = dmu_dt;
+        o.duu = duu;
+    } else {
+        const RT vdp = static_cast<RT>(1.0 / (3.0 * s.charge)) / drift_den;
+        const RT vdx = vdp * (dbz_dy * ib2 - RT(2) * bz * db_dy * ib3);
+        const RT vdy = vdp * (-dbz_dx * ib2 + RT(2) * bz * db_dx * ib3);
+        o.dx_dt = vx + vdx + dkxx_dx + dkxy_dy;
+        o.dy_dt = vy + vdy + dkxy_dx + dkyy_dy;
+        o.dp_dt = -p_d * static_cast<double>(dvx_dx + dvy_dy) / 3.0;
+    }
+    return true;
+}
+
+// Adaptive step candidates (src/main.cpp:1243-1277, :1603-1665) before the
+// [dt_min, dt_max] clamp. The momentum candidate is formed in double.
+template <bool FOCUSED, typename RT>
+__device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p, const kpt_device_settings& s) {
+    if (FOCUSED) {
+        const RT ds = T.skperp > RT(0) ? T.skperp : T.skpara;
+        if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0 && T.dmu_dt != RT(0) && ds > RT(0)) {
+            RT d = rmin(sqr(static_cast<RT>(0.5 * s.dx) / ds), sqr(static_cast<RT>(0.5 * s.dy) / ds));
+            d = rmin(d, sqr(ds / T.dx_dt));
+            d = rmin(d, sqr(ds / T.dy_dt));
+            d = rmin(d, RT(0.1) / rabs(T.dmu_dt));
+            d = rmin(d, RT(2) * T.duu / sqr(T.dmu_dt));
+            return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
+        }
+        return s.dt_min;
+    }
+    if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0) {
+        const RT as = T.skperp > RT(0) ? T.skperp : T.skpara;
+        RT d = rmin(sqr(static_cast<RT>(0.5 * s.dx) / T.skpara), sqr(static_cast<RT>(0.5 * s.dy) / T.skpara));
+        d = rmin(d, sqr(as / T.dx_dt));
+        d = rmin(d, sqr(as / T.dy_dt));
+        return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
+    }
+    return s.dt_min;
+}
+
+// Bilinear weights, both frames and the time blend of one sample.
+template <int C, typename T>
+__device__ __forceinline__ void sample_blend(const T* __restrict__ flo, const T* __restrict__ fhi, bool blend,
+                                             double uw, int64_t n00, int nxp, uint64_t plane, double wx,
+                                             double wy, T* c) {
+    const T w00 = static_cast<T>((1.0 - wx) * (1.0 - wy));
+    const T w10 = static_cast<T>(wx * (1.0 - wy));
+    const T w01 = static_cast<T>((1.0 - wx) * wy);
+    const T w11 = static_cast<T>(wx * wy);
+    sample_corners<C, T>(flo, n00, nxp, plane, w00, w10, w01, w11, c);
+    if (blend) {
+        T hi[C];
+        sample_corners<C, T>(fhi, n00, nxp, plane, w00, w10, w01, w11, hi);
+        const double lw = 1.0 - uw;
+        const T uwt = static_cast<T>(uw), lwt = static_cast<T>(lw);
+#pragma unroll
+        for (int k = 0; k < C; ++k) c[k] = lwt * c[k] + uwt * hi[k];
+    }
+}
+
+// The same sample from the FP64 frames for the rare near-null lanes of the FP32
+// push: four components at a time, so this branch does not set the register
+// budget of the whole kernel.
+template <int C>
+__device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restrict__ flo,
+                                                          const double* __restrict__ fhi, bool blend,
+                                                          double uw, int64_t n00, int nxp, uint64_t plane,
+                                                          double wx, double wy, double* c) {
+    const double w00 = (1.0 - wx) * (1.0 - wy), w10 = wx * (1.0 - wy);
+    const double w01 = (1.0 - wx) * wy, w11 = wx * wy;
+    const double lw = 1.0 - uw;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const double* q = flo + k * plane + n00;
+        double v = w00 * q[0] + w10 * q[1] + w01 * q[nxp] + w11 * q[nxp + 1];
+        if (blend) {
+            const double* h = fhi + k * plane + n00;
+            v = lw * v + uw * (w00 * h[0] + w10 * h[1] + w01 * h[nxp] + w11 * h[nxp + 1]);
+        }
+        c[k] = v;
+        if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The push: one lane advances one particle through the whole interval
+// (move_parker_particles_one_interval, src/main.cpp:1113-1325, and the shared
+// mover move_particles_one_interval, src/main.cpp:1364-1753).
+// MODEL: 16 or 21 packed components. FOCUSED: focused branch. FT: storage,
+// interpolation and coefficient arithmetic type (x, y, t, p and mu are carried in
+// double in both). CLOSURE: pitch-angle closure of the focused branch.
+// ---------------------------------------------------------------------------
+// Waves per SIMD of the push: the unified VGPR/AGPR file holds 512 registers
+// per lane, so 2 waves get 256 each and keep every corner load of a sub-step in
+// flight; more waves trade loads in flight per wave for latency hiding.
+#ifndef KPT_PUSH_WAVES_FP64
+#define KPT_PUSH_WAVES_FP64 2
+#endif
+#ifndef KPT_PUSH_WAVES_FP32
+#define KPT_PUSH_WAVES_FP32 2
+#endif
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
-__global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
+    push_kernel(PushArgs a) {
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
+    constexpr bool FP32 = sizeof(FT) == 4;
     constexpr int CP = MODEL;  // components read per corner
     const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     uint64_t my_steps = 0;
@@ -280,6 +559,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
         const int nxp = a.g.nx + 2;
         const double interval_start = a.t0;
         const double interval_end = a.t_end;
+        const bool blend = s.time_interpolation != 0;
         double x = a.P.x[idx], y = a.P.y[idx], p = a.P.p[idx], t = a.P.t[idx];
         double dt = a.P.step_dt[idx];
         double diagnostic_dt = dt;
@@ -308,190 +588,39 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                 break;
             }
             const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
-            const FT w00 = static_cast<FT>((1.0 - wx) * (1.0 - wy));
-            const FT w10 = static_cast<FT>(wx * (1.0 - wy));
-            const FT w01 = static_cast<FT>((1.0 - wx) * wy);
-            const FT w11 = static_cast<FT>(wx * wy);
+            double uw = 0.0;
+            if (blend) {
+                const double rt = (t - interval_start) / s.mhd_dt;
+                uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
+            }
 
-            double c[CP];
+            StepTerms<FT> T;
+            bool ok;
             {
-                FT lo[CP];
-                sample_corners<CP, FT>(flo, n00, nxp, a.plane, w00, w10, w01, w11, lo);
-                if (s.time_interpolation != 0) {
-                    FT hi[CP];
-                    sample_corners<CP, FT>(fhi, n00, nxp, a.plane, w00, w10, w01, w11, hi);
-                    const double rt = (t - interval_start) / s.mhd_dt;
-                    const double uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
-                    const double lw = 1.0 - uw;
-#pragma unroll
-                    for (int k = 0; k < CP; ++k)
-                        c[k] = lw * static_cast<double>(lo[k]) + uw * static_cast<double>(hi[k]);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < CP; ++k) c[k] = static_cast<double>(lo[k]);
-                }
-            }
-
-            double vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = 0.0, dvy_dx = 0.0, dvy_dy,
-                                              dvz_dx = 0.0, dvz_dy = 0.0, dbx_dx, dbx_dy, dby_dx,
-                                              dby_dy, dbz_dx, dbz_dy, db_dx, db_dy;
-            if constexpr (MODEL == 16) {
-                vx = c[0]; vy = c[1]; bx = c[2]; by = c[3]; bz = c[4]; b = c[5];
-                dvx_dx = c[6]; dvy_dy = c[7]; dbx_dx = c[8]; dbx_dy = c[9];
-                dby_dx = c[10]; dby_dy = c[11]; dbz_dx = c[12]; dbz_dy = c[13];
-                db_dx = c[14]; db_dy = c[15];
-            } else {
-                vx = c[0]; vy = c[1]; bx = c[3]; by = c[4]; bz = c[5]; b = c[6];
-                dvx_dx = c[7]; dvx_dy = c[8]; dvy_dx = c[9]; dvy_dy = c[10];
-                dvz_dx = c[11]; dvz_dy = c[12]; dbx_dx = c[13]; dbx_dy = c[14];
-                dby_dx = c[15]; dby_dy = c[16]; dbz_dx = c[17]; dbz_dy = c[18];
-                db_dx = c[19]; db_dy = c[20];
-            }
-
-            if (!isfinite(p) || p <= 0.0 || !isfinite(b) || b <= 0.0) {
-                status = 2;
-                break;
-            }
-            const double ib = 1.0 / b;
-            const double ib2 = ib * ib;
-            const double ib3 = ib2 * ib;
-
-            // kappa model (src/main.cpp:1177-1199)
-            double knorm = 1.0;
-            if (s.magnetic_dependency == 1) knorm *= pow(b, s.gamma_turb - 2.0);
-            if (s.momentum_dependency == 1) knorm *= pow(p / s.p0, s.pindex);
-            const double kpara = s.kpara0 * knorm;
-            const double kperp = kpara * s.kperp_over_kpara;
-            if (!isfinite(kpara) || kpara <= 0.0 || !isfinite(kperp) || kperp < 0.0) {
-                status = 2;
-                break;
-            }
-            const double skpara = sqrt(2.0 * kpara);
-            const double skperp = sqrt(2.0 * kperp);
-            const double skpara_perp = MODEL == 16 ? sqrt(2.0 * (kpara - kperp))
-                                                   : (kpara > kperp ? sqrt(2.0 * (kpara - kperp)) : 0.0);
-            const double kpp = FOCUSED ? -kperp : kpara - kperp;
-
-            double dkdx = 0.0, dkdy = 0.0;
-            if (s.magnetic_dependency == 1) {
-                const double me = s.gamma_turb - 2.0;
-                dkdx = db_dx * ib * me;
-                dkdy = db_dy * ib * me;
-            }
-            // divergence of the diffusion tensor (src/main.cpp:1210-1223)
-            const double dkxx_dx = kperp * dkdx + kpp * dkdx * bx * bx * ib2 +
-                                   2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
-            const double dkyy_dy = kperp * dkdy + kpp * dkdy * by * by * ib2 +
-                                   2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
-            const double dkxy_dx = kpp * dkdx * bx * by * ib2 +
-                                   kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
-            const double dkxy_dy = kpp * dkdy * bx * by * ib2 +
-                                   kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
-            const double drift_den =
-                sqrt(sqr(s.drift_param1 * s.p0 / p) + sqr(s.drift_param2 * s.p0 * s.p0 / (p * p)));
-
-            double dx_dt, dy_dt, dp_dt, dmu_dt = 0.0, duu = 0.0;
-            // focusing-rate pieces and QLT constants reused by the Picard passes
-            double f_speed = 0.0, f_divv = 0.0, f_bb = 0.0, f_bv = 0.0, f_divb = 0.0;
-            double q_xi = 0.0, q_xig = 0.0, q_pref = 0.0;
-            if (FOCUSED) {
-                // reduced focused branch (src/main.cpp:1499-1589)
-                const double mu2 = pitch_mu * pitch_mu;
-                const double muf1 = 0.5 * (1.0 - mu2);
-                const double muf2 = 0.5 * (3.0 * mu2 - 1.0);
-                const double kx = bx * dbx_dx + by * dbx_dy;
-                const double ky = bx * dby_dx + by * dby_dy;
-                const double kz = bx * dbz_dx + by * dbz_dy;
-                const double bdot_curvb = bx * dbz_dy - by * dbz_dx + bz * (dby_dx - dbx_dy);
-                const double vdp = 1.0 / s.charge / drift_den;
-                const double vdx = vdp * (muf1 * (-bz * db_dy) * ib2 + mu2 * (by * kz - bz * ky) * ib3 +
-                                          muf1 * bx * bdot_curvb * ib3);
-                const double vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
-                                          muf1 * by * bdot_curvb * ib3);
-                // QLT: relativistic speed p / gamma (m = c = 1)
-                const double gamma_p = QLT ? sqrt(1.0 + p * p) : 1.0;
-                const double speed = QLT ? p / gamma_p : s.particle_v0 * p / s.p0;
-                if (!isfinite(speed) || speed <= 0.0) {
-                    status = 2;
-                    break;
-                }
-                const double vpb = speed * pitch_mu * ib;
-                dx_dt = vx + vdx + vpb * bx + dkxx_dx + dkxy_dy;
-                dy_dt = vy + vdy + vpb * by + dkxy_dx + dkyy_dy;
-                const double divv = dvx_dx + dvy_dy;
-                const double bb_gradv = (bx * (bx * dvx_dx + by * dvx_dy) + by * (bx * dvy_dx + by * dvy_dy) +
-                                         bz * (bx * dvz_dx + by * dvz_dy)) * ib2;
-                const double bv_gradv = (bx * (vx * dvx_dx + vy * dvx_dy) + by * (vx * dvy_dx + vy * dvy_dy) +
-                                         bz * (vx * dvz_dx + vy * dvz_dy)) * ib;
-                dp_dt = p * -(muf1 * divv + muf2 * bb_gradv + pitch_mu * bv_gradv / speed);
-                const double div_bnorm = -(bx * db_dx + by * db_dy) * ib2;
-                dmu_dt = speed * div_bnorm + pitch_mu * divv - 3.0 * pitch_mu * bb_gradv -
-                         2.0 * bv_gradv / speed;
-                dmu_dt *= 0.5 * (1.0 - mu2);
-                double duu_du = 0.0;
-                if constexpr (QLT) {
-                    f_speed = speed;
-                    f_divv = divv;
-                    f_bb = bb_gradv;
-                    f_bv = bv_gradv;
-                    f_divb = div_bnorm;
-                    const double omega = fabs(s.charge) * b / gamma_p;
-                    q_xi = omega * s.qlt_correlation_length / speed;
-                    q_xig = pow(q_xi, s.qlt_spectral_index);
-                    q_pref = 0.25 * M_PI * omega * s.qlt_sigma2 * a.qlt_norm;
-                    qlt_scattering(pitch_mu, q_xi, q_xig, q_pref, s.qlt_spectral_index, s.qlt_h0, duu, duu_du);
-                } else {
-                    const double abs_mu = fabs(pitch_mu);
-                    const double dtmp = pow(abs_mu, s.gamma_turb - 1.0) + s.pitch_angle_scattering_h0;
-                    duu = s.duu0 * (1.0 - mu2) * dtmp;
-                    if (pitch_mu != 0.0) {
-                        const double g2 = (1.0 - mu2) * pow(abs_mu, s.gamma_turb - 2.0);
-                        duu_du = s.duu0 * (-2.0 * pitch_mu * dtmp + (pitch_mu > 0.0 ? g2 : -g2));
+                FT c[CP];
+                sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                if constexpr (FP32) {
+                    if (!(c[MODEL == 16 ? 5 : 6] >= static_cast<float>(s.fp64_null_b))) {
+                        // near a null: the FP64 frames and FP64 arithmetic for this sub-step
+                        double cd[CP];
+                        sample_blend_fp64_grouped<CP>(a.field_lo64, a.field_hi64, blend, uw, n00, nxp, a.plane,
+                                                      wx, wy, cd);
+                        StepTerms<double> Td;
+                        ok = eval_terms<MODEL, FOCUSED, QLT, double>(cd, p, pitch_mu, s, a.qlt_norm, Td);
+                        narrow_terms(Td, T);
+                    } else {
+                        ok = eval_terms<MODEL, FOCUSED, QLT, float>(c, p, pitch_mu, s, a.qlt_norm, T);
                     }
-                    double duu_norm = 1.0;
-                    if (s.magnetic_dependency == 1) duu_norm *= pow(b, 2.0 - s.gamma_turb);
-                    if (s.momentum_dependency == 1) duu_norm *= pow(p / s.p0, s.gamma_turb - 1.0);
-                    duu *= duu_norm;
-                    duu_du *= duu_norm;
-                }
-                dmu_dt += duu_du;
-                if (!isfinite(dmu_dt) || !isfinite(duu) || duu < 0.0) {
-                    status = 2;
-                    break;
-                }
-            } else {
-                // Parker branch (src/main.cpp:1225-1241)
-                const double vdp = 1.0 / (3.0 * s.charge) / drift_den;
-                const double vdx = vdp * (dbz_dy * ib2 - 2.0 * bz * db_dy * ib3);
-                const double vdy = vdp * (-dbz_dx * ib2 + 2.0 * bz * db_dx * ib3);
-                dx_dt = vx + vdx + dkxx_dx + dkxy_dy;
-                dy_dt = vy + vdy + dkxy_dx + dkyy_dy;
-                dp_dt = -p * (dvx_dx + dvy_dy) / 3.0;
-            }
-            (void)dvz_dx;
-
-            // adaptive step (src/main.cpp:1243-1277, :1603-1665)
-            if (FOCUSED) {
-                const double ds = skperp > 0.0 ? skperp : skpara;
-                if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0 && dmu_dt != 0.0 && ds > 0.0) {
-                    dt = fmin(sqr(0.5 * s.dx / ds), sqr(0.5 * s.dy / ds));
-                    dt = fmin(dt, sqr(ds / dx_dt));
-                    dt = fmin(dt, sqr(ds / dy_dt));
-                    dt = fmin(dt, 0.1 * p / fabs(dp_dt));
-                    dt = fmin(dt, 0.1 / fabs(dmu_dt));
-                    dt = fmin(dt, 2.0 * duu / sqr(dmu_dt));
                 } else {
-                    dt = s.dt_min;
+                    ok = eval_terms<MODEL, FOCUSED, QLT, double>(c, p, pitch_mu, s, a.qlt_norm, T);
                 }
-            } else if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
-                const double as = skperp > 0.0 ? skperp : skpara;
-                dt = fmin(sqr(0.5 * s.dx / skpara), sqr(0.5 * s.dy / skpara));
-                dt = fmin(dt, sqr(as / dx_dt));
-                dt = fmin(dt, sqr(as / dy_dt));
-                dt = fmin(dt, 0.1 * p / fabs(dp_dt));
-            } else {
-                dt = s.dt_min;
             }
+            if (!ok) {
+                status = 2;
+                break;
+            }
+
+            dt = candidate_dt<FOCUSED>(T, p, s);
             if (dt < s.dt_min) dt = s.dt_min;
             if (dt > s.dt_max) dt = s.dt_max;
             diagnostic_dt = dt;
@@ -502,7 +631,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                 break;
             }
 
-            double r0, r1, r2, r3;
+            FT r0, r1, r2, r3;
             const uint32_t k = static_cast<uint32_t>(step_guard - 1);
             if (a.noise != nullptr) {
                 if (k >= a.noise_steps) {
@@ -510,65 +639,71 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
                     break;
                 }
                 const double* row = a.noise + (static_cast<uint64_t>(idx) * a.noise_steps + k) * 4;
-                r0 = row[0];
-                r1 = row[1];
-                r2 = row[2];
-                r3 = row[3];
+                r0 = static_cast<FT>(row[0]);
+                r1 = static_cast<FT>(row[1]);
+                r2 = static_cast<FT>(row[2]);
+                r3 = static_cast<FT>(row[3]);
             } else {
                 uint32_t rb[4];
                 philox4x32_10(static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), a.frame, k,
                               a.key[0], a.key[1], rb);
-                r0 = -kSqrt3 + (kSqrt3 - -kSqrt3) * u01(rb[0]);
-                r1 = -kSqrt3 + (kSqrt3 - -kSqrt3) * u01(rb[1]);
-                r2 = -kSqrt3 + (kSqrt3 - -kSqrt3) * u01(rb[2]);
-                r3 = -kSqrt3 + (kSqrt3 - -kSqrt3) * u01(rb[3]);
+                r0 = draw_sqrt3<FT>(rb[0]);
+                r1 = draw_sqrt3<FT>(rb[1]);
+                r2 = draw_sqrt3<FT>(rb[2]);
+                r3 = draw_sqrt3<FT>(rb[3]);
             }
-            const double sdt = sqrt(dt);
-            double delta_x, delta_y;
+            const FT dtr = static_cast<FT>(dt);
+            const FT sdt = rsqrt_(dtr);
+            FT delta_x, delta_y;
             if (FOCUSED) {
                 // src/main.cpp:1671-1702 (ran1, ran2, ignored, pitch)
-                const double bxn = bx * ib, byn = by * ib, bzn = bz * ib;
-                const double bxy = sqrt(bxn * bxn + byn * byn);
-                double sx = 0.0, sy = 0.0;
-                if (bxy > 0.0) {
-                    const double ibxy = 1.0 / bxy;
-                    sx = skperp * ibxy * sdt * (-bxn * bzn * r0 - by * r1);
-                    sy = skperp * ibxy * sdt * (-byn * bzn * r0 + bx * r1);
+                const FT bxn = T.bx * T.ib, byn = T.by * T.ib, bzn = T.bz * T.ib;
+                const FT bxy = rsqrt_(bxn * bxn + byn * byn);
+                FT sx = FT(0), sy = FT(0);
+                if (bxy > FT(0)) {
+                    const FT ibxy = FT(1) / bxy;
+                    sx = T.skperp * ibxy * sdt * (-bxn * bzn * r0 - T.by * r1);
+                    sy = T.skperp * ibxy * sdt * (-byn * bzn * r0 + T.bx * r1);
                 }
-                delta_x = dx_dt * dt + sx;
-                delta_y = dy_dt * dt + sy;
+                delta_x = T.dx_dt * dtr + sx;
+                delta_y = T.dy_dt * dtr + sy;
                 if constexpr (QLT) {
                     // FocusTransportSolver::picard_pitch_angle_update (FocusSolver.hpp:700-719):
                     // three passes, drift and diffusion at the trial mu, one noise value.
                     const double mu0 = pitch_mu;
-                    double rate = dmu_dt, D = duu, trial = mu0;
+                    FT rate = T.dmu_dt, D = T.duu;
+                    double trial = mu0;
 #pragma unroll 1
                     for (int it = 0; it < 3; ++it) {
                         if (it > 0) {
-                            const double m = reflect_mu(trial, s.mu_max);
-                            double dD;
-                            qlt_scattering(m, q_xi, q_xig, q_pref, s.qlt_spectral_index, s.qlt_h0, D, dD);
-                            rate = 0.5 * (1.0 - m * m) *
-                                       (f_speed * f_divb + m * f_divv - 3.0 * m * f_bb - 2.0 * f_bv / f_speed) +
+                            const FT m = static_cast<FT>(reflect_mu(trial, s.mu_max));
+                            FT dD;
+                            qlt_scattering<FT>(m, T.q_xi, T.q_xig, T.q_pref, static_cast<FT>(s.qlt_spectral_index),
+                                               static_cast<FT>(s.qlt_h0), D, dD);
+                            rate = FT(0.5) * (FT(1) - m * m) *
+                                       (T.f_speed * T.f_divb + m * T.f_divv - FT(3) * m * T.f_bb -
+                                        FT(2) * T.f_bv / T.f_speed) +
                                    dD;
                         }
-                        const double scale = D > 0.0 ? sqrt(2.0 * D * dt) : 0.0;
-                        trial = reflect_mu(mu0 + dt * rate + scale * r3, s.mu_max);
+                        const FT scale = D > FT(0) ? rsqrt_(FT(2) * D * dtr) : FT(0);
+                        trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3),
+                                           s.mu_max);
                     }
                     pitch_mu = trial;
                 } else {
-                    const double m = pitch_mu + dmu_dt * dt + r3 * sqrt(2.0 * duu) * sdt;
+                    const double m = pitch_mu + static_cast<double>(T.dmu_dt * dtr) +
+                                     static_cast<double>(r3 * rsqrt_(FT(2) * T.duu) * sdt);
                     pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
                 }
             } else {
                 // src/main.cpp:1279-1292 (ran1, ran2, ran3, ignored)
-                delta_x = dx_dt * dt + r0 * skperp * sdt + r2 * skpara_perp * sdt * bx * ib;
-                delta_y = dy_dt * dt + r1 * skperp * sdt + r2 * skpara_perp * sdt * by * ib;
+                delta_x = T.dx_dt * dtr + r0 * T.skperp * sdt + r2 * T.skpara_perp * sdt * T.bx * T.ib;
+                delta_y = T.dy_dt * dtr + r1 * T.skperp * sdt + r2 * T.skpara_perp * sdt * T.by * T.ib;
             }
             (void)r3;
-            x += delta_x;
-            y += delta_y;
-            p += dp_dt * dt;
+            x += static_cast<double>(delta_x);
+            y += static_cast<double>(delta_y);
+            p += T.dp_dt * dt;
             t += dt;
             ++my_steps;
             if (p < 0.25 * s.p0) p = 0.25 * s.p0;
@@ -599,7 +734,6 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs a) {
         atomicAdd(&a.counters[2], static_cast<unsigned long long>(act));
     }
 }
-
 // ---------------------------------------------------------------------------
 // Injection (src/main.cpp:1021-1044): x, y, t, mu from one Philox block with
 // counter (root id, 0xFFFFFFFF, frame, root id >> 32).

@@ -59,6 +59,7 @@ class DeviceSettings(C.Structure):
         ("closure", C.c_int32), ("reserved1", C.c_int32),
         ("qlt_sigma2", C.c_double), ("qlt_correlation_length", C.c_double),
         ("qlt_spectral_index", C.c_double), ("qlt_h0", C.c_double),
+        ("fp64_null_b", C.c_double),
     ]
 
     def copy(self) -> "DeviceSettings":
