@@ -31,6 +31,7 @@ struct FieldSlot {
     float* c32 = nullptr;
     size_t nodes_cap = 0;
     size_t raw_cap = 0;
+    uint64_t version = 0;  // changes with every upload into the slot
 };
 
 }  // namespace
@@ -45,6 +46,12 @@ struct kpt_ctx {
     uint64_t next_root = 0;
     kpt::ParticleArrays P{};
     FieldSlot slot[2];
+    uint64_t field_versions = 0;
+    // FP32 frame-interleaved pair of the lower and upper frames (built by the
+    // FP32 time-blended push when either slot changed)
+    float* pair32 = nullptr;
+    size_t pair32_cap = 0;
+    uint64_t pair_versions[2] = {0, 0};
     double* noise = nullptr;
     uint64_t noise_particles = 0;
     uint32_t noise_steps = 0;
@@ -288,6 +295,7 @@ kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, i
     f.model = model;
     KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    f.version = ++ctx->field_versions;
     f.valid = true;
     return KPT_OK;
 }
@@ -402,6 +410,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.c64);
         dev_free(f.c32);
     }
+    dev_free(ctx->pair32);
     if (ctx->p2_allocated) {
         kpt::ParticleArrays& Q = ctx->P2;
         for (void* p : {static_cast<void*>(Q.x), static_cast<void*>(Q.y), static_cast<void*>(Q.p),
@@ -729,6 +738,24 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
                                             : a.field_lo;
     a.field_lo64 = lo.c64;
     a.field_hi64 = s->time_interpolation != 0 ? hi.c64 : lo.c64;
+    a.field_pair = nullptr;
+    if (fp32 && s->time_interpolation != 0) {
+        const size_t elems = kpt::coefficient_plane(lo.g.nx, lo.g.ny) * static_cast<size_t>(lo.model);
+        if (2 * elems > ctx->pair32_cap) {
+            dev_free(ctx->pair32);
+            ctx->pair32 = nullptr;
+            ctx->pair32_cap = 0;
+            ctx->pair_versions[0] = ctx->pair_versions[1] = 0;
+            KPT_HIP(ctx, dev_alloc(&ctx->pair32, 2 * elems));
+            ctx->pair32_cap = 2 * elems;
+        }
+        if (ctx->pair_versions[0] != lo.version || ctx->pair_versions[1] != hi.version) {
+            KPT_HIP(ctx, kpt::launch_pair(lo.c32, hi.c32, elems, ctx->pair32, ctx->stream));
+            ctx->pair_versions[0] = lo.version;
+            ctx->pair_versions[1] = hi.version;
+        }
+        a.field_pair = ctx->pair32;
+    }
     a.plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
     a.g = lo.g;
     a.s = *s;

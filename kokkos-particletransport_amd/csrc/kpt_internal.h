@@ -50,6 +50,7 @@ struct PushArgs {
     uint64_t count;
     const void* field_lo;  // double or float, component-planar (see coefficient_plane)
     const void* field_hi;
+    const float* field_pair;   // FP32 time-blended push: lower/upper interleaved, or nullptr
     const double* field_lo64;  // FP64 frames, read by FP32 lanes below fp64_null_b
     const double* field_hi64;
     uint64_t plane;
@@ -71,6 +72,8 @@ struct PushArgs {
 hipError_t launch_pack(const double* raw /* 6 planes nx*ny */, const GridDesc& g, int model,
                        double* out64, float* out32, hipStream_t stream);
 hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream);
+// out[2 e + 0] = lo[e], out[2 e + 1] = hi[e] for e < n
+hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream);
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);

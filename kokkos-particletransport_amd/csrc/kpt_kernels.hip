@@ -231,6 +231,8 @@ __device__ __forceinline__ double rsqrt_(double v) { return sqrt(v); }
 __device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ double rrcp(double v) { return 1.0 / v; }
 __device__ __forceinline__ float rrcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double rpow(double a, double e) { return pow(a, e); }
 __device__ __forceinline__ float rpow(float a, float e) {
     // a^e = 2^(e log2 a) for normal positive a; anything else takes powf.
@@ -285,11 +287,21 @@ __device__ __forceinline__ void qlt_scattering(RT mu, RT xi, RT xi_g, RT pref, R
     const RT mu_h = rabs(mu) + h0;
     const RT pm1 = rpow(mu_h, gk - RT(1));  // mu_h^(gamma_k - 1)
     const RT den = pm1 * mu_h + xi_g;       // mu_h^gamma_k + xi^gamma_k
-    D = pref * one_minus_mu2 * xi * pm1 / den;
+    D = rdiv(pref * one_minus_mu2 * xi * pm1, den);
     if (D == RT(0) || mu == RT(0)) return;
     const RT sign_mu = mu > RT(0) ? RT(1) : RT(-1);
-    const RT shape = sign_mu * ((gk - RT(1)) / mu_h - gk * pm1 / den);
-    dD = D * (RT(-2) * mu / one_minus_mu2 + shape);
+    const RT shape = sign_mu * (rdiv(gk - RT(1), mu_h) - rdiv(gk * pm1, den));
+    dD = D * (rdiv(RT(-2) * mu, one_minus_mu2) + shape);
+}
+
+// p / p0: double division in the FP64 path, a multiply by the hoisted 1 / p0 in
+// the FP32 path.
+template <typename RT>
+__device__ __forceinline__ RT p_ratio(double p, double p0) {
+    if constexpr (sizeof(RT) == 4)
+        return static_cast<RT>(p * (1.0 / p0));
+    else
+        return p / p0;
 }
 
 // Terms of one sub-step at the particle's sampled coefficients.
@@ -344,7 +356,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
 
     RT knorm = RT(1);
     if (s.magnetic_dependency == 1) knorm *= rpow(b, static_cast<RT>(s.gamma_turb - 2.0));
-    if (s.momentum_dependency == 1) knorm *= rpow(static_cast<RT>(p_d / s.p0), static_cast<RT>(s.pindex));
+    if (s.momentum_dependency == 1) knorm *= rpow(p_ratio<RT>(p_d, s.p0), static_cast<RT>(s.pindex));
     const RT kpara = static_cast<RT>(s.kpara0) * knorm;
     const RT kperp = kpara * static_cast<RT>(s.kperp_over_kpara);
     if (!rfinite(kpara) || kpara <= RT(0) || !rfinite(kperp) || kperp < RT(0)) return false;
@@ -368,8 +380,8 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
                        kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - RT(2) * bx * by * db_dx * ib3);
     const RT dkxy_dy = kpp * dkdy * bx * by * ib2 +
                        kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - RT(2) * bx * by * db_dy * ib3);
-    const RT drift_den = rsqrt_(sqr(static_cast<RT>(s.drift_param1 * s.p0) / p) +
-                                sqr(static_cast<RT>(s.drift_param2 * s.p0 * s.p0) / (p * p)));
+    const RT drift_den = rsqrt_(sqr(rdiv(static_cast<RT>(s.drift_param1 * s.p0), p)) +
+                                sqr(rdiv(static_cast<RT>(s.drift_param2 * s.p0 * s.p0), p * p)));
 
     o.bx = bx;
     o.by = by;
@@ -387,14 +399,14 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
         const RT ky = bx * dby_dx + by * dby_dy;
         const RT kz = bx * dbz_dx + by * dbz_dy;
         const RT bdot_curvb = bx * dbz_dy - by * dbz_dx + bz * (dby_dx - dbx_dy);
-        const RT vdp = static_cast<RT>(1.0 / s.charge) / drift_den;
+        const RT vdp = rdiv(static_cast<RT>(1.0 / s.charge), drift_den);
         const RT vdx = vdp * (muf1 * (-bz * db_dy) * ib2 + mu2 * (by * kz - bz * ky) * ib3 +
                               muf1 * bx * bdot_curvb * ib3);
         const RT vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
                               muf1 * by * bdot_curvb * ib3);
         // QLT: relativistic speed p / gamma (m = c = 1)
         const RT gamma_p = QLT ? rsqrt_(RT(1) + p * p) : RT(1);
-        const RT speed = QLT ? p / gamma_p : static_cast<RT>(s.particle_v0) * p / static_cast<RT>(s.p0);
+        const RT speed = QLT ? rdiv(p, gamma_p) : rdiv(static_cast<RT>(s.particle_v0) * p, static_cast<RT>(s.p0));
         if (!rfinite(speed) || speed <= RT(0)) return false;
         const RT vpb = speed * mu * ib;
         o.dx_dt = vx + vdx + vpb * bx + dkxx_dx + dkxy_dy;
@@ -404,9 +416,10 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
                              bz * (bx * dvz_dx + by * dvz_dy)) * ib2;
         const RT bv_gradv = (bx * (vx * dvx_dx + vy * dvx_dy) + by * (vx * dvy_dx + vy * dvy_dy) +
                              bz * (vx * dvz_dx + vy * dvz_dy)) * ib;
-        o.dp_dt = p_d * -static_cast<double>(muf1 * divv + muf2 * bb_gradv + mu * bv_gradv / speed);
+        const RT bv_over_v = rdiv(bv_gradv, speed);
+        o.dp_dt = p_d * -static_cast<double>(muf1 * divv + muf2 * bb_gradv + mu * bv_over_v);
         const RT div_bnorm = -(bx * db_dx + by * db_dy) * ib2;
-        RT dmu_dt = speed * div_bnorm + mu * divv - RT(3) * mu * bb_gradv - RT(2) * bv_gradv / speed;
+        RT dmu_dt = speed * div_bnorm + mu * divv - RT(3) * mu * bb_gradv - RT(2) * bv_over_v;
         dmu_dt *= RT(0.5) * (RT(1) - mu2);
         RT duu, duu_du = RT(0);
         if constexpr (QLT) {
@@ -415,8 +428,8 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
             o.f_bb = bb_gradv;
             o.f_bv = bv_gradv;
             o.f_divb = div_bnorm;
-            const RT omega = static_cast<RT>(fabs(s.charge)) * b / gamma_p;
-            o.q_xi = omega * static_cast<RT>(s.qlt_correlation_length) / speed;
+            const RT omega = rdiv(static_cast<RT>(fabs(s.charge)) * b, gamma_p);
+            o.q_xi = rdiv(omega * static_cast<RT>(s.qlt_correlation_length), speed);
             o.q_xig = rpow(o.q_xi, static_cast<RT>(s.qlt_spectral_index));
             o.q_pref = static_cast<RT>(0.25 * M_PI) * omega * static_cast<RT>(s.qlt_sigma2) *
                        static_cast<RT>(qlt_norm);
@@ -434,7 +447,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
             RT duu_norm = RT(1);
             if (s.magnetic_dependency == 1) duu_norm *= rpow(b, static_cast<RT>(2.0 - s.gamma_turb));
             if (s.momentum_dependency == 1)
-                duu_norm *= rpow(static_cast<RT>(p_d / s.p0), static_cast<RT>(s.gamma_turb - 1.0));
+                duu_norm *= rpow(p_ratio<RT>(p_d, s.p0), static_cast<RT>(s.gamma_turb - 1.0));
             duu *= duu_norm;
             duu_du *= duu_norm;
         }
@@ -443,12 +456,15 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
         o.dmu_dt = dmu_dt;
         o.duu = duu;
     } else {
-        const RT vdp = static_cast<RT>(1.0 / (3.0 * s.charge)) / drift_den;
+        const RT vdp = rdiv(static_cast<RT>(1.0 / (3.0 * s.charge)), drift_den);
         const RT vdx = vdp * (dbz_dy * ib2 - RT(2) * bz * db_dy * ib3);
         const RT vdy = vdp * (-dbz_dx * ib2 + RT(2) * bz * db_dx * ib3);
         o.dx_dt = vx + vdx + dkxx_dx + dkxy_dy;
         o.dy_dt = vy + vdy + dkxy_dx + dkyy_dy;
-        o.dp_dt = -p_d * static_cast<double>(dvx_dx + dvy_dy) / 3.0;
+        if constexpr (sizeof(RT) == 4)
+            o.dp_dt = -p_d * static_cast<double>((dvx_dx + dvy_dy) * RT(1.0 / 3.0));
+        else
+            o.dp_dt = -p_d * static_cast<double>(dvx_dx + dvy_dy) / 3.0;
     }
     return true;
 }
@@ -460,20 +476,20 @@ __device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p,
     if (FOCUSED) {
         const RT ds = T.skperp > RT(0) ? T.skperp : T.skpara;
         if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0 && T.dmu_dt != RT(0) && ds > RT(0)) {
-            RT d = rmin(sqr(static_cast<RT>(0.5 * s.dx) / ds), sqr(static_cast<RT>(0.5 * s.dy) / ds));
-            d = rmin(d, sqr(ds / T.dx_dt));
-            d = rmin(d, sqr(ds / T.dy_dt));
-            d = rmin(d, RT(0.1) / rabs(T.dmu_dt));
-            d = rmin(d, RT(2) * T.duu / sqr(T.dmu_dt));
+            RT d = rmin(sqr(rdiv(static_cast<RT>(0.5 * s.dx), ds)), sqr(rdiv(static_cast<RT>(0.5 * s.dy), ds)));
+            d = rmin(d, sqr(rdiv(ds, T.dx_dt)));
+            d = rmin(d, sqr(rdiv(ds, T.dy_dt)));
+            d = rmin(d, rdiv(RT(0.1), rabs(T.dmu_dt)));
+            d = rmin(d, rdiv(RT(2) * T.duu, sqr(T.dmu_dt)));
             return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
         }
         return s.dt_min;
     }
     if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0) {
         const RT as = T.skperp > RT(0) ? T.skperp : T.skpara;
-        RT d = rmin(sqr(static_cast<RT>(0.5 * s.dx) / T.skpara), sqr(static_cast<RT>(0.5 * s.dy) / T.skpara));
-        d = rmin(d, sqr(as / T.dx_dt));
-        d = rmin(d, sqr(as / T.dy_dt));
+        RT d = rmin(sqr(rdiv(static_cast<RT>(0.5 * s.dx), T.skpara)), sqr(rdiv(static_cast<RT>(0.5 * s.dy), T.skpara)));
+        d = rmin(d, sqr(rdiv(as, T.dx_dt)));
+        d = rmin(d, sqr(rdiv(as, T.dy_dt)));
         return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
     }
     return s.dt_min;
@@ -496,6 +512,27 @@ __device__ __forceinline__ void sample_blend(const T* __restrict__ flo, const T*
         const T uwt = static_cast<T>(uw), lwt = static_cast<T>(lw);
 #pragma unroll
         for (int k = 0; k < C; ++k) c[k] = lwt * c[k] + uwt * hi[k];
+    }
+}
+
+// FP32 time-blended sample from the frame-interleaved pair buffer
+// pair[2 (k plane + node) + f], f = 0 lower, 1 upper: one row of the stencil,
+// (lo_i, hi_i, lo_i+1, hi_i+1), is one 16-byte load; the bilinear and time
+// weights are folded into eight products.
+template <int C>
+__device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, int64_t n00, int nxp,
+                                              uint64_t plane, double uw, double wx, double wy, float* c) {
+    const double lw = 1.0 - uw;
+    const float a00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * lw), b00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * uw);
+    const float a10 = static_cast<float>(wx * (1.0 - wy) * lw), b10 = static_cast<float>(wx * (1.0 - wy) * uw);
+    const float a01 = static_cast<float>((1.0 - wx) * wy * lw), b01 = static_cast<float>((1.0 - wx) * wy * uw);
+    const float a11 = static_cast<float>(wx * wy * lw), b11 = static_cast<float>(wx * wy * uw);
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const float* q = pair + 2 * (k * plane + n00);
+        const float* r = q + 2 * nxp;
+        c[k] = a00 * q[0] + b00 * q[1] + a10 * q[2] + b10 * q[3] + a01 * r[0] + b01 * r[1] + a11 * r[2] +
+               b11 * r[3];
     }
 }
 
@@ -590,7 +627,7 @@ __global__ void __launch_bounds__(kBlock)
             const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
             double uw = 0.0;
             if (blend) {
-                const double rt = (t - interval_start) / s.mhd_dt;
+                const double rt = FP32 ? (t - interval_start) * (1.0 / s.mhd_dt) : (t - interval_start) / s.mhd_dt;
                 uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
             }
 
@@ -598,7 +635,14 @@ __global__ void __launch_bounds__(kBlock)
             bool ok;
             {
                 FT c[CP];
-                sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                if constexpr (FP32) {
+                    if (a.field_pair != nullptr)
+                        sample_pair32<CP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                    else
+                        sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                } else {
+                    sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                }
                 if constexpr (FP32) {
                     if (!(c[MODEL == 16 ? 5 : 6] >= static_cast<float>(s.fp64_null_b))) {
                         // near a null: the FP64 frames and FP64 arithmetic for this sub-step
@@ -620,9 +664,16 @@ __global__ void __launch_bounds__(kBlock)
                 break;
             }
 
-            dt = candidate_dt<FOCUSED>(T, p, s);
-            if (dt < s.dt_min) dt = s.dt_min;
-            if (dt > s.dt_max) dt = s.dt_max;
+            if (s.dt_min < s.dt_max) {
+                dt = candidate_dt<FOCUSED>(T, p, s);
+                if (dt < s.dt_min) dt = s.dt_min;
+                if (dt > s.dt_max) dt = s.dt_max;
+            } else {
+                // dt_min == dt_max: the clamp fixes the step whatever the
+                // candidates are (they are never NaN: the first is finite and
+                // fmin drops NaN operands).
+                dt = s.dt_min;
+            }
             diagnostic_dt = dt;
             const double remaining = interval_end - t;
             if (dt > remaining) dt = remaining;
@@ -661,7 +712,7 @@ __global__ void __launch_bounds__(kBlock)
                 const FT bxy = rsqrt_(bxn * bxn + byn * byn);
                 FT sx = FT(0), sy = FT(0);
                 if (bxy > FT(0)) {
-                    const FT ibxy = FT(1) / bxy;
+                    const FT ibxy = rrcp(bxy);
                     sx = T.skperp * ibxy * sdt * (-bxn * bzn * r0 - T.by * r1);
                     sy = T.skperp * ibxy * sdt * (-byn * bzn * r0 + T.bx * r1);
                 }
@@ -682,7 +733,7 @@ __global__ void __launch_bounds__(kBlock)
                                                static_cast<FT>(s.qlt_h0), D, dD);
                             rate = FT(0.5) * (FT(1) - m * m) *
                                        (T.f_speed * T.f_divb + m * T.f_divv - FT(3) * m * T.f_bb -
-                                        FT(2) * T.f_bv / T.f_speed) +
+                                        FT(2) * rdiv(T.f_bv, T.f_speed)) +
                                    dD;
                         }
                         const FT scale = D > FT(0) ? rsqrt_(FT(2) * D * dtr) : FT(0);
@@ -1057,6 +1108,20 @@ hipError_t launch_pack(const double* raw, const GridDesc& g, int model, double* 
     const uint64_t nodes = static_cast<uint64_t>(g.nx + 2) * (g.ny + 2);
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for(nodes)), dim3(kBlock), 0, stream, raw, g, model,
                        coefficient_plane(g.nx, g.ny), out64, out32);
+    return hipGetLastError();
+}
+
+// Frame-interleaved FP32 pair of the two coefficient frames of an interval.
+__global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ lo, const float* __restrict__ hi,
+                                                      uint64_t n, float2* __restrict__ out) {
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    if (e < n) out[e] = make_float2(lo[e], hi[e]);
+}
+
+hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                       lo, hi, n, reinterpret_cast<float2*>(out));
     return hipGetLastError();
 }
 
