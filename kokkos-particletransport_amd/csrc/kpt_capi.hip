@@ -768,6 +768,8 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.noise_steps = ctx->noise_steps;
     a.counters = ctx->counters;
     a.qlt_norm = s->qlt_spectral_index * std::sin(M_PI / s->qlt_spectral_index) / M_PI;
+    a.k64 = kpt::make_step_const<double>(*s, a.qlt_norm);
+    a.k32 = kpt::make_step_const<float>(*s, a.qlt_norm);
     (void)hipSetDevice(ctx->device);
     const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
     const int chunks = s->push_chunks > 1 ? s->push_chunks : 1;

@@ -45,6 +45,54 @@ struct ParticleArrays {
     uint32_t* substep;  // sub-steps taken in the current interval (push chunks only)
 };
 
+// Constants of the sub-step arithmetic in its own type, formed once on the host
+// (kpt_push_interval) so that the float push reads 32-bit scalars directly.
+template <typename RT>
+struct StepConst {
+    RT kpara0, kperp_ratio;
+    RT exp_b_kappa;        // gamma_turb - 2: kappa ~ B^(gamma-2)
+    RT pindex;             // kappa ~ (p/p0)^pindex
+    RT drift_c1, drift_c2; // drift_param1 p0, drift_param2 p0^2
+    RT inv_q, inv_3q, abs_q;
+    RT v0, p0;
+    RT half_dx, half_dy;
+    RT duu0, duu_h0;       // reduced closure
+    RT exp_mu1, exp_mu2;   // gamma_turb - 1, gamma_turb - 2
+    RT exp_b_duu;          // 2 - gamma_turb
+    RT qlt_lc, qlt_gk, qlt_h0;
+    RT qlt_pref;           // (pi / 4) sigma2 gamma_k sin(pi / gamma_k) / pi
+    RT null_b;             // fp64_null_b
+};
+
+template <typename RT>
+inline StepConst<RT> make_step_const(const kpt_device_settings& s, double qlt_norm) {
+    StepConst<RT> k;
+    k.kpara0 = static_cast<RT>(s.kpara0);
+    k.kperp_ratio = static_cast<RT>(s.kperp_over_kpara);
+    k.exp_b_kappa = static_cast<RT>(s.gamma_turb - 2.0);
+    k.pindex = static_cast<RT>(s.pindex);
+    k.drift_c1 = static_cast<RT>(s.drift_param1 * s.p0);
+    k.drift_c2 = static_cast<RT>(s.drift_param2 * s.p0 * s.p0);
+    k.inv_q = static_cast<RT>(1.0 / s.charge);
+    k.inv_3q = static_cast<RT>(1.0 / (3.0 * s.charge));
+    k.abs_q = static_cast<RT>(s.charge < 0.0 ? -s.charge : s.charge);
+    k.v0 = static_cast<RT>(s.particle_v0);
+    k.p0 = static_cast<RT>(s.p0);
+    k.half_dx = static_cast<RT>(0.5 * s.dx);
+    k.half_dy = static_cast<RT>(0.5 * s.dy);
+    k.duu0 = static_cast<RT>(s.duu0);
+    k.duu_h0 = static_cast<RT>(s.pitch_angle_scattering_h0);
+    k.exp_mu1 = static_cast<RT>(s.gamma_turb - 1.0);
+    k.exp_mu2 = static_cast<RT>(s.gamma_turb - 2.0);
+    k.exp_b_duu = static_cast<RT>(2.0 - s.gamma_turb);
+    k.qlt_lc = static_cast<RT>(s.qlt_correlation_length);
+    k.qlt_gk = static_cast<RT>(s.qlt_spectral_index);
+    k.qlt_h0 = static_cast<RT>(s.qlt_h0);
+    k.qlt_pref = static_cast<RT>(0.25 * 3.14159265358979323846 * s.qlt_sigma2 * qlt_norm);
+    k.null_b = static_cast<RT>(s.fp64_null_b);
+    return k;
+}
+
 struct PushArgs {
     ParticleArrays P;
     uint64_t count;
@@ -65,6 +113,8 @@ struct PushArgs {
     const double* noise;  // injected draw table or nullptr
     uint32_t noise_steps;
     double qlt_norm;      // gamma_k sin(pi / gamma_k) / pi (pitch_angle_spectrum_normalization)
+    StepConst<double> k64;
+    StepConst<float> k32;
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
 };
 

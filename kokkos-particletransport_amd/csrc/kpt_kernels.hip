@@ -332,7 +332,7 @@ __device__ __forceinline__ void narrow_terms(const StepTerms<S>& a, StepTerms<D>
 // particle failed (status 2).
 template <int MODEL, bool FOCUSED, bool QLT, typename RT>
 __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d, const kpt_device_settings& s,
-                                           double qlt_norm, StepTerms<RT>& o) {
+                                           const StepConst<RT>& k, StepTerms<RT>& o) {
     RT vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = RT(0), dvy_dx = RT(0), dvy_dy, dvz_dx = RT(0), dvz_dy = RT(0),
                                   dbx_dx, dbx_dy, dby_dx, dby_dy, dbz_dx, dbz_dy, db_dx, db_dy;
     if constexpr (MODEL == 16) {
@@ -355,10 +355,10 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
     const RT ib3 = ib2 * ib;
 
     RT knorm = RT(1);
-    if (s.magnetic_dependency == 1) knorm *= rpow(b, static_cast<RT>(s.gamma_turb - 2.0));
-    if (s.momentum_dependency == 1) knorm *= rpow(p_ratio<RT>(p_d, s.p0), static_cast<RT>(s.pindex));
-    const RT kpara = static_cast<RT>(s.kpara0) * knorm;
-    const RT kperp = kpara * static_cast<RT>(s.kperp_over_kpara);
+    if (s.magnetic_dependency == 1) knorm *= rpow(b, k.exp_b_kappa);
+    if (s.momentum_dependency == 1) knorm *= rpow(p_ratio<RT>(p_d, s.p0), k.pindex);
+    const RT kpara = k.kpara0 * knorm;
+    const RT kperp = kpara * k.kperp_ratio;
     if (!rfinite(kpara) || kpara <= RT(0) || !rfinite(kperp) || kperp < RT(0)) return false;
     o.skpara = rsqrt_(RT(2) * kpara);
     o.skperp = rsqrt_(RT(2) * kperp);
@@ -368,7 +368,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
 
     RT dkdx = RT(0), dkdy = RT(0);
     if (s.magnetic_dependency == 1) {
-        const RT me = static_cast<RT>(s.gamma_turb - 2.0);
+        const RT me = k.exp_b_kappa;
         dkdx = db_dx * ib * me;
         dkdy = db_dy * ib * me;
     }
@@ -380,8 +380,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
                        kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - RT(2) * bx * by * db_dx * ib3);
     const RT dkxy_dy = kpp * dkdy * bx * by * ib2 +
                        kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - RT(2) * bx * by * db_dy * ib3);
-    const RT drift_den = rsqrt_(sqr(rdiv(static_cast<RT>(s.drift_param1 * s.p0), p)) +
-                                sqr(rdiv(static_cast<RT>(s.drift_param2 * s.p0 * s.p0), p * p)));
+    const RT drift_den = rsqrt_(sqr(rdiv(k.drift_c1, p)) + sqr(rdiv(k.drift_c2, p * p)));
 
     o.bx = bx;
     o.by = by;
@@ -399,14 +398,14 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
         const RT ky = bx * dby_dx + by * dby_dy;
         const RT kz = bx * dbz_dx + by * dbz_dy;
         const RT bdot_curvb = bx * dbz_dy - by * dbz_dx + bz * (dby_dx - dbx_dy);
-        const RT vdp = rdiv(static_cast<RT>(1.0 / s.charge), drift_den);
+        const RT vdp = rdiv(k.inv_q, drift_den);
         const RT vdx = vdp * (muf1 * (-bz * db_dy) * ib2 + mu2 * (by * kz - bz * ky) * ib3 +
                               muf1 * bx * bdot_curvb * ib3);
         const RT vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
                               muf1 * by * bdot_curvb * ib3);
         // QLT: relativistic speed p / gamma (m = c = 1)
         const RT gamma_p = QLT ? rsqrt_(RT(1) + p * p) : RT(1);
-        const RT speed = QLT ? rdiv(p, gamma_p) : rdiv(static_cast<RT>(s.particle_v0) * p, static_cast<RT>(s.p0));
+        const RT speed = QLT ? rdiv(p, gamma_p) : rdiv(k.v0 * p, k.p0);
         if (!rfinite(speed) || speed <= RT(0)) return false;
         const RT vpb = speed * mu * ib;
         o.dx_dt = vx + vdx + vpb * bx + dkxx_dx + dkxy_dy;
@@ -428,26 +427,22 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
             o.f_bb = bb_gradv;
             o.f_bv = bv_gradv;
             o.f_divb = div_bnorm;
-            const RT omega = rdiv(static_cast<RT>(fabs(s.charge)) * b, gamma_p);
-            o.q_xi = rdiv(omega * static_cast<RT>(s.qlt_correlation_length), speed);
-            o.q_xig = rpow(o.q_xi, static_cast<RT>(s.qlt_spectral_index));
-            o.q_pref = static_cast<RT>(0.25 * M_PI) * omega * static_cast<RT>(s.qlt_sigma2) *
-                       static_cast<RT>(qlt_norm);
-            qlt_scattering<RT>(mu, o.q_xi, o.q_xig, o.q_pref, static_cast<RT>(s.qlt_spectral_index),
-                               static_cast<RT>(s.qlt_h0), duu, duu_du);
+            const RT omega = rdiv(k.abs_q * b, gamma_p);
+            o.q_xi = rdiv(omega * k.qlt_lc, speed);
+            o.q_xig = rpow(o.q_xi, k.qlt_gk);
+            o.q_pref = omega * k.qlt_pref;
+            qlt_scattering<RT>(mu, o.q_xi, o.q_xig, o.q_pref, k.qlt_gk, k.qlt_h0, duu, duu_du);
         } else {
             const RT abs_mu = rabs(mu);
-            const RT dtmp = rpow(abs_mu, static_cast<RT>(s.gamma_turb - 1.0)) +
-                            static_cast<RT>(s.pitch_angle_scattering_h0);
-            duu = static_cast<RT>(s.duu0) * (RT(1) - mu2) * dtmp;
+            const RT dtmp = rpow(abs_mu, k.exp_mu1) + k.duu_h0;
+            duu = k.duu0 * (RT(1) - mu2) * dtmp;
             if (mu != RT(0)) {
-                const RT g2 = (RT(1) - mu2) * rpow(abs_mu, static_cast<RT>(s.gamma_turb - 2.0));
-                duu_du = static_cast<RT>(s.duu0) * (RT(-2) * mu * dtmp + (mu > RT(0) ? g2 : -g2));
+                const RT g2 = (RT(1) - mu2) * rpow(abs_mu, k.exp_mu2);
+                duu_du = k.duu0 * (RT(-2) * mu * dtmp + (mu > RT(0) ? g2 : -g2));
             }
             RT duu_norm = RT(1);
-            if (s.magnetic_dependency == 1) duu_norm *= rpow(b, static_cast<RT>(2.0 - s.gamma_turb));
-            if (s.momentum_dependency == 1)
-                duu_norm *= rpow(p_ratio<RT>(p_d, s.p0), static_cast<RT>(s.gamma_turb - 1.0));
+            if (s.magnetic_dependency == 1) duu_norm *= rpow(b, k.exp_b_duu);
+            if (s.momentum_dependency == 1) duu_norm *= rpow(p_ratio<RT>(p_d, s.p0), k.exp_mu1);
             duu *= duu_norm;
             duu_du *= duu_norm;
         }
@@ -456,7 +451,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
         o.dmu_dt = dmu_dt;
         o.duu = duu;
     } else {
-        const RT vdp = rdiv(static_cast<RT>(1.0 / (3.0 * s.charge)), drift_den);
+        const RT vdp = rdiv(k.inv_3q, drift_den);
         const RT vdx = vdp * (dbz_dy * ib2 - RT(2) * bz * db_dy * ib3);
         const RT vdy = vdp * (-dbz_dx * ib2 + RT(2) * bz * db_dx * ib3);
         o.dx_dt = vx + vdx + dkxx_dx + dkxy_dy;
@@ -469,14 +464,22 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
     return true;
 }
 
+template <typename RT>
+__device__ __forceinline__ const StepConst<RT>& step_const(const PushArgs& a);
+template <>
+__device__ __forceinline__ const StepConst<double>& step_const<double>(const PushArgs& a) { return a.k64; }
+template <>
+__device__ __forceinline__ const StepConst<float>& step_const<float>(const PushArgs& a) { return a.k32; }
+
 // Adaptive step candidates (src/main.cpp:1243-1277, :1603-1665) before the
 // [dt_min, dt_max] clamp. The momentum candidate is formed in double.
 template <bool FOCUSED, typename RT>
-__device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p, const kpt_device_settings& s) {
+__device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p, const kpt_device_settings& s,
+                                               const StepConst<RT>& k) {
     if (FOCUSED) {
         const RT ds = T.skperp > RT(0) ? T.skperp : T.skpara;
         if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0 && T.dmu_dt != RT(0) && ds > RT(0)) {
-            RT d = rmin(sqr(rdiv(static_cast<RT>(0.5 * s.dx), ds)), sqr(rdiv(static_cast<RT>(0.5 * s.dy), ds)));
+            RT d = rmin(sqr(rdiv(k.half_dx, ds)), sqr(rdiv(k.half_dy, ds)));
             d = rmin(d, sqr(rdiv(ds, T.dx_dt)));
             d = rmin(d, sqr(rdiv(ds, T.dy_dt)));
             d = rmin(d, rdiv(RT(0.1), rabs(T.dmu_dt)));
@@ -487,7 +490,7 @@ __device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p,
     }
     if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0) {
         const RT as = T.skperp > RT(0) ? T.skperp : T.skpara;
-        RT d = rmin(sqr(rdiv(static_cast<RT>(0.5 * s.dx), T.skpara)), sqr(rdiv(static_cast<RT>(0.5 * s.dy), T.skpara)));
+        RT d = rmin(sqr(rdiv(k.half_dx, T.skpara)), sqr(rdiv(k.half_dy, T.skpara)));
         d = rmin(d, sqr(rdiv(as, T.dx_dt)));
         d = rmin(d, sqr(rdiv(as, T.dy_dt)));
         return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
@@ -644,19 +647,19 @@ __global__ void __launch_bounds__(kBlock)
                     sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
                 }
                 if constexpr (FP32) {
-                    if (!(c[MODEL == 16 ? 5 : 6] >= static_cast<float>(s.fp64_null_b))) {
+                    if (!(c[MODEL == 16 ? 5 : 6] >= a.k32.null_b)) {
                         // near a null: the FP64 frames and FP64 arithmetic for this sub-step
                         double cd[CP];
                         sample_blend_fp64_grouped<CP>(a.field_lo64, a.field_hi64, blend, uw, n00, nxp, a.plane,
                                                       wx, wy, cd);
                         StepTerms<double> Td;
-                        ok = eval_terms<MODEL, FOCUSED, QLT, double>(cd, p, pitch_mu, s, a.qlt_norm, Td);
+                        ok = eval_terms<MODEL, FOCUSED, QLT, double>(cd, p, pitch_mu, s, a.k64, Td);
                         narrow_terms(Td, T);
                     } else {
-                        ok = eval_terms<MODEL, FOCUSED, QLT, float>(c, p, pitch_mu, s, a.qlt_norm, T);
+                        ok = eval_terms<MODEL, FOCUSED, QLT, float>(c, p, pitch_mu, s, a.k32, T);
                     }
                 } else {
-                    ok = eval_terms<MODEL, FOCUSED, QLT, double>(c, p, pitch_mu, s, a.qlt_norm, T);
+                    ok = eval_terms<MODEL, FOCUSED, QLT, double>(c, p, pitch_mu, s, a.k64, T);
                 }
             }
             if (!ok) {
@@ -665,7 +668,7 @@ __global__ void __launch_bounds__(kBlock)
             }
 
             if (s.dt_min < s.dt_max) {
-                dt = candidate_dt<FOCUSED>(T, p, s);
+                dt = candidate_dt<FOCUSED>(T, p, s, step_const<FT>(a));
                 if (dt < s.dt_min) dt = s.dt_min;
                 if (dt > s.dt_max) dt = s.dt_max;
             } else {
@@ -729,8 +732,8 @@ __global__ void __launch_bounds__(kBlock)
                         if (it > 0) {
                             const FT m = static_cast<FT>(reflect_mu(trial, s.mu_max));
                             FT dD;
-                            qlt_scattering<FT>(m, T.q_xi, T.q_xig, T.q_pref, static_cast<FT>(s.qlt_spectral_index),
-                                               static_cast<FT>(s.qlt_h0), D, dD);
+                            qlt_scattering<FT>(m, T.q_xi, T.q_xig, T.q_pref, step_const<FT>(a).qlt_gk,
+                                               step_const<FT>(a).qlt_h0, D, dD);
                             rate = FT(0.5) * (FT(1) - m * m) *
                                        (T.f_speed * T.f_divb + m * T.f_divv - FT(3) * m * T.f_bb -
                                         FT(2) * rdiv(T.f_bv, T.f_speed)) +
