@@ -377,6 +377,63 @@ kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h
 kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint64_t* count,
                                   double charge, double split_ratio, char* err, size_t err_len);
 
+/* ---- learned pitch-angle closure (KPT_CLOSURE_MLP) ------------------------------
+ * No counterpart in the reference. A small network stands in for the resonance
+ * part of the slab-QLT D_mumu of LegencyModel::evaluate_pitch_angle_scattering
+ * (include/LegencyModel.hpp:539-689):
+ *   D = (pi/4) Omega sigma2 C_gamma (1 - mu^2) R(|mu|, xi, gamma_k),
+ *   R = xi mu_h^(gamma_k - 1) / (mu_h^gamma_k + xi^gamma_k),
+ * network inputs (ln|mu|, ln xi, gamma_k), outputs (ln R - out_mean) / out_scale
+ * and |mu| d ln R / d|mu|; 3 -> 64 -> 64 -> 64 -> 2 with SiLU. It is trained on
+ * the GPU (kpt_mlp_train) on exact-QLT samples and evaluated inside the FP32
+ * focused push with BF16 MFMA and FP32 accumulation; samples outside the hull
+ * (and every sample when the push's gamma_k or h0 differ from the hull's) take
+ * the exact formula. */
+enum { KPT_MLP_INPUTS = 3, KPT_MLP_HIDDEN = 64, KPT_MLP_OUTPUTS = 2, KPT_MLP_PARAMS = 8706 };
+
+typedef struct kpt_mlp_hull {
+    double mu_lo;                 /* |mu| >= mu_lo (and < 1)            default 0.02  */
+    double log_xi_lo, log_xi_hi;  /* ln xi range                        ln 1e-7, 0    */
+    double gamma_lo, gamma_hi;    /* gamma_k range                      1.5, 2        */
+    double h0;                    /* resonance regularisation trained   1e-6          */
+    double out_mean, out_scale;   /* ln R = out_mean + out_scale * y0 (set by training) */
+} kpt_mlp_hull;
+
+typedef struct kpt_mlp_train_opts {
+    uint64_t samples;  /* training set size, default 2^22 */
+    uint32_t steps;    /* Adam steps, default 8000 */
+    uint32_t batch;    /* default 4096 */
+    double lr;         /* peak learning rate, cosine decay to 0, default 3e-3 */
+    uint64_t seed;
+    kpt_mlp_hull hull;
+} kpt_mlp_train_opts;
+
+typedef struct kpt_mlp_report {
+    double final_loss;               /* mean squared error of the last batch (scaled targets) */
+    double val_rms_lnr, val_max_lnr; /* held-out samples, BF16-MFMA inference vs exact FP64  */
+    double val_rms_y1, val_max_y1;
+    double val_rms_rel_d;            /* rms relative error of D over the held-out set        */
+    double train_ms;
+    uint32_t steps;
+    uint32_t validation_samples;
+} kpt_mlp_report;
+
+kpt_status kpt_mlp_default_train_opts(kpt_mlp_train_opts* opts);
+/* Train on the GPU: exact-QLT samples from a Philox stream, forward / backward /
+ * Adam kernels, then build the BF16 MFMA image used by the push. */
+kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_report* report);
+/* Load / read the FP32 parameters (KPT_MLP_PARAMS floats) and the hull. */
+kpt_status kpt_mlp_set(kpt_ctx* ctx, const float* params, size_t n, const kpt_mlp_hull* hull);
+kpt_status kpt_mlp_get(kpt_ctx* ctx, float* params, size_t n, kpt_mlp_hull* hull);
+/* Gradient of the mean squared error over n samples (features n x 3 and scaled
+ * targets n x 2, host arrays) with respect to the parameters, from the training
+ * kernels; n <= 65536. */
+kpt_status kpt_mlp_gradient(kpt_ctx* ctx, const float* feats, const float* targets, uint64_t n, float* grad,
+                            double* loss);
+/* Network outputs for n host feature rows: mfma = 0 the FP32 training forward
+ * kernel, mfma = 1 the BF16 MFMA wavefront evaluation of the push. */
+kpt_status kpt_mlp_infer(kpt_ctx* ctx, const float* feats, uint64_t n, int32_t mfma, float* out);
+
 /* ---- CSV outputs (host only) ------------------------------------------------ */
 
 /* Per-frame timing and split counters: FrameTiming (src/main.cpp:355-363). */

@@ -52,6 +52,12 @@ struct kpt_ctx {
     float* pair32 = nullptr;
     size_t pair32_cap = 0;
     uint64_t pair_versions[2] = {0, 0};
+    // learned closure: FP32 parameters (device and host copies), hull, MFMA image
+    bool mlp_valid = false;
+    kpt_mlp_hull mlp_hull{};
+    float* mlp_params = nullptr;
+    void* mlp_image = nullptr;
+    std::vector<float> mlp_host;
     double* noise = nullptr;
     uint64_t noise_particles = 0;
     uint32_t noise_steps = 0;
@@ -411,6 +417,8 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.c32);
     }
     dev_free(ctx->pair32);
+    dev_free(ctx->mlp_params);
+    dev_free(ctx->mlp_image);
     if (ctx->p2_allocated) {
         kpt::ParticleArrays& Q = ctx->P2;
         for (void* p : {static_cast<void*>(Q.x), static_cast<void*>(Q.y), static_cast<void*>(Q.p),
@@ -770,6 +778,17 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.qlt_norm = s->qlt_spectral_index * std::sin(M_PI / s->qlt_spectral_index) / M_PI;
     a.k64 = kpt::make_step_const<double>(*s, a.qlt_norm);
     a.k32 = kpt::make_step_const<float>(*s, a.qlt_norm);
+    a.mlp_image = nullptr;
+    a.mlp_on = 0;
+    if (s->closure == KPT_CLOSURE_MLP) {
+        if (!fp32) return fail(ctx, KPT_ERR_ARG, "the learned pitch-angle closure runs in the FP32 push.");
+        if (!ctx->mlp_valid) return fail(ctx, KPT_ERR_STATE, "no pitch-angle network is loaded (kpt_mlp_train / kpt_mlp_set).");
+        const kpt_mlp_hull& h = ctx->mlp_hull;
+        a.mlp_image = ctx->mlp_image;
+        a.mlp_scale = kpt::mlp_scale(h);
+        a.mlp_on = (s->qlt_spectral_index >= h.gamma_lo && s->qlt_spectral_index <= h.gamma_hi &&
+                    s->qlt_h0 == h.h0) ? 1 : 0;
+    }
     (void)hipSetDevice(ctx->device);
     const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
     const int chunks = s->push_chunks > 1 ? s->push_chunks : 1;
@@ -1145,6 +1164,282 @@ kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, doub
     st = kpt_snapshot_read_host(path, &h, &count, charge, split_ratio, err, sizeof(err));
     if (st != KPT_OK) return fail(ctx, st, err);
     return kpt_particles_upload(ctx, &h, count);
+}
+
+}  // extern "C"
+
+namespace {
+
+bool hull_ok(const kpt_mlp_hull& h) {
+    return std::isfinite(h.mu_lo) && h.mu_lo > 0.0 && h.mu_lo < 1.0 && std::isfinite(h.log_xi_lo) &&
+           std::isfinite(h.log_xi_hi) && h.log_xi_hi > h.log_xi_lo && std::isfinite(h.gamma_lo) &&
+           std::isfinite(h.gamma_hi) && h.gamma_hi > h.gamma_lo && h.gamma_lo > 1.0 && std::isfinite(h.h0) &&
+           h.h0 > 0.0 && std::isfinite(h.out_mean) && std::isfinite(h.out_scale) && h.out_scale > 0.0;
+}
+
+// Install FP32 parameters: device copy, host copy and the BF16 MFMA image.
+kpt_status mlp_install(kpt_ctx* ctx, const float* params, const kpt_mlp_hull& h) {
+    if (!ctx->mlp_params) KPT_HIP(ctx, dev_alloc(&ctx->mlp_params, kpt::kMlpParams));
+    if (!ctx->mlp_image) KPT_HIP(ctx, hipMalloc(&ctx->mlp_image, kpt::kMlpImageBytes));
+    ctx->mlp_host.assign(params, params + kpt::kMlpParams);
+    std::vector<unsigned char> image(kpt::kMlpImageBytes, 0);
+    kpt::mlp_build_image(params, image.data());
+    KPT_HIP(ctx, hipMemcpy(ctx->mlp_params, params, kpt::kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    KPT_HIP(ctx, hipMemcpy(ctx->mlp_image, image.data(), image.size(), hipMemcpyHostToDevice));
+    ctx->mlp_hull = h;
+    ctx->mlp_valid = true;
+    return KPT_OK;
+}
+
+// Scoped device buffers of the training and test calls.
+struct DevBufs {
+    std::vector<void*> ptrs;
+    ~DevBufs() {
+        for (void* p : ptrs) dev_free(p);
+    }
+    template <typename T>
+    hipError_t get(T** p, size_t elems) {
+        hipError_t e = dev_alloc(p, elems);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_mlp_default_train_opts(kpt_mlp_train_opts* o) {
+    if (!o) return KPT_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->samples = 1ull << 22;
+    o->steps = 8000;
+    o->batch = 4096;
+    o->lr = 3.0e-3;
+    o->seed = 0x4D4C50ull;
+    o->hull.mu_lo = 0.02;
+    o->hull.log_xi_lo = std::log(1.0e-7);
+    o->hull.log_xi_hi = 0.0;
+    o->hull.gamma_lo = 1.5;
+    o->hull.gamma_hi = 2.0;
+    o->hull.h0 = 1.0e-6;
+    o->hull.out_mean = 0.0;
+    o->hull.out_scale = 1.0;
+    return KPT_OK;
+}
+
+kpt_status kpt_mlp_set(kpt_ctx* ctx, const float* params, size_t n, const kpt_mlp_hull* hull) {
+    if (!ctx || !params || !hull) return KPT_ERR_ARG;
+    if (n != static_cast<size_t>(kpt::kMlpParams))
+        return fail(ctx, KPT_ERR_ARG, "the pitch-angle network has KPT_MLP_PARAMS parameters.");
+    if (!hull_ok(*hull)) return fail(ctx, KPT_ERR_ARG, "invalid pitch-angle network hull.");
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(ctx, KPT_ERR_ARG, "non-finite network parameter.");
+    (void)hipSetDevice(ctx->device);
+    return mlp_install(ctx, params, *hull);
+}
+
+kpt_status kpt_mlp_get(kpt_ctx* ctx, float* params, size_t n, kpt_mlp_hull* hull) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (!ctx->mlp_valid) return fail(ctx, KPT_ERR_STATE, "no pitch-angle network is loaded.");
+    if (params) {
+        if (n != static_cast<size_t>(kpt::kMlpParams))
+            return fail(ctx, KPT_ERR_ARG, "the pitch-angle network has KPT_MLP_PARAMS parameters.");
+        std::memcpy(params, ctx->mlp_host.data(), n * sizeof(float));
+    }
+    if (hull) *hull = ctx->mlp_hull;
+    return KPT_OK;
+}
+
+kpt_status kpt_mlp_gradient(kpt_ctx* ctx, const float* feats, const float* targets, uint64_t n, float* grad,
+                            double* loss) {
+    if (!ctx || !feats || !targets || !grad) return KPT_ERR_ARG;
+    if (!ctx->mlp_valid) return fail(ctx, KPT_ERR_STATE, "no pitch-angle network is loaded.");
+    if (n == 0 || n > 65536) return fail(ctx, KPT_ERR_ARG, "gradient batch must hold 1..65536 samples.");
+    (void)hipSetDevice(ctx->device);
+    DevBufs b;
+    float *x, *y, *z, *yhat, *dz, *dy, *g, *lp;
+    const size_t parts = kpt::mlp_loss_partials(n);
+    KPT_HIP(ctx, b.get(&x, n * 3));
+    KPT_HIP(ctx, b.get(&y, n * 2));
+    KPT_HIP(ctx, b.get(&z, n * 3 * kpt::kMlpH));
+    KPT_HIP(ctx, b.get(&yhat, n * 2));
+    KPT_HIP(ctx, b.get(&dz, n * 3 * kpt::kMlpH));
+    KPT_HIP(ctx, b.get(&dy, n * 2));
+    KPT_HIP(ctx, b.get(&g, kpt::kMlpParams));
+    KPT_HIP(ctx, b.get(&lp, parts));
+    KPT_HIP(ctx, hipMemcpyAsync(x, feats, n * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(y, targets, n * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, x, n, z, yhat, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_backward(ctx->mlp_params, x, z, yhat, y, n, dz, dy, g, lp, ctx->stream));
+    std::vector<float> lh(parts);
+    KPT_HIP(ctx, hipMemcpyAsync(grad, g, kpt::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(lh.data(), lp, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (loss) {
+        double t = 0.0;
+        for (float v : lh) t += v;
+        *loss = t;
+    }
+    return KPT_OK;
+}
+
+kpt_status kpt_mlp_infer(kpt_ctx* ctx, const float* feats, uint64_t n, int32_t mfma, float* out) {
+    if (!ctx || !feats || !out) return KPT_ERR_ARG;
+    if (!ctx->mlp_valid) return fail(ctx, KPT_ERR_STATE, "no pitch-angle network is loaded.");
+    if (n == 0) return KPT_OK;
+    (void)hipSetDevice(ctx->device);
+    DevBufs b;
+    float *x, *o;
+    KPT_HIP(ctx, b.get(&x, n * 3));
+    KPT_HIP(ctx, b.get(&o, n * 2));
+    KPT_HIP(ctx, hipMemcpyAsync(x, feats, n * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (mfma)
+        KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, x, n, o, ctx->stream));
+    else
+        KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, x, n, nullptr, o, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(out, o, n * 2 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_report* report) {
+    if (!ctx || !opts) return KPT_ERR_ARG;
+    kpt_mlp_hull h = opts->hull;
+    h.out_mean = 0.0;
+    h.out_scale = 1.0;
+    if (!hull_ok(h)) return fail(ctx, KPT_ERR_ARG, "invalid pitch-angle network hull.");
+    const uint64_t N = opts->samples;
+    const uint32_t B = opts->batch;
+    if (B == 0 || B > 65536 || N < B || N > (1ull << 26) || opts->steps == 0 || !(opts->lr > 0.0))
+        return fail(ctx, KPT_ERR_ARG, "training needs 1 <= batch <= 65536, batch <= samples <= 2^26, steps > 0, lr > 0.");
+    (void)hipSetDevice(ctx->device);
+    DevBufs b;
+    float *X, *Y, *z, *yhat, *dz, *dy, *g, *m, *v, *lp, *params;
+    double* raw;
+    const size_t parts = kpt::mlp_loss_partials(B);
+    KPT_HIP(ctx, b.get(&X, N * 3));
+    KPT_HIP(ctx, b.get(&Y, N * 2));
+    KPT_HIP(ctx, b.get(&raw, N));
+    KPT_HIP(ctx, b.get(&z, static_cast<size_t>(B) * 3 * kpt::kMlpH));
+    KPT_HIP(ctx, b.get(&yhat, static_cast<size_t>(B) * 2));
+    KPT_HIP(ctx, b.get(&dz, static_cast<size_t>(B) * 3 * kpt::kMlpH));
+    KPT_HIP(ctx, b.get(&dy, static_cast<size_t>(B) * 2));
+    KPT_HIP(ctx, b.get(&g, kpt::kMlpParams));
+    KPT_HIP(ctx, b.get(&m, kpt::kMlpParams));
+    KPT_HIP(ctx, b.get(&v, kpt::kMlpParams));
+    KPT_HIP(ctx, b.get(&lp, parts));
+    KPT_HIP(ctx, b.get(&params, kpt::kMlpParams));
+
+    // training set: exact slab-QLT targets, ln R standardised over the set
+    KPT_HIP(ctx, kpt::launch_mlp_generate(X, Y, raw, N, opts->seed, 0, h, ctx->stream));
+    std::vector<double> lnr(N);
+    KPT_HIP(ctx, hipMemcpyAsync(lnr.data(), raw, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double mean = 0.0, var = 0.0;
+    for (double r : lnr) mean += r;
+    mean /= static_cast<double>(N);
+    for (double r : lnr) var += (r - mean) * (r - mean);
+    const double sd = std::sqrt(var / static_cast<double>(N));
+    h.out_mean = mean;
+    h.out_scale = sd > 0.0 ? sd : 1.0;
+    KPT_HIP(ctx, kpt::launch_mlp_normalize(Y, N, static_cast<float>(h.out_mean),
+                                           static_cast<float>(1.0 / h.out_scale), ctx->stream));
+
+    // initial weights: uniform +-sqrt(3 / fan_in) from splitmix64(seed), zero biases
+    std::vector<float> p0(kpt::kMlpParams, 0.0f);
+    uint64_t st = opts->seed ^ 0xA5A5A5A5DEADBEEFull;
+    auto init = [&](int off, int rows, int cols) {
+        const double a = std::sqrt(3.0 / cols);
+        for (int i = 0; i < rows * cols; ++i) {
+            const double u = (static_cast<double>(splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+            p0[off + i] = static_cast<float>(a * (2.0 * u - 1.0));
+        }
+    };
+    init(kpt::kOffW1, kpt::kMlpH, kpt::kMlpIn);
+    init(kpt::kOffW2, kpt::kMlpH, kpt::kMlpH);
+    init(kpt::kOffW3, kpt::kMlpH, kpt::kMlpH);
+    init(kpt::kOffW4, kpt::kMlpOut, kpt::kMlpH);
+    KPT_HIP(ctx, hipMemcpyAsync(params, p0.data(), p0.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    KPT_HIP(ctx, hipMemsetAsync(m, 0, kpt::kMlpParams * sizeof(float), ctx->stream));
+    KPT_HIP(ctx, hipMemsetAsync(v, 0, kpt::kMlpParams * sizeof(float), ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+    KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    const uint64_t batches = N / B;
+    double b1c = 1.0, b2c = 1.0;
+    for (uint32_t it = 1; it <= opts->steps; ++it) {
+        const uint64_t off = ((it - 1) % batches) * B;
+        const float lr = static_cast<float>(opts->lr * 0.5 * (1.0 + std::cos(M_PI * it / opts->steps)));
+        b1c *= 0.9;
+        b2c *= 0.999;
+        KPT_HIP(ctx, kpt::launch_mlp_forward(params, X + 3 * off, B, z, yhat, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_backward(params, X + 3 * off, z, yhat, Y + 2 * off, B, dz, dy, g, lp,
+                                              ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_adam(params, m, v, g, lr, static_cast<float>(1.0 - b1c),
+                                          static_cast<float>(1.0 - b2c), ctx->stream));
+    }
+    KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<float> lh(parts), pf(kpt::kMlpParams);
+    KPT_HIP(ctx, hipMemcpyAsync(lh.data(), lp, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(pf.data(), params, pf.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float train_ms = 0.0f;
+    KPT_HIP(ctx, hipEventElapsedTime(&train_ms, ctx->ev0, ctx->ev1));
+    for (float w : pf)
+        if (!std::isfinite(w)) return fail(ctx, KPT_ERR_DEVICE, "pitch-angle network training diverged.");
+    kpt_status s = mlp_install(ctx, pf.data(), h);
+    if (s != KPT_OK) return s;
+
+    // held-out check of the installed BF16 MFMA evaluation against exact FP64 targets
+    const uint64_t V = 65536;
+    float *vx, *vy, *vo;
+    double* vraw;
+    KPT_HIP(ctx, b.get(&vx, V * 3));
+    KPT_HIP(ctx, b.get(&vy, V * 2));
+    KPT_HIP(ctx, b.get(&vo, V * 2));
+    KPT_HIP(ctx, b.get(&vraw, V));
+    KPT_HIP(ctx, kpt::launch_mlp_generate(vx, vy, vraw, V, opts->seed, N, h, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, vx, V, vo, ctx->stream));
+    std::vector<double> vr(V);
+    std::vector<float> ty(2 * V), to(2 * V);
+    KPT_HIP(ctx, hipMemcpyAsync(vr.data(), vraw, V * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(ty.data(), vy, 2 * V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(to.data(), vo, 2 * V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (report) {
+        std::memset(report, 0, sizeof(*report));
+        double loss = 0.0;
+        for (float w : lh) loss += w;
+        double s_lnr = 0, m_lnr = 0, s_y1 = 0, m_y1 = 0, s_d = 0;
+        for (uint64_t i = 0; i < V; ++i) {
+            const double e0 = h.out_mean + h.out_scale * to[2 * i] - vr[i];
+            const double e1 = static_cast<double>(to[2 * i + 1]) - ty[2 * i + 1];
+            const double ed = std::expm1(e0);
+            s_lnr += e0 * e0;
+            m_lnr = std::max(m_lnr, std::fabs(e0));
+            s_y1 += e1 * e1;
+            m_y1 = std::max(m_y1, std::fabs(e1));
+            s_d += ed * ed;
+        }
+        report->final_loss = loss;
+        report->val_rms_lnr = std::sqrt(s_lnr / V);
+        report->val_max_lnr = m_lnr;
+        report->val_rms_y1 = std::sqrt(s_y1 / V);
+        report->val_max_y1 = m_y1;
+        report->val_rms_rel_d = std::sqrt(s_d / V);
+        report->train_ms = train_ms;
+        report->steps = opts->steps;
+        report->validation_samples = static_cast<uint32_t>(V);
+    }
+    return KPT_OK;
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../../include/kpt.h"
+#include "kpt_mlp.h"
 
 namespace kpt {
 
@@ -115,6 +116,11 @@ struct PushArgs {
     double qlt_norm;      // gamma_k sin(pi / gamma_k) / pi (pitch_angle_spectrum_normalization)
     StepConst<double> k64;
     StepConst<float> k32;
+    // learned closure (KPT_CLOSURE_MLP): network image and feature scaling;
+    // mlp_on = 0 sends every evaluation to the exact formula
+    const void* mlp_image;
+    MlpScale mlp_scale;
+    int32_t mlp_on;
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
 };
 

@@ -12,44 +12,14 @@
 #include <cmath>
 #include <cstdint>
 
+#include "kpt_device_common.h"
 #include "kpt_internal.h"
+#include "kpt_mlp_device.h"
 
 namespace kpt {
 namespace {
 
 constexpr int kBlock = 256;
-
-// ---------------------------------------------------------------------------
-// Philox4x32-10 (Salmon et al., SC'11) in registers: 10 rounds of two 32x32->64
-// multiplies, Weyl-incremented key.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    out[0] = c0;
-    out[1] = c1;
-    out[2] = c2;
-    out[3] = c3;
-}
-
-// (r + 1/2) / 2^32: open-interval U(0, 1) from 32 bits.
-__device__ __forceinline__ double u01(uint32_t r) {
-    return (static_cast<double>(r) + 0.5) * (1.0 / 4294967296.0);
-}
 
 constexpr double kSqrt3 = 1.7320508075688772935;
 
@@ -304,7 +274,9 @@ __device__ __forceinline__ RT p_ratio(double p, double p0) {
         return p / p0;
 }
 
-// Terms of one sub-step at the particle's sampled coefficients.
+// Terms of one sub-step at the particle's sampled coefficients. With the QLT
+// and learned closures, dmu_dt holds only the focusing rate and duu is zero on
+// return from eval_terms; the caller adds the closure's D and dD/dmu.
 template <typename RT>
 struct StepTerms {
     RT dx_dt, dy_dt, dmu_dt, duu;
@@ -431,7 +403,8 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
             o.q_xi = rdiv(omega * k.qlt_lc, speed);
             o.q_xig = rpow(o.q_xi, k.qlt_gk);
             o.q_pref = omega * k.qlt_pref;
-            qlt_scattering<RT>(mu, o.q_xi, o.q_xig, o.q_pref, k.qlt_gk, k.qlt_h0, duu, duu_du);
+            // the closure (exact or learned) is evaluated by the caller
+            duu = RT(0);
         } else {
             const RT abs_mu = rabs(mu);
             const RT dtmp = rpow(abs_mu, k.exp_mu1) + k.duu_h0;
@@ -666,6 +639,17 @@ __global__ void __launch_bounds__(kBlock)
                 status = 2;
                 break;
             }
+            if constexpr (QLT) {
+                FT D, dD;
+                qlt_scattering<FT>(static_cast<FT>(pitch_mu), T.q_xi, T.q_xig, T.q_pref, step_const<FT>(a).qlt_gk,
+                                   step_const<FT>(a).qlt_h0, D, dD);
+                T.dmu_dt += dD;
+                T.duu = D;
+                if (!rfinite(T.dmu_dt) || !rfinite(D) || D < FT(0)) {
+                    status = 2;
+                    break;
+                }
+            }
 
             if (s.dt_min < s.dt_max) {
                 dt = candidate_dt<FOCUSED>(T, p, s, step_const<FT>(a));
@@ -788,6 +772,256 @@ __global__ void __launch_bounds__(kBlock)
         atomicAdd(&a.counters[2], static_cast<unsigned long long>(act));
     }
 }
+// Closure from the network outputs at pitch cosine mu (|mu| in the hull):
+// D = P (1 - mu^2) R, dD/dmu = P R ((1 - mu^2) sgn(mu) y1 / |mu| - 2 mu),
+// ln R = out_mean + out_scale y0.
+__device__ __forceinline__ void mlp_closure(float y0, float y1, float mu, float pref, const MlpScale& ms, float& D,
+                                            float& dD) {
+    const float omm = 1.0f - mu * mu;
+    const float r = __builtin_amdgcn_exp2f((ms.out_mean + ms.out_scale * y0) * 1.4426950408889634f);
+    const float sgn = mu > 0.0f ? 1.0f : -1.0f;
+    D = pref * omm * r;
+    dD = pref * r * (omm * sgn * y1 * __builtin_amdgcn_rcpf(fabsf(mu)) - 2.0f * mu);
+}
+
+// One closure evaluation at mu: the network's outputs where the sample lies in
+// the hull, the exact slab-QLT formula elsewhere.
+__device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, float mu, const StepTerms<float>& T,
+                                           const StepConst<float>& K, const MlpScale& ms, float& D, float& dD) {
+    if (in_hull)
+        mlp_closure(y0, y1, mu, T.q_pref, ms, D, dD);
+    else
+        qlt_scattering<float>(mu, T.q_xi, T.q_xig, T.q_pref, K.qlt_gk, K.qlt_h0, D, dD);
+}
+
+// ---------------------------------------------------------------------------
+// Focused FP32 push with the learned closure. Same mover as push_kernel<21,
+// true, float, QLT>, with the sub-step loop run wave-uniformly: every lane of
+// the wavefront takes part in the three network evaluations of a sub-step (D at
+// the initial mu and at the two Picard trial values), lanes that are finished
+// or failed with dummy features.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_FP32)))
+    push_kernel_mlp(PushArgs a) {
+    __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
+    mlp_load_lds(a.mlp_image, mlp_smem);
+    __syncthreads();
+    const MlpLds im = mlp_lds_view(mlp_smem);
+    const float* b4 = reinterpret_cast<const float*>(static_cast<const char*>(a.mlp_image) + kMlpLdsBytes);
+    const float b40 = b4[0], b41 = b4[1];
+    const MlpScale ms = a.mlp_scale;
+    const StepConst<float>& K = a.k32;
+    const kpt_device_settings& s = a.s;
+    constexpr int CP = 21;
+    const bool use_net = a.mlp_on != 0;
+
+    const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    const bool live = idx < static_cast<int64_t>(a.count) && a.P.status[idx] == 0;
+    uint64_t my_steps = 0;
+    uint32_t my_fail = 0, my_active = live ? 1u : 0u;
+    const int nxp = a.g.nx + 2;
+    const double interval_start = a.t0, interval_end = a.t_end, t_stop = a.t_stop;
+    const bool blend = s.time_interpolation != 0;
+    double x = 0.0, y = 0.0, p = 1.0, t = interval_end, dt = 0.0, diagnostic_dt = 0.0, pitch_mu = 0.0;
+    uint64_t id = 0;
+    int step_guard = 0, status = 0;
+    if (live) {
+        x = a.P.x[idx];
+        y = a.P.y[idx];
+        p = a.P.p[idx];
+        t = a.P.t[idx];
+        dt = a.P.step_dt[idx];
+        diagnostic_dt = dt;
+        pitch_mu = reflect_mu(a.P.mu[idx], s.mu_max);
+        id = a.P.id[idx];
+        step_guard = a.first_chunk ? 0 : static_cast<int>(a.P.substep[idx]);
+    }
+
+    for (;;) {
+        bool run = live && status == 0 && t < t_stop && step_guard < s.max_particle_steps_per_interval;
+        if (!__any(run)) break;
+
+        // phase 1: sample, rates, features at the initial mu
+        StepTerms<float> T{};
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lxi = 0.0f;
+        bool h0 = false;
+        if (run) {
+            ++step_guard;
+            x = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
+            y = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
+            int i0, j0;
+            double wx, wy;
+            if (!axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) ||
+                !axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy)) {
+                status = 2;
+                run = false;
+            } else {
+                const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
+                double uw = 0.0;
+                if (blend) {
+                    const double rt = (t - interval_start) * (1.0 / s.mhd_dt);
+                    uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
+                }
+                float c[CP];
+                if (a.field_pair != nullptr)
+                    sample_pair32<CP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                else
+                    sample_blend<CP, float>(static_cast<const float*>(a.field_lo), static_cast<const float*>(a.field_hi),
+                                            blend, uw, n00, nxp, a.plane, wx, wy, c);
+                bool ok;
+                if (!(c[6] >= K.null_b)) {
+                    double cd[CP];
+                    sample_blend_fp64_grouped<CP>(a.field_lo64, a.field_hi64, blend, uw, n00, nxp, a.plane, wx, wy, cd);
+                    StepTerms<double> Td;
+                    ok = eval_terms<21, true, true, double>(cd, p, pitch_mu, s, a.k64, Td);
+                    narrow_terms(Td, T);
+                } else {
+                    ok = eval_terms<21, true, true, float>(c, p, pitch_mu, s, K, T);
+                }
+                if (!ok) {
+                    status = 2;
+                    run = false;
+                } else {
+                    lxi = __builtin_amdgcn_logf(T.q_xi) * 0.6931471805599453f;
+                    h0 = use_net && mlp_features(ms, fabsf(static_cast<float>(pitch_mu)), lxi, K.qlt_gk, f0, f1, f2);
+                }
+            }
+        }
+        float y0, y1;
+        mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
+
+        // phase 2: closure at mu0, step size, noise, transverse increments, first Picard pass
+        float r3 = 0.0f, dtr = 0.0f, delta_x = 0.0f, delta_y = 0.0f;
+        const double mu0 = pitch_mu;
+        double trial = mu0;
+        bool h1 = false;
+        f0 = f1 = f2 = 0.0f;
+        if (run) {
+            float D, dD;
+            closure_at(h0, y0, y1, static_cast<float>(mu0), T, K, ms, D, dD);
+            T.dmu_dt += dD;
+            T.duu = D;
+            if (!rfinite(T.dmu_dt) || !rfinite(D) || D < 0.0f) {
+                status = 2;
+                run = false;
+            }
+            if (run) {
+                if (s.dt_min < s.dt_max) {
+                    dt = candidate_dt<true>(T, p, s, K);
+                    if (dt < s.dt_min) dt = s.dt_min;
+                    if (dt > s.dt_max) dt = s.dt_max;
+                } else {
+                    dt = s.dt_min;
+                }
+                diagnostic_dt = dt;
+                const double remaining = interval_end - t;
+                if (dt > remaining) dt = remaining;
+                if (!isfinite(dt) || dt <= 0.0) {
+                    status = 2;
+                    run = false;
+                }
+            }
+            if (run) {
+                float r0, r1;
+                const uint32_t k = static_cast<uint32_t>(step_guard - 1);
+                if (a.noise != nullptr) {
+                    if (k >= a.noise_steps) {
+                        status = 2;
+                        run = false;
+                    } else {
+                        const double* row = a.noise + (static_cast<uint64_t>(idx) * a.noise_steps + k) * 4;
+                        r0 = static_cast<float>(row[0]);
+                        r1 = static_cast<float>(row[1]);
+                        r3 = static_cast<float>(row[3]);
+                    }
+                } else {
+                    uint32_t rb[4];
+                    philox4x32_10(static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), a.frame, k, a.key[0],
+                                  a.key[1], rb);
+                    r0 = draw_sqrt3<float>(rb[0]);
+                    r1 = draw_sqrt3<float>(rb[1]);
+                    r3 = draw_sqrt3<float>(rb[3]);
+                }
+                if (run) {
+                    dtr = static_cast<float>(dt);
+                    const float sdt = rsqrt_(dtr);
+                    const float bxn = T.bx * T.ib, byn = T.by * T.ib, bzn = T.bz * T.ib;
+                    const float bxy = rsqrt_(bxn * bxn + byn * byn);
+                    float sx = 0.0f, sy = 0.0f;
+                    if (bxy > 0.0f) {
+                        const float ibxy = rrcp(bxy);
+                        sx = T.skperp * ibxy * sdt * (-bxn * bzn * r0 - T.by * r1);
+                        sy = T.skperp * ibxy * sdt * (-byn * bzn * r0 + T.bx * r1);
+                    }
+                    delta_x = T.dx_dt * dtr + sx;
+                    delta_y = T.dy_dt * dtr + sy;
+                    const float scale = D > 0.0f ? rsqrt_(2.0f * D * dtr) : 0.0f;
+                    trial = reflect_mu(mu0 + static_cast<double>(dtr * T.dmu_dt) + static_cast<double>(scale * r3),
+                                       s.mu_max);
+                    const float m1 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    h1 = use_net && mlp_features(ms, fabsf(m1), lxi, K.qlt_gk, f0, f1, f2);
+                }
+            }
+        }
+
+        // phases 3 and 4: the two Picard passes at the trial values
+#pragma unroll 1
+        for (int it = 1; it < 3; ++it) {
+            mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
+            bool hn = false;
+            f0 = f1 = f2 = 0.0f;
+            if (run) {
+                const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
+                float D, dD;
+                closure_at(h1, y0, y1, m, T, K, ms, D, dD);
+                const float rate = 0.5f * (1.0f - m * m) *
+                                       (T.f_speed * T.f_divb + m * T.f_divv - 3.0f * m * T.f_bb -
+                                        2.0f * rdiv(T.f_bv, T.f_speed)) +
+                                   dD;
+                const float scale = D > 0.0f ? rsqrt_(2.0f * D * dtr) : 0.0f;
+                trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3), s.mu_max);
+                if (it == 1) {
+                    const float m2 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    hn = use_net && mlp_features(ms, fabsf(m2), lxi, K.qlt_gk, f0, f1, f2);
+                }
+            }
+            h1 = hn;
+        }
+
+        if (run) {
+            pitch_mu = trial;
+            x += static_cast<double>(delta_x);
+            y += static_cast<double>(delta_y);
+            p += T.dp_dt * dt;
+            t += dt;
+            ++my_steps;
+            if (p < 0.25 * s.p0) p = 0.25 * s.p0;
+            if (!isfinite(x) || !isfinite(y) || !isfinite(p) || !isfinite(t)) status = 2;
+        }
+    }
+
+    if (live) {
+        if (step_guard >= s.max_particle_steps_per_interval && t < interval_end) status = 2;
+        a.P.x[idx] = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
+        a.P.y[idx] = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
+        a.P.p[idx] = p;
+        a.P.mu[idx] = pitch_mu;
+        a.P.t[idx] = t;
+        a.P.step_dt[idx] = diagnostic_dt;
+        a.P.status[idx] = status;
+        a.P.substep[idx] = static_cast<uint32_t>(step_guard);
+        my_fail = status != 0 ? 1u : 0u;
+    }
+    const unsigned long long steps = wave_sum(static_cast<unsigned long long>(my_steps));
+    const unsigned int fails = wave_sum(my_fail);
+    const unsigned int act = wave_sum(my_active);
+    if ((threadIdx.x & 63) == 0 && act != 0) {
+        atomicAdd(&a.counters[0], steps);
+        atomicAdd(&a.counters[1], static_cast<unsigned long long>(fails));
+        atomicAdd(&a.counters[2], static_cast<unsigned long long>(act));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Injection (src/main.cpp:1021-1044): x, y, t, mu from one Philox block with
 // counter (root id, 0xFFFFFFFF, frame, root id >> 32).
@@ -1137,6 +1371,9 @@ hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, in
             hipLaunchKernelGGL((push_kernel<16, false, float, R>), grid, block, 0, stream, a);
         else
             hipLaunchKernelGGL((push_kernel<16, false, double, R>), grid, block, 0, stream, a);
+    } else if (focused && closure == KPT_CLOSURE_MLP) {
+        if (!fp32 || a.mlp_image == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(push_kernel_mlp, grid, block, 0, stream, a);
     } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
             hipLaunchKernelGGL((push_kernel<21, true, float, Q>), grid, block, 0, stream, a);

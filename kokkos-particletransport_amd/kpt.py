@@ -134,8 +134,45 @@ class InjectOpts(C.Structure):
     ]
 
 
+class MlpHull(C.Structure):
+    """kpt_mlp_hull: training hull and output scaling of the pitch-angle network."""
+    _fields_ = [
+        ("mu_lo", C.c_double), ("log_xi_lo", C.c_double), ("log_xi_hi", C.c_double),
+        ("gamma_lo", C.c_double), ("gamma_hi", C.c_double), ("h0", C.c_double),
+        ("out_mean", C.c_double), ("out_scale", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [
+        ("samples", C.c_uint64), ("steps", C.c_uint32), ("batch", C.c_uint32),
+        ("lr", C.c_double), ("seed", C.c_uint64), ("hull", MlpHull),
+    ]
+
+
+class MlpReport(C.Structure):
+    _fields_ = [
+        ("final_loss", C.c_double), ("val_rms_lnr", C.c_double), ("val_max_lnr", C.c_double),
+        ("val_rms_y1", C.c_double), ("val_max_y1", C.c_double), ("val_rms_rel_d", C.c_double),
+        ("train_ms", C.c_double), ("steps", C.c_uint32), ("validation_samples", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MLP_PARAMS = 8706
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
 
 
 def particles_struct(parts: dict) -> ParticlesHost:
@@ -217,6 +254,14 @@ EXPORTS = {
     "kpt_snapshot_read_host": (C.c_int, [C.c_char_p, C.POINTER(ParticlesHost),
                                          C.POINTER(C.c_uint64), C.c_double, C.c_double,
                                          C.c_char_p, C.c_size_t]),
+    "kpt_mlp_default_train_opts": (C.c_int, [C.POINTER(MlpTrainOpts)]),
+    "kpt_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(MlpTrainOpts), C.POINTER(MlpReport)]),
+    "kpt_mlp_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.POINTER(MlpHull)]),
+    "kpt_mlp_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.POINTER(MlpHull)]),
+    "kpt_mlp_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "kpt_mlp_infer": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64, C.c_int32,
+                                C.POINTER(C.c_float)]),
 }
 
 
@@ -254,6 +299,12 @@ def make_device_settings(host: HostSettings, x_edges, y_edges) -> DeviceSettings
     _check_static(load().kpt_make_device_settings(C.byref(host), len(xe) - 1, len(ye) - 1,
                                                   _dp(xe), _dp(ye), C.byref(out)))
     return out
+
+
+def mlp_default_train_opts() -> MlpTrainOpts:
+    o = MlpTrainOpts()
+    _check_static(load().kpt_mlp_default_train_opts(C.byref(o)))
+    return o
 
 
 def philox4x32_10(ctr, key):
@@ -461,6 +512,40 @@ class Context:
 
     def clear_noise(self):
         self._check(self.lib.kpt_clear_noise(self.h))
+
+    # ---- learned pitch-angle closure -----------------------------------------
+    def mlp_train(self, opts: "MlpTrainOpts | None" = None) -> dict:
+        o = opts or mlp_default_train_opts()
+        r = MlpReport()
+        self._check(self.lib.kpt_mlp_train(self.h, C.byref(o), C.byref(r)))
+        return r.as_dict()
+
+    def mlp_set(self, params, hull: MlpHull):
+        w = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.kpt_mlp_set(self.h, _fp(w), w.size, C.byref(hull)))
+
+    def mlp_get(self):
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        h = MlpHull()
+        self._check(self.lib.kpt_mlp_get(self.h, _fp(w), w.size, C.byref(h)))
+        return w, h
+
+    def mlp_gradient(self, feats, targets):
+        x = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(targets, dtype=np.float32)
+        assert x.ndim == 2 and x.shape[1] == 3 and y.shape == (x.shape[0], 2)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.kpt_mlp_gradient(self.h, _fp(x), _fp(y), x.shape[0], _fp(g),
+                                              C.byref(loss)))
+        return g, loss.value
+
+    def mlp_infer(self, feats, mfma=True):
+        x = np.ascontiguousarray(feats, dtype=np.float32)
+        assert x.ndim == 2 and x.shape[1] == 3
+        out = np.empty((x.shape[0], 2), dtype=np.float32)
+        self._check(self.lib.kpt_mlp_infer(self.h, _fp(x), x.shape[0], 1 if mfma else 0, _fp(out)))
+        return out
 
     def snapshot_write(self, path, charge, split_ratio, p0):
         self._check(self.lib.kpt_snapshot_write(self.h, path.encode(), charge, split_ratio, p0))

@@ -1,0 +1,193 @@
+"""GPU tests of the learned pitch-angle closure: training gradient against the
+CPU oracle, BF16-MFMA inference against its CPU restatement, training accuracy
+against exact slab-QLT, the push with the learned closure against the FP64
+oracle (statistically), and the A/B of what the network gains over the exact
+formula in the same FP32 push."""
+import time
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import kpt
+import kpt_mlp_oracle as M
+import kpt_oracle as O
+import workloads as W
+from helpers import chi2_two_sample
+
+pytestmark = pytest.mark.gpu
+
+HULL0 = kpt.mlp_default_train_opts().hull
+
+
+def _random_net(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(M.NPARAM)
+    p[M.OFF_W1:M.OFF_B1] = rng.standard_normal(M.OFF_B1 - M.OFF_W1) * np.sqrt(1 / 3)
+    for a, b in ((M.OFF_W2, M.OFF_B2), (M.OFF_W3, M.OFF_B3), (M.OFF_W4, M.OFF_B4)):
+        p[a:b] = rng.standard_normal(b - a) * np.sqrt(1 / 64)
+    for a, b in ((M.OFF_B1, M.OFF_W2), (M.OFF_B2, M.OFF_W3), (M.OFF_B3, M.OFF_W4), (M.OFF_B4, M.NPARAM)):
+        p[a:b] = rng.standard_normal(b - a) * 0.1
+    return p.astype(np.float32)
+
+
+def test_training_gradient_matches_oracle():
+    """Forward + backward training kernels against the float64 oracle gradient
+    (itself pinned by finite differences) on a 517-sample batch."""
+    p = _random_net(1)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (517, 3)).astype(np.float32)
+    y = rng.standard_normal((517, 2)).astype(np.float32)
+    with kpt.Context(16) as c:
+        c.mlp_set(p, HULL0)
+        g, loss = c.mlp_gradient(x, y)
+    lref, gref = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+    assert loss == pytest.approx(lref, rel=1e-5)
+    scale = np.max(np.abs(gref))
+    assert np.max(np.abs(g - gref)) < 2e-5 * scale, np.max(np.abs(g - gref)) / scale
+    # every block of parameters carries gradient (no dropped term)
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_W2, M.OFF_B2), (M.OFF_W3, M.OFF_B3), (M.OFF_W4, M.OFF_B4)):
+        assert np.max(np.abs(g[a:b])) > 1e-4 * scale
+
+
+def test_mfma_inference_matches_bf16_restatement():
+    """The wavefront BF16 MFMA evaluation (n not a multiple of 64) against the
+    CPU restatement with BF16 weights and activations, and against the FP32
+    forward kernel within BF16 rounding."""
+    p = _random_net(3)
+    x = np.random.default_rng(4).uniform(-1, 1, (4000, 3)).astype(np.float32)
+    with kpt.Context(16) as c:
+        c.mlp_set(p, HULL0)
+        ym = c.mlp_infer(x, mfma=True)
+        yf = c.mlp_infer(x, mfma=False)
+    _, y64 = M.forward(p.astype(np.float64), x)
+    yb = M.forward_bf16(p, x)
+    scale = max(1.0, float(np.max(np.abs(y64))))
+    np.testing.assert_allclose(yf, y64, rtol=1e-5, atol=1e-5 * scale)
+    assert np.max(np.abs(ym - yb)) < 1e-2 * scale
+    assert np.mean(np.abs(ym - yb)) < 1e-3 * scale
+    assert np.max(np.abs(ym - y64)) < 3e-2 * scale
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """One training run on the GPU (2^20 exact-QLT samples, 4000 Adam steps)."""
+    o = kpt.mlp_default_train_opts()
+    o.samples = 1 << 20
+    o.steps = 4000
+    with kpt.Context(16) as c:
+        t0 = time.perf_counter()
+        rep = c.mlp_train(o)
+        rep["wall_s"] = time.perf_counter() - t0
+        params, hull = c.mlp_get()
+    return params, hull, rep
+
+
+def test_training_reaches_exact_qlt(trained):
+    """Held-out accuracy of the BF16-MFMA network against exact FP64 slab-QLT,
+    from the training report and from an independent check on fresh samples."""
+    params, hull, rep = trained
+    print("training report:", rep)
+    assert rep["val_rms_lnr"] < 0.05 and rep["val_rms_rel_d"] < 0.05 and rep["val_rms_y1"] < 0.05, rep
+    h = hull.as_dict()
+    rng = np.random.default_rng(9)
+    mu = rng.uniform(h["mu_lo"], 1.0, 20000)
+    lxi = rng.uniform(h["log_xi_lo"], h["log_xi_hi"], 20000)
+    gk = rng.uniform(h["gamma_lo"], h["gamma_hi"], 20000)
+    x = M.features(mu, lxi, gk, h).astype(np.float32)
+    with kpt.Context(16) as c:
+        c.mlp_set(params, hull)
+        y = c.mlp_infer(x, mfma=True)
+    ln_r, y1 = M.exact_targets(mu, lxi, gk, h["h0"])
+    err = h["out_mean"] + h["out_scale"] * y[:, 0] - ln_r
+    assert np.sqrt(np.mean(err ** 2)) < 0.05
+    assert np.sqrt(np.mean((y[:, 1] - y1) ** 2)) < 0.05
+
+
+def _qlt_case(n=32768, steps=400, seed=3):
+    xe, ye, f0 = W.island_field(64, 32, seed=seed)
+    _, _, f1 = W.island_field(64, 32, seed=seed + 1)
+    s = W.c3_device_settings(xe, ye, precision=kpt.PREC_FP32)
+    s.mhd_dt = steps * s.dt_min
+    parts = W.c3_particles(n, s, seed=seed)
+    return xe, ye, f0, f1, s, parts
+
+
+def _push(xe, ye, f0, f1, s, parts, seed, net=None, noise=None):
+    with kpt.Context(len(parts["x"]), seed=seed) as c:
+        if net is not None:
+            c.mlp_set(*net)
+        c.field_upload(kpt.SLOT_LOWER, 21, xe, ye, f0)
+        c.field_upload(kpt.SLOT_UPPER, 21, xe, ye, f1)
+        c.upload(parts)
+        if noise is not None:
+            c.set_noise(noise)
+        st = c.push_interval(s, 0.0, 0)
+        return c.download(), st
+
+
+def test_mlp_push_outside_hull_equals_exact_push(trained):
+    """With the push's h0 different from the trained one every evaluation takes
+    the exact formula: the wave-uniform MLP kernel then reproduces the exact
+    FP32 QLT kernel (same noise table) to float rounding; the two kernels are
+    compiled separately, so contraction of a*b+c into FMA may differ."""
+    params, hull, _ = trained
+    xe, ye, f0, f1, s, parts = _qlt_case(n=4096, steps=20)
+    s.qlt_h0 = 2e-6
+    noise = np.random.default_rng(5).uniform(-np.sqrt(3), np.sqrt(3), (4096, 24, 4))
+    s.max_particle_steps_per_interval = 24
+    exact, _ = _push(xe, ye, f0, f1, s, parts, 1, noise=noise)
+    s2 = s.copy()
+    s2.closure = kpt.CLOSURE_MLP
+    got, st = _push(xe, ye, f0, f1, s2, parts, 1, net=(params, hull), noise=noise)
+    assert st.failures == 0
+    np.testing.assert_array_equal(got["t"], exact["t"])
+    np.testing.assert_array_equal(got["status"], exact["status"])
+    for k, scale in (("x", s.x_width), ("y", s.y_width), ("p", None), ("mu", 1.0)):
+        den = np.abs(exact[k]) if scale is None else scale
+        assert np.max(np.abs(got[k] - exact[k]) / den) < 1e-6, k
+
+
+def test_mlp_push_matches_oracle_and_reports_gain(trained):
+    """Focused transport with the learned closure (FP32, 32 768 particles, 400
+    fixed steps, p in [1, 1e3], mu uniform): pitch-angle, log-p and spatial
+    histograms against the FP64 exact-QLT oracle, chi-square p > 0.01 and KS
+    p > 0.01. Then the A/B of the FP32 push with and without the network on
+    262 144 particles of the same case."""
+    params, hull, _ = trained
+    xe, ye, f0, f1, s, parts = _qlt_case()
+    n = len(parts["x"])
+    want = {k: v.copy() for k, v in parts.items()}
+    s64 = s.copy()
+    s64.precision = kpt.PREC_FP64
+    O.push_interval(O.Field(xe, ye, f0, 21), O.Field(xe, ye, f1, 21), s64, 0.0, 0, 99, want)
+    smlp = s.copy()
+    smlp.closure = kpt.CLOSURE_MLP
+    got, st = _push(xe, ye, f0, f1, smlp, parts, 5, net=(params, hull))
+    assert st.failures == 0 and st.particle_steps == 400 * n
+    assert np.all(np.abs(got["mu"]) <= 1.0)
+
+    def hists(q):
+        hm = np.histogram(q["mu"], bins=32, range=(-1, 1))[0]
+        hp = np.histogram(np.log(q["p"]), bins=32, range=(np.log(0.9), np.log(1.1e3)))[0]
+        hxy = np.histogram2d(q["y"], q["x"], bins=(16, 32),
+                             range=((s.y_lower, s.y_upper), (s.x_lower, s.x_upper)))[0]
+        return hm, hp, hxy
+
+    for a, b in zip(hists(want), hists(got)):
+        assert chi2_two_sample(a, b)[2] > 0.01
+    for k in ("mu", "p", "y"):
+        assert stats.ks_2samp(want[k], got[k]).pvalue > 0.01, k
+
+    # A/B: the same FP32 push with the exact formula and with the network
+    big = W.c3_particles(262144, s, seed=8)
+    rates = {}
+    for name, closure in (("exact", kpt.CLOSURE_QLT), ("mlp", kpt.CLOSURE_MLP)):
+        s2 = s.copy()
+        s2.closure = closure
+        s2.push_chunks = 4
+        _, st = _push(xe, ye, f0, f1, s2, big, 7, net=(params, hull))
+        rates[name] = st.particle_steps / (st.kernel_ms * 1e-3)
+    print(f"FP32 push, exact QLT {rates['exact']:.3e} steps/s, learned closure {rates['mlp']:.3e} "
+          f"steps/s, ratio {rates['mlp'] / rates['exact']:.3f}")
+    assert rates["mlp"] > 0 and rates["exact"] > 0
