@@ -9,8 +9,10 @@ Workload `focused` (default; the headline): focused transport on the
 particles with mu ~ U[-1, 1] and p log-uniform on [1, 1e3], slab-QLT
 pitch-angle scattering (gamma_k = 5/3, sigma2 = 0.1, Lc = 0.02), fixed
 dt = 5e-5 (2000 sub-steps per interval). The measured path is the FP32 bulk
-push with the chosen closure (--closure); the same particles are then pushed by
-the plain FP64 exact-QLT push for the speed-up.
+push with the learned D_mumu closure (--closure mlp; the network is trained on
+the GPU before the timed region) or with the exact formula (--closure qlt). The
+same particles are then pushed by the FP32 exact-QLT push (when the measured
+path is the learned one) and by the plain FP64 exact-QLT push, for the ratios.
 
 Workload `parker`: Parker transport at production shape, 4 194 304 particles,
 the driver's default physics with adaptive dt, FP64 or FP32.
@@ -116,8 +118,8 @@ def copy_probe(mib=512, repeats=3):
 
 
 def timed_push(ctx, s, warmup, steps, first_frame=0):
-    """Warm-up and timed intervals; returns (steps, failures, elapsed s, kernel ms per
-    interval, state bytes of the launches, active after)."""
+    """Warm-up and timed intervals; returns (sub-steps, failures, elapsed s, kernel ms
+    per interval, (launches, particles active at entry) per interval)."""
     frame = first_frame
     for _ in range(warmup):
         st = ctx.push_interval(s, frame * s.mhd_dt, frame)
@@ -130,11 +132,25 @@ def timed_push(ctx, s, warmup, steps, first_frame=0):
         total += st.particle_steps
         fails += st.failures
         kernel_ms.append(st.kernel_ms)
-        active.append(st.active_in * st.launches)
+        active.append((st.launches, st.active_in))
         log(f"  frame {frame}: {st.particle_steps} steps, {st.kernel_ms:.1f} ms")
         frame += 1
     elapsed = time.perf_counter() - t0
     return total, fails, elapsed, kernel_ms, active
+
+
+def profiled_traffic(workload):
+    """HBM bytes per push launch of this workload from the committed rocprofv3
+    counter summaries (lines `traffic_bytes_per_launch <workload> <bytes>`), or None."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
+                    return float(parts[2])
+    return None
 
 
 def main():
@@ -150,11 +166,14 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None,
                     help="default: fp32 for focused, fp64 for parker")
-    ap.add_argument("--closure", choices=["qlt", "mlp"], default="qlt")
+    ap.add_argument("--closure", choices=["qlt", "mlp"], default="mlp")
+    ap.add_argument("--mlp-steps", type=int, default=8000, help="Adam steps of the network training")
     ap.add_argument("--no-fp64-leg", action="store_true",
                     help="focused: skip the plain FP64 push of the same particles")
+    ap.add_argument("--no-fp32-leg", action="store_true",
+                    help="focused with --closure mlp: skip the FP32 exact-QLT push of the same particles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--chunks", type=int, default=8,
+    ap.add_argument("--chunks", type=int, default=16,
                     help="time chunks per interval, each preceded by a cell sort (push_chunks)")
     args = ap.parse_args()
     if args.copy_probe:
@@ -189,8 +208,16 @@ def main():
     log(f"bench: {args.workload} {args.nx}x{args.ny}, {n} particles, {prec}"
         + (f", closure {args.closure}" if focused else ""))
 
-    fp64_leg = None
+    fp64_leg = fp32_leg = mlp_report = None
     with kpt.Context(n, seed=114514) as ctx:
+        if focused and args.closure == "mlp":
+            log("training the D_mumu network on the GPU ...")
+            o = kpt.mlp_default_train_opts()
+            o.steps = args.mlp_steps
+            t_train = time.perf_counter()
+            mlp_report = ctx.mlp_train(o)
+            mlp_report["wall_s"] = time.perf_counter() - t_train
+            log(f"  {mlp_report}")
         ctx.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
         ctx.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
         if focused:
@@ -202,6 +229,14 @@ def main():
             start = ctx.download()
         steps, failures, elapsed, kernel_ms, active = timed_push(ctx, s, args.warmup, args.steps)
         summary = ctx.summary()
+        if focused and args.closure == "mlp" and not args.no_fp32_leg:
+            log("FP32 exact-QLT push of the same particles ...")
+            s32 = s.copy()
+            s32.closure = kpt.CLOSURE_QLT
+            ctx.upload(start)
+            st32, f32, el32, km32, _ = timed_push(ctx, s32, args.warmup, args.steps)
+            fp32_leg = {"value": st32 / el32, "sub_steps": int(st32), "failures": int(f32),
+                        "kernel_ms_avg": float(np.mean(km32))}
         if focused and not args.no_fp64_leg:
             log("plain FP64 push of the same particles ...")
             s64 = s.copy()
@@ -213,13 +248,16 @@ def main():
                         "kernel_ms_avg": float(np.mean(km64))}
 
     value = steps / elapsed
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    launches = sum(a[0] for a in active)
+    avg_launch_s = float(np.sum(kernel_ms)) / 1e3 / launches
     per_step = step_bytes(model, fp32)
-    launch_bytes = (steps / args.steps) * per_step + float(np.mean(active)) * state_bytes(model)
-    achieved = launch_bytes / avg_kernel_s / 1e9
+    launch_bytes = steps / launches * per_step + float(np.mean([a[1] for a in active])) * state_bytes(model)
+    achieved = launch_bytes / avg_launch_s / 1e9
+    workload = (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}_island_{n}" if focused else
+                f"parker_{prec}_{args.nx}x{args.ny}_island_{n}")
+    traffic = profiled_traffic(workload)
     cfg = {
-        "workload": (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}_island_{n}" if focused else
-                     f"parker_{prec}_{args.nx}x{args.ny}_island_{n}"),
+        "workload": workload,
         "field": f"{args.nx}x{args.ny} uniform grid, 2 time-blended frames",
         "particles": n,
         "interval": s.mhd_dt,
@@ -231,6 +269,13 @@ def main():
     if focused:
         cfg["closure"] = args.closure
         cfg["dt"] = s.dt_min
+        if mlp_report:
+            cfg["mlp_training"] = {k: mlp_report[k] for k in ("steps", "train_ms", "final_loss",
+                                                          "val_rms_lnr", "val_max_lnr",
+                                                          "val_rms_rel_d", "val_rms_y1")}
+        if fp32_leg:
+            cfg["fp32_exact_qlt"] = fp32_leg
+            cfg["ratio_vs_fp32_exact_qlt"] = value / fp32_leg["value"]
         if fp64_leg:
             cfg["fp64_plain"] = fp64_leg
             cfg["speedup_vs_fp64_plain"] = value / fp64_leg["value"]
@@ -255,8 +300,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel_ms_avg": avg_kernel_s * 1e3,
+            "traffic": traffic,
+            "kernel": "push_kernel_mlp" if (focused and args.closure == "mlp") else "push_kernel",
+            "kernel_ms_avg": avg_launch_s * 1e3,
+            "launches": int(launches),
+            "algorithmic_bytes_per_launch": launch_bytes,
             "bytes_per_step": per_step,
         },
     }

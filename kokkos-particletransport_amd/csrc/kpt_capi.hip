@@ -1265,8 +1265,9 @@ kpt_status kpt_mlp_gradient(kpt_ctx* ctx, const float* feats, const float* targe
     if (n == 0 || n > 65536) return fail(ctx, KPT_ERR_ARG, "gradient batch must hold 1..65536 samples.");
     (void)hipSetDevice(ctx->device);
     DevBufs b;
-    float *x, *y, *z, *yhat, *dz, *dy, *g, *lp;
+    float *x, *y, *z, *yhat, *dz, *dy, *g, *gp, *lp;
     const size_t parts = kpt::mlp_loss_partials(n);
+    KPT_HIP(ctx, b.get(&gp, kpt::mlp_grad_partials(n)));
     KPT_HIP(ctx, b.get(&x, n * 3));
     KPT_HIP(ctx, b.get(&y, n * 2));
     KPT_HIP(ctx, b.get(&z, n * 3 * kpt::kMlpH));
@@ -1278,7 +1279,7 @@ kpt_status kpt_mlp_gradient(kpt_ctx* ctx, const float* feats, const float* targe
     KPT_HIP(ctx, hipMemcpyAsync(x, feats, n * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     KPT_HIP(ctx, hipMemcpyAsync(y, targets, n * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, x, n, z, yhat, ctx->stream));
-    KPT_HIP(ctx, kpt::launch_mlp_backward(ctx->mlp_params, x, z, yhat, y, n, dz, dy, g, lp, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_backward(ctx->mlp_params, x, z, yhat, y, n, dz, dy, gp, g, lp, ctx->stream));
     std::vector<float> lh(parts);
     KPT_HIP(ctx, hipMemcpyAsync(grad, g, kpt::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipMemcpyAsync(lh.data(), lp, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1322,9 +1323,10 @@ kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_r
         return fail(ctx, KPT_ERR_ARG, "training needs 1 <= batch <= 65536, batch <= samples <= 2^26, steps > 0, lr > 0.");
     (void)hipSetDevice(ctx->device);
     DevBufs b;
-    float *X, *Y, *z, *yhat, *dz, *dy, *g, *m, *v, *lp, *params;
+    float *X, *Y, *z, *yhat, *dz, *dy, *g, *gp, *m, *v, *lp, *params;
     double* raw;
     const size_t parts = kpt::mlp_loss_partials(B);
+    KPT_HIP(ctx, b.get(&gp, kpt::mlp_grad_partials(B)));
     KPT_HIP(ctx, b.get(&X, N * 3));
     KPT_HIP(ctx, b.get(&Y, N * 2));
     KPT_HIP(ctx, b.get(&raw, N));
@@ -1381,7 +1383,7 @@ kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_r
         b1c *= 0.9;
         b2c *= 0.999;
         KPT_HIP(ctx, kpt::launch_mlp_forward(params, X + 3 * off, B, z, yhat, ctx->stream));
-        KPT_HIP(ctx, kpt::launch_mlp_backward(params, X + 3 * off, z, yhat, Y + 2 * off, B, dz, dy, g, lp,
+        KPT_HIP(ctx, kpt::launch_mlp_backward(params, X + 3 * off, z, yhat, Y + 2 * off, B, dz, dy, gp, g, lp,
                                               ctx->stream));
         KPT_HIP(ctx, kpt::launch_mlp_adam(params, m, v, g, lr, static_cast<float>(1.0 - b1c),
                                           static_cast<float>(1.0 - b2c), ctx->stream));

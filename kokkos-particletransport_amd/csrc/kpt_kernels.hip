@@ -801,7 +801,10 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // the initial mu and at the two Picard trial values), lanes that are finished
 // or failed with dummy features.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_FP32)))
+#ifndef KPT_PUSH_WAVES_MLP
+#define KPT_PUSH_WAVES_MLP 2
+#endif
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
     push_kernel_mlp(PushArgs a) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     mlp_load_lds(a.mlp_image, mlp_smem);

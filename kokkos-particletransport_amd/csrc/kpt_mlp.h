@@ -114,12 +114,13 @@ hipError_t launch_mlp_normalize(float* targets, uint64_t n, float mean, float in
 hipError_t launch_mlp_forward(const float* params, const float* x, uint64_t n, float* z, float* yhat,
                               hipStream_t stream);
 hipError_t launch_mlp_backward(const float* params, const float* x, const float* z, const float* yhat,
-                               const float* y, uint64_t n, float* dz, float* dy, float* grad, float* loss_partials,
-                               hipStream_t stream);
+                               const float* y, uint64_t n, float* dz, float* dy, float* grad_partials, float* grad,
+                               float* loss_partials, hipStream_t stream);
 hipError_t launch_mlp_adam(float* params, float* m, float* v, const float* grad, float lr, float b1c, float b2c,
                            hipStream_t stream);
 hipError_t launch_mlp_infer(const void* lds_image, const float* x, uint64_t n, float* out, hipStream_t stream);
 size_t mlp_loss_partials(uint64_t n);
+size_t mlp_grad_partials(uint64_t n);  // floats of per-chunk gradient partial sums
 // Host: BF16 MFMA fragments and bias initialisers from FP32 parameters.
 void mlp_build_image(const float* params, void* image /* kMlpImageBytes */);
 

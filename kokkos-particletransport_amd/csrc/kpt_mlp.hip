@@ -151,38 +151,57 @@ __global__ void __launch_bounds__(kBlk) bwd_data_kernel(const float* __restrict_
     }
 }
 
-// One lane per parameter: gradient = sum over the batch in sample order.
+// One lane per parameter and batch chunk: partial[chunk][q] = sum over the
+// chunk's samples in sample order; grad = sum over chunks in chunk order.
+constexpr uint64_t kGradChunk = 128;
+
 __global__ void __launch_bounds__(kBlk) bwd_weight_kernel(const float* __restrict__ x, const float* __restrict__ z,
                                                           const float* __restrict__ dz, const float* __restrict__ dy,
-                                                          uint64_t n, float* __restrict__ grad) {
+                                                          uint64_t n_total, float* __restrict__ partial) {
     const int q = blockIdx.x * kBlk + threadIdx.x;
     if (q >= kMlpParams) return;
+    const uint64_t b0 = blockIdx.y * kGradChunk;
+    const uint64_t n = b0 + kGradChunk < n_total ? b0 + kGradChunk : n_total;
+    x += 3 * b0;
+    z += b0 * kZ;
+    dz += b0 * kZ;
+    dy += 2 * b0;
+    const uint64_t m = n - b0;
     float acc = 0.0f;
     if (q < kOffB1) {  // W1[i][j]
         const int i = q / kMlpIn, j = q % kMlpIn;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + i] * x[3 * b + j];
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + i] * x[3 * b + j];
     } else if (q < kOffW2) {
         const int i = q - kOffB1;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + i];
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + i];
     } else if (q < kOffB2) {
         const int i = (q - kOffW2) / kMlpH, j = (q - kOffW2) % kMlpH;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + kMlpH + i] * silu(z[b * kZ + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + kMlpH + i] * silu(z[b * kZ + j]);
     } else if (q < kOffW3) {
         const int i = q - kOffB2;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + kMlpH + i];
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + kMlpH + i];
     } else if (q < kOffB3) {
         const int i = (q - kOffW3) / kMlpH, j = (q - kOffW3) % kMlpH;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + 2 * kMlpH + i] * silu(z[b * kZ + kMlpH + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + 2 * kMlpH + i] * silu(z[b * kZ + kMlpH + j]);
     } else if (q < kOffW4) {
         const int i = q - kOffB3;
-        for (uint64_t b = 0; b < n; ++b) acc += dz[b * kZ + 2 * kMlpH + i];
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + 2 * kMlpH + i];
     } else if (q < kOffB4) {
         const int i = (q - kOffW4) / kMlpH, j = (q - kOffW4) % kMlpH;
-        for (uint64_t b = 0; b < n; ++b) acc += dy[2 * b + i] * silu(z[b * kZ + 2 * kMlpH + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dy[2 * b + i] * silu(z[b * kZ + 2 * kMlpH + j]);
     } else {
         const int i = q - kOffB4;
-        for (uint64_t b = 0; b < n; ++b) acc += dy[2 * b + i];
+        for (uint64_t b = 0; b < m; ++b) acc += dy[2 * b + i];
     }
+    partial[blockIdx.y * static_cast<uint64_t>(kMlpParams) + q] = acc;
+}
+
+__global__ void __launch_bounds__(kBlk) grad_reduce_kernel(const float* __restrict__ partial, uint32_t chunks,
+                                                           float* __restrict__ grad) {
+    const int q = blockIdx.x * kBlk + threadIdx.x;
+    if (q >= kMlpParams) return;
+    float acc = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) acc += partial[c * static_cast<uint64_t>(kMlpParams) + q];
     grad[q] = acc;
 }
 
@@ -255,14 +274,19 @@ hipError_t launch_mlp_forward(const float* params, const float* x, uint64_t n, f
 }
 
 size_t mlp_loss_partials(uint64_t n) { return blocks(n); }
+size_t mlp_grad_partials(uint64_t n) { return ((n + kGradChunk - 1) / kGradChunk) * kMlpParams; }
 
 hipError_t launch_mlp_backward(const float* params, const float* x, const float* z, const float* yhat,
-                               const float* y, uint64_t n, float* dz, float* dy, float* grad, float* loss_partials,
-                               hipStream_t stream) {
+                               const float* y, uint64_t n, float* dz, float* dy, float* grad_partials, float* grad,
+                               float* loss_partials, hipStream_t stream) {
     if (n == 0) return hipSuccess;
+    const uint32_t chunks = static_cast<uint32_t>((n + kGradChunk - 1) / kGradChunk);
     hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks(n)), dim3(kBlk), 0, stream, params, z, yhat, y, n, dz, dy,
                        loss_partials);
-    hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks(kMlpParams)), dim3(kBlk), 0, stream, x, z, dz, dy, n, grad);
+    hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks(kMlpParams), chunks), dim3(kBlk), 0, stream, x, z, dz, dy, n,
+                       grad_partials);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks(kMlpParams)), dim3(kBlk), 0, stream, grad_partials, chunks,
+                       grad);
     return hipGetLastError();
 }
 

@@ -93,7 +93,10 @@ __device__ __forceinline__ void mlp_block(const MlpLds& im, int lane, const bf16
 // the two lane halves involve all 64 lanes.
 __device__ __forceinline__ void mlp_eval_wave(const MlpLds& im, float f0, float f1, float f2, float b40, float b41,
                                               float& o0, float& o1) {
-    const int lane = static_cast<int>(threadIdx.x & 63u);
+    int lane = static_cast<int>(threadIdx.x & 63u);
+    // An opaque lane index keeps the fragment and bias reads from LDS at each
+    // evaluation: hoisted out of a caller's loop they would pin ~150 registers.
+    asm volatile("" : "+v"(lane));
     const float g0 = __shfl_xor(f0, 32), g1 = __shfl_xor(f1, 32), g2 = __shfl_xor(f2, 32);
     bf16x8 x0 = {}, x1 = {};
     if (lane < 32) {
@@ -108,6 +111,9 @@ __device__ __forceinline__ void mlp_eval_wave(const MlpLds& im, float f0, float 
     }
     float p0, p1, q0, q1;
     mlp_block(im, lane, x0, b40, b41, p0, p1);  // samples of lanes 0..31
+#ifndef KPT_MLP_INTERLEAVE_BLOCKS
+    __builtin_amdgcn_sched_barrier(0);          // one block's working set at a time
+#endif
     mlp_block(im, lane, x1, b40, b41, q0, q1);  // samples of lanes 32..63, results in lanes 0..31
     const float r0 = __shfl_xor(q0, 32), r1 = __shfl_xor(q1, 32);
     o0 = lane < 32 ? p0 : r0;
