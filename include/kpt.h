@@ -377,6 +377,58 @@ kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h
 kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint64_t* count,
                                   double charge, double split_ratio, char* err, size_t err_len);
 
+/* ---- generic Parker solver (global time step) ------------------------------------
+ * ParkerSolver::advance_stochastic (include/ParkerSolver.hpp:265-299) on the
+ * packed 18-component basis of its coefficient fields
+ * (ParkerCoefficient::make_legacy_transport_coefficient_fields,
+ * include/ParkerCoefficient.hpp:1090-1140): B, V, curl(B/B^2), the primary and
+ * secondary div(kappa_ij) bases, kappa_par and kappa_perp at gamma = 1
+ * (LegencyModel, include/LegencyModel.hpp:359-507, constant turbulence of
+ * include/TurbulenceProperties.hpp:59-210) and div V, cell-centred on the slot's
+ * uploaded B and V (no interpolation shift). One step: dt = courant * min over
+ * alive particles of the local limits (diffusion (dx/2)^2 / (2 kappa_eff),
+ * advection dx / |u|, momentum 3 / |div V|), then x += dt u + dW with
+ * Cov(dW) = 2 kappa dt split into perpendicular and field-aligned Gaussians,
+ * p = max(p - dt p div V / 3, 0), periodic wrap. Where the reference aborts
+ * (|B| = 0, invalid kappa), the building call returns an error and the step
+ * marks the particle with status 2. */
+typedef struct kpt_generic_parker_params {
+    int32_t perpendicular_model;           /* 0 README formula, 1 constant ratio  */
+    int32_t reserved;
+    double parallel_prefactor;             /* 1.622                               */
+    double perpendicular_prefactor;        /* 0.198                               */
+    double perpendicular_parallel_ratio;   /* used by the constant-ratio model    */
+    double minimum_magnetic_field_magnitude;
+    double minimum_solar_wind_speed;
+    double minimum_sigma2;
+    double minimum_magnetic_field_squared_for_drift;
+    double sigma2;                         /* constant turbulence: 0.5            */
+    double l2d;                            /* 1                                   */
+    double lslab;                          /* 1                                   */
+    double lc;                             /* <= 0: lc = lslab                    */
+    double rest_mass;                      /* ParticleProperties: 1               */
+    double charge;                         /* 1                                   */
+    double speed_of_light;                 /* 1                                   */
+    double courant;                        /* SolverBase courant number: 0.3      */
+} kpt_generic_parker_params;
+
+enum { KPT_GENERIC_PARKER_COMPONENTS = 18 };
+
+kpt_status kpt_generic_parker_default_params(kpt_generic_parker_params* out);
+/* Build the packed basis of `slot` from the B and V uploaded there. */
+kpt_status kpt_generic_parker_build(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot);
+/* The packed basis of nodes i in [-1, nx], j in [-1, ny] as
+ * double[(ny+2)][(nx+2)][18]. */
+kpt_status kpt_generic_parker_download(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len);
+/* compute_adaptive_time_step: courant * min over alive particles. */
+kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
+                                        double* dt);
+/* advance_stochastic: one step of every alive particle. `step` keys the Philox
+ * stream (id, step) and selects the row of an injected noise table, which then
+ * holds the four Gaussians (g_x, g_y, g_z, g_parallel) of each step. */
+kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
+                                   uint32_t step, double* dt_used, uint64_t* failures);
+
 /* ---- learned pitch-angle closure (KPT_CLOSURE_MLP) ------------------------------
  * No counterpart in the reference. A small network stands in for the resonance
  * part of the slab-QLT D_mumu of LegencyModel::evaluate_pitch_angle_scattering

@@ -32,6 +32,10 @@ struct FieldSlot {
     size_t nodes_cap = 0;
     size_t raw_cap = 0;
     uint64_t version = 0;  // changes with every upload into the slot
+    // generic Parker basis built from raw (kpt_generic_parker_build)
+    double* gen = nullptr;
+    size_t gen_cap = 0;
+    bool gen_valid = false;
 };
 
 }  // namespace
@@ -302,6 +306,7 @@ kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, i
     KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     f.version = ++ctx->field_versions;
+    f.gen_valid = false;
     f.valid = true;
     return KPT_OK;
 }
@@ -415,6 +420,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.raw);
         dev_free(f.c64);
         dev_free(f.c32);
+        dev_free(f.gen);
     }
     dev_free(ctx->pair32);
     dev_free(ctx->mlp_params);
@@ -1441,6 +1447,178 @@ kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_r
         report->steps = opts->steps;
         report->validation_samples = static_cast<uint32_t>(V);
     }
+    return KPT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+kpt_status check_generic(kpt_ctx* ctx, const kpt_generic_parker_params* p) {
+    if (!p) return fail(ctx, KPT_ERR_ARG, "generic Parker parameters pointer is null.");
+    // LegencyModel::DiffusionModelParameters::validate (LegencyModel.hpp:80-105)
+    if (!std::isfinite(p->parallel_prefactor) || p->parallel_prefactor <= 0.0 ||
+        !std::isfinite(p->perpendicular_prefactor) || p->perpendicular_prefactor <= 0.0 ||
+        !std::isfinite(p->perpendicular_parallel_ratio) || p->perpendicular_parallel_ratio < 0.0 ||
+        !std::isfinite(p->minimum_magnetic_field_magnitude) || p->minimum_magnetic_field_magnitude < 0.0 ||
+        !std::isfinite(p->minimum_solar_wind_speed) || p->minimum_solar_wind_speed < 0.0 ||
+        !std::isfinite(p->minimum_sigma2) || p->minimum_sigma2 < 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "LegencyModel::DiffusionModelParameters: coefficients and floors must be finite positive or "
+                    "non-negative values.");
+    if (p->perpendicular_model != 0 && p->perpendicular_model != 1)
+        return fail(ctx, KPT_ERR_ARG, "invalid perpendicular diffusion model (0 README formula, 1 constant ratio).");
+    if (p->perpendicular_model == 1 && p->perpendicular_parallel_ratio <= 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "LegencyModel::DiffusionModelParameters: constant perpendicular ratio must be positive when "
+                    "selected.");
+    // TurbulencePropertyModel::validate and ParticleProperties::validate
+    if (!std::isfinite(p->sigma2) || p->sigma2 <= 0.0 || !std::isfinite(p->l2d) || p->l2d <= 0.0 ||
+        !std::isfinite(p->lslab) || p->lslab <= 0.0 || !std::isfinite(p->lc))
+        return fail(ctx, KPT_ERR_ARG, "TurbulencePropertyModel: coefficients must be positive finite values.");
+    if (!std::isfinite(p->rest_mass) || p->rest_mass <= 0.0 || !std::isfinite(p->charge) ||
+        !std::isfinite(p->speed_of_light) || p->speed_of_light <= 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "ParticleProperties: rest mass and speed of light must be positive finite dimensionless values, "
+                    "charge must be finite, and the energy scale must be positive and finite.");
+    if (p->charge == 0.0)
+        return fail(ctx, KPT_ERR_ARG, "ParkerSolver: Parker drift requires a finite nonzero particle charge.");
+    if (!std::isfinite(p->courant) || p->courant <= 0.0)
+        return fail(ctx, KPT_ERR_ARG, "ParkerSolver: Courant scale must be positive and finite.");
+    if (!std::isfinite(p->minimum_magnetic_field_squared_for_drift) || p->minimum_magnetic_field_squared_for_drift < 0.0)
+        return fail(ctx, KPT_ERR_ARG, "ParkerCoefficient: curl(B/B^2) floor must be finite and non-negative.");
+    return KPT_OK;
+}
+
+kpt_status generic_slot(kpt_ctx* ctx, int32_t slot, const FieldSlot** out) {
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    const FieldSlot& f = ctx->slot[slot];
+    if (!f.valid || !f.gen_valid)
+        return fail(ctx, KPT_ERR_STATE, "ParkerSolver: coefficient fields are not configured.");
+    *out = &f;
+    return KPT_OK;
+}
+
+// courant * min over alive particles; 0 when none gives a finite bound.
+kpt_status generic_dt(kpt_ctx* ctx, const kpt_generic_parker_params* p, const FieldSlot& f, double* dt) {
+    const double large = 1.0e300;
+    unsigned long long bits;
+    std::memcpy(&bits, &large, 8);
+    KPT_HIP(ctx, hipMemcpyAsync(ctx->counters + 6, &bits, 8, hipMemcpyHostToDevice, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_generic_min_dt(ctx->P, ctx->count, f.gen, f.g, *p, ctx->counters + 6, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(&bits, ctx->counters + 6, 8, hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double m;
+    std::memcpy(&m, &bits, 8);
+    *dt = (!std::isfinite(m) || m >= 0.5 * large) ? 0.0 : p->courant * m;
+    return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_generic_parker_default_params(kpt_generic_parker_params* p) {
+    if (!p) return KPT_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->perpendicular_model = 0;
+    p->parallel_prefactor = 1.622;   // DiffusionModelParameters (LegencyModel.hpp:27-35)
+    p->perpendicular_prefactor = 0.198;
+    p->sigma2 = 0.5;                 // TurbulencePropertyModel (TurbulenceProperties.hpp:59-70)
+    p->l2d = 1.0;
+    p->lslab = 1.0;
+    p->lc = -1.0;
+    p->rest_mass = 1.0;              // ParticleProperties (ParticleSystem.hpp:145-150)
+    p->charge = 1.0;
+    p->speed_of_light = 1.0;
+    p->courant = 0.3;                // SolverBase (SolverBase.hpp:156)
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_parker_build(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_generic(ctx, p);
+    if (st != KPT_OK) return st;
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    FieldSlot& f = ctx->slot[slot];
+    if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    (void)hipSetDevice(ctx->device);
+    const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_PARKER_COMPONENTS;
+    if (elems > f.gen_cap) {
+        dev_free(f.gen);
+        f.gen = nullptr;
+        f.gen_cap = 0;
+        KPT_HIP(ctx, dev_alloc(&f.gen, elems));
+        f.gen_cap = elems;
+    }
+    f.gen_valid = false;
+    int* bad = reinterpret_cast<int*>(ctx->counters + 7);
+    KPT_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_generic_pack(f.raw, f.g, *p, f.gen, bad, ctx->stream));
+    int h = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h)
+        return fail(ctx, KPT_ERR_ARG,
+                    "ParkerCoefficient / LegencyModel: |B| must be positive (or floored) and V, Lc, Omega and sigma2 "
+                    "positive wherever the coefficients are evaluated.");
+    f.gen_valid = true;
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_parker_download(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len) {
+    if (!ctx || !out) return KPT_ERR_ARG;
+    const FieldSlot* f;
+    kpt_status st = generic_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    const size_t plane = kpt::coefficient_plane(f->g.nx, f->g.ny);
+    const int C = KPT_GENERIC_PARKER_COMPONENTS;
+    if (out_len < plane * C) return fail(ctx, KPT_ERR_ARG, "output buffer too small.");
+    std::vector<double> tmp(plane * C);
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemcpy(tmp.data(), f->gen, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t n = 0; n < plane; ++n)
+        for (int k = 0; k < C; ++k) out[n * C + k] = tmp[k * plane + n];
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
+                                        double* dt) {
+    if (!ctx || !dt) return KPT_ERR_ARG;
+    kpt_status st = check_generic(ctx, p);
+    if (st != KPT_OK) return st;
+    const FieldSlot* f;
+    st = generic_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    (void)hipSetDevice(ctx->device);
+    return generic_dt(ctx, p, *f, dt);
+}
+
+kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot, uint32_t step,
+                                   double* dt_used, uint64_t* failures) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_generic(ctx, p);
+    if (st != KPT_OK) return st;
+    const FieldSlot* f;
+    st = generic_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    if (ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
+        return fail(ctx, KPT_ERR_STATE, "noise table does not cover this step of every stored particle.");
+    (void)hipSetDevice(ctx->device);
+    double dt = 0.0;
+    st = generic_dt(ctx, p, *f, &dt);
+    if (st != KPT_OK) return st;
+    if (!std::isfinite(dt) || dt <= 0.0)
+        return fail(ctx, KPT_ERR_STATE, "ParkerSolver: adaptive stochastic time step is not positive and finite.");
+    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters + 5, 0, sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_generic_step(ctx->P, ctx->count, f->gen, f->g, *p, dt, step, key, ctx->noise,
+                                          ctx->noise_steps, ctx->counters + 5, ctx->stream));
+    unsigned long long nf = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&nf, ctx->counters + 5, 8, hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (dt_used) *dt_used = dt;
+    if (failures) *failures = nf;
     return KPT_OK;
 }
 

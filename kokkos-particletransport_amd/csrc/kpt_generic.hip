@@ -1,0 +1,349 @@
+// kpt_generic.hip — the generic Parker solver on the GPU (FP64): its packed
+// coefficient basis and its global-time-step stochastic step.
+//
+// Basis (ParkerSolver::build_packed_coefficient_basis, include/ParkerSolver.hpp:
+// 452-640, over ParkerCoefficient::make_legacy_transport_coefficient_fields,
+// include/ParkerCoefficient.hpp:1090-1140), cell-centred, component-planar like
+// the driver frames (plane k, nodes i in [-1, nx], j in [-1, ny]):
+//   0-2 B, 3-5 V, 6-8 curl(B / B^2), 9-11 primary and 12-14 secondary
+//   div(kappa_ij) basis, 15 kappa_par(gamma = 1), 16 kappa_perp(gamma = 1), 17 div V.
+// Each stored node is computed at its periodic image in the physical domain, so
+// ghost nodes equal the reference's periodic ghost fill exactly. The grid is the
+// driver's uniform Cartesian one (Lame coefficients 1, Jacobian 1); z is the
+// invariant direction.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kpt_device_common.h"
+#include "kpt_grid_device.h"
+#include "kpt_internal.h"
+
+namespace kpt {
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kGC = KPT_GENERIC_PARKER_COMPONENTS;
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    const int w = i % n;
+    return w < 0 ? w + n : w;
+}
+
+// Cell-centre coordinate (AnalyticCoordinateGrid, include/AnalyticCoordinateGrid.hpp:159-176).
+__device__ __forceinline__ double cell_coord(const GridDesc& g, int d, int k) {
+    return 0.5 * ((g.start[d] + k * g.step[d]) + (g.start[d] + (k + 1.0) * g.step[d]));
+}
+
+// centered_three_point_derivative (include/ParkerCoefficient.hpp:215-233).
+__device__ __forceinline__ double three_point(double xl, double xc, double xu, double fl, double fc, double fu) {
+    const double lw = xc - xl, uw = xu - xc, tw = xu - xl;
+    return (-uw / (lw * tw)) * fl + ((uw - lw) / (lw * uw)) * fc + (lw / (uw * tw)) * fu;
+}
+
+struct RawView {
+    const double* raw;  // 6 planes of nx * ny (Bx, By, Bz, Vx, Vy, Vz), x fastest
+    int nx, ny;
+    __device__ double at(int c, int i, int j) const {
+        return raw[static_cast<int64_t>(c) * nx * ny + static_cast<int64_t>(wrapi(j, ny)) * nx + wrapi(i, nx)];
+    }
+};
+
+struct PointEval {
+    const RawView& r;
+    const kpt_generic_parker_params& p;
+    int* bad;
+
+    __device__ double bmag(int i, int j) const {
+        double s = 0.0;
+        for (int c = 0; c < 3; ++c) s += r.at(c, i, j) * r.at(c, i, j);
+        return sqrt(s);
+    }
+    // magnetic_direction_component (ParkerCoefficient.hpp:255-270)
+    __device__ double bdir(int c, int i, int j) const {
+        const double m = bmag(i, j);
+        const double f = p.minimum_magnetic_field_magnitude;
+        if (m <= 0.0 && f <= 0.0) *bad = 1;
+        return r.at(c, i, j) / (m > f ? m : f);
+    }
+    // magnetic_field_over_magnitude_squared_component (ParkerCoefficient.hpp:279-306)
+    __device__ double over_b2(int c, int i, int j) const {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += r.at(k, i, j) * r.at(k, i, j);
+        const double f = p.minimum_magnetic_field_squared_for_drift;
+        if (s <= 0.0 && f <= 0.0) *bad = 1;
+        return r.at(c, i, j) / (s > f ? s : f);
+    }
+    // evaluate_gamma_independent_coefficients (LegencyModel.hpp:430-470) with the
+    // constant turbulence state of TurbulencePropertyModel::evaluate_constant.
+    __device__ void kappa(int i, int j, double& kpar, double& kperp) const {
+        auto floor_pos = [](double v, double f) { return f <= 0.0 ? v : (v < f ? f : v); };
+        double vs = 0.0;
+        for (int c = 3; c < 6; ++c) vs += r.at(c, i, j) * r.at(c, i, j);
+        const double b = floor_pos(bmag(i, j), p.minimum_magnetic_field_magnitude);
+        const double v = floor_pos(sqrt(vs), p.minimum_solar_wind_speed);
+        const double s2 = floor_pos(p.sigma2, p.minimum_sigma2);
+        const double lc = p.lc > 0.0 ? p.lc : p.lslab;
+        if (b <= 0.0) *bad = 1;
+        const double omega = fabs(p.charge) * b / (p.rest_mass * p.speed_of_light);
+        if (v <= 0.0 || lc <= 0.0 || omega <= 0.0 || s2 <= 0.0) *bad = 1;
+        kpar = p.parallel_prefactor * pow(v, 4.0 / 3.0) * pow(lc, 2.0 / 3.0) / (pow(omega, 1.0 / 3.0) * s2);
+        if (p.perpendicular_model == 1) {
+            kperp = p.perpendicular_parallel_ratio * kpar;
+        } else {
+            if (p.l2d <= 0.0 || kpar <= 0.0) *bad = 1;
+            kperp = (v / 3.0) * pow(3.0 * kpar / v, 1.0 / 3.0) * pow(p.perpendicular_prefactor * s2 * p.l2d, 2.0 / 3.0);
+        }
+    }
+    // diffusion_tensor_basis_component (ParkerCoefficient.hpp:611-640):
+    // basis 0 parallel, 1 perpendicular, 2 constant ratio
+    __device__ double tensor(int basis, int a, int b, int i, int j) const {
+        const double ba = bdir(a, i, j), bb = bdir(b, i, j);
+        const double proj = ba * bb, id = a == b ? 1.0 : 0.0;
+        double kpar, kperp;
+        kappa(i, j, kpar, kperp);
+        if (basis == 0) return kpar * proj;
+        if (basis == 1) return kperp * (id - proj);
+        const double rr = p.perpendicular_parallel_ratio;
+        return kpar * (rr * id + (1.0 - rr) * proj);
+    }
+};
+
+// Central derivative along dim of f(i, j) at physical (i, j); z -> 0.
+template <typename F>
+__device__ __forceinline__ double deriv(const GridDesc& g, int i, int j, int dim, F f) {
+    if (dim >= 2 || (dim == 0 ? g.nx : g.ny) <= 1) return 0.0;
+    const int c = dim == 0 ? i : j;
+    const double fl = dim == 0 ? f(i - 1, j) : f(i, j - 1);
+    const double fu = dim == 0 ? f(i + 1, j) : f(i, j + 1);
+    return three_point(cell_coord(g, dim, c - 1), cell_coord(g, dim, c), cell_coord(g, dim, c + 1), fl, f(i, j), fu);
+}
+
+__global__ void __launch_bounds__(kBlk) generic_pack_kernel(const double* __restrict__ raw, GridDesc g,
+                                                            kpt_generic_parker_params p, uint64_t plane,
+                                                            double* __restrict__ out, int* __restrict__ bad) {
+    const int nxp = g.nx + 2, nyp = g.ny + 2;
+    const int64_t node = blockIdx.x * static_cast<int64_t>(kBlk) + threadIdx.x;
+    if (node >= static_cast<int64_t>(nxp) * nyp) return;
+    const int i = wrapi(static_cast<int>(node % nxp) - 1, g.nx);
+    const int j = wrapi(static_cast<int>(node / nxp) - 1, g.ny);
+    const RawView r{raw, g.nx, g.ny};
+    int flag = 0;
+    const PointEval e{r, p, &flag};
+    double v[kGC];
+    for (int c = 0; c < 3; ++c) {
+        v[c] = r.at(c, i, j);
+        v[3 + c] = r.at(3 + c, i, j);
+    }
+    for (int c = 0; c < 3; ++c) {
+        const int da = (c + 1) % 3, db = (c + 2) % 3;
+        v[6 + c] = deriv(g, i, j, da, [&](int a, int b) { return e.over_b2(db, a, b); }) -
+                   deriv(g, i, j, db, [&](int a, int b) { return e.over_b2(da, a, b); });
+    }
+    const bool ratio = p.perpendicular_model == 1;
+    const int b0 = ratio ? 2 : 0;
+    for (int c = 0; c < 3; ++c) {
+        double prim = 0.0, sec = 0.0;
+        for (int d = 0; d < 2; ++d) {
+            prim += deriv(g, i, j, d, [&](int a, int b) { return e.tensor(b0, c, d, a, b); });
+            if (!ratio) sec += deriv(g, i, j, d, [&](int a, int b) { return e.tensor(1, c, d, a, b); });
+        }
+        v[9 + c] = prim;
+        v[12 + c] = ratio ? prim : sec;
+    }
+    e.kappa(i, j, v[15], v[16]);
+    double s = 0.0;
+    for (int d = 0; d < 2; ++d) s += deriv(g, i, j, d, [&](int a, int b) { return r.at(3 + d, a, b); });
+    v[17] = s;
+    for (int k = 0; k < kGC; ++k) out[k * plane + node] = v[k];
+    if (flag) atomicOr(bad, 1);
+}
+
+struct GenTerms {
+    double u[2], b[3], kpar, kperp, keff[2], divv, p, prate;
+};
+
+// evaluate_terms_from_packed (ParkerSolver.hpp:771-875) at a wrapped position.
+__device__ __forceinline__ bool generic_terms(const double* __restrict__ f, uint64_t plane, const GridDesc& g,
+                                              const kpt_generic_parker_params& P, double x, double y, double p_in,
+                                              GenTerms& T) {
+    int i0, j0;
+    double wx, wy;
+    if (!axis_stencil_cc(x, g, 0, g.nx, i0, wx) || !axis_stencil_cc(y, g, 1, g.ny, j0, wy)) return false;
+    const int nxp = g.nx + 2;
+    const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
+    const double w00 = (1.0 - wx) * (1.0 - wy), w10 = wx * (1.0 - wy), w01 = (1.0 - wx) * wy, w11 = wx * wy;
+    double v[kGC];
+#pragma unroll
+    for (int k = 0; k < kGC; ++k) {
+        const double* q = f + k * plane + n00;
+        v[k] = w00 * q[0] + w10 * q[1] + w01 * q[nxp] + w11 * q[nxp + 1];
+    }
+    const double pm = fabs(p_in);
+    const double nm = pm / (P.rest_mass * P.speed_of_light);
+    const double gamma = sqrt(1.0 + nm * nm);
+    const double speed = pm / (gamma * P.rest_mass);
+    const double bm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(isfinite(bm) && bm > 0.0)) return false;
+    const double ib = 1.0 / bm;
+    for (int c = 0; c < 3; ++c) T.b[c] = v[c] * ib;
+    const double fpar = pow(gamma, 1.0 / 3.0);
+    const bool ratio = P.perpendicular_model == 1;
+    const double fperp = ratio ? fpar : pow(gamma, 1.0 / 9.0);
+    T.kpar = v[15] * fpar;
+    T.kperp = v[16] * fperp;
+    if (!(isfinite(T.kpar) && T.kpar >= 0.0 && isfinite(T.kperp) && T.kperp >= 0.0)) return false;
+    for (int d = 0; d < 2; ++d) {
+        T.keff[d] = T.kperp + (T.kpar - T.kperp) * T.b[d] * T.b[d];
+        if (!(isfinite(T.keff[d]) && T.keff[d] >= 0.0)) return false;
+    }
+    T.divv = v[17];
+    const double drift = pm * speed * P.speed_of_light / (3.0 * P.charge);
+    for (int d = 0; d < 2; ++d) {
+        const double adv = ratio ? fpar * v[9 + d] : fpar * v[9 + d] + fperp * v[12 + d];
+        T.u[d] = v[3 + d] + drift * v[6 + d] + adv;
+    }
+    T.p = pm;
+    T.prate = -(pm / 3.0) * T.divv;
+    return true;
+}
+
+// local_stable_time_step (ParkerSolver.hpp:956-1007).
+__device__ __forceinline__ double generic_local_dt(const GridDesc& g, const GenTerms& T) {
+    const double large = 1.0e300;
+    double ddt = large, adt = large;
+    for (int d = 0; d < 2; ++d) {
+        const double dx = fabs(g.step[d]);
+        double di = large, ai = large;
+        if (T.keff[d] > 0.0) {
+            const double lim = 0.5 * dx;
+            di = lim * lim / (2.0 * T.keff[d]);
+        }
+        const double a = fabs(T.u[d]);
+        if (a > 0.0) ai = 1.0 * dx / a;
+        ddt = ddt < di ? ddt : di;
+        adt = adt < ai ? adt : ai;
+    }
+    double mdt = large;
+    const double dvm = fabs(T.divv);
+    if (dvm > 0.0) mdt = 3.0 * 1.0 / dvm;
+    const double m = ddt < adt ? ddt : adt;
+    return m < mdt ? m : mdt;
+}
+
+// MappedCoordinateGrid::wrap_coordinate for the physical domain of axis d.
+__device__ __forceinline__ double wrap_axis(const GridDesc& g, int d, double x) {
+    double w = x - floor((x - g.phys_lo[d]) * g.inv_width[d]) * g.width[d];
+    if (w >= g.phys_hi[d]) w -= g.width[d];
+    if (w < g.phys_lo[d]) w += g.width[d];
+    return w;
+}
+
+__global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint64_t count,
+                                                          const double* __restrict__ f, uint64_t plane, GridDesc g,
+                                                          kpt_generic_parker_params p,
+                                                          unsigned long long* __restrict__ min_bits) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
+    double m = 1.0e300;
+    if (i < count && P.status[i] == 0) {
+        GenTerms T;
+        if (generic_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], T))
+            m = generic_local_dt(g, T);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o < m ? o : m;
+    }
+    // positive doubles order like their bit patterns: the minimum is exact and
+    // independent of the order of the atomics
+    if ((threadIdx.x & 63) == 0) atomicMin(min_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+__global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, uint64_t count,
+                                                            const double* __restrict__ f, uint64_t plane, GridDesc g,
+                                                            kpt_generic_parker_params p, double dt, uint32_t step,
+                                                            uint32_t k0, uint32_t k1, const double* __restrict__ noise,
+                                                            uint32_t noise_steps,
+                                                            unsigned long long* __restrict__ failures) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
+    uint32_t fail = 0;
+    if (i < count && P.status[i] == 0) {
+        const double x = wrap_axis(g, 0, P.x[i]), y = wrap_axis(g, 1, P.y[i]);
+        GenTerms T;
+        if (!generic_terms(f, plane, g, p, x, y, P.p[i], T)) {
+            P.status[i] = 2;
+            fail = 1;
+        } else {
+            double n[4];
+            if (noise != nullptr) {
+                const double* row = noise + (i * noise_steps + step) * 4;
+                for (int d = 0; d < 4; ++d) n[d] = row[d];
+            } else {
+                const uint64_t id = P.id[i];
+                uint32_t r[4];
+                philox4x32_10(static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), 0x47454E50u, step, k0, k1,
+                              r);
+                for (int k = 0; k < 2; ++k) {
+                    const double rad = sqrt(-2.0 * log(u01(r[2 * k])));
+                    const double ang = 6.283185307179586476925286766559 * u01(r[2 * k + 1]);
+                    n[2 * k] = rad * cos(ang);
+                    n[2 * k + 1] = rad * sin(ang);
+                }
+            }
+            // sample_magnetic_diffusion_increment (ParkerSolver.hpp:443-477)
+            double iso = 0.0;
+            for (int c = 0; c < 3; ++c) iso += n[c] * T.b[c];
+            const double sperp = T.kperp > 0.0 ? sqrt(2.0 * T.kperp * dt) : 0.0;
+            const double spar = T.kpar > 0.0 ? sqrt(2.0 * T.kpar * dt) : 0.0;
+            const double repl = spar * n[3] - sperp * iso;
+            double pos[2] = {x, y};
+            for (int d = 0; d < 2; ++d) {
+                const double det = dt * T.u[d];
+                const double sto = sperp * n[d] + repl * T.b[d];
+                pos[d] += det + sto;
+            }
+            P.x[i] = wrap_axis(g, 0, pos[0]);
+            P.y[i] = wrap_axis(g, 1, pos[1]);
+            const double pn = T.p + dt * T.prate;
+            P.p[i] = pn > 0.0 ? pn : 0.0;
+        }
+    }
+    unsigned int w = fail;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) w += __shfl_xor(w, off, 64);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(failures, static_cast<unsigned long long>(w));
+}
+
+inline unsigned blocks(uint64_t n) { return static_cast<unsigned>((n + kBlk - 1) / kBlk); }
+
+}  // namespace
+
+hipError_t launch_generic_pack(const double* raw, const GridDesc& g, const kpt_generic_parker_params& p,
+                               double* out, int* bad, hipStream_t stream) {
+    const uint64_t plane = coefficient_plane(g.nx, g.ny);
+    hipLaunchKernelGGL(generic_pack_kernel, dim3(blocks(plane)), dim3(kBlk), 0, stream, raw, g, p, plane, out, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_generic_min_dt(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                                 const kpt_generic_parker_params& p, unsigned long long* min_bits,
+                                 hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(generic_dt_kernel, dim3(blocks(count)), dim3(kBlk), 0, stream, P, count, f,
+                       coefficient_plane(g.nx, g.ny), g, p, min_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                               const kpt_generic_parker_params& p, double dt, uint32_t step, const uint32_t key[2],
+                               const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                               hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(generic_step_kernel, dim3(blocks(count)), dim3(kBlk), 0, stream, P, count, f,
+                       coefficient_plane(g.nx, g.ny), g, p, dt, step, key[0], key[1], noise, noise_steps, failures);
+    return hipGetLastError();
+}
+
+}  // namespace kpt

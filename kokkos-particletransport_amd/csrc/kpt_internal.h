@@ -128,6 +128,17 @@ struct PushArgs {
 hipError_t launch_pack(const double* raw /* 6 planes nx*ny */, const GridDesc& g, int model,
                        double* out64, float* out32, hipStream_t stream);
 hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream);
+// Generic Parker solver (kpt_generic.hip): packed 18-component basis, global
+// minimum of the local time-step bounds (as double bits), one stochastic step.
+hipError_t launch_generic_pack(const double* raw, const GridDesc& g, const kpt_generic_parker_params& p,
+                               double* out, int* bad, hipStream_t stream);
+hipError_t launch_generic_min_dt(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                                 const kpt_generic_parker_params& p, unsigned long long* min_bits,
+                                 hipStream_t stream);
+hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                               const kpt_generic_parker_params& p, double dt, uint32_t step, const uint32_t key[2],
+                               const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                               hipStream_t stream);
 // out[2 e + 0] = lo[e], out[2 e + 1] = hi[e] for e < n
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream);
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,

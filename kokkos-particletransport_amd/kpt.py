@@ -134,6 +134,25 @@ class InjectOpts(C.Structure):
     ]
 
 
+class GenericParkerParams(C.Structure):
+    """kpt_generic_parker_params: model controls of the generic Parker solver."""
+    _fields_ = [
+        ("perpendicular_model", C.c_int32), ("reserved", C.c_int32),
+        ("parallel_prefactor", C.c_double), ("perpendicular_prefactor", C.c_double),
+        ("perpendicular_parallel_ratio", C.c_double),
+        ("minimum_magnetic_field_magnitude", C.c_double), ("minimum_solar_wind_speed", C.c_double),
+        ("minimum_sigma2", C.c_double), ("minimum_magnetic_field_squared_for_drift", C.c_double),
+        ("sigma2", C.c_double), ("l2d", C.c_double), ("lslab", C.c_double), ("lc", C.c_double),
+        ("rest_mass", C.c_double), ("charge", C.c_double), ("speed_of_light", C.c_double),
+        ("courant", C.c_double),
+    ]
+
+    def copy(self) -> "GenericParkerParams":
+        c = GenericParkerParams()
+        C.pointer(c)[0] = self
+        return c
+
+
 class MlpHull(C.Structure):
     """kpt_mlp_hull: training hull and output scaling of the pitch-angle network."""
     _fields_ = [
@@ -254,6 +273,14 @@ EXPORTS = {
     "kpt_snapshot_read_host": (C.c_int, [C.c_char_p, C.POINTER(ParticlesHost),
                                          C.POINTER(C.c_uint64), C.c_double, C.c_double,
                                          C.c_char_p, C.c_size_t]),
+    "kpt_generic_parker_default_params": (C.c_int, [C.POINTER(GenericParkerParams)]),
+    "kpt_generic_parker_build": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32]),
+    "kpt_generic_parker_download": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                              C.c_size_t]),
+    "kpt_generic_parker_time_step": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32,
+                                               C.POINTER(C.c_double)]),
+    "kpt_generic_parker_step": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32,
+                                          C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kpt_mlp_default_train_opts": (C.c_int, [C.POINTER(MlpTrainOpts)]),
     "kpt_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(MlpTrainOpts), C.POINTER(MlpReport)]),
     "kpt_mlp_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.POINTER(MlpHull)]),
@@ -299,6 +326,12 @@ def make_device_settings(host: HostSettings, x_edges, y_edges) -> DeviceSettings
     _check_static(load().kpt_make_device_settings(C.byref(host), len(xe) - 1, len(ye) - 1,
                                                   _dp(xe), _dp(ye), C.byref(out)))
     return out
+
+
+def generic_default_params() -> GenericParkerParams:
+    p = GenericParkerParams()
+    _check_static(load().kpt_generic_parker_default_params(C.byref(p)))
+    return p
 
 
 def mlp_default_train_opts() -> MlpTrainOpts:
@@ -512,6 +545,27 @@ class Context:
 
     def clear_noise(self):
         self._check(self.lib.kpt_clear_noise(self.h))
+
+    # ---- generic Parker solver ------------------------------------------------
+    def generic_build(self, params: "GenericParkerParams", slot=0):
+        self._check(self.lib.kpt_generic_parker_build(self.h, C.byref(params), slot))
+
+    def generic_coefficients(self, slot, nx, ny):
+        out = np.empty((ny + 2) * (nx + 2) * 18)
+        self._check(self.lib.kpt_generic_parker_download(self.h, slot, _dp(out), out.size))
+        return out.reshape(ny + 2, nx + 2, 18)
+
+    def generic_time_step(self, params: "GenericParkerParams", slot=0) -> float:
+        dt = C.c_double()
+        self._check(self.lib.kpt_generic_parker_time_step(self.h, C.byref(params), slot, C.byref(dt)))
+        return dt.value
+
+    def generic_step(self, params: "GenericParkerParams", step, slot=0):
+        """One advance_stochastic step; returns (dt, failures)."""
+        dt, nf = C.c_double(), C.c_uint64()
+        self._check(self.lib.kpt_generic_parker_step(self.h, C.byref(params), slot, step, C.byref(dt),
+                                                     C.byref(nf)))
+        return dt.value, nf.value
 
     # ---- learned pitch-angle closure -----------------------------------------
     def mlp_train(self, opts: "MlpTrainOpts | None" = None) -> dict:

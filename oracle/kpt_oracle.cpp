@@ -950,3 +950,353 @@ void kpto_summary(const kpt_particles_host* P, uint64_t count, kpt_summary* s) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Generic ParkerSolver path (include/ParkerSolver.hpp, ParkerCoefficient.hpp,
+// LegencyModel.hpp, FieldOperator.hpp) on the uniform periodic 2-D grid, three
+// vector components, cell-centred fields with two ghost layers.
+// ===========================================================================
+namespace {
+
+// centered_three_point_derivative (include/ParkerCoefficient.hpp:215-233,
+// include/FieldOperator.hpp:86-103).
+double centered_three_point(double xl, double xc, double xu, double fl, double fc, double fu) {
+    const double lw = xc - xl, uw = xu - xc, tw = xu - xl;
+    return (-uw / (lw * tw)) * fl + ((uw - lw) / (lw * uw)) * fc + (lw / (uw * tw)) * fu;
+}
+
+// Central derivative along `dim` (0, 1; 0 for the invariant z) of a per-point
+// quantity f(i, j) at physical point (i, j).
+template <typename F>
+double gen_derivative(const OGrid& g, int i, int j, int dim, F f) {
+    if (dim >= 2 || g.n[dim] <= 1) return 0.0;
+    int lo[2] = {i, j}, up[2] = {i, j};
+    lo[dim] -= 1;
+    up[dim] += 1;
+    const int c = dim == 0 ? i : j;
+    return centered_three_point(g.cell(dim, c - 1), g.cell(dim, c), g.cell(dim, c + 1), f(lo[0], lo[1]),
+                                f(i, j), f(up[0], up[1]));
+}
+
+double gen_bmag(const OField& B, int i, int j) {
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) s += B.at(i, j, c) * B.at(i, j, c);
+    return std::sqrt(s);
+}
+
+// magnetic_direction_component (ParkerCoefficient.hpp:255-270).
+double gen_bdir(const OField& B, int i, int j, int c, double bmin, bool& bad) {
+    const double m = gen_bmag(B, i, j);
+    if (m <= 0.0 && bmin <= 0.0) bad = true;
+    return B.at(i, j, c) / (m > bmin ? m : bmin);
+}
+
+// LegencyModel::parallel_diffusion_gamma_one / perpendicular_diffusion_gamma_one
+// (LegencyModel.hpp:359-424) with the floors of evaluate_gamma_independent_coefficients.
+void gen_kappa(const kpt_generic_parker_params& P, double bmag, double vsw, double& kpar, double& kperp,
+               bool& bad) {
+    auto floor_pos = [](double v, double f) { return f <= 0.0 ? v : (v < f ? f : v); };
+    const double b = floor_pos(bmag, P.minimum_magnetic_field_magnitude);
+    const double v = floor_pos(vsw, P.minimum_solar_wind_speed);
+    const double s2 = floor_pos(P.sigma2, P.minimum_sigma2);
+    const double lc = P.lc > 0.0 ? P.lc : P.lslab;
+    if (b <= 0.0) bad = true;
+    const double omega = std::fabs(P.charge) * b / (P.rest_mass * P.speed_of_light);
+    if (v <= 0.0 || lc <= 0.0 || omega <= 0.0 || s2 <= 0.0) bad = true;
+    kpar = P.parallel_prefactor * std::pow(v, 4.0 / 3.0) * std::pow(lc, 2.0 / 3.0) /
+           (std::pow(omega, 1.0 / 3.0) * s2);
+    if (P.perpendicular_model == 1) {
+        kperp = P.perpendicular_parallel_ratio * kpar;
+    } else {
+        if (P.l2d <= 0.0 || kpar <= 0.0) bad = true;
+        kperp = (v / 3.0) * std::pow(3.0 * kpar / v, 1.0 / 3.0) *
+                std::pow(P.perpendicular_prefactor * s2 * P.l2d, 2.0 / 3.0);
+    }
+}
+
+OField gen_alloc(const OGrid& g, int comps) {
+    OField f;
+    f.g = g;
+    f.comps = comps;
+    f.ext[0] = g.n[0] + 2 * kGhost;
+    f.ext[1] = g.n[1] + 2 * kGhost;
+    f.data.assign(static_cast<size_t>(f.ext[0]) * f.ext[1] * comps, 0.0);
+    return f;
+}
+
+// make_legacy_transport_coefficient_fields (ParkerCoefficient.hpp:1090-1140) packed
+// as ParkerSolver::build_packed_coefficient_basis (ParkerSolver.hpp:452-640):
+// B(0-2) V(3-5) curl(B/B^2)(6-8) primary(9-11) secondary(12-14) kpar1 kperp1 divV.
+bool gen_build(const OField& B, const OField& V, const kpt_generic_parker_params& P, OField& out) {
+    const OGrid& g = B.g;
+    const int nx = g.n[0], ny = g.n[1];
+    bool bad = false;
+    OField curl = gen_alloc(g, 3), kap = gen_alloc(g, 2), div = gen_alloc(g, 6), dv = gen_alloc(g, 1);
+    // curl(B / B^2) (ParkerCoefficient.hpp:390-419, :476-511)
+    auto over_b2 = [&](int c) {
+        return [&, c](int i, int j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += B.at(i, j, k) * B.at(i, j, k);
+            const double fl = P.minimum_magnetic_field_squared_for_drift;
+            if (s <= 0.0 && fl <= 0.0) bad = true;
+            return B.at(i, j, c) / (s > fl ? s : fl);
+        };
+    };
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const int da = (c + 1) % 3, db = (c + 2) % 3;
+                curl.at(i, j, c) = gen_derivative(g, i, j, da, over_b2(db)) - gen_derivative(g, i, j, db, over_b2(da));
+            }
+    fill_ghosts_periodic(curl);
+    // gamma-independent kappa fields (LegencyModel.hpp:801-871)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double vs = 0.0;
+            for (int k = 0; k < 3; ++k) vs += V.at(i, j, k) * V.at(i, j, k);
+            gen_kappa(P, gen_bmag(B, i, j), std::sqrt(vs), kap.at(i, j, 0), kap.at(i, j, 1), bad);
+        }
+    fill_ghosts_periodic(kap);
+    // div(kappa_ij) bases (ParkerCoefficient.hpp:611-640, :800-869): Cartesian,
+    // so div_i = sum_j d_j T_ij. basis 0 parallel, 1 perpendicular, 2 constant ratio.
+    auto tensor = [&](int basis, int a, int b) {
+        return [&, basis, a, b](int i, int j) {
+            const double bi = gen_bdir(B, i, j, a, P.minimum_magnetic_field_magnitude, bad);
+            const double bj = gen_bdir(B, i, j, b, P.minimum_magnetic_field_magnitude, bad);
+            const double proj = bi * bj, id = a == b ? 1.0 : 0.0;
+            if (basis == 0) return kap.at(i, j, 0) * proj;
+            if (basis == 1) return kap.at(i, j, 1) * (id - proj);
+            const double r = P.perpendicular_parallel_ratio;
+            return kap.at(i, j, 0) * (r * id + (1.0 - r) * proj);
+        };
+    };
+    const int b0 = P.perpendicular_model == 1 ? 2 : 0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double prim = 0.0, sec = 0.0;
+                for (int d = 0; d < 2; ++d) {
+                    prim += gen_derivative(g, i, j, d, tensor(b0, c, d));
+                    if (P.perpendicular_model != 1) sec += gen_derivative(g, i, j, d, tensor(1, c, d));
+                }
+                div.at(i, j, c) = prim;
+                div.at(i, j, 3 + c) = P.perpendicular_model == 1 ? prim : sec;
+            }
+    fill_ghosts_periodic(div);
+    // FieldOperator::fill_divergence (FieldOperator.hpp:259-290, :487-513)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double s = 0.0;
+            for (int d = 0; d < 2; ++d)
+                s += gen_derivative(g, i, j, d, [&, d](int a, int b) { return V.at(a, b, d); });
+            dv.at(i, j, 0) = s;
+        }
+    fill_ghosts_periodic(dv);
+    out = gen_alloc(g, KPT_GENERIC_PARKER_COMPONENTS);
+    for (int j = -kGhost; j < ny + kGhost; ++j)
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                out.at(i, j, c) = B.at(i, j, c);
+                out.at(i, j, 3 + c) = V.at(i, j, c);
+                out.at(i, j, 6 + c) = curl.at(i, j, c);
+                out.at(i, j, 9 + c) = div.at(i, j, c);
+                out.at(i, j, 12 + c) = div.at(i, j, 3 + c);
+            }
+            out.at(i, j, 15) = kap.at(i, j, 0);
+            out.at(i, j, 16) = kap.at(i, j, 1);
+            out.at(i, j, 17) = dv.at(i, j, 0);
+        }
+    return !bad;
+}
+
+struct GenTerms {
+    bool ok = false;
+    double u[3] = {0, 0, 0}, b[3] = {0, 0, 0}, kpar = 0, kperp = 0, keff[2] = {0, 0}, divv = 0, p = 0,
+           prate = 0;
+};
+
+// evaluate_terms_from_packed (ParkerSolver.hpp:771-875).
+GenTerms gen_terms(const OField& F, const kpt_generic_parker_params& P, const double pos[2], double p_in) {
+    GenTerms T;
+    double v[KPT_GENERIC_PARKER_COMPONENTS];
+    sample(F, pos, v);
+    const double pm = std::fabs(p_in);
+    const double nm = pm / (P.rest_mass * P.speed_of_light);
+    const double gamma = std::sqrt(1.0 + nm * nm);
+    const double speed = pm / (gamma * P.rest_mass);
+    const double bm = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(std::isfinite(bm) && bm > 0.0)) return T;
+    const double ib = 1.0 / bm;
+    for (int c = 0; c < 3; ++c) T.b[c] = v[c] * ib;
+    double adv[3];
+    const double fpar = std::pow(gamma, 1.0 / 3.0);
+    if (P.perpendicular_model == 1) {
+        for (int c = 0; c < 3; ++c) adv[c] = fpar * v[9 + c];
+    } else {
+        const double fperp = std::pow(gamma, 1.0 / 9.0);
+        for (int c = 0; c < 3; ++c) adv[c] = fpar * v[9 + c] + fperp * v[12 + c];
+    }
+    T.kpar = v[15] * fpar;
+    T.kperp = v[16] * (P.perpendicular_model == 1 ? fpar : std::pow(gamma, 1.0 / 9.0));
+    if (!(std::isfinite(T.kpar) && T.kpar >= 0.0 && std::isfinite(T.kperp) && T.kperp >= 0.0)) return T;
+    for (int d = 0; d < 2; ++d) {
+        T.keff[d] = T.kperp + (T.kpar - T.kperp) * T.b[d] * T.b[d];
+        if (!(std::isfinite(T.keff[d]) && T.keff[d] >= 0.0)) return T;
+    }
+    T.divv = v[17];
+    const double drift = pm * speed * P.speed_of_light / (3.0 * P.charge);
+    for (int c = 0; c < 3; ++c) T.u[c] = v[3 + c] + drift * v[6 + c] + adv[c];
+    T.p = pm;
+    T.prate = -(pm / 3.0) * T.divv;
+    T.ok = true;
+    return T;
+}
+
+// local_stable_time_step (ParkerSolver.hpp:956-1007).
+double gen_local_dt(const OGrid& g, const GenTerms& T) {
+    const double large = 1.0e300;
+    double ddt = large, adt = large;
+    for (int d = 0; d < 2; ++d) {
+        const double dx = std::fabs(g.step[d]);
+        double di = large, ai = large;
+        if (T.keff[d] > 0.0) {
+            const double lim = 0.5 * dx;
+            di = lim * lim / (2.0 * T.keff[d]);
+        }
+        const double a = std::fabs(T.u[d]);
+        if (a > 0.0) ai = 1.0 * dx / a;
+        ddt = ddt < di ? ddt : di;
+        adt = adt < ai ? adt : ai;
+    }
+    double mdt = large;
+    const double dvm = std::fabs(T.divv);
+    if (dvm > 0.0) mdt = 3.0 * 1.0 / dvm;
+    const double m = ddt < adt ? ddt : adt;
+    return m < mdt ? m : mdt;
+}
+
+// Four standard normals: Box-Muller on the four Philox uniforms (0, 1), or the
+// injected row (g_x, g_y, g_z, g_parallel).
+void gen_normals(const double* table, uint32_t steps, uint64_t i, uint64_t id, uint32_t step, const uint32_t key[2],
+                 double n[4]) {
+    if (table) {
+        const double* row = table + (static_cast<size_t>(i) * steps + step) * 4;
+        for (int d = 0; d < 4; ++d) n[d] = row[d];
+        return;
+    }
+    const uint32_t ctr[4] = {static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), 0x47454E50u, step};
+    uint32_t r[4];
+    philox10(ctr, key, r);
+    for (int k = 0; k < 2; ++k) {
+        const double rad = std::sqrt(-2.0 * std::log(u01(r[2 * k])));
+        const double ang = 6.283185307179586476925286766559 * u01(r[2 * k + 1]);
+        n[2 * k] = rad * std::cos(ang);
+        n[2 * k + 1] = rad * std::sin(ang);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void kpto_generic_default_params(kpt_generic_parker_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->perpendicular_model = 0;
+    p->parallel_prefactor = 1.622;
+    p->perpendicular_prefactor = 0.198;
+    p->sigma2 = 0.5;
+    p->l2d = 1.0;
+    p->lslab = 1.0;
+    p->lc = -1.0;
+    p->rest_mass = 1.0;
+    p->charge = 1.0;
+    p->speed_of_light = 1.0;
+    p->courant = 0.3;
+}
+
+// Packed generic basis from B and V planes [ny][nx]; nullptr where the reference
+// aborts (|B| = 0 without floor, non-positive kappa inputs).
+void* kpto_generic_create(int32_t nx, int32_t ny, const double* xe, const double* ye, const double* const planes[6],
+                          const kpt_generic_parker_params* p) {
+    OField B, V;
+    for (OField* f : {&B, &V}) {
+        make_grid(f->g, nx, ny, xe, ye);
+        *f = gen_alloc(f->g, 3);
+    }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                B.at(i, j, c) = planes[c][static_cast<size_t>(j) * nx + i];
+                V.at(i, j, c) = planes[3 + c][static_cast<size_t>(j) * nx + i];
+            }
+    fill_ghosts_periodic(B);
+    fill_ghosts_periodic(V);
+    OField* out = new OField();
+    if (!gen_build(B, V, *p, *out)) {
+        delete out;
+        return nullptr;
+    }
+    return out;
+}
+
+// compute_adaptive_time_step (ParkerSolver.hpp:1084-1130): courant * min, 0 when
+// no alive particle gives a finite bound.
+double kpto_generic_time_step(const void* field, const kpt_generic_parker_params* p, const kpt_particles_host* P,
+                              uint64_t count) {
+    const OField& F = *static_cast<const OField*>(field);
+    double m = 1.0e300;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
+        const GenTerms T = gen_terms(F, *p, pos, P->p[i]);
+        if (!T.ok) continue;
+        const double d = gen_local_dt(F.g, T);
+        m = m < d ? m : d;
+    }
+    if (!std::isfinite(m) || m >= 0.5e300) return 0.0;
+    return p->courant * m;
+}
+
+// advance_stochastic (ParkerSolver.hpp:265-299, :1462-1512) with the increment of
+// sample_magnetic_diffusion_increment (:443-477). Returns dt (0 = no step).
+double kpto_generic_step(const void* field, const kpt_generic_parker_params* p, kpt_particles_host* P, uint64_t count,
+                         uint32_t step, uint64_t seed, const double* noise, uint32_t noise_steps,
+                         uint64_t* failures) {
+    const OField& F = *static_cast<const OField*>(field);
+    const double dt = kpto_generic_time_step(field, p, P, count);
+    if (failures) *failures = 0;
+    if (!(dt > 0.0)) return 0.0;
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    uint64_t fails = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
+        const GenTerms T = gen_terms(F, *p, pos, P->p[i]);
+        if (!T.ok) {
+            P->status[i] = 2;
+            ++fails;
+            continue;
+        }
+        double n[4];
+        gen_normals(noise, noise_steps, i, P->id ? P->id[i] : i, step, key, n);
+        double iso = 0.0;
+        for (int c = 0; c < 3; ++c) iso += n[c] * T.b[c];
+        const double sperp = T.kperp > 0.0 ? std::sqrt(2.0 * T.kperp * dt) : 0.0;
+        const double spar = T.kpar > 0.0 ? std::sqrt(2.0 * T.kpar * dt) : 0.0;
+        const double repl = spar * n[3] - sperp * iso;
+        double x[2] = {pos[0], pos[1]};
+        for (int d = 0; d < 2; ++d) {
+            const double det = dt * T.u[d];
+            const double sto = sperp * n[d] + repl * T.b[d];
+            x[d] += det + sto;
+        }
+        P->x[i] = F.g.wrap(0, x[0]);
+        P->y[i] = F.g.wrap(1, x[1]);
+        const double pn = T.p + dt * T.prate;
+        P->p[i] = pn > 0.0 ? pn : 0.0;
+    }
+    if (failures) *failures = fails;
+    return dt;
+}
+
+}  // extern "C"

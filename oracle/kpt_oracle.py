@@ -68,6 +68,14 @@ def load():
             "kpto_summary": (None, [P(kpt.ParticlesHost), C.c_uint64, P(kpt.Summary)]),
             "kpto_qlt_scattering": (None, [C.c_double] * 7 + [dp]),
             "kpto_reflect_mu": (C.c_double, [C.c_double, C.c_double]),
+            "kpto_generic_default_params": (None, [P(kpt.GenericParkerParams)]),
+            "kpto_generic_create": (C.c_void_p, [C.c_int32, C.c_int32, dp, dp, P(dp),
+                                                 P(kpt.GenericParkerParams)]),
+            "kpto_generic_time_step": (C.c_double, [C.c_void_p, P(kpt.GenericParkerParams),
+                                                    P(kpt.ParticlesHost), C.c_uint64]),
+            "kpto_generic_step": (C.c_double, [C.c_void_p, P(kpt.GenericParkerParams),
+                                               P(kpt.ParticlesHost), C.c_uint64, C.c_uint32,
+                                               C.c_uint64, dp, C.c_uint32, P(C.c_uint64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -186,3 +194,45 @@ def summary(parts, count) -> dict:
     s = kpt.Summary()
     load().kpto_summary(C.byref(kpt.particles_struct(parts)), count, C.byref(s))
     return s.as_dict()
+
+
+def generic_default_params():
+    p = kpt.GenericParkerParams()
+    load().kpto_generic_default_params(C.byref(p))
+    return p
+
+
+class GenericField(Field):
+    """Packed 18-component basis of the generic Parker solver on a periodic grid
+    (make_legacy_transport_coefficient_fields + build_packed_coefficient_basis)."""
+
+    def __init__(self, x_edges, y_edges, planes, params):
+        lib = load()
+        self.xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+        self.ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+        self._planes = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+        self.nx, self.ny, self.model = len(self.xe) - 1, len(self.ye) - 1, 18
+        arr = (C.POINTER(C.c_double) * 6)(*[_dp(a) for a in self._planes])
+        self.params = params
+        self.h = lib.kpto_generic_create(self.nx, self.ny, _dp(self.xe), _dp(self.ye), arr,
+                                         C.byref(params))
+        if not self.h:
+            raise ValueError("generic Parker coefficients: the reference aborts on this field")
+
+    def time_step(self, parts):
+        h = kpt.particles_struct(parts)
+        return load().kpto_generic_time_step(self.h, C.byref(self.params), C.byref(h),
+                                             len(parts["x"]))
+
+    def step(self, parts, step, seed=0, noise=None):
+        """One advance_stochastic step (modifies parts). Returns (dt, failures)."""
+        h = kpt.particles_struct(parts)
+        fails = C.c_uint64()
+        nz, ns = None, 0
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            assert nz.ndim == 3 and nz.shape[2] == 4 and nz.shape[0] >= len(parts["x"])
+            ns = nz.shape[1]
+        dt = load().kpto_generic_step(self.h, C.byref(self.params), C.byref(h), len(parts["x"]),
+                                      step, seed, _dp(nz), ns, C.byref(fails))
+        return dt, fails.value
