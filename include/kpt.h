@@ -173,7 +173,9 @@ typedef struct kpt_host_settings {
 
 /* Particle arrays on the host, structure of arrays (the fields of
  * ReconnectionParticleStorage, src/main.cpp:288-333, plus the particle id that
- * keys the counter-based RNG). Any pointer may be NULL on download to skip it. */
+ * keys the counter-based RNG and the energy-split baseline of ParticleSystem,
+ * include/ParticleSystem.hpp:347). Any pointer may be NULL on download to
+ * skip it. */
 typedef struct kpt_particles_host {
     double* x;
     double* y;
@@ -185,6 +187,8 @@ typedef struct kpt_particles_host {
     int32_t* status;
     int32_t* split_level;
     uint64_t* id;
+    double* e0;  /* initial_kinetic_energy: baseline of the energy-growth split;
+                    NULL on upload stores 0 (an unset baseline) */
 } kpt_particles_host;
 
 typedef struct kpt_push_stats {
@@ -428,6 +432,55 @@ kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_p
  * holds the four Gaussians (g_x, g_y, g_z, g_parallel) of each step. */
 kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
                                    uint32_t step, double* dt_used, uint64_t* failures);
+
+/* ---- ParticleSystem lifecycle (include/ParticleSystem.hpp) ----------------------- */
+
+/* ParticleProperties (include/ParticleSystem.hpp:145-213). */
+typedef struct kpt_particle_properties {
+    double rest_mass;        /* 1 */
+    double charge;           /* 1 */
+    double speed_of_light;   /* 1 */
+    double energy_scale_erg; /* 1 */
+} kpt_particle_properties;
+
+/* ParticleSplittingPolicy (include/ParticleSystem.hpp:280-295). */
+typedef struct kpt_energy_split_policy {
+    double energy_split_ratio;   /* > 1; 2 */
+    double minimum_child_weight; /* >= 0; 0 */
+} kpt_energy_split_policy;
+
+/* ParticleSplitStatistics (include/ParticleSystem.hpp:268-275). */
+typedef struct kpt_split_statistics {
+    uint64_t active_count;
+    uint64_t split_count;
+    uint64_t skipped_capacity_count;
+    uint64_t skipped_invalid_baseline_count;
+    uint64_t skipped_minimum_weight_count;
+} kpt_split_statistics;
+
+kpt_status kpt_default_particle_properties(kpt_particle_properties* out);
+kpt_status kpt_default_energy_split_policy(kpt_energy_split_policy* out);
+
+/* split_particles_on_energy_growth (include/ParticleSystem.hpp:802-917): an alive
+ * parent with E_k / e0 >= ratio and floor(ln(E_k/e0)/ln ratio) > level appends one
+ * child at the tail; both carry half the weight and level + 1. A non-finite or
+ * non-positive e0 is replaced by E_k and counted. Children are appended in parent
+ * slot order and take fresh root ids (the ids key the RNG streams). */
+kpt_status kpt_split_energy_growth(kpt_ctx* ctx, const kpt_particle_properties* props,
+                                   const kpt_energy_split_policy* policy, kpt_split_statistics* stats);
+/* reset_split_baselines (include/ParticleSystem.hpp:630-653): e0 = E_k(|p|) and
+ * split level 0 for every stored particle. */
+kpt_status kpt_reset_split_baselines(kpt_ctx* ctx, const kpt_particle_properties* props);
+/* apply_grid_boundary_conditions (include/ParticleSystem.hpp:661-706) on the grid of
+ * field `slot`: alive particles are wrapped along periodic axes (bit 0: x, bit 1:
+ * y of periodic_mask) and get status 1 (Escaped) when they leave the physical
+ * domain [lo, hi] along a non-periodic axis. */
+kpt_status kpt_apply_boundaries(kpt_ctx* ctx, int32_t slot, int32_t periodic_mask, uint64_t* escaped);
+/* gamma_from_momentum_magnitude, speed_from_momentum_magnitude and
+ * kinetic_energy_from_momentum_magnitude (include/ParticleSystem.hpp:170-195) of the
+ * first `count` stored particles; any output may be NULL. */
+kpt_status kpt_particle_kinematics(kpt_ctx* ctx, const kpt_particle_properties* props, double* gamma,
+                                   double* speed, double* kinetic_energy, uint64_t count);
 
 /* ---- learned pitch-angle closure (KPT_CLOSURE_MLP) ------------------------------
  * No counterpart in the reference. A small network stands in for the resonance

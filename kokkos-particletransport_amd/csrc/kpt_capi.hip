@@ -381,6 +381,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     if ((e = dev_alloc(&P.status, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&P.split_level, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&P.id, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&P.e0, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&P.substep, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
     ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
@@ -399,6 +400,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     }
     if ((e = hipMemset(P.split_level, 0, capacity * 4)) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(P.weight, 0, capacity * 8)) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(P.e0, 0, capacity * 8)) != hipSuccess) return bail(e, "hipMemset");
     *out = ctx;
     return KPT_OK;
 }
@@ -411,7 +413,8 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
     for (void* p : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p),
                     static_cast<void*>(P.t), static_cast<void*>(P.step_dt), static_cast<void*>(P.weight),
                     static_cast<void*>(P.mu), static_cast<void*>(P.status), static_cast<void*>(P.split_level),
-                    static_cast<void*>(P.id), static_cast<void*>(P.substep), static_cast<void*>(ctx->counters),
+                    static_cast<void*>(P.id), static_cast<void*>(P.e0), static_cast<void*>(P.substep),
+                    static_cast<void*>(ctx->counters),
                     static_cast<void*>(ctx->split_scratch), static_cast<void*>(ctx->hist_partials),
                     static_cast<void*>(ctx->hist_out), ctx->summary_partials,
                     static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
@@ -431,7 +434,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
                         static_cast<void*>(Q.t), static_cast<void*>(Q.step_dt), static_cast<void*>(Q.weight),
                         static_cast<void*>(Q.mu), static_cast<void*>(Q.status),
                         static_cast<void*>(Q.split_level), static_cast<void*>(Q.id),
-                        static_cast<void*>(Q.substep)})
+                        static_cast<void*>(Q.e0), static_cast<void*>(Q.substep)})
             dev_free(p);
     }
     for (int k = 0; k < 2; ++k) {
@@ -660,6 +663,10 @@ kpt_status kpt_particles_upload(kpt_ctx* ctx, const kpt_particles_host* h, uint6
         KPT_HIP(ctx, hipMemcpy(P.mu, h->mu, count * 8, hipMemcpyHostToDevice));
         KPT_HIP(ctx, hipMemcpy(P.status, h->status, count * 4, hipMemcpyHostToDevice));
         KPT_HIP(ctx, hipMemcpy(P.split_level, h->split_level, count * 4, hipMemcpyHostToDevice));
+        if (h->e0)
+            KPT_HIP(ctx, hipMemcpy(P.e0, h->e0, count * 8, hipMemcpyHostToDevice));
+        else
+            KPT_HIP(ctx, hipMemset(P.e0, 0, count * 8));
     }
     uint64_t max_root = 0;
     if (h->id) {
@@ -697,6 +704,7 @@ kpt_status kpt_particles_download(kpt_ctx* ctx, kpt_particles_host* h, uint64_t 
     if (h->status) KPT_HIP(ctx, hipMemcpy(h->status, P.status, count * 4, hipMemcpyDeviceToHost));
     if (h->split_level) KPT_HIP(ctx, hipMemcpy(h->split_level, P.split_level, count * 4, hipMemcpyDeviceToHost));
     if (h->id) KPT_HIP(ctx, hipMemcpy(h->id, P.id, count * 8, hipMemcpyDeviceToHost));
+    if (h->e0) KPT_HIP(ctx, hipMemcpy(h->e0, P.e0, count * 8, hipMemcpyDeviceToHost));
     return KPT_OK;
 }
 
@@ -922,6 +930,7 @@ kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_do
         KPT_HIP(ctx, dev_alloc(&Q.status, cap));
         KPT_HIP(ctx, dev_alloc(&Q.split_level, cap));
         KPT_HIP(ctx, dev_alloc(&Q.id, cap));
+        KPT_HIP(ctx, dev_alloc(&Q.e0, cap));
         KPT_HIP(ctx, dev_alloc(&Q.substep, cap));
         for (int k = 0; k < 2; ++k) {
             KPT_HIP(ctx, dev_alloc(&ctx->sort.keys[k], cap));
@@ -1132,6 +1141,7 @@ kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint6
         if (h->p) std::memcpy(&h->p[i], r + 68, 8);
         if (h->mu) std::memcpy(&h->mu[i], r + 76, 8);
         if (h->weight) std::memcpy(&h->weight[i], r + 84, 8);
+        if (h->e0) std::memcpy(&h->e0[i], r + 92, 8);
     }
     *count_io = count;
     return KPT_OK;
@@ -1163,10 +1173,11 @@ kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, doub
                                                " particles, but configured capacity is " +
                                                std::to_string(ctx->capacity) + ".");
     }
-    std::vector<double> x(count), y(count), p(count), t(count, restart_time), dt(count, dt_min), w(count), mu(count);
+    std::vector<double> x(count), y(count), p(count), t(count, restart_time), dt(count, dt_min), w(count), mu(count),
+        e0(count);
     std::vector<int32_t> status(count), level(count);
     kpt_particles_host h{x.data(), y.data(), p.data(), t.data(), dt.data(), w.data(), mu.data(),
-                         status.data(), level.data(), nullptr};
+                         status.data(), level.data(), nullptr, e0.data()};
     st = kpt_snapshot_read_host(path, &h, &count, charge, split_ratio, err, sizeof(err));
     if (st != KPT_OK) return fail(ctx, st, err);
     return kpt_particles_upload(ctx, &h, count);
@@ -1620,6 +1631,134 @@ kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params
     if (dt_used) *dt_used = dt;
     if (failures) *failures = nf;
     return KPT_OK;
+}
+
+}  // extern "C"
+
+// ---- ParticleSystem lifecycle -------------------------------------------------
+
+namespace {
+
+// ParticleProperties::validate (include/ParticleSystem.hpp:155-165).
+kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr) {
+    if (!pr) return fail(ctx, KPT_ERR_ARG, "particle properties pointer is null.");
+    if (!std::isfinite(pr->rest_mass) || pr->rest_mass <= 0.0 || !std::isfinite(pr->charge) ||
+        !std::isfinite(pr->speed_of_light) || pr->speed_of_light <= 0.0 || !std::isfinite(pr->energy_scale_erg) ||
+        pr->energy_scale_erg <= 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "ParticleProperties: rest mass and speed of light must be positive finite dimensionless values, "
+                    "charge must be finite, and the energy scale must be positive and finite.");
+    return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_default_particle_properties(kpt_particle_properties* out) {
+    if (!out) return KPT_ERR_ARG;
+    out->rest_mass = 1.0;
+    out->charge = 1.0;
+    out->speed_of_light = 1.0;
+    out->energy_scale_erg = 1.0;
+    return KPT_OK;
+}
+
+kpt_status kpt_default_energy_split_policy(kpt_energy_split_policy* out) {
+    if (!out) return KPT_ERR_ARG;
+    out->energy_split_ratio = 2.0;
+    out->minimum_child_weight = 0.0;
+    return KPT_OK;
+}
+
+kpt_status kpt_split_energy_growth(kpt_ctx* ctx, const kpt_particle_properties* props,
+                                   const kpt_energy_split_policy* policy, kpt_split_statistics* stats) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_properties(ctx, props);
+    if (st != KPT_OK) return st;
+    if (!policy) return fail(ctx, KPT_ERR_ARG, "split policy pointer is null.");
+    // ParticleSplittingPolicy::validate (include/ParticleSystem.hpp:287-294)
+    if (!std::isfinite(policy->energy_split_ratio) || policy->energy_split_ratio <= 1.0 ||
+        !std::isfinite(policy->minimum_child_weight) || policy->minimum_child_weight < 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "ParticleSplittingPolicy: energy split ratio must be greater than one and minimum child weight "
+                    "must be finite and non-negative.");
+    kpt_split_statistics out{};
+    if (ctx->count == 0 || ctx->count >= ctx->capacity) {
+        if (stats) *stats = out;
+        return KPT_OK;
+    }
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 5 * sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_energy_split(ctx->P, ctx->count, ctx->capacity, ctx->next_root, *props,
+                                          policy->energy_split_ratio, std::log(policy->energy_split_ratio),
+                                          policy->minimum_child_weight, ctx->split_scratch, ctx->counters,
+                                          ctx->stream));
+    unsigned long long c[5];
+    KPT_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    out.active_count = c[0];
+    out.split_count = c[1];
+    out.skipped_capacity_count = c[2];
+    out.skipped_invalid_baseline_count = c[3];
+    out.skipped_minimum_weight_count = c[4];
+    ctx->count += c[1];
+    ctx->next_root += c[1];
+    if (stats) *stats = out;
+    return KPT_OK;
+}
+
+kpt_status kpt_reset_split_baselines(kpt_ctx* ctx, const kpt_particle_properties* props) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_properties(ctx, props);
+    if (st != KPT_OK) return st;
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, kpt::launch_reset_baselines(ctx->P, ctx->count, *props, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+kpt_status kpt_apply_boundaries(kpt_ctx* ctx, int32_t slot, int32_t periodic_mask, uint64_t* escaped) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    if (periodic_mask < 0 || periodic_mask > 3) return fail(ctx, KPT_ERR_ARG, "periodic mask must be in [0, 3].");
+    const FieldSlot& f = ctx->slot[slot];
+    if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_boundaries(ctx->P, ctx->count, f.g, periodic_mask, ctx->counters, ctx->stream));
+    unsigned long long e = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&e, ctx->counters, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (escaped) *escaped = e;
+    return KPT_OK;
+}
+
+kpt_status kpt_particle_kinematics(kpt_ctx* ctx, const kpt_particle_properties* props, double* gamma,
+                                   double* speed, double* kinetic_energy, uint64_t count) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_properties(ctx, props);
+    if (st != KPT_OK) return st;
+    if (count > ctx->count) return fail(ctx, KPT_ERR_ARG, "count exceeds stored particles.");
+    if (count == 0) return KPT_OK;
+    (void)hipSetDevice(ctx->device);
+    double* out[3] = {gamma, speed, kinetic_energy};
+    double* dev[3] = {nullptr, nullptr, nullptr};
+    kpt_status rc = KPT_OK;
+    for (int k = 0; k < 3 && rc == KPT_OK; ++k) {
+        if (!out[k]) continue;
+        const hipError_t e = dev_alloc(&dev[k], count);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "hipMalloc");
+    }
+    if (rc == KPT_OK) {
+        hipError_t e = kpt::launch_kinematics(ctx->P.p, count, *props, dev[0], dev[1], dev[2], ctx->stream);
+        for (int k = 0; k < 3 && e == hipSuccess; ++k)
+            if (out[k]) e = hipMemcpyAsync(out[k], dev[k], count * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "particle kinematics");
+    }
+    for (double* d : dev) dev_free(d);
+    return rc;
 }
 
 }  // extern "C"

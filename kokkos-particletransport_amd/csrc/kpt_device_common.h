@@ -32,6 +32,13 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // (r + 1/2) / 2^32: open-interval U(0, 1) from 32 bits.
 __device__ __forceinline__ double u01(uint32_t r) {
     return (static_cast<double>(r) + 0.5) * (1.0 / 4294967296.0);

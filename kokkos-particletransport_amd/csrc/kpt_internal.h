@@ -43,6 +43,7 @@ struct ParticleArrays {
     int32_t* status;
     int32_t* split_level;
     uint64_t* id;
+    double* e0;         // split baseline: initial kinetic energy
     uint32_t* substep;  // sub-steps taken in the current interval (push chunks only)
 };
 
@@ -156,6 +157,21 @@ hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capaci
                         const kpt_device_settings& s, const SplitTable& tab, uint32_t* scratch,
                         unsigned long long* counters /* [0] split, [1] skipped */,
                         hipStream_t stream);
+// Exclusive scan of n uint32 tile counts in place; the total lands in [n].
+hipError_t launch_scan_tiles(uint32_t* tile_counts, uint32_t n, hipStream_t stream);
+// ParticleSystem lifecycle passes. The energy split needs esplit_scratch_elems(count)
+// uint32 of scratch; counters [0..4] follow ParticleSplitStatistics.
+size_t esplit_scratch_elems(uint64_t count);
+hipError_t launch_energy_split(const ParticleArrays& P, uint64_t count, uint64_t capacity, uint64_t first_root,
+                               const kpt_particle_properties& pr, double ratio_min, double log_ratio,
+                               double min_child_weight, uint32_t* scratch, unsigned long long* counters,
+                               hipStream_t stream);
+hipError_t launch_reset_baselines(const ParticleArrays& P, uint64_t count, const kpt_particle_properties& pr,
+                                  hipStream_t stream);
+hipError_t launch_boundaries(const ParticleArrays& P, uint64_t count, const GridDesc& g, int mask,
+                             unsigned long long* escaped, hipStream_t stream);
+hipError_t launch_kinematics(const double* p, uint64_t count, const kpt_particle_properties& pr, double* gamma,
+                             double* speed, double* ekin, hipStream_t stream);
 // Histograms: `partials` holds hist_partial_elems(bins) doubles.
 size_t hist_partial_elems(int bins);
 hipError_t launch_hist_logp(const ParticleArrays& P, uint64_t count, int bins, double log_pmin,

@@ -39,13 +39,6 @@ __device__ __forceinline__ double wrap_periodic(double v, double lo, double hi, 
     return w;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // Coefficient packing: make_parker_reconnection_coefficients /
 // make_reconnection_coefficients (src/main.cpp:839-957) on the periodic grid.
@@ -1027,6 +1020,9 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(ParticleArrays P, uint64
     P.status[i] = 0;
     P.split_level[i] = 0;
     P.id[i] = root;
+    // split baseline in the driver's units (rest mass = c = 1), the E0 its
+    // snapshots record (src/main.cpp:2160-2170)
+    P.e0[i] = sqrt(1.0 + P.p[i] * P.p[i]) - 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1126,6 +1122,7 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, u
             P.weight[i] = w;
             P.weight[child] = w;
             P.mu[child] = P.mu[i];
+            P.e0[child] = P.e0[i];
             P.status[child] = 0;
             P.split_level[i] = new_level;
             P.split_level[child] = new_level;
@@ -1390,6 +1387,11 @@ hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capaci
     hipLaunchKernelGGL(split_scan_kernel, dim3(1), dim3(1024), 0, stream, scratch, tiles);
     hipLaunchKernelGGL(split_apply_kernel, dim3(tiles), dim3(kBlock), 0, stream, P, count, capacity, s, tab,
                        scratch, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_tiles(uint32_t* tile_counts, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(split_scan_kernel, dim3(1), dim3(1024), 0, stream, tile_counts, n);
     return hipGetLastError();
 }
 

@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(ParticleArrays src, Part
     dst.mu[i] = src.mu[s];
     dst.status[i] = src.status[s];
     dst.split_level[i] = src.split_level[s];
+    dst.e0[i] = src.e0[s];
     dst.id[i] = src.id[s];
     dst.substep[i] = src.substep[s];
 }

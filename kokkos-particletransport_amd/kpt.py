@@ -30,7 +30,7 @@ CLOSURE_MLP = 2
 PARTICLE_FIELDS = (
     ("x", np.float64), ("y", np.float64), ("p", np.float64), ("t", np.float64),
     ("step_dt", np.float64), ("weight", np.float64), ("mu", np.float64),
-    ("status", np.int32), ("split_level", np.int32), ("id", np.uint64),
+    ("status", np.int32), ("split_level", np.int32), ("id", np.uint64), ("e0", np.float64),
 )
 
 
@@ -93,6 +93,7 @@ class ParticlesHost(C.Structure):
         ("step_dt", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
         ("mu", C.POINTER(C.c_double)), ("status", C.POINTER(C.c_int32)),
         ("split_level", C.POINTER(C.c_int32)), ("id", C.POINTER(C.c_uint64)),
+        ("e0", C.POINTER(C.c_double)),
     ]
 
 
@@ -151,6 +152,27 @@ class GenericParkerParams(C.Structure):
         c = GenericParkerParams()
         C.pointer(c)[0] = self
         return c
+
+
+class ParticleProperties(C.Structure):
+    """kpt_particle_properties (ParticleProperties, include/ParticleSystem.hpp:145-213)."""
+    _fields_ = [("rest_mass", C.c_double), ("charge", C.c_double), ("speed_of_light", C.c_double),
+                ("energy_scale_erg", C.c_double)]
+
+
+class EnergySplitPolicy(C.Structure):
+    """kpt_energy_split_policy (ParticleSplittingPolicy, include/ParticleSystem.hpp:280-295)."""
+    _fields_ = [("energy_split_ratio", C.c_double), ("minimum_child_weight", C.c_double)]
+
+
+class SplitStatistics(C.Structure):
+    """kpt_split_statistics (ParticleSplitStatistics, include/ParticleSystem.hpp:268-275)."""
+    _fields_ = [("active_count", C.c_uint64), ("split_count", C.c_uint64),
+                ("skipped_capacity_count", C.c_uint64), ("skipped_invalid_baseline_count", C.c_uint64),
+                ("skipped_minimum_weight_count", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class MlpHull(C.Structure):
@@ -281,6 +303,14 @@ EXPORTS = {
                                                C.POINTER(C.c_double)]),
     "kpt_generic_parker_step": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32,
                                           C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_default_particle_properties": (C.c_int, [C.POINTER(ParticleProperties)]),
+    "kpt_default_energy_split_policy": (C.c_int, [C.POINTER(EnergySplitPolicy)]),
+    "kpt_split_energy_growth": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties),
+                                          C.POINTER(EnergySplitPolicy), C.POINTER(SplitStatistics)]),
+    "kpt_reset_split_baselines": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties)]),
+    "kpt_apply_boundaries": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint64)]),
+    "kpt_particle_kinematics": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]),
     "kpt_mlp_default_train_opts": (C.c_int, [C.POINTER(MlpTrainOpts)]),
     "kpt_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(MlpTrainOpts), C.POINTER(MlpReport)]),
     "kpt_mlp_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.POINTER(MlpHull)]),
@@ -331,6 +361,18 @@ def make_device_settings(host: HostSettings, x_edges, y_edges) -> DeviceSettings
 def generic_default_params() -> GenericParkerParams:
     p = GenericParkerParams()
     _check_static(load().kpt_generic_parker_default_params(C.byref(p)))
+    return p
+
+
+def default_particle_properties() -> ParticleProperties:
+    p = ParticleProperties()
+    _check_static(load().kpt_default_particle_properties(C.byref(p)))
+    return p
+
+
+def default_energy_split_policy() -> EnergySplitPolicy:
+    p = EnergySplitPolicy()
+    _check_static(load().kpt_default_energy_split_policy(C.byref(p)))
     return p
 
 
@@ -527,6 +569,12 @@ class Context:
         self._check(self.lib.kpt_histogram_xy(self.h, bx, by, x_lo, x_hi, y_lo, y_hi, _dp(out)))
         return out.reshape(by, bx)
 
+    def snapshot_write(self, path, charge, split_ratio, p0):
+        self._check(self.lib.kpt_snapshot_write(self.h, path.encode(), charge, split_ratio, p0))
+
+    def snapshot_read(self, path, charge, split_ratio, restart_time, dt_min):
+        self._check(self.lib.kpt_snapshot_read(self.h, path.encode(), charge, split_ratio, restart_time, dt_min))
+
     def sort(self, s: DeviceSettings, compact=False) -> int:
         """Sort by stencil cell (inactive last); returns the active count."""
         act = C.c_uint64()
@@ -566,6 +614,27 @@ class Context:
         self._check(self.lib.kpt_generic_parker_step(self.h, C.byref(params), slot, step, C.byref(dt),
                                                      C.byref(nf)))
         return dt.value, nf.value
+
+    # ---- ParticleSystem lifecycle ---------------------------------------------
+    def split_energy_growth(self, props: "ParticleProperties", policy: "EnergySplitPolicy") -> dict:
+        st = SplitStatistics()
+        self._check(self.lib.kpt_split_energy_growth(self.h, C.byref(props), C.byref(policy), C.byref(st)))
+        return st.as_dict()
+
+    def reset_split_baselines(self, props: "ParticleProperties"):
+        self._check(self.lib.kpt_reset_split_baselines(self.h, C.byref(props)))
+
+    def apply_boundaries(self, slot, periodic_mask) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.kpt_apply_boundaries(self.h, slot, periodic_mask, C.byref(out)))
+        return out.value
+
+    def kinematics(self, props: "ParticleProperties", count=None):
+        """(gamma, speed, kinetic energy) of the first `count` stored particles."""
+        n = self.count if count is None else count
+        g, v, e = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.kpt_particle_kinematics(self.h, C.byref(props), _dp(g), _dp(v), _dp(e), n))
+        return g, v, e
 
     # ---- learned pitch-angle closure -----------------------------------------
     def mlp_train(self, opts: "MlpTrainOpts | None" = None) -> dict:

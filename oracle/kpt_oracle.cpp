@@ -826,6 +826,7 @@ uint64_t kpto_inject(const kpt_device_settings* s, const kpt_inject_opts* o, uin
         P->status[i] = 0;
         P->split_level[i] = 0;
         if (P->id) P->id[i] = root;
+        if (P->e0) P->e0[i] = std::sqrt(1.0 + pinit * pinit) - 1.0;
     }
     return n;
 }
@@ -863,6 +864,7 @@ uint64_t kpto_split(const kpt_device_settings* s, kpt_particles_host* P, uint64_
         P->weight[parent] = w;
         P->weight[child] = w;
         P->mu[child] = P->mu[parent];
+        if (P->e0) P->e0[child] = P->e0[parent];
         P->status[child] = 0;
         P->split_level[parent] = new_level;
         P->split_level[child] = new_level;
@@ -1297,6 +1299,114 @@ double kpto_generic_step(const void* field, const kpt_generic_parker_params* p, 
     }
     if (failures) *failures = fails;
     return dt;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ParticleSystem lifecycle (include/ParticleSystem.hpp): energy-growth split,
+// baseline reset and grid boundary conditions, serial over slots.
+// ---------------------------------------------------------------------------
+namespace {
+
+// ParticleProperties::kinetic_energy_from_momentum_magnitude (:190-194).
+double kinetic_energy(double pm, const kpt_particle_properties& pr) {
+    const double nm = pm / (pr.rest_mass * pr.speed_of_light);
+    const double gamma = std::sqrt(1.0 + nm * nm);
+    return (gamma - 1.0) * pr.rest_mass * pr.speed_of_light * pr.speed_of_light;
+}
+
+}  // namespace
+
+extern "C" {
+
+// split_particles_on_energy_growth (include/ParticleSystem.hpp:802-917) with the
+// parents taken in slot order; children take ids first_root, first_root + 1, ...
+// Returns the new count; stats[5] = alive, split, skipped capacity, invalid
+// baseline, light child.
+uint64_t kpto_split_energy_growth(const kpt_particle_properties* pr, double ratio_min, double min_child_weight,
+                                  kpt_particles_host* P, uint64_t count, uint64_t capacity, uint64_t first_root,
+                                  uint64_t stats[5]) {
+    for (int k = 0; k < 5; ++k) stats[k] = 0;
+    if (count == 0 || count >= capacity) return count;
+    const double log_ratio = std::log(ratio_min);
+    uint64_t appended = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        ++stats[0];
+        const double pm = P->p[i] >= 0.0 ? P->p[i] : -P->p[i];
+        const double energy = kinetic_energy(pm, *pr);
+        const double base = P->e0[i];
+        if (!std::isfinite(base) || base <= 0.0) {
+            if (std::isfinite(energy) && energy > 0.0) P->e0[i] = energy;
+            ++stats[3];
+            continue;
+        }
+        const double ratio = energy / base;
+        if (!std::isfinite(ratio) || ratio < ratio_min) continue;
+        const int target = static_cast<int>(std::floor(std::log(ratio) / log_ratio));
+        if (target <= P->split_level[i]) continue;
+        const double cw = 0.5 * P->weight[i];
+        if (!std::isfinite(cw) || cw <= 0.0 || cw < min_child_weight) {
+            ++stats[4];
+            continue;
+        }
+        const uint64_t rank = appended++;
+        const uint64_t child = count + rank;
+        if (child >= capacity) {
+            ++stats[2];
+            continue;
+        }
+        P->x[child] = P->x[i];
+        P->y[child] = P->y[i];
+        P->p[child] = P->p[i];
+        P->t[child] = P->t[i];
+        P->step_dt[child] = P->step_dt[i];
+        P->mu[child] = P->mu[i];
+        P->e0[child] = base;
+        P->weight[i] = cw;
+        P->weight[child] = cw;
+        P->status[child] = 0;
+        P->split_level[i] += 1;
+        P->split_level[child] = P->split_level[i];
+        if (P->id) P->id[child] = first_root + rank;
+        ++stats[1];
+    }
+    return count + stats[1];
+}
+
+// reset_split_baselines (include/ParticleSystem.hpp:633-653).
+void kpto_reset_split_baselines(const kpt_particle_properties* pr, kpt_particles_host* P, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i) {
+        P->e0[i] = kinetic_energy(P->p[i] >= 0.0 ? P->p[i] : -P->p[i], *pr);
+        P->split_level[i] = 0;
+    }
+}
+
+// apply_grid_boundary_conditions (include/ParticleSystem.hpp:661-706) on the
+// grid of edges xe[0..nx], ye[0..ny]. Returns the number of particles escaped.
+uint64_t kpto_apply_boundaries(int32_t nx, int32_t ny, const double* xe, const double* ye, int32_t mask,
+                               kpt_particles_host* P, uint64_t count) {
+    OGrid g;
+    make_grid(g, nx, ny, xe, ye);
+    uint64_t escaped = 0;
+    double* pos[2] = {P->x, P->y};
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        bool out = false;
+        for (int d = 0; d < 2; ++d) {
+            const double x = pos[d][i];
+            if (mask & (1 << d))
+                pos[d][i] = g.wrap(d, x);
+            else if (!(x >= g.phys_lo[d] && x <= g.phys_hi[d]))
+                out = true;
+        }
+        if (out) {
+            P->status[i] = 1;
+            ++escaped;
+        }
+    }
+    return escaped;
 }
 
 }  // extern "C"

@@ -68,6 +68,12 @@ def load():
             "kpto_summary": (None, [P(kpt.ParticlesHost), C.c_uint64, P(kpt.Summary)]),
             "kpto_qlt_scattering": (None, [C.c_double] * 7 + [dp]),
             "kpto_reflect_mu": (C.c_double, [C.c_double, C.c_double]),
+            "kpto_split_energy_growth": (C.c_uint64, [P(kpt.ParticleProperties), C.c_double, C.c_double,
+                                                      P(kpt.ParticlesHost), C.c_uint64, C.c_uint64, C.c_uint64,
+                                                      P(C.c_uint64)]),
+            "kpto_reset_split_baselines": (None, [P(kpt.ParticleProperties), P(kpt.ParticlesHost), C.c_uint64]),
+            "kpto_apply_boundaries": (C.c_uint64, [C.c_int32, C.c_int32, dp, dp, C.c_int32,
+                                                   P(kpt.ParticlesHost), C.c_uint64]),
             "kpto_generic_default_params": (None, [P(kpt.GenericParkerParams)]),
             "kpto_generic_create": (C.c_void_p, [C.c_int32, C.c_int32, dp, dp, P(dp),
                                                  P(kpt.GenericParkerParams)]),
@@ -174,6 +180,31 @@ def split(s, parts, count, capacity):
     a, b = C.c_uint64(), C.c_uint64()
     n = load().kpto_split(C.byref(s), C.byref(h), count, capacity, C.byref(a), C.byref(b))
     return n, a.value, b.value
+
+
+_SPLIT_STATS = ("active_count", "split_count", "skipped_capacity_count", "skipped_invalid_baseline_count",
+                "skipped_minimum_weight_count")
+
+
+def split_energy_growth(props, policy, parts, count, capacity, first_root):
+    """ParticleSystem::split_particles_on_energy_growth on a capacity-sized dict.
+    Returns (new count, statistics dict)."""
+    h = kpt.particles_struct(parts)
+    st = (C.c_uint64 * 5)()
+    n = load().kpto_split_energy_growth(C.byref(props), policy.energy_split_ratio, policy.minimum_child_weight,
+                                        C.byref(h), count, capacity, first_root, st)
+    return n, dict(zip(_SPLIT_STATS, list(st)))
+
+
+def reset_split_baselines(props, parts, count):
+    load().kpto_reset_split_baselines(C.byref(props), C.byref(kpt.particles_struct(parts)), count)
+
+
+def apply_boundaries(x_edges, y_edges, mask, parts, count):
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    return load().kpto_apply_boundaries(len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye), mask,
+                                        C.byref(kpt.particles_struct(parts)), count)
 
 
 def histogram_logp(parts, count, bins, pmin, pmax):
