@@ -433,6 +433,50 @@ kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_p
 kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
                                    uint32_t step, double* dt_used, uint64_t* failures);
 
+/* ---- generic focused-transport solver (global time step) -------------------------
+ * FocusTransportSolver (include/FocusSolver.hpp:74-1890) on the packed basis of
+ * FocusCoefficient::make_legacy_transport_coefficient_fields_with_time_derivative
+ * (include/FocusCoefficient.hpp:858-990), FP64, on the periodic grid of a field
+ * slot (cell-centred, like the generic Parker solver). Packed components:
+ *   0-2 B, 3-5 b, 6-8 V, 9-11 b x grad ln|B| / |B|, 12-14 b x (b.grad)b / |B|,
+ *   15-17 div(kappa_perp(gamma=1) (I - bb)), 18 |B|, 19 kappa_par(gamma=1),
+ *   20 kappa_perp(gamma=1), 21 div V, 22 b.grad ln|B|, 23 bb:grad V,
+ *   24 b.dV/dt + b.(V.grad)V, 25 sigma2, 26 Lc. */
+typedef struct kpt_generic_focus_params {
+    kpt_generic_parker_params model;      /* diffusion model, turbulence, particle, courant */
+    double minimum_magnetic_field_magnitude; /* floor of b, ln|B| and the drift bases: 0 */
+    /* PitchAngleScatteringParameters (include/LegencyModel.hpp:196-223) */
+    double spectral_index;                /* 5/3  */
+    double resonance_regularization_h0;   /* 1e-6 */
+    double scattering_minimum_magnetic_field_magnitude;
+    double scattering_minimum_sigma2;
+    double scattering_minimum_correlation_length;
+    double scattering_minimum_speed;
+    double maximum_pitch_angle_cosine;    /* set_maximum_pitch_angle_cosine: 1 */
+} kpt_generic_focus_params;
+
+enum { KPT_GENERIC_FOCUS_COMPONENTS = 27 };
+
+kpt_status kpt_generic_focus_default_params(kpt_generic_focus_params* out);
+/* Build the packed basis of `slot` from the B and V uploaded there; dvdt is NULL
+ * (dV/dt = 0, make_legacy_transport_coefficient_fields) or three planes
+ * [3][ny][nx] of dV/dt, x fastest. */
+kpt_status kpt_generic_focus_build(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                   const double* dvdt);
+/* The packed basis of nodes i in [-1, nx], j in [-1, ny] as double[(ny+2)][(nx+2)][27]. */
+kpt_status kpt_generic_focus_download(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len);
+/* compute_adaptive_time_step (include/FocusSolver.hpp:208): courant * min over
+ * alive particles of the six local bounds; 0 when none is finite. */
+kpt_status kpt_generic_focus_time_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                       double* dt);
+/* advance_stochastic (include/FocusSolver.hpp:276): one step of every alive
+ * particle (x, y, p, mu). `step` keys the Philox stream and selects the row of
+ * an injected noise table: (g_x, g_y, g_z) of the perpendicular increment and
+ * the pitch-angle normal. Particles where the reference aborts (|B| <= 0, zero
+ * speed, invalid coefficients) get status 2 and are counted in failures. */
+kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                  uint32_t step, double* dt_used, uint64_t* failures);
+
 /* ---- ParticleSystem lifecycle (include/ParticleSystem.hpp) ----------------------- */
 
 /* ParticleProperties (include/ParticleSystem.hpp:145-213). */

@@ -36,6 +36,10 @@ struct FieldSlot {
     double* gen = nullptr;
     size_t gen_cap = 0;
     bool gen_valid = false;
+    // generic focused basis built from raw (kpt_generic_focus_build)
+    double* foc = nullptr;
+    size_t foc_cap = 0;
+    bool foc_valid = false;
 };
 
 }  // namespace
@@ -307,6 +311,7 @@ kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, i
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     f.version = ++ctx->field_versions;
     f.gen_valid = false;
+    f.foc_valid = false;
     f.valid = true;
     return KPT_OK;
 }
@@ -424,6 +429,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.c64);
         dev_free(f.c32);
         dev_free(f.gen);
+        dev_free(f.foc);
     }
     dev_free(ctx->pair32);
     dev_free(ctx->mlp_params);
@@ -1759,6 +1765,174 @@ kpt_status kpt_particle_kinematics(kpt_ctx* ctx, const kpt_particle_properties* 
     }
     for (double* d : dev) dev_free(d);
     return rc;
+}
+
+}  // extern "C"
+
+// ---- generic focused-transport solver ------------------------------------------
+
+namespace {
+
+kpt_status check_focus(kpt_ctx* ctx, const kpt_generic_focus_params* p) {
+    if (!p) return fail(ctx, KPT_ERR_ARG, "generic focused parameters pointer is null.");
+    kpt_status st = check_generic(ctx, &p->model);
+    if (st != KPT_OK) return st;
+    if (!std::isfinite(p->minimum_magnetic_field_magnitude) || p->minimum_magnetic_field_magnitude < 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "FocusCoefficient: magnetic-field magnitude floor must be finite and non-negative.");
+    // PitchAngleScatteringParameters::validate (include/LegencyModel.hpp:206-222)
+    if (!std::isfinite(p->spectral_index) || p->spectral_index <= 1.0 ||
+        !std::isfinite(p->resonance_regularization_h0) || p->resonance_regularization_h0 <= 0.0 ||
+        !std::isfinite(p->scattering_minimum_magnetic_field_magnitude) ||
+        p->scattering_minimum_magnetic_field_magnitude < 0.0 || !std::isfinite(p->scattering_minimum_sigma2) ||
+        p->scattering_minimum_sigma2 < 0.0 || !std::isfinite(p->scattering_minimum_correlation_length) ||
+        p->scattering_minimum_correlation_length < 0.0 || !std::isfinite(p->scattering_minimum_speed) ||
+        p->scattering_minimum_speed < 0.0)
+        return fail(ctx, KPT_ERR_ARG,
+                    "LegencyModel::PitchAngleScatteringParameters: coefficients and floors must be finite, with "
+                    "spectral_index > 1 and resonance_regularization_h0 > 0.");
+    if (!std::isfinite(p->maximum_pitch_angle_cosine) || p->maximum_pitch_angle_cosine <= 0.0 ||
+        p->maximum_pitch_angle_cosine > 1.0)
+        return fail(ctx, KPT_ERR_ARG, "FocusTransportSolver: maximum |mu| must satisfy 0 < value <= 1.");
+    return KPT_OK;
+}
+
+kpt_status focus_slot(kpt_ctx* ctx, int32_t slot, const FieldSlot** out) {
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    const FieldSlot& f = ctx->slot[slot];
+    if (!f.valid || !f.foc_valid)
+        return fail(ctx, KPT_ERR_STATE, "FocusTransportSolver: coefficient fields are not configured.");
+    *out = &f;
+    return KPT_OK;
+}
+
+kpt_status focus_dt(kpt_ctx* ctx, const kpt_generic_focus_params* p, const FieldSlot& f, double* dt) {
+    const double large = 1.0e300;
+    unsigned long long bits;
+    std::memcpy(&bits, &large, 8);
+    KPT_HIP(ctx, hipMemcpyAsync(ctx->counters + 6, &bits, 8, hipMemcpyHostToDevice, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_focus_min_dt(ctx->P, ctx->count, f.foc, f.g, *p, ctx->counters + 6, ctx->stream));
+    KPT_HIP(ctx, hipMemcpyAsync(&bits, ctx->counters + 6, 8, hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double m;
+    std::memcpy(&m, &bits, 8);
+    *dt = (!std::isfinite(m) || m >= 0.5 * large) ? 0.0 : p->model.courant * m;
+    return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_generic_focus_default_params(kpt_generic_focus_params* p) {
+    if (!p) return KPT_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    kpt_generic_parker_default_params(&p->model);
+    p->spectral_index = 5.0 / 3.0;             // PitchAngleScatteringParameters (LegencyModel.hpp:196-203)
+    p->resonance_regularization_h0 = 1.0e-6;
+    p->maximum_pitch_angle_cosine = 1.0;       // FocusTransportSolver (FocusSolver.hpp:145)
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_focus_build(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                   const double* dvdt) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_focus(ctx, p);
+    if (st != KPT_OK) return st;
+    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
+    FieldSlot& f = ctx->slot[slot];
+    if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    (void)hipSetDevice(ctx->device);
+    const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_FOCUS_COMPONENTS;
+    if (elems > f.foc_cap) {
+        dev_free(f.foc);
+        f.foc = nullptr;
+        f.foc_cap = 0;
+        KPT_HIP(ctx, dev_alloc(&f.foc, elems));
+        f.foc_cap = elems;
+    }
+    f.foc_valid = false;
+    double* dv = nullptr;
+    const size_t cells = static_cast<size_t>(f.g.nx) * f.g.ny;
+    if (dvdt) {
+        KPT_HIP(ctx, dev_alloc(&dv, 3 * cells));
+        const hipError_t e = hipMemcpy(dv, dvdt, 3 * cells * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            dev_free(dv);
+            return hip_fail(ctx, e, "hipMemcpy");
+        }
+    }
+    int* bad = reinterpret_cast<int*>(ctx->counters + 7);
+    int h = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), ctx->stream);
+    if (e == hipSuccess) e = kpt::launch_focus_pack(f.raw, dv, f.g, *p, f.foc, bad, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(dv);
+    if (e != hipSuccess) return hip_fail(ctx, e, "generic focused basis");
+    if (h)
+        return fail(ctx, KPT_ERR_ARG,
+                    "FocusCoefficient / LegencyModel: |B| must exceed the floor and V, Lc, Omega and sigma2 must be "
+                    "positive wherever the coefficients are evaluated.");
+    f.foc_valid = true;
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_focus_download(kpt_ctx* ctx, int32_t slot, double* out, size_t out_len) {
+    if (!ctx || !out) return KPT_ERR_ARG;
+    const FieldSlot* f;
+    kpt_status st = focus_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    const size_t plane = kpt::coefficient_plane(f->g.nx, f->g.ny);
+    const int C = KPT_GENERIC_FOCUS_COMPONENTS;
+    if (out_len < plane * C) return fail(ctx, KPT_ERR_ARG, "output buffer too small.");
+    std::vector<double> tmp(plane * C);
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipMemcpy(tmp.data(), f->foc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t n = 0; n < plane; ++n)
+        for (int k = 0; k < C; ++k) out[n * C + k] = tmp[k * plane + n];
+    return KPT_OK;
+}
+
+kpt_status kpt_generic_focus_time_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                       double* dt) {
+    if (!ctx || !dt) return KPT_ERR_ARG;
+    kpt_status st = check_focus(ctx, p);
+    if (st != KPT_OK) return st;
+    const FieldSlot* f;
+    st = focus_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    (void)hipSetDevice(ctx->device);
+    return focus_dt(ctx, p, *f, dt);
+}
+
+kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot, uint32_t step,
+                                  double* dt_used, uint64_t* failures) {
+    if (!ctx) return KPT_ERR_ARG;
+    kpt_status st = check_focus(ctx, p);
+    if (st != KPT_OK) return st;
+    const FieldSlot* f;
+    st = focus_slot(ctx, slot, &f);
+    if (st != KPT_OK) return st;
+    if (ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
+        return fail(ctx, KPT_ERR_STATE, "noise table does not cover this step of every stored particle.");
+    (void)hipSetDevice(ctx->device);
+    double dt = 0.0;
+    st = focus_dt(ctx, p, *f, &dt);
+    if (st != KPT_OK) return st;
+    if (!std::isfinite(dt) || dt <= 0.0)
+        return fail(ctx, KPT_ERR_STATE,
+                    "FocusTransportSolver: adaptive stochastic time step is not positive and finite.");
+    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
+    KPT_HIP(ctx, hipMemsetAsync(ctx->counters + 5, 0, sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_focus_step(ctx->P, ctx->count, f->foc, f->g, *p, dt, step, key, ctx->noise,
+                                        ctx->noise_steps, ctx->counters + 5, ctx->stream));
+    unsigned long long nf = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&nf, ctx->counters + 5, 8, hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (dt_used) *dt_used = dt;
+    if (failures) *failures = nf;
+    return KPT_OK;
 }
 
 }  // extern "C"

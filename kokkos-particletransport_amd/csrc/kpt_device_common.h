@@ -32,6 +32,26 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
+// FocusTransportSolver::reflected_pitch_angle_cosine (include/FocusSolver.hpp:
+// 454-470): mirror at +-upper. The mirror loop is bounded (16 passes); a value
+// still outside is folded by the closed-form triangle wave of period 4 upper,
+// so every lane leaves this function after a fixed amount of work.
+__device__ __forceinline__ double reflect_mu(double mu, double upper) {
+    const double lower = -upper;
+    for (int k = 0; k < 16 && (mu > upper || mu < lower); ++k) {
+        if (mu > upper) mu = 2.0 * upper - mu;
+        if (mu < lower) mu = 2.0 * lower - mu;
+    }
+    if (mu > upper || mu < lower) {
+        const double period = 4.0 * upper;
+        double m = mu + upper;
+        m -= period * floor(m / period);
+        if (m > 2.0 * upper) m = period - m;
+        mu = m - upper;
+    }
+    return mu;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -17,6 +17,7 @@
 #include <cstdint>
 
 #include "kpt_device_common.h"
+#include "kpt_generic_device.h"
 #include "kpt_grid_device.h"
 #include "kpt_internal.h"
 
@@ -25,100 +26,6 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kGC = KPT_GENERIC_PARKER_COMPONENTS;
-
-__device__ __forceinline__ int wrapi(int i, int n) {
-    const int w = i % n;
-    return w < 0 ? w + n : w;
-}
-
-// Cell-centre coordinate (AnalyticCoordinateGrid, include/AnalyticCoordinateGrid.hpp:159-176).
-__device__ __forceinline__ double cell_coord(const GridDesc& g, int d, int k) {
-    return 0.5 * ((g.start[d] + k * g.step[d]) + (g.start[d] + (k + 1.0) * g.step[d]));
-}
-
-// centered_three_point_derivative (include/ParkerCoefficient.hpp:215-233).
-__device__ __forceinline__ double three_point(double xl, double xc, double xu, double fl, double fc, double fu) {
-    const double lw = xc - xl, uw = xu - xc, tw = xu - xl;
-    return (-uw / (lw * tw)) * fl + ((uw - lw) / (lw * uw)) * fc + (lw / (uw * tw)) * fu;
-}
-
-struct RawView {
-    const double* raw;  // 6 planes of nx * ny (Bx, By, Bz, Vx, Vy, Vz), x fastest
-    int nx, ny;
-    __device__ double at(int c, int i, int j) const {
-        return raw[static_cast<int64_t>(c) * nx * ny + static_cast<int64_t>(wrapi(j, ny)) * nx + wrapi(i, nx)];
-    }
-};
-
-struct PointEval {
-    const RawView& r;
-    const kpt_generic_parker_params& p;
-    int* bad;
-
-    __device__ double bmag(int i, int j) const {
-        double s = 0.0;
-        for (int c = 0; c < 3; ++c) s += r.at(c, i, j) * r.at(c, i, j);
-        return sqrt(s);
-    }
-    // magnetic_direction_component (ParkerCoefficient.hpp:255-270)
-    __device__ double bdir(int c, int i, int j) const {
-        const double m = bmag(i, j);
-        const double f = p.minimum_magnetic_field_magnitude;
-        if (m <= 0.0 && f <= 0.0) *bad = 1;
-        return r.at(c, i, j) / (m > f ? m : f);
-    }
-    // magnetic_field_over_magnitude_squared_component (ParkerCoefficient.hpp:279-306)
-    __device__ double over_b2(int c, int i, int j) const {
-        double s = 0.0;
-        for (int k = 0; k < 3; ++k) s += r.at(k, i, j) * r.at(k, i, j);
-        const double f = p.minimum_magnetic_field_squared_for_drift;
-        if (s <= 0.0 && f <= 0.0) *bad = 1;
-        return r.at(c, i, j) / (s > f ? s : f);
-    }
-    // evaluate_gamma_independent_coefficients (LegencyModel.hpp:430-470) with the
-    // constant turbulence state of TurbulencePropertyModel::evaluate_constant.
-    __device__ void kappa(int i, int j, double& kpar, double& kperp) const {
-        auto floor_pos = [](double v, double f) { return f <= 0.0 ? v : (v < f ? f : v); };
-        double vs = 0.0;
-        for (int c = 3; c < 6; ++c) vs += r.at(c, i, j) * r.at(c, i, j);
-        const double b = floor_pos(bmag(i, j), p.minimum_magnetic_field_magnitude);
-        const double v = floor_pos(sqrt(vs), p.minimum_solar_wind_speed);
-        const double s2 = floor_pos(p.sigma2, p.minimum_sigma2);
-        const double lc = p.lc > 0.0 ? p.lc : p.lslab;
-        if (b <= 0.0) *bad = 1;
-        const double omega = fabs(p.charge) * b / (p.rest_mass * p.speed_of_light);
-        if (v <= 0.0 || lc <= 0.0 || omega <= 0.0 || s2 <= 0.0) *bad = 1;
-        kpar = p.parallel_prefactor * pow(v, 4.0 / 3.0) * pow(lc, 2.0 / 3.0) / (pow(omega, 1.0 / 3.0) * s2);
-        if (p.perpendicular_model == 1) {
-            kperp = p.perpendicular_parallel_ratio * kpar;
-        } else {
-            if (p.l2d <= 0.0 || kpar <= 0.0) *bad = 1;
-            kperp = (v / 3.0) * pow(3.0 * kpar / v, 1.0 / 3.0) * pow(p.perpendicular_prefactor * s2 * p.l2d, 2.0 / 3.0);
-        }
-    }
-    // diffusion_tensor_basis_component (ParkerCoefficient.hpp:611-640):
-    // basis 0 parallel, 1 perpendicular, 2 constant ratio
-    __device__ double tensor(int basis, int a, int b, int i, int j) const {
-        const double ba = bdir(a, i, j), bb = bdir(b, i, j);
-        const double proj = ba * bb, id = a == b ? 1.0 : 0.0;
-        double kpar, kperp;
-        kappa(i, j, kpar, kperp);
-        if (basis == 0) return kpar * proj;
-        if (basis == 1) return kperp * (id - proj);
-        const double rr = p.perpendicular_parallel_ratio;
-        return kpar * (rr * id + (1.0 - rr) * proj);
-    }
-};
-
-// Central derivative along dim of f(i, j) at physical (i, j); z -> 0.
-template <typename F>
-__device__ __forceinline__ double deriv(const GridDesc& g, int i, int j, int dim, F f) {
-    if (dim >= 2 || (dim == 0 ? g.nx : g.ny) <= 1) return 0.0;
-    const int c = dim == 0 ? i : j;
-    const double fl = dim == 0 ? f(i - 1, j) : f(i, j - 1);
-    const double fu = dim == 0 ? f(i + 1, j) : f(i, j + 1);
-    return three_point(cell_coord(g, dim, c - 1), cell_coord(g, dim, c), cell_coord(g, dim, c + 1), fl, f(i, j), fu);
-}
 
 __global__ void __launch_bounds__(kBlk) generic_pack_kernel(const double* __restrict__ raw, GridDesc g,
                                                             kpt_generic_parker_params p, uint64_t plane,
@@ -232,14 +139,6 @@ __device__ __forceinline__ double generic_local_dt(const GridDesc& g, const GenT
     return m < mdt ? m : mdt;
 }
 
-// MappedCoordinateGrid::wrap_coordinate for the physical domain of axis d.
-__device__ __forceinline__ double wrap_axis(const GridDesc& g, int d, double x) {
-    double w = x - floor((x - g.phys_lo[d]) * g.inv_width[d]) * g.width[d];
-    if (w >= g.phys_hi[d]) w -= g.width[d];
-    if (w < g.phys_lo[d]) w += g.width[d];
-    return w;
-}
-
 __global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint64_t count,
                                                           const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                           kpt_generic_parker_params p,
@@ -251,14 +150,7 @@ __global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint
         if (generic_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], T))
             m = generic_local_dt(g, T);
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(m, off, 64);
-        m = o < m ? o : m;
-    }
-    // positive doubles order like their bit patterns: the minimum is exact and
-    // independent of the order of the atomics
-    if ((threadIdx.x & 63) == 0) atomicMin(min_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    wave_min_to(m, min_bits);
 }
 
 __global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, uint64_t count,
@@ -281,16 +173,7 @@ __global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, ui
                 const double* row = noise + (i * noise_steps + step) * 4;
                 for (int d = 0; d < 4; ++d) n[d] = row[d];
             } else {
-                const uint64_t id = P.id[i];
-                uint32_t r[4];
-                philox4x32_10(static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), 0x47454E50u, step, k0, k1,
-                              r);
-                for (int k = 0; k < 2; ++k) {
-                    const double rad = sqrt(-2.0 * log(u01(r[2 * k])));
-                    const double ang = 6.283185307179586476925286766559 * u01(r[2 * k + 1]);
-                    n[2 * k] = rad * cos(ang);
-                    n[2 * k + 1] = rad * sin(ang);
-                }
+                philox_normals4(P.id[i], 0x47454E50u, step, k0, k1, n);
             }
             // sample_magnetic_diffusion_increment (ParkerSolver.hpp:443-477)
             double iso = 0.0;
@@ -310,9 +193,7 @@ __global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, ui
             P.p[i] = pn > 0.0 ? pn : 0.0;
         }
     }
-    unsigned int w = fail;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) w += __shfl_xor(w, off, 64);
+    const unsigned int w = wave_sum(fail);
     if ((threadIdx.x & 63) == 0 && w) atomicAdd(failures, static_cast<unsigned long long>(w));
 }
 

@@ -157,6 +157,16 @@ hipError_t launch_split(const ParticleArrays& P, uint64_t count, uint64_t capaci
                         const kpt_device_settings& s, const SplitTable& tab, uint32_t* scratch,
                         unsigned long long* counters /* [0] split, [1] skipped */,
                         hipStream_t stream);
+// Generic focused-transport solver (kpt_focus.hip): the 27-component basis, the
+// per-particle stable-dt minimum (as double bits) and one step.
+hipError_t launch_focus_pack(const double* raw, const double* dvdt, const GridDesc& g,
+                             const kpt_generic_focus_params& p, double* out, int* bad, hipStream_t stream);
+hipError_t launch_focus_min_dt(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                               const kpt_generic_focus_params& p, unsigned long long* min_bits, hipStream_t stream);
+hipError_t launch_focus_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
+                             const kpt_generic_focus_params& p, double dt, uint32_t step, const uint32_t key[2],
+                             const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                             hipStream_t stream);
 // Exclusive scan of n uint32 tile counts in place; the total lands in [n].
 hipError_t launch_scan_tiles(uint32_t* tile_counts, uint32_t n, hipStream_t stream);
 // ParticleSystem lifecycle passes. The energy split needs esplit_scratch_elems(count)

@@ -191,26 +191,6 @@ __device__ __forceinline__ float draw_sqrt3<float>(uint32_t r) {
     return fmaf(u, static_cast<float>(2.0 * kSqrt3), static_cast<float>(-kSqrt3));
 }
 
-// FocusTransportSolver::reflected_pitch_angle_cosine (include/FocusSolver.hpp:
-// 454-470): mirror at +-upper. The mirror loop is bounded (16 passes); a value
-// still outside is folded by the closed-form triangle wave of period 4 upper,
-// so every lane leaves this function after a fixed amount of work.
-__device__ __forceinline__ double reflect_mu(double mu, double upper) {
-    const double lower = -upper;
-    for (int k = 0; k < 16 && (mu > upper || mu < lower); ++k) {
-        if (mu > upper) mu = 2.0 * upper - mu;
-        if (mu < lower) mu = 2.0 * lower - mu;
-    }
-    if (mu > upper || mu < lower) {
-        const double period = 4.0 * upper;
-        double m = mu + upper;
-        m -= period * floor(m / period);
-        if (m > 2.0 * upper) m = period - m;
-        mu = m - upper;
-    }
-    return mu;
-}
-
 // Slab-QLT D_mumu and dD_mumu/dmu (include/LegencyModel.hpp:557-643) for one
 // trial mu, with omega, xi = omega Lc / v and xi^gamma_k fixed for the sub-step.
 // `pref` = (pi / 4) omega sigma2 gamma_k sin(pi / gamma_k) / pi.

@@ -154,6 +154,22 @@ class GenericParkerParams(C.Structure):
         return c
 
 
+class GenericFocusParams(C.Structure):
+    """kpt_generic_focus_params: the generic focused-transport solver's controls."""
+    _fields_ = [
+        ("model", GenericParkerParams), ("minimum_magnetic_field_magnitude", C.c_double),
+        ("spectral_index", C.c_double), ("resonance_regularization_h0", C.c_double),
+        ("scattering_minimum_magnetic_field_magnitude", C.c_double), ("scattering_minimum_sigma2", C.c_double),
+        ("scattering_minimum_correlation_length", C.c_double), ("scattering_minimum_speed", C.c_double),
+        ("maximum_pitch_angle_cosine", C.c_double),
+    ]
+
+    def copy(self) -> "GenericFocusParams":
+        c = GenericFocusParams()
+        C.pointer(c)[0] = self
+        return c
+
+
 class ParticleProperties(C.Structure):
     """kpt_particle_properties (ParticleProperties, include/ParticleSystem.hpp:145-213)."""
     _fields_ = [("rest_mass", C.c_double), ("charge", C.c_double), ("speed_of_light", C.c_double),
@@ -303,6 +319,14 @@ EXPORTS = {
                                                C.POINTER(C.c_double)]),
     "kpt_generic_parker_step": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32,
                                           C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_generic_focus_default_params": (C.c_int, [C.POINTER(GenericFocusParams)]),
+    "kpt_generic_focus_build": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32,
+                                          C.POINTER(C.c_double)]),
+    "kpt_generic_focus_download": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_size_t]),
+    "kpt_generic_focus_time_step": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32,
+                                              C.POINTER(C.c_double)]),
+    "kpt_generic_focus_step": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32, C.c_uint32,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kpt_default_particle_properties": (C.c_int, [C.POINTER(ParticleProperties)]),
     "kpt_default_energy_split_policy": (C.c_int, [C.POINTER(EnergySplitPolicy)]),
     "kpt_split_energy_growth": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties),
@@ -361,6 +385,12 @@ def make_device_settings(host: HostSettings, x_edges, y_edges) -> DeviceSettings
 def generic_default_params() -> GenericParkerParams:
     p = GenericParkerParams()
     _check_static(load().kpt_generic_parker_default_params(C.byref(p)))
+    return p
+
+
+def generic_focus_default_params() -> GenericFocusParams:
+    p = GenericFocusParams()
+    _check_static(load().kpt_generic_focus_default_params(C.byref(p)))
     return p
 
 
@@ -613,6 +643,28 @@ class Context:
         dt, nf = C.c_double(), C.c_uint64()
         self._check(self.lib.kpt_generic_parker_step(self.h, C.byref(params), slot, step, C.byref(dt),
                                                      C.byref(nf)))
+        return dt.value, nf.value
+
+    # ---- generic focused-transport solver --------------------------------------
+    def focus_build(self, params: "GenericFocusParams", slot=0, dvdt=None):
+        d = None if dvdt is None else np.ascontiguousarray(np.stack([np.asarray(a, np.float64) for a in dvdt]))
+        self._check(self.lib.kpt_generic_focus_build(self.h, C.byref(params), slot, _dp(d)))
+
+    def focus_coefficients(self, slot, nx, ny):
+        out = np.empty((ny + 2) * (nx + 2) * 27)
+        self._check(self.lib.kpt_generic_focus_download(self.h, slot, _dp(out), out.size))
+        return out.reshape(ny + 2, nx + 2, 27)
+
+    def focus_time_step(self, params: "GenericFocusParams", slot=0) -> float:
+        dt = C.c_double()
+        self._check(self.lib.kpt_generic_focus_time_step(self.h, C.byref(params), slot, C.byref(dt)))
+        return dt.value
+
+    def focus_step(self, params: "GenericFocusParams", step, slot=0):
+        """One FocusTransportSolver::advance_stochastic step; returns (dt, failures)."""
+        dt, nf = C.c_double(), C.c_uint64()
+        self._check(self.lib.kpt_generic_focus_step(self.h, C.byref(params), slot, step, C.byref(dt),
+                                                    C.byref(nf)))
         return dt.value, nf.value
 
     # ---- ParticleSystem lifecycle ---------------------------------------------

@@ -1410,3 +1410,400 @@ uint64_t kpto_apply_boundaries(int32_t nx, int32_t ny, const double* xe, const d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Generic focused-transport solver: the packed 27-component basis of
+// FocusCoefficient::make_legacy_transport_coefficient_fields_with_time_derivative
+// (include/FocusCoefficient.hpp:858-990) as FocusTransportSolver::
+// build_packed_coefficient_basis (include/FocusSolver.hpp:106-136, :497-590)
+// packs it, and the solver's global-dt step (include/FocusSolver.hpp:276-299,
+// :635-719, :905-1033, :1149-1232, :1560-1620). Intermediate fields keep the
+// reference's order: |B| and b and ln|B| on every stored point, derivatives on
+// the physical points followed by the periodic ghost fill.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kFC = KPT_GENERIC_FOCUS_COMPONENTS;
+
+double floor_positive(double v, double f) { return f <= 0.0 ? v : (v < f ? f : v); }
+
+// physical_vector_gradient_component, Cartesian (FocusCoefficient.hpp:184-216):
+// d V_out / d x_der, zero for the invariant z.
+double vec_grad(const OField& f, int i, int j, int out, int der) {
+    if (der >= 2) return 0.0;
+    return gen_derivative(f.g, i, j, der, [&](int a, int b) { return f.at(a, b, out); });
+}
+
+bool foc_build(const OField& B, const OField& V, const OField* dvdt, const kpt_generic_focus_params& P, OField& out) {
+    const kpt_generic_parker_params& M = P.model;
+    const OGrid& g = B.g;
+    const int nx = g.n[0], ny = g.n[1];
+    const double ff = P.minimum_magnetic_field_magnitude;
+    bool bad = false;
+    OField mag = gen_alloc(g, 1), dir = gen_alloc(g, 3), lnb = gen_alloc(g, 1), glnb = gen_alloc(g, 3),
+           curv = gen_alloc(g, 3), drift = gen_alloc(g, 6), kap = gen_alloc(g, 2), dperp = gen_alloc(g, 3),
+           dv = gen_alloc(g, 1), con = gen_alloc(g, 3);
+    // make_magnitude_field, make_direction_field (Abort at |B| <= floor),
+    // make_log_magnetic_field (FieldOperator.hpp:616-640, :700-800; FocusCoefficient.hpp:486-520)
+    for (int j = -kGhost; j < ny + kGhost; ++j)
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            const double m = gen_bmag(B, i, j);
+            mag.at(i, j, 0) = m;
+            if (m <= ff) {
+                bad = true;
+                continue;
+            }
+            const double inv = 1.0 / m;
+            for (int c = 0; c < 3; ++c) dir.at(i, j, c) = B.at(i, j, c) * inv;
+            const double safe = m > ff ? m : ff;
+            if (safe <= 0.0) bad = true;
+            lnb.at(i, j, 0) = std::log(safe);
+        }
+    if (bad) return false;
+    // FieldOperator::make_gradient_field of ln|B|; magnetic curvature (b.grad)b
+    // (FocusCoefficient.hpp:430-450, :545-580)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            for (int d = 0; d < 2; ++d)
+                glnb.at(i, j, d) = gen_derivative(g, i, j, d, [&](int a, int b) { return lnb.at(a, b, 0); });
+            glnb.at(i, j, 2) = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double r = 0.0;
+                for (int d = 0; d < 3; ++d) r += dir.at(i, j, d) * vec_grad(dir, i, j, c, d);
+                curv.at(i, j, c) = r;
+            }
+        }
+    fill_ghosts_periodic(glnb);
+    fill_ghosts_periodic(curv);
+    // fill_focused_drift_basis_fields (FocusCoefficient.hpp:600-660), every point
+    auto cross = [](const double a[3], const double b[3], int c) {
+        return c == 0 ? a[1] * b[2] - a[2] * b[1] : (c == 1 ? a[2] * b[0] - a[0] * b[2] : a[0] * b[1] - a[1] * b[0]);
+    };
+    for (int j = -kGhost; j < ny + kGhost; ++j)
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            const double mm = mag.at(i, j, 0) > ff ? mag.at(i, j, 0) : ff;
+            if (mm <= 0.0) bad = true;
+            const double inv = 1.0 / mm;
+            const double bb[3] = {dir.at(i, j, 0), dir.at(i, j, 1), dir.at(i, j, 2)};
+            const double gg[3] = {glnb.at(i, j, 0), glnb.at(i, j, 1), glnb.at(i, j, 2)};
+            const double cc[3] = {curv.at(i, j, 0), curv.at(i, j, 1), curv.at(i, j, 2)};
+            for (int c = 0; c < 3; ++c) {
+                drift.at(i, j, c) = cross(bb, gg, c) * inv;
+                drift.at(i, j, 3 + c) = cross(bb, cc, c) * inv;
+            }
+        }
+    // gamma-independent kappa (LegencyModel.hpp:801-871) and
+    // div(kappa_perp (I - bb)) with the diffusion floor (FocusCoefficient.hpp:840-856)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double vs = 0.0;
+            for (int k = 0; k < 3; ++k) vs += V.at(i, j, k) * V.at(i, j, k);
+            gen_kappa(M, gen_bmag(B, i, j), std::sqrt(vs), kap.at(i, j, 0), kap.at(i, j, 1), bad);
+        }
+    fill_ghosts_periodic(kap);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double s = 0.0;
+                for (int d = 0; d < 2; ++d)
+                    s += gen_derivative(g, i, j, d, [&](int a, int b) {
+                        const double bi = gen_bdir(B, a, b, c, M.minimum_magnetic_field_magnitude, bad);
+                        const double bj = gen_bdir(B, a, b, d, M.minimum_magnetic_field_magnitude, bad);
+                        return kap.at(a, b, 1) * ((c == d ? 1.0 : 0.0) - bi * bj);
+                    });
+                dperp.at(i, j, c) = s;
+            }
+    fill_ghosts_periodic(dperp);
+    // div V; focusing b.grad ln|B|, bb:grad V, b.dV/dt + b.(V.grad)V
+    // (FocusCoefficient.hpp:218-330, :700-778)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double s = 0.0;
+            for (int d = 0; d < 2; ++d)
+                s += gen_derivative(g, i, j, d, [&](int a, int b) { return V.at(a, b, d); });
+            dv.at(i, j, 0) = s;
+            double foc = 0.0, bbg = 0.0, conv = 0.0, part = 0.0;
+            for (int c = 0; c < 3; ++c) foc += dir.at(i, j, c) * glnb.at(i, j, c);
+            for (int a = 0; a < 3; ++a)
+                for (int d = 0; d < 3; ++d) bbg += dir.at(i, j, a) * dir.at(i, j, d) * vec_grad(V, i, j, a, d);
+            if (dvdt)
+                for (int c = 0; c < 3; ++c) part += dir.at(i, j, c) * dvdt->at(i, j, c);
+            for (int a = 0; a < 3; ++a)
+                for (int d = 0; d < 3; ++d) conv += dir.at(i, j, a) * V.at(i, j, d) * vec_grad(V, i, j, a, d);
+            con.at(i, j, 0) = foc;
+            con.at(i, j, 1) = bbg;
+            con.at(i, j, 2) = part + conv;
+        }
+    fill_ghosts_periodic(dv);
+    fill_ghosts_periodic(con);
+    if (bad) return false;
+    out = gen_alloc(g, kFC);
+    const double lc = M.lc > 0.0 ? M.lc : M.lslab;
+    for (int j = -kGhost; j < ny + kGhost; ++j)
+        for (int i = -kGhost; i < nx + kGhost; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                out.at(i, j, c) = B.at(i, j, c);
+                out.at(i, j, 3 + c) = dir.at(i, j, c);
+                out.at(i, j, 6 + c) = V.at(i, j, c);
+                out.at(i, j, 9 + c) = drift.at(i, j, c);
+                out.at(i, j, 12 + c) = drift.at(i, j, 3 + c);
+                out.at(i, j, 15 + c) = dperp.at(i, j, c);
+            }
+            out.at(i, j, 18) = mag.at(i, j, 0);
+            out.at(i, j, 19) = kap.at(i, j, 0);
+            out.at(i, j, 20) = kap.at(i, j, 1);
+            out.at(i, j, 21) = dv.at(i, j, 0);
+            out.at(i, j, 22) = con.at(i, j, 0);
+            out.at(i, j, 23) = con.at(i, j, 1);
+            out.at(i, j, 24) = con.at(i, j, 2);
+            out.at(i, j, 25) = M.sigma2;
+            out.at(i, j, 26) = lc;
+        }
+    return true;
+}
+
+struct FocTerms {
+    bool ok = false;
+    double b[3] = {0, 0, 0}, u[2] = {0, 0}, keff[2] = {0, 0};
+    double p = 0, mu = 0, gamma = 0, speed = 0, kpar = 0, kperp = 0, prate = 0;
+    double bmag = 0, divv = 0, bbg = 0, bdv = 0, foc = 0, s2 = 0, lc = 0;
+    double D = 0, dD = 0, mu_rate = 0;
+};
+
+// evaluate_pitch_angle_at_mu (FocusSolver.hpp:665-694) with
+// LegencyModel::evaluate_pitch_angle_scattering (LegencyModel.hpp:650-689).
+bool foc_pitch(const kpt_generic_focus_params& P, const FocTerms& T, double trial, double& D, double& dD,
+               double& rate) {
+    const kpt_generic_parker_params& M = P.model;
+    const double mu = reflect_mu(trial, P.maximum_pitch_angle_cosine);
+    const double omm = 1.0 - mu * mu;
+    if (T.speed <= 0.0) return false;
+    const double focusing = 0.5 * omm * (-T.speed * T.foc + mu * T.divv - 3.0 * mu * T.bbg - 2.0 * T.bdv / T.speed);
+    const double sb = floor_positive(T.bmag, P.scattering_minimum_magnetic_field_magnitude);
+    const double ss = floor_positive(T.s2, P.scattering_minimum_sigma2);
+    const double sl = floor_positive(T.lc, P.scattering_minimum_correlation_length);
+    const double sv = floor_positive(T.speed, P.scattering_minimum_speed);
+    if (sb <= 0.0 || ss <= 0.0 || sl <= 0.0 || sv <= 0.0) return false;
+    const double omega = std::fabs(M.charge) * sb / (M.rest_mass * M.speed_of_light) / T.gamma;
+    const QltD q = qlt_scattering(mu, omega, sv, ss, sl, P.spectral_index, P.resonance_regularization_h0);
+    D = q.D;
+    dD = q.dD;
+    rate = focusing + q.dD;
+    return true;
+}
+
+// evaluate_terms_from_packed (FocusSolver.hpp:905-1033) at a wrapped position.
+FocTerms foc_terms(const OField& F, const kpt_generic_focus_params& P, const double pos[2], double p_in,
+                   double mu_in) {
+    const kpt_generic_parker_params& M = P.model;
+    FocTerms T;
+    double v[kFC];
+    sample(F, pos, v);
+    T.p = std::fabs(p_in);
+    T.mu = reflect_mu(mu_in, P.maximum_pitch_angle_cosine);
+    const double nm = T.p / (M.rest_mass * M.speed_of_light);
+    T.gamma = std::sqrt(1.0 + nm * nm);
+    T.speed = T.p / (T.gamma * M.rest_mass);
+    T.bmag = v[18];
+    if (!(std::isfinite(T.bmag) && T.bmag > 0.0)) return T;
+    const double dm = std::sqrt(v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+    if (!(std::isfinite(dm) && dm > 0.0)) return T;
+    const double idm = 1.0 / dm;
+    for (int c = 0; c < 3; ++c) T.b[c] = v[3 + c] * idm;
+    const double fperp = M.perpendicular_model == 1 ? std::pow(T.gamma, 1.0 / 3.0) : std::pow(T.gamma, 1.0 / 9.0);
+    T.kpar = v[19] * std::pow(T.gamma, 1.0 / 3.0);
+    T.kperp = v[20] * fperp;
+    if (!(std::isfinite(T.kpar) && T.kpar >= 0.0 && std::isfinite(T.kperp) && T.kperp >= 0.0)) return T;
+    const double mu2 = T.mu * T.mu;
+    const double gdf = 0.5 * (1.0 - mu2), cdf = mu2;
+    const double pref = T.p * T.speed * M.speed_of_light / M.charge;
+    for (int d = 0; d < 2; ++d) {
+        const double stream = T.speed * T.mu * T.b[d];
+        const double drift = pref * (gdf * v[9 + d] + cdf * v[12 + d]);
+        const double dadv = fperp * v[15 + d];
+        T.u[d] = stream + v[6 + d] + drift + dadv;
+        const double proj = 1.0 - T.b[d] * T.b[d];
+        T.keff[d] = proj > 0.0 ? T.kperp * proj : 0.0;
+    }
+    T.divv = v[21];
+    T.bbg = v[23];
+    T.bdv = v[24];
+    T.foc = v[22];
+    T.s2 = v[25];
+    T.lc = v[26];
+    const double omm = 1.0 - mu2;
+    const double accel = T.speed > 0.0 ? T.mu * T.bdv / T.speed : 0.0;
+    const double mf = 0.5 * omm * (T.divv - T.bbg) + mu2 * T.bbg + accel;
+    T.prate = -T.p * mf;
+    if (!foc_pitch(P, T, T.mu, T.D, T.dD, T.mu_rate)) return T;
+    T.ok = true;
+    return T;
+}
+
+// local_stable_time_step (FocusSolver.hpp:1149-1232).
+double foc_local_dt(const OGrid& g, const FocTerms& T) {
+    const double large = 1.0e300;
+    double ddt = large, adt = large;
+    for (int d = 0; d < 2; ++d) {
+        const double dx = std::fabs(g.step[d]);
+        double di = large, ai = large;
+        if (T.keff[d] > 0.0) {
+            const double lim = 0.5 * dx;
+            di = lim * lim / (2.0 * T.keff[d]);
+        }
+        const double a = std::fabs(T.u[d]);
+        if (a > 0.0) ai = 1.0 * dx / a;
+        ddt = ddt < di ? ddt : di;
+        adt = adt < ai ? adt : ai;
+    }
+    double mdt = large, padt = large, pddt = large, pbdt = large;
+    const double pr = std::fabs(T.prate);
+    if (T.p > 0.0 && pr > 0.0) mdt = 1.0 * T.p / pr;
+    const double mr = std::fabs(T.mu_rate);
+    if (mr > 0.0) padt = 10.0 * 0.1 / mr;
+    if (T.D > 0.0) pddt = 10.0 * 0.1 * 0.1 / (2.0 * T.D);
+    if (T.D > 0.0 && mr > 0.0) pbdt = 10.0 * 2.0 * T.D / (mr * mr);
+    const double a = ddt < adt ? ddt : adt;
+    const double b = a < mdt ? a : mdt;
+    const double c = pddt < pbdt ? pddt : pbdt;
+    const double e = padt < c ? padt : c;
+    return b < e ? b : e;
+}
+
+// Four standard normals of step `step` (Box-Muller on one Philox block with
+// counter (id_lo, id_hi, 0x464F4355, step)) or a noise-table row.
+void foc_normals(const double* table, uint32_t steps, uint64_t i, uint64_t id, uint32_t step, const uint32_t key[2],
+                 double n[4]) {
+    if (table) {
+        const double* row = table + (static_cast<size_t>(i) * steps + step) * 4;
+        for (int d = 0; d < 4; ++d) n[d] = row[d];
+        return;
+    }
+    const uint32_t ctr[4] = {static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), 0x464F4355u, step};
+    uint32_t r[4];
+    philox10(ctr, key, r);
+    for (int k = 0; k < 2; ++k) {
+        const double rad = std::sqrt(-2.0 * std::log(u01(r[2 * k])));
+        const double ang = 6.283185307179586476925286766559 * u01(r[2 * k + 1]);
+        n[2 * k] = rad * std::cos(ang);
+        n[2 * k + 1] = rad * std::sin(ang);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void kpto_generic_focus_default_params(kpt_generic_focus_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    kpto_generic_default_params(&p->model);
+    p->spectral_index = 5.0 / 3.0;
+    p->resonance_regularization_h0 = 1.0e-6;
+    p->maximum_pitch_angle_cosine = 1.0;
+}
+
+// dvdt: nullptr or three planes [3][ny][nx]. nullptr where the reference aborts.
+void* kpto_generic_focus_create(int32_t nx, int32_t ny, const double* xe, const double* ye,
+                                const double* const planes[6], const double* dvdt,
+                                const kpt_generic_focus_params* p) {
+    OField B, V, T;
+    for (OField* f : {&B, &V, &T}) {
+        make_grid(f->g, nx, ny, xe, ye);
+        *f = gen_alloc(f->g, 3);
+    }
+    const size_t plane = static_cast<size_t>(nx) * ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const size_t k = static_cast<size_t>(j) * nx + i;
+                B.at(i, j, c) = planes[c][k];
+                V.at(i, j, c) = planes[3 + c][k];
+                if (dvdt) T.at(i, j, c) = dvdt[c * plane + k];
+            }
+    fill_ghosts_periodic(B);
+    fill_ghosts_periodic(V);
+    fill_ghosts_periodic(T);
+    OField* out = new OField();
+    if (!foc_build(B, V, dvdt ? &T : nullptr, *p, *out)) {
+        delete out;
+        return nullptr;
+    }
+    return out;
+}
+
+double kpto_generic_focus_time_step(const void* field, const kpt_generic_focus_params* p, const kpt_particles_host* P,
+                                    uint64_t count) {
+    const OField& F = *static_cast<const OField*>(field);
+    double m = 1.0e300;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
+        const FocTerms T = foc_terms(F, *p, pos, P->p[i], P->mu[i]);
+        if (!T.ok) continue;
+        const double d = foc_local_dt(F.g, T);
+        m = m < d ? m : d;
+    }
+    if (!std::isfinite(m) || m >= 0.5e300) return 0.0;
+    return p->model.courant * m;
+}
+
+// Terms of one particle for tests: out = {dt_local, u_x, u_y, p_rate, mu_rate, D,
+// dD, kpar, kperp, keff_x, keff_y, b_x, b_y, b_z}; returns 0 where the reference aborts.
+int kpto_generic_focus_terms(const void* field, const kpt_generic_focus_params* p, double x, double y, double pm,
+                             double mu, double* out) {
+    const OField& F = *static_cast<const OField*>(field);
+    const double pos[2] = {F.g.wrap(0, x), F.g.wrap(1, y)};
+    const FocTerms T = foc_terms(F, *p, pos, pm, mu);
+    if (!T.ok) return 0;
+    const double v[14] = {foc_local_dt(F.g, T), T.u[0], T.u[1], T.prate, T.mu_rate, T.D, T.dD,
+                          T.kpar, T.kperp, T.keff[0], T.keff[1], T.b[0], T.b[1], T.b[2]};
+    for (int k = 0; k < 14; ++k) out[k] = v[k];
+    return 1;
+}
+
+double kpto_generic_focus_step(const void* field, const kpt_generic_focus_params* p, kpt_particles_host* P,
+                               uint64_t count, uint32_t step, uint64_t seed, const double* noise,
+                               uint32_t noise_steps, uint64_t* failures) {
+    const OField& F = *static_cast<const OField*>(field);
+    const double dt = kpto_generic_focus_time_step(field, p, P, count);
+    if (failures) *failures = 0;
+    if (!(dt > 0.0)) return 0.0;
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    uint64_t fails = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] != 0) continue;
+        const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
+        const FocTerms T = foc_terms(F, *p, pos, P->p[i], P->mu[i]);
+        if (!T.ok) {
+            P->status[i] = 2;
+            ++fails;
+            continue;
+        }
+        double n[4];
+        foc_normals(noise, noise_steps, i, P->id ? P->id[i] : i, step, key, n);
+        // sample_perpendicular_diffusion_increment (FocusSolver.hpp:635-659)
+        double par = 0.0;
+        for (int c = 0; c < 3; ++c) par += n[c] * T.b[c];
+        const double scale = T.kperp > 0.0 ? std::sqrt(2.0 * T.kperp * dt) : 0.0;
+        double x[2] = {pos[0], pos[1]};
+        for (int d = 0; d < 2; ++d) x[d] += dt * T.u[d] + scale * (n[d] - par * T.b[d]);
+        P->x[i] = F.g.wrap(0, x[0]);
+        P->y[i] = F.g.wrap(1, x[1]);
+        const double pn = T.p + dt * T.prate;
+        P->p[i] = pn > 0.0 ? pn : 0.0;
+        // picard_pitch_angle_update (FocusSolver.hpp:700-719)
+        const double mu0 = reflect_mu(T.mu, p->maximum_pitch_angle_cosine);
+        double trial = mu0;
+        for (int it = 0; it < 3; ++it) {
+            double D = 0.0, dD = 0.0, rate = 0.0;
+            foc_pitch(*p, T, trial, D, dD, rate);  // the inputs that can fail were checked at T.mu
+            const double sc = D > 0.0 ? std::sqrt(2.0 * D * dt) : 0.0;
+            trial = reflect_mu(mu0 + dt * rate + sc * n[3], p->maximum_pitch_angle_cosine);
+        }
+        P->mu[i] = trial;
+    }
+    if (failures) *failures = fails;
+    return dt;
+}
+
+}  // extern "C"

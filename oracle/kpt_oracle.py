@@ -75,6 +75,16 @@ def load():
             "kpto_apply_boundaries": (C.c_uint64, [C.c_int32, C.c_int32, dp, dp, C.c_int32,
                                                    P(kpt.ParticlesHost), C.c_uint64]),
             "kpto_generic_default_params": (None, [P(kpt.GenericParkerParams)]),
+            "kpto_generic_focus_default_params": (None, [P(kpt.GenericFocusParams)]),
+            "kpto_generic_focus_create": (C.c_void_p, [C.c_int32, C.c_int32, dp, dp, P(dp), dp,
+                                                       P(kpt.GenericFocusParams)]),
+            "kpto_generic_focus_time_step": (C.c_double, [C.c_void_p, P(kpt.GenericFocusParams),
+                                                          P(kpt.ParticlesHost), C.c_uint64]),
+            "kpto_generic_focus_terms": (C.c_int, [C.c_void_p, P(kpt.GenericFocusParams), C.c_double, C.c_double,
+                                                   C.c_double, C.c_double, dp]),
+            "kpto_generic_focus_step": (C.c_double, [C.c_void_p, P(kpt.GenericFocusParams), P(kpt.ParticlesHost),
+                                                     C.c_uint64, C.c_uint32, C.c_uint64, dp, C.c_uint32,
+                                                     P(C.c_uint64)]),
             "kpto_generic_create": (C.c_void_p, [C.c_int32, C.c_int32, dp, dp, P(dp),
                                                  P(kpt.GenericParkerParams)]),
             "kpto_generic_time_step": (C.c_double, [C.c_void_p, P(kpt.GenericParkerParams),
@@ -266,4 +276,58 @@ class GenericField(Field):
             ns = nz.shape[1]
         dt = load().kpto_generic_step(self.h, C.byref(self.params), C.byref(h), len(parts["x"]),
                                       step, seed, _dp(nz), ns, C.byref(fails))
+        return dt, fails.value
+
+
+def generic_focus_default_params():
+    p = kpt.GenericFocusParams()
+    load().kpto_generic_focus_default_params(C.byref(p))
+    return p
+
+
+FOCUS_TERMS = ("dt_local", "u_x", "u_y", "p_rate", "mu_rate", "D", "dD", "kpar", "kperp", "keff_x", "keff_y",
+               "b_x", "b_y", "b_z")
+
+
+class GenericFocusField(Field):
+    """Packed 27-component basis of the generic focused-transport solver on a
+    periodic grid (make_legacy_transport_coefficient_fields_with_time_derivative
+    + FocusTransportSolver::build_packed_coefficient_basis)."""
+
+    def __init__(self, x_edges, y_edges, planes, params, dvdt=None):
+        lib = load()
+        self.xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+        self.ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+        self._planes = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+        self.nx, self.ny, self.model = len(self.xe) - 1, len(self.ye) - 1, 27
+        self._dvdt = None if dvdt is None else np.ascontiguousarray(np.stack([np.asarray(a, np.float64)
+                                                                               for a in dvdt]))
+        arr = (C.POINTER(C.c_double) * 6)(*[_dp(a) for a in self._planes])
+        self.params = params
+        self.h = lib.kpto_generic_focus_create(self.nx, self.ny, _dp(self.xe), _dp(self.ye), arr,
+                                               _dp(self._dvdt), C.byref(params))
+        if not self.h:
+            raise ValueError("generic focused coefficients: the reference aborts on this field")
+
+    def time_step(self, parts):
+        h = kpt.particles_struct(parts)
+        return load().kpto_generic_focus_time_step(self.h, C.byref(self.params), C.byref(h), len(parts["x"]))
+
+    def terms(self, x, y, p, mu):
+        """Local terms of one particle (dict over FOCUS_TERMS), None where the reference aborts."""
+        out = np.empty(len(FOCUS_TERMS))
+        ok = load().kpto_generic_focus_terms(self.h, C.byref(self.params), x, y, p, mu, _dp(out))
+        return dict(zip(FOCUS_TERMS, out.tolist())) if ok else None
+
+    def step(self, parts, step, seed=0, noise=None):
+        """One advance_stochastic step (modifies parts). Returns (dt, failures)."""
+        h = kpt.particles_struct(parts)
+        fails = C.c_uint64()
+        nz, ns = None, 0
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            assert nz.ndim == 3 and nz.shape[2] == 4 and nz.shape[0] >= len(parts["x"])
+            ns = nz.shape[1]
+        dt = load().kpto_generic_focus_step(self.h, C.byref(self.params), C.byref(h), len(parts["x"]), step, seed,
+                                            _dp(nz), ns, C.byref(fails))
         return dt, fails.value
