@@ -433,6 +433,64 @@ kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_p
 kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
                                    uint32_t step, double* dt_used, uint64_t* failures);
 
+/* ---- batched ensemble: many independent runs in one persistent launch ----------
+ * No counterpart in the reference, which runs one configuration per process. An
+ * ensemble holds R runs, each with its own field frame, settings, Philox seed and
+ * particle range. kpt_ensemble_push advances every run through whole intervals
+ * in ONE persistent launch per interval: workgroups take 256-particle tiles from
+ * per-XCD work queues. A run is either the Parker FP32 push (field model
+ * PARKER16) or the focused FP32 push with the slab-QLT closure (FOCUSED21,
+ * closure KPT_CLOSURE_QLT, or KPT_CLOSURE_MLP with the context's network). Each
+ * run holds one frame: time interpolation is off and the interval is pushed in
+ * one chunk. A run's result equals a kpt_push_interval of the same run in a
+ * context of its own (same settings, seed, frame, particles). The frame loop can
+ * be captured once in a hipGraph and replayed. */
+typedef struct kpt_ensemble kpt_ensemble;
+
+typedef struct kpt_ensemble_tuning {
+    int32_t grid_blocks;  /* persistent workgroups of one launch (0: two per CU)           */
+    int32_t groups;       /* work queues: 1 (one global queue) or 8 (one per XCD, runs
+                             assigned longest-first, idle XCDs steal)                     */
+    double ms;            /* autotune: time of the chosen configuration on the window      */
+} kpt_ensemble_tuning;
+
+typedef struct kpt_ensemble_run_stats {
+    uint64_t particle_steps; /* sub-steps executed by the run in the call */
+    uint64_t failures;       /* particles that went to status 2           */
+    uint64_t active_in;      /* status-0 particles summed over the frames */
+} kpt_ensemble_run_stats;
+
+/* The ensemble uses ctx's device, stream and learned closure; destroy it before ctx. */
+kpt_status kpt_ensemble_create(kpt_ctx* ctx, int32_t runs, const uint64_t* capacities, kpt_ensemble** out);
+kpt_status kpt_ensemble_destroy(kpt_ensemble* e);
+kpt_status kpt_ensemble_field_upload(kpt_ensemble* e, int32_t run, int32_t model, int32_t nx, int32_t ny,
+                                     const double* x_edges, const double* y_edges, const double* bx,
+                                     const double* by, const double* bz, const double* vx, const double* vy,
+                                     const double* vz);
+/* Settings and Philox seed of one run (precision must be KPT_PREC_FP32). */
+kpt_status kpt_ensemble_set_run(kpt_ensemble* e, int32_t run, const kpt_device_settings* s, uint64_t seed);
+kpt_status kpt_ensemble_particles_upload(kpt_ensemble* e, int32_t run, const kpt_particles_host* h,
+                                         uint64_t count);
+kpt_status kpt_ensemble_particles_download(kpt_ensemble* e, int32_t run, kpt_particles_host* h, uint64_t count);
+kpt_status kpt_ensemble_count(kpt_ensemble* e, int32_t run, uint64_t* count);
+/* inject_uniform_particles into one run (its settings, seed and root counter). */
+kpt_status kpt_ensemble_inject(kpt_ensemble* e, int32_t run, uint64_t n, double t0, uint32_t frame,
+                               uint64_t* injected);
+/* Advance every run through `frames` intervals, frame f = first_frame .. + frames - 1
+ * starting at t0 = f * mhd_dt of the run. use_graph != 0 records one interval in a
+ * hipGraph and replays it. per_run (runs entries, may be NULL) receives the
+ * counters; kernel_ms the device time of the push launches. */
+kpt_status kpt_ensemble_push(kpt_ensemble* e, uint32_t first_frame, uint32_t frames, int32_t use_graph,
+                             kpt_ensemble_run_stats* per_run, double* kernel_ms);
+/* Time the candidate launch shapes on the first `window` (0 < window <= 1) of the
+ * interval starting at frame `frame`, keep the fastest, and leave the particles as
+ * they were. `table` (may be NULL) receives up to `table_len` candidates tried. */
+kpt_status kpt_ensemble_autotune(kpt_ensemble* e, uint32_t frame, double window, kpt_ensemble_tuning* chosen,
+                                 kpt_ensemble_tuning* table, int32_t table_len, int32_t* tried);
+kpt_status kpt_ensemble_set_tuning(kpt_ensemble* e, const kpt_ensemble_tuning* t);
+kpt_status kpt_ensemble_histogram_logp(kpt_ensemble* e, int32_t run, int32_t bins, double pmin, double pmax,
+                                       double* out);
+
 /* ---- generic focused-transport solver (global time step) -------------------------
  * FocusTransportSolver (include/FocusSolver.hpp:74-1890) on the packed basis of
  * FocusCoefficient::make_legacy_transport_coefficient_fields_with_time_derivative

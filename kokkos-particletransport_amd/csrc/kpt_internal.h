@@ -125,6 +125,22 @@ struct PushArgs {
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
 };
 
+// One run of a batched ensemble: its full push arguments (particles point into
+// the ensemble's arrays) and which lane function advances it.
+enum EnsembleKind : int32_t { kEnsParker32 = 0, kEnsFocusedQlt32 = 1, kEnsFocusedMlp32 = 2 };
+struct EnsembleRun {
+    PushArgs a;
+    int32_t kind;
+    int32_t pad;
+};
+// Frame-loop state of an ensemble in device memory: intervals run so far and the
+// first frame of the loop.
+struct EnsembleLoop {
+    uint32_t first_frame;
+    uint32_t done;
+    double window;  // fraction of the interval pushed (1 in the frame loop)
+};
+
 // Device-side entry points (kpt_kernels.hip). All launch on `stream`.
 hipError_t launch_pack(const double* raw /* 6 planes nx*ny */, const GridDesc& g, int model,
                        double* out64, float* out32, hipStream_t stream);
@@ -167,6 +183,15 @@ hipError_t launch_focus_step(const ParticleArrays& P, uint64_t count, const doub
                              const kpt_generic_focus_params& p, double dt, uint32_t step, const uint32_t key[2],
                              const double* noise, uint32_t noise_steps, unsigned long long* failures,
                              hipStream_t stream);
+// Ensemble: `tiles` lists (run, first particle) of every 256-particle tile,
+// grouped per work queue: group g owns tiles [group_start[g], group_start[g+1]).
+// launch_ensemble_advance sets every run's interval from `loop` (frame
+// first_frame + done, t0 = frame mhd_dt) and clears the queue counters;
+// launch_ensemble runs one persistent launch of `grid` workgroups.
+hipError_t launch_ensemble_advance(EnsembleRun* runs, int n_runs, EnsembleLoop* loop, unsigned int* queues,
+                                   int groups, hipStream_t stream);
+hipError_t launch_ensemble(const EnsembleRun* runs, const uint2* tiles, const uint32_t* group_start, int groups,
+                           unsigned int* queues, int grid, const void* mlp_image, int has_mlp, hipStream_t stream);
 // Exclusive scan of n uint32 tile counts in place; the total lands in [n].
 hipError_t launch_scan_tiles(uint32_t* tile_counts, uint32_t n, hipStream_t stream);
 // ParticleSystem lifecycle passes. The energy split needs esplit_scratch_elems(count)

@@ -12,6 +12,12 @@
 #include <cmath>
 #include <cstdint>
 
+// Multiply-adds are fused only inside one source expression, never across
+// statements: the push's rounding then does not depend on the kernel that
+// inlines it (a run pushed alone and inside the batched ensemble agree bit for
+// bit).
+#pragma clang fp contract(on)
+
 #include "kpt_device_common.h"
 #include "kpt_internal.h"
 #include "kpt_mlp_device.h"
@@ -499,14 +505,14 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
 #ifndef KPT_PUSH_WAVES_FP32
 #define KPT_PUSH_WAVES_FP32 2
 #endif
+// One lane of the push: particle `idx` of the set in `a` (lanes past a.count
+// or with status != 0 only join the wave reduction of the counters). Every lane
+// of a wavefront calls it together.
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
-__global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
-    push_kernel(PushArgs a) {
+__device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) {
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr bool FP32 = sizeof(FT) == 4;
     constexpr int CP = MODEL;  // components read per corner
-    const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     uint64_t my_steps = 0;
     uint32_t my_fail = 0, my_active = 0;
 
@@ -718,6 +724,14 @@ __global__ void __launch_bounds__(kBlock)
         atomicAdd(&a.counters[2], static_cast<unsigned long long>(act));
     }
 }
+
+template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
+    push_kernel(PushArgs a) {
+    push_lane<MODEL, FOCUSED, FT, CLOSURE>(a, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+}
+
 // Closure from the network outputs at pitch cosine mu (|mu| in the hull):
 // D = P (1 - mu^2) R, dD/dmu = P R ((1 - mu^2) sgn(mu) y1 / |mu| - 2 mu),
 // ln R = out_mean + out_scale y0.
@@ -750,21 +764,17 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 #ifndef KPT_PUSH_WAVES_MLP
 #define KPT_PUSH_WAVES_MLP 2
 #endif
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
-    push_kernel_mlp(PushArgs a) {
-    __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
-    mlp_load_lds(a.mlp_image, mlp_smem);
-    __syncthreads();
-    const MlpLds im = mlp_lds_view(mlp_smem);
-    const float* b4 = reinterpret_cast<const float*>(static_cast<const char*>(a.mlp_image) + kMlpLdsBytes);
-    const float b40 = b4[0], b41 = b4[1];
+// One lane of the learned-closure push (the network image already staged in
+// LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
+// together: the sub-step loop and the network evaluations are wave-uniform.
+__device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& im, const float b40, const float b41,
+                                              const int64_t idx) {
     const MlpScale ms = a.mlp_scale;
     const StepConst<float>& K = a.k32;
     const kpt_device_settings& s = a.s;
     constexpr int CP = 21;
     const bool use_net = a.mlp_on != 0;
 
-    const int64_t idx = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     const bool live = idx < static_cast<int64_t>(a.count) && a.P.status[idx] == 0;
     uint64_t my_steps = 0;
     uint32_t my_fail = 0, my_active = live ? 1u : 0u;
@@ -836,8 +846,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
                 }
             }
         }
-        float y0, y1;
-        mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
+        float y0 = 0.0f, y1 = 0.0f;
+        if (use_net) mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);  // use_net is uniform over the launch
 
         // phase 2: closure at mu0, step size, noise, transverse increments, first Picard pass
         float r3 = 0.0f, dtr = 0.0f, delta_x = 0.0f, delta_y = 0.0f;
@@ -916,7 +926,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
         // phases 3 and 4: the two Picard passes at the trial values
 #pragma unroll 1
         for (int it = 1; it < 3; ++it) {
-            mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
+            if (use_net) mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
             bool hn = false;
             f0 = f1 = f2 = 0.0f;
             if (run) {
@@ -969,6 +979,89 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
         atomicAdd(&a.counters[1], static_cast<unsigned long long>(fails));
         atomicAdd(&a.counters[2], static_cast<unsigned long long>(act));
     }
+}
+
+// The network image of one launch goes to LDS once per workgroup.
+__device__ __forceinline__ MlpLds mlp_stage(const void* image, uint4* smem, float& b40, float& b41) {
+    mlp_load_lds(image, smem);
+    __syncthreads();
+    const float* b4 = reinterpret_cast<const float*>(static_cast<const char*>(image) + kMlpLdsBytes);
+    b40 = b4[0];
+    b41 = b4[1];
+    return mlp_lds_view(smem);
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
+    push_kernel_mlp(PushArgs a) {
+    __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
+    float b40, b41;
+    const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
+    push_lane_mlp(a, im, b40, b41, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: one persistent launch advances many independent runs.
+// Workgroups take 256-particle tiles from work queues; with 8 queues, queue g
+// holds the tiles of the runs assigned to XCD g (workgroups are dispatched to
+// the XCDs round-robin, so workgroup b starts at queue b % 8 and keeps its
+// run's field in that XCD's L2), and a workgroup whose queue is empty takes
+// tiles from the next queues. Every workgroup leaves once all queues are
+// exhausted.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
+    ensemble_kernel(const EnsembleRun* __restrict__ runs, const uint2* __restrict__ tiles,
+                    const uint32_t* __restrict__ group_start, int groups, unsigned int* __restrict__ queues,
+                    const void* __restrict__ mlp_image, int has_mlp) {
+    __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
+    __shared__ uint32_t s_tile;
+    float b40 = 0.0f, b41 = 0.0f;
+    MlpLds im{};
+    if (has_mlp) im = mlp_stage(mlp_image, mlp_smem, b40, b41);
+    int g = static_cast<int>(blockIdx.x % static_cast<unsigned>(groups));
+    int visited = 0;
+    while (visited < groups) {
+        const uint32_t size = group_start[g + 1] - group_start[g];
+        if (threadIdx.x == 0) s_tile = atomicAdd(&queues[g], 1u);
+        __syncthreads();
+        const uint32_t k = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (k >= size) {  // this queue is exhausted: move to the next one
+            g = g + 1 == groups ? 0 : g + 1;
+            ++visited;
+            continue;
+        }
+        const uint2 t = tiles[group_start[g] + k];
+        const uint32_t r = __builtin_amdgcn_readfirstlane(t.x);
+        const int64_t idx = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(t.y)) + threadIdx.x;
+        const EnsembleRun& R = runs[r];
+        if (R.kind == kEnsParker32)
+            push_lane<16, false, float, KPT_CLOSURE_REDUCED>(R.a, idx);
+        else if (R.kind == kEnsFocusedQlt32)
+            push_lane<21, true, float, KPT_CLOSURE_QLT>(R.a, idx);
+        else
+            push_lane_mlp(R.a, im, b40, b41, idx);
+    }
+}
+
+// Sets every run's interval for the next launch of the frame loop and clears
+// the queue counters (a kernel rather than a memset, so the loop can be
+// captured in a graph).
+__global__ void __launch_bounds__(kBlock) ensemble_advance_kernel(EnsembleRun* __restrict__ runs, int n_runs,
+                                                                  EnsembleLoop* __restrict__ loop,
+                                                                  unsigned int* __restrict__ queues, int groups) {
+    const uint32_t frame = loop->first_frame + loop->done;
+    const double window = loop->window;
+    for (int r = threadIdx.x; r < n_runs; r += kBlock) {
+        PushArgs& a = runs[r].a;
+        a.frame = frame;
+        a.t0 = static_cast<double>(frame) * a.s.mhd_dt;
+        a.t_end = a.t0 + a.s.mhd_dt;
+        a.t_stop = window >= 1.0 ? a.t_end : a.t0 + a.s.mhd_dt * window;
+        a.first_chunk = 1;
+    }
+    for (int q = threadIdx.x; q < groups; q += kBlock) queues[q] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) loop->done = loop->done + 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1344,6 +1437,20 @@ hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, in
         else
             hipLaunchKernelGGL((push_kernel<21, false, double, R>), grid, block, 0, stream, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_ensemble_advance(EnsembleRun* runs, int n_runs, EnsembleLoop* loop, unsigned int* queues,
+                                   int groups, hipStream_t stream) {
+    hipLaunchKernelGGL(ensemble_advance_kernel, dim3(1), dim3(kBlock), 0, stream, runs, n_runs, loop, queues, groups);
+    return hipGetLastError();
+}
+
+hipError_t launch_ensemble(const EnsembleRun* runs, const uint2* tiles, const uint32_t* group_start, int groups,
+                           unsigned int* queues, int grid, const void* mlp_image, int has_mlp, hipStream_t stream) {
+    if (groups < 1 || grid < 1 || (has_mlp && mlp_image == nullptr)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ensemble_kernel, dim3(grid), dim3(kBlock), 0, stream, runs, tiles, group_start, groups, queues,
+                       mlp_image, has_mlp);
     return hipGetLastError();
 }
 

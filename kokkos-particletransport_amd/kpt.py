@@ -170,6 +170,18 @@ class GenericFocusParams(C.Structure):
         return c
 
 
+class EnsembleTuning(C.Structure):
+    """kpt_ensemble_tuning: launch shape of the ensemble's persistent push."""
+    _fields_ = [("grid_blocks", C.c_int32), ("groups", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"grid_blocks": self.grid_blocks, "groups": self.groups, "ms": self.ms}
+
+
+class EnsembleRunStats(C.Structure):
+    _fields_ = [("particle_steps", C.c_uint64), ("failures", C.c_uint64), ("active_in", C.c_uint64)]
+
+
 class ParticleProperties(C.Structure):
     """kpt_particle_properties (ParticleProperties, include/ParticleSystem.hpp:145-213)."""
     _fields_ = [("rest_mass", C.c_double), ("charge", C.c_double), ("speed_of_light", C.c_double),
@@ -327,6 +339,23 @@ EXPORTS = {
                                               C.POINTER(C.c_double)]),
     "kpt_generic_focus_step": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32, C.c_uint32,
                                          C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_ensemble_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
+    "kpt_ensemble_destroy": (C.c_int, [C.c_void_p]),
+    "kpt_ensemble_field_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+                                  + [C.POINTER(C.c_double)] * 8),
+    "kpt_ensemble_set_run": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(DeviceSettings), C.c_uint64]),
+    "kpt_ensemble_particles_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ParticlesHost), C.c_uint64]),
+    "kpt_ensemble_particles_download": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ParticlesHost), C.c_uint64]),
+    "kpt_ensemble_count": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    "kpt_ensemble_inject": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_double, C.c_uint32,
+                                      C.POINTER(C.c_uint64)]),
+    "kpt_ensemble_push": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(EnsembleRunStats),
+                                    C.POINTER(C.c_double)]),
+    "kpt_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.POINTER(EnsembleTuning),
+                                        C.POINTER(EnsembleTuning), C.c_int32, C.POINTER(C.c_int32)]),
+    "kpt_ensemble_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(EnsembleTuning)]),
+    "kpt_ensemble_histogram_logp": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                              C.POINTER(C.c_double)]),
     "kpt_default_particle_properties": (C.c_int, [C.POINTER(ParticleProperties)]),
     "kpt_default_energy_split_policy": (C.c_int, [C.POINTER(EnergySplitPolicy)]),
     "kpt_split_energy_growth": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties),
@@ -728,3 +757,85 @@ class Context:
     def snapshot_read(self, path, charge, split_ratio, restart_time, dt_min):
         self._check(self.lib.kpt_snapshot_read(self.h, path.encode(), charge, split_ratio,
                                                restart_time, dt_min))
+
+
+class Ensemble:
+    """Batched ensemble of independent runs on one context (kpt_ensemble_*).
+    Destroy it (or leave its `with` block) before the context."""
+
+    def __init__(self, ctx: "Context", capacities):
+        self.ctx, self.lib = ctx, ctx.lib
+        caps = (C.c_uint64 * len(capacities))(*[int(c) for c in capacities])
+        self.runs = len(capacities)
+        self.h = C.c_void_p()
+        ctx._check(self.lib.kpt_ensemble_create(ctx.h, self.runs, caps, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.kpt_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        self.ctx._check(st)
+
+    def field_upload(self, run, model, x_edges, y_edges, planes):
+        xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+        ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+        ps = [np.ascontiguousarray(a, dtype=np.float64) for a in planes]
+        self._check(self.lib.kpt_ensemble_field_upload(self.h, run, model, len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye),
+                                                       *[_dp(a) for a in ps]))
+
+    def set_run(self, run, s: "DeviceSettings", seed):
+        self._check(self.lib.kpt_ensemble_set_run(self.h, run, C.byref(s), seed))
+
+    def upload(self, run, parts: dict):
+        h = particles_struct(parts)
+        self._check(self.lib.kpt_ensemble_particles_upload(self.h, run, C.byref(h), len(parts["x"])))
+
+    def count(self, run) -> int:
+        n = C.c_uint64()
+        self._check(self.lib.kpt_ensemble_count(self.h, run, C.byref(n)))
+        return n.value
+
+    def download(self, run) -> dict:
+        n = self.count(run)
+        parts = empty_particles(n)
+        h = particles_struct(parts)
+        self._check(self.lib.kpt_ensemble_particles_download(self.h, run, C.byref(h), n))
+        return parts
+
+    def inject(self, run, n, t0, frame) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.kpt_ensemble_inject(self.h, run, n, t0, frame, C.byref(out)))
+        return out.value
+
+    def push(self, first_frame, frames=1, graph=False):
+        """Returns (per-run stats list of dicts, kernel ms)."""
+        st = (EnsembleRunStats * self.runs)()
+        ms = C.c_double()
+        self._check(self.lib.kpt_ensemble_push(self.h, first_frame, frames, int(graph), st, C.byref(ms)))
+        return [{"particle_steps": s.particle_steps, "failures": s.failures, "active_in": s.active_in}
+                for s in st], ms.value
+
+    def autotune(self, frame, window=0.25):
+        chosen = EnsembleTuning()
+        table = (EnsembleTuning * 16)()
+        tried = C.c_int32()
+        self._check(self.lib.kpt_ensemble_autotune(self.h, frame, window, C.byref(chosen), table, 16,
+                                                   C.byref(tried)))
+        return chosen.as_dict(), [table[k].as_dict() for k in range(tried.value)]
+
+    def set_tuning(self, grid_blocks, groups):
+        t = EnsembleTuning(grid_blocks, groups, 0.0)
+        self._check(self.lib.kpt_ensemble_set_tuning(self.h, C.byref(t)))
+
+    def histogram_logp(self, run, bins, pmin, pmax):
+        out = np.empty(bins)
+        self._check(self.lib.kpt_ensemble_histogram_logp(self.h, run, bins, pmin, pmax, _dp(out)))
+        return out

@@ -146,3 +146,70 @@ def c3_particles(n, s, seed=11, p_range=None) -> dict:
     parts["split_level"][:] = 0
     parts["id"][:] = np.arange(n, dtype=np.uint64)
     return parts
+
+
+C5 = dict(runs=64, nx=512, ny=256, count_range=(1.0e3, 1.0e6), bz_range=(0.0, 0.5), ratio_range=(0.01, 0.2),
+          inflow_range=(0.05, 0.15), steps_per_interval=50, frames=10)
+
+
+def c5_runs(seed=2024, runs=None, nx=None, ny=None, count_range=None) -> list:
+    """The batched-ensemble workload: per run a seeded island-chain field variant
+    (guide field Bz ~ U[0, 0.5], inflow ~ U[0.05, 0.15]), kperp/kpar ~ U[0.01, 0.2],
+    a particle count log-uniform on count_range; even runs Parker, odd runs
+    focused transport with the learned closure."""
+    runs = C5["runs"] if runs is None else runs
+    nx = C5["nx"] if nx is None else nx
+    ny = C5["ny"] if ny is None else ny
+    lo, hi = C5["count_range"] if count_range is None else count_range
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(runs):
+        bz = rng.uniform(*C5["bz_range"])
+        ratio = rng.uniform(*C5["ratio_range"])
+        inflow = rng.uniform(*C5["inflow_range"])
+        count = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        xe, ye, planes = island_field(nx, ny, bz=bz, inflow=inflow, seed=seed * 1000 + r)
+        out.append(dict(run=r, bz=bz, kperp_over_kpara=ratio, inflow=inflow, count=count, focused=r % 2 == 1,
+                        xe=xe, ye=ye, planes=planes, seed=seed * 1000 + r))
+    return out
+
+
+def c5_settings(spec, steps_per_interval=None, closure=None) -> "kpt.DeviceSettings":
+    """FP32 settings of one ensemble run: the C1 Parker physics or the C3 focused
+    QLT physics with the run's kperp/kpar, fixed dt = mhd_dt / steps_per_interval,
+    one frame (no time interpolation)."""
+    n = C5["steps_per_interval"] if steps_per_interval is None else steps_per_interval
+    h = c3_host_settings() if spec["focused"] else c1_host_settings()
+    h.kperp_over_kpara = spec["kperp_over_kpara"]
+    h.dt_min_rel = h.dt_max_rel = 1.0 / n
+    h.time_interpolation = 0
+    if spec["focused"]:
+        s = c3_device_settings(spec["xe"], spec["ye"], closure=kpt.CLOSURE_MLP if closure is None else closure,
+                               precision=kpt.PREC_FP32)
+        s2 = kpt.make_device_settings(h, spec["xe"], spec["ye"])
+        for k in ("kperp_over_kpara", "dt_min", "dt_max", "time_interpolation", "kpara0"):
+            if hasattr(s, k):
+                setattr(s, k, getattr(s2, k))
+        return s
+    s = kpt.make_device_settings(h, spec["xe"], spec["ye"])
+    s.precision = kpt.PREC_FP32
+    return s
+
+
+def c5_particles(spec, s, frame=0) -> dict:
+    """The run's particles at the start of interval `frame`: Parker runs at p = 1
+    in |y| < 0.1, focused runs with the C3 distribution."""
+    n = spec["count"]
+    if spec["focused"]:
+        parts = c3_particles(n, s, seed=spec["seed"])
+    else:
+        rng = np.random.default_rng(spec["seed"])
+        parts = kpt.empty_particles(n)
+        parts["x"][:] = rng.uniform(s.x_lower, s.x_upper, n)
+        parts["y"][:] = rng.uniform(-0.1, 0.1, n)
+        parts["p"][:] = 1.0
+        parts["step_dt"][:] = s.dt_min
+        parts["weight"][:] = 1.0
+        parts["id"][:] = np.arange(n, dtype=np.uint64)
+    parts["t"][:] = frame * s.mhd_dt
+    return parts
