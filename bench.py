@@ -17,6 +17,15 @@ path is the learned one) and by the plain FP64 exact-QLT push, for the ratios.
 Workload `parker`: Parker transport at production shape, 4 194 304 particles,
 the driver's default physics with adaptive dt, FP64 or FP32.
 
+Workload `ensemble`: 64 independent runs, each on its own seeded 512 x 256
+island-chain field variant (guide field Bz ~ U[0, 0.5], inflow ~ U[0.05, 0.15],
+kperp/kpar ~ U[0.01, 0.2]), particle counts log-uniform on [1e3, 1e6]; even runs
+Parker, odd runs focused transport with the learned closure; FP32 bulk, fixed
+dt = mhd_dt / 50. One bench step is a cell sort of every run followed by a
+500-sub-step frame loop (10 intervals) of every run: one persistent launch per
+interval, the loop replayed from a hipGraph, the launch shape chosen by the
+autotuner first.
+
 Prints one JSON line. `value` is particle-steps/s over the timed steps (exact
 sub-step counts from the kernel). `roofline` prices the push kernel's algorithmic
 bytes (frames x 4 corners x components x word size per sub-step, plus the
@@ -153,6 +162,117 @@ def profiled_traffic(workload):
     return None
 
 
+def run_ensemble(args):
+    """The batched-ensemble workload (see the module docstring)."""
+    specs = W.c5_runs(runs=args.runs or W.C5["runs"])
+    frames = W.C5["frames"]
+    settings = [W.c5_settings(spec) for spec in specs]
+    total = sum(spec["count"] for spec in specs)
+    log(f"bench: ensemble of {len(specs)} runs, {total} particles, {frames} x {W.C5['steps_per_interval']} "
+        f"sub-steps per step")
+    with kpt.Context(16, seed=114514) as ctx:
+        log("training the D_mumu network on the GPU ...")
+        o = kpt.mlp_default_train_opts()
+        o.steps = args.mlp_steps
+        report = ctx.mlp_train(o)
+        with kpt.Ensemble(ctx, [spec["count"] for spec in specs]) as e:
+            for r, (spec, s) in enumerate(zip(specs, settings)):
+                model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
+                e.field_upload(r, model, spec["xe"], spec["ye"], spec["planes"])
+                e.set_run(r, s, spec["seed"])
+                e.upload(r, W.c5_particles(spec, s, frame=0))
+            e.sort()
+            chosen, table = e.autotune(0, window=0.2)
+            log(f"  autotune: chosen {chosen}; tried {table}")
+            frame = 0
+            for _ in range(args.warmup):
+                e.sort()
+                st, ms = e.push(frame, frames=frames, graph=True)
+                log(f"  warmup frames {frame}..{frame + frames - 1}: {sum(x['particle_steps'] for x in st)} steps, "
+                    f"{ms:.1f} ms")
+                frame += frames
+            steps, fails, kms = 0, 0, 0.0
+            per_kind = {"parker": 0, "focused": 0}
+            bytes_alg = 0.0
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                e.sort()  # part of the step: the cell order of every run, then the frame loop
+                st, ms = e.push(frame, frames=frames, graph=True)
+                frame += frames
+                kms += ms
+                for spec, x in zip(specs, st):
+                    steps += x["particle_steps"]
+                    fails += x["failures"]
+                    model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
+                    per_kind["focused" if spec["focused"] else "parker"] += x["particle_steps"]
+                    bytes_alg += x["particle_steps"] * step_bytes(model, True, frames=1) + \
+                        x["active_in"] * state_bytes(model)
+                log(f"  frames {frame - frames}..{frame - 1}: {sum(x['particle_steps'] for x in st)} steps, "
+                    f"{ms:.1f} ms")
+            elapsed = time.perf_counter() - t0
+    value = steps / elapsed
+    launches = args.steps * frames
+    avg_launch_s = kms / 1e3 / launches
+    achieved = bytes_alg / launches / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (64 seeded island-chain field variants; C1 Parker and C3 focused particle sets)",
+        "config": {
+            "workload": f"ensemble_{len(specs)}_runs_512x256_fp32_{total}",
+            "runs": len(specs),
+            "particles": total,
+            "sub_steps_per_step": frames * W.C5["steps_per_interval"],
+            "sub_steps": int(steps),
+            "parker_sub_steps": int(per_kind["parker"]),
+            "focused_mlp_sub_steps": int(per_kind["focused"]),
+            "failures": int(fails),
+            "tuning": chosen,
+            "autotune_table": table,
+            "graph": True,
+            "mlp_training": {k: report[k] for k in ("steps", "train_ms", "val_rms_rel_d")},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(f"ensemble_{len(specs)}_runs_512x256_fp32_{total}"),
+            "kernel": "ensemble_kernel",
+            "kernel_ms_avg": avg_launch_s * 1e3,
+            "launches": int(launches),
+            "algorithmic_bytes_per_launch": bytes_alg / launches,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        parts = []
+        for focused in (False, True):
+            spec = max((sp for sp in specs if sp["focused"] == focused), key=lambda sp: sp["count"])
+            s = W.c5_settings(spec)
+            model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
+            parts.append(cpu_baseline(spec["xe"], spec["ye"], spec["planes"], spec["planes"], model, s,
+                                      W.c5_particles(spec, s, frame=0), target_seconds=6.0))
+        share = [per_kind["parker"] / max(steps, 1), per_kind["focused"] / max(steps, 1)]
+        rate = 1.0 / sum(w / b["value"] for w, b in zip(share, parts))  # the measured mix of sub-steps
+        out["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": parts[0]["cores"],
+                               "kind": "port",
+                               "sample": f"the largest Parker run and the largest focused run (exact QLT), combined "
+                                         f"with the measured shares {share[0]:.3f} / {share[1]:.3f} of sub-steps: "
+                                         + " | ".join(b["sample"] for b in parts)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true",
@@ -160,7 +280,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["focused", "parker"], default="focused")
+    ap.add_argument("--workload", choices=["focused", "parker", "ensemble"], default="focused")
+    ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's size")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
@@ -185,6 +306,9 @@ def main():
               "diagnostic interval; not built).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
     focused = args.workload == "focused"
     prec = args.precision or ("fp32" if focused else "fp64")
     fp32 = prec == "fp32"

@@ -490,6 +490,11 @@ kpt_status kpt_ensemble_autotune(kpt_ensemble* e, uint32_t frame, double window,
 kpt_status kpt_ensemble_set_tuning(kpt_ensemble* e, const kpt_ensemble_tuning* t);
 kpt_status kpt_ensemble_histogram_logp(kpt_ensemble* e, int32_t run, int32_t bins, double pmin, double pmax,
                                        double* out);
+/* Reorder every run's particles by the lower-left node of their stencil (stable;
+ * inactive particles last), so the gathers of one tile share cache lines.
+ * Results per particle id do not change. `active` (may be NULL) receives the
+ * number of status-0 particles over all runs. */
+kpt_status kpt_ensemble_sort(kpt_ensemble* e, uint64_t* active);
 
 /* ---- generic focused-transport solver (global time step) -------------------------
  * FocusTransportSolver (include/FocusSolver.hpp:74-1890) on the packed basis of

@@ -1956,6 +1956,9 @@ struct kpt_ensemble {
     kpt::ParticleArrays P{};
     kpt::ParticleArrays B{};  // autotune backup, allocated on first use
     bool backup_allocated = false;
+    kpt::ParticleArrays P2{};  // cell sort target, allocated on first use
+    bool p2_allocated = false;
+    kpt::SortScratch sort{};
     kpt::EnsembleRun* d_runs = nullptr;
     unsigned long long* d_counters = nullptr;  // 4 per run
     uint2* d_tiles = nullptr;
@@ -2216,6 +2219,12 @@ kpt_status kpt_ensemble_destroy(kpt_ensemble* e) {
     drop_graph(e);
     free_arrays(e->P);
     if (e->backup_allocated) free_arrays(e->B);
+    if (e->p2_allocated) free_arrays(e->P2);
+    for (int k = 0; k < 2; ++k) {
+        dev_free(e->sort.keys[k]);
+        dev_free(e->sort.vals[k]);
+    }
+    dev_free(e->sort.temp);
     for (FieldSlot& f : e->field) {
         dev_free(f.raw);
         dev_free(f.c64);
@@ -2488,3 +2497,40 @@ kpt_status kpt_ensemble_histogram_logp(kpt_ensemble* e, int32_t run, int32_t bin
 }
 
 }  // extern "C"
+
+extern "C" kpt_status kpt_ensemble_sort(kpt_ensemble* e, uint64_t* active) {
+    if (!e) return KPT_ERR_ARG;
+    kpt_ctx* ctx = e->ctx;
+    (void)hipSetDevice(ctx->device);
+    for (int r = 0; r < e->runs; ++r) {
+        if (!e->field[r].valid) return fail(ctx, KPT_ERR_STATE, "ensemble run " + std::to_string(r) + ": field is not loaded.");
+        if (!e->set[r]) return fail(ctx, KPT_ERR_STATE, "ensemble run " + std::to_string(r) + ": settings are not set.");
+    }
+    if (!e->p2_allocated) {
+        const uint64_t maxcap = *std::max_element(e->cap.begin(), e->cap.end());
+        KPT_HIP(ctx, alloc_arrays(e->P2, e->total_cap));
+        for (int k = 0; k < 2; ++k) {
+            KPT_HIP(ctx, dev_alloc(&e->sort.keys[k], std::max<uint64_t>(maxcap, 1)));
+            KPT_HIP(ctx, dev_alloc(&e->sort.vals[k], std::max<uint64_t>(maxcap, 1)));
+        }
+        e->sort.temp_bytes = kpt::sort_temp_bytes(std::max<uint64_t>(maxcap, 1));
+        KPT_HIP(ctx, hipMalloc(&e->sort.temp, std::max<size_t>(e->sort.temp_bytes, 1)));
+        e->p2_allocated = true;
+    }
+    unsigned long long* act_dev = ctx->counters + 4;
+    KPT_HIP(ctx, hipMemsetAsync(act_dev, 0, sizeof(unsigned long long), ctx->stream));
+    for (int r = 0; r < e->runs; ++r) {
+        if (e->count[r] == 0) continue;
+        const kpt_device_settings& s = e->s[r];
+        KPT_HIP(ctx, kpt::launch_sort(arrays_at(e->P, e->off[r]), arrays_at(e->P2, e->off[r]), e->count[r],
+                                      e->field[r].g, s.interpolation_shift_x, s.interpolation_shift_y,
+                                      std::numeric_limits<double>::infinity(), e->sort, act_dev, ctx->stream));
+    }
+    unsigned long long act = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&act, act_dev, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::swap(e->P, e->P2);
+    e->dirty = true;  // the run table points into the other particle set now
+    if (active) *active = act;
+    return KPT_OK;
+}

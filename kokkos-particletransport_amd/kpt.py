@@ -356,6 +356,7 @@ EXPORTS = {
     "kpt_ensemble_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(EnsembleTuning)]),
     "kpt_ensemble_histogram_logp": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                               C.POINTER(C.c_double)]),
+    "kpt_ensemble_sort": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kpt_default_particle_properties": (C.c_int, [C.POINTER(ParticleProperties)]),
     "kpt_default_energy_split_policy": (C.c_int, [C.POINTER(EnergySplitPolicy)]),
     "kpt_split_energy_growth": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties),
@@ -839,3 +840,9 @@ class Ensemble:
         out = np.empty(bins)
         self._check(self.lib.kpt_ensemble_histogram_logp(self.h, run, bins, pmin, pmax, _dp(out)))
         return out
+
+    def sort(self) -> int:
+        """Cell-sort every run's particles; returns the active count over all runs."""
+        n = C.c_uint64()
+        self._check(self.lib.kpt_ensemble_sort(self.h, C.byref(n)))
+        return n.value

@@ -157,3 +157,24 @@ def test_ensemble_histogram_and_rejected_runs():
             e.set_run(2, red, 1)
             with pytest.raises(kpt.KptError):
                 e.push(FRAME)
+
+
+def test_sorted_ensemble_equals_standalone_by_id():
+    """The per-run cell sort only reorders: after sort + push every particle id
+    carries the same state as in the unsorted standalone push."""
+    specs = _specs()
+    with kpt.Context(16) as c:
+        c.mlp_set(_params(), _hull())
+        with _build(c, specs) as e:
+            act = e.sort()
+            assert act == sum(s["count"] for s in specs)
+            e.push(FRAME, frames=1)
+            got = [e.download(r) for r in range(len(specs))]
+    for r, spec in enumerate(specs):
+        want, _ = _alone(spec)
+        if spec["count"] == 0:
+            continue
+        og, ow = np.argsort(got[r]["id"]), np.argsort(want["id"])
+        assert not np.array_equal(got[r]["id"], want["id"]) or spec["count"] < 300
+        for k in FIELDS:
+            np.testing.assert_array_equal(got[r][k][og], want[k][ow], err_msg=f"run {r} ({spec['kind']}) {k}")
