@@ -366,8 +366,15 @@ void kpt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * version-4 KPTPRT file of the stored particles. */
 kpt_status kpt_snapshot_write(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
                               double p0);
+/* No counterpart in the reference: the same v4 file with each record's id field
+ * holding the particle id (which keys the particle's random stream) and its E0
+ * field holding the stored split baseline. kpt_snapshot_read of such a file
+ * followed by the same frames reproduces an uninterrupted run bit for bit. */
+kpt_status kpt_snapshot_write_resumable(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
+                                        double p0);
 /* load_reconnection_particle_snapshot (src/main.cpp:2009-2126): replace the
- * particle set; time = restart_time, step_dt = dt_min. */
+ * particle set; time = restart_time, step_dt = dt_min. The records' ids become
+ * the particle ids (slot indices in a file of the driver). */
 kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
                              double restart_time, double dt_min);
 

@@ -61,6 +61,7 @@ struct RunSettings {
     double qlt_spectral_index{5.0 / 3.0};
     uint32_t mlp_train_steps{8000};
     int device{0};
+    bool resumable_snapshots{false};
 
     int interval_count() const { return end_frame - start_frame; }
     bool walltime_limit_enabled() const { return walltime_limit_seconds > 0.0; }
@@ -219,6 +220,8 @@ void print_help() {
         << "  --qlt-sigma2 X --qlt-correlation-length X --qlt-spectral-index X\n"
         << "                                slab-QLT turbulence (default 0.1, 0.02, 5/3)\n"
         << "  --mlp-train-steps N           Adam steps of the learned closure (default 8000)\n"
+        << "  --resumable-snapshots         snapshots carry particle ids and split baselines,\n"
+        << "                                so a restart continues the run bit for bit\n"
         << "  --device N                    GPU ordinal (default 0)\n";
 }
 
@@ -308,6 +311,8 @@ RunSettings parse(int argc, char** argv) {
             s.split_particles = false;
         } else if (a == "--no-time-interp") {
             s.time_interpolation = false;
+        } else if (a == "--resumable-snapshots") {
+            s.resumable_snapshots = true;
         } else if (a == "--precision") {
             const std::string p = val();
             if (p == "fp64") s.precision = KPT_PREC_FP64;
@@ -447,7 +452,10 @@ int run(const RunSettings& rs) {
     log.f.open(rs.output_dir + "/run.log", rs.restart_enabled() ? (std::ios::out | std::ios::app) : std::ios::out);
     if (!log.f) throw std::runtime_error("Failed to open run log output.");
 
-    const uint64_t seed = rs.restart_enabled()
+    // A restart reseeds like the reference (src/main.cpp:2644-2650), except with
+    // resumable snapshots: their particles keep their ids, and the streams keyed
+    // by (seed, id, frame, sub-step) then continue the interrupted run.
+    const uint64_t seed = rs.restart_enabled() && !rs.resumable_snapshots
                               ? rs.seed + 0x9e3779b97f4a7c15ull * static_cast<uint64_t>(rs.start_frame + 1)
                               : rs.seed;
     Ctx ctx;
@@ -473,8 +481,14 @@ int run(const RunSettings& rs) {
         s.qlt_spectral_index = rs.qlt_spectral_index;
     }
     if (rs.restart_enabled()) {
+        // The particles of the closed interval start_frame - 1 carry its end time
+        // t0 + mhd_dt exactly (the last sub-step is truncated to it); the restart
+        // time is that value, which differs from start_frame * mhd_dt by at most
+        // one rounding.
+        const double restart_time =
+            rs.start_frame > 0 ? static_cast<double>(rs.start_frame - 1) * rs.host.mhd_dt + rs.host.mhd_dt : 0.0;
         ctx.check(kpt_snapshot_read(ctx.c, rs.restart_particle_snapshot_path.c_str(), rs.host.charge,
-                                    rs.host.split_ratio, rs.start_frame * rs.host.mhd_dt, s.dt_min),
+                                    rs.host.split_ratio, restart_time, s.dt_min),
                   "restart snapshot");
         uint64_t n = 0;
         kpt_particles_count(ctx.c, &n);
@@ -583,7 +597,10 @@ int run(const RunSettings& rs) {
             ((frame + 1 - rs.start_frame) % rs.particle_snapshot_interval == 0 || finishing)) {
             const auto ts = Clock::now();
             const std::string path = frame_path(rs.output_dir, "particles_", frame + 1, ".bin");
-            ctx.check(kpt_snapshot_write(ctx.c, path.c_str(), rs.host.charge, rs.host.split_ratio, rs.host.p0),
+            ctx.check(rs.resumable_snapshots
+                          ? kpt_snapshot_write_resumable(ctx.c, path.c_str(), rs.host.charge, rs.host.split_ratio,
+                                                         rs.host.p0)
+                          : kpt_snapshot_write(ctx.c, path.c_str(), rs.host.charge, rs.host.split_ratio, rs.host.p0),
                       "snapshot");
             std::ostringstream l;
             l << "wrote particle snapshot " << path << " in " << since(ts) << " s";

@@ -1042,9 +1042,16 @@ void kpt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // ---- v4 snapshots --------------------------------------------------------------
 
-kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h, uint64_t count,
-                                   uint64_t capacity, double charge, double split_ratio, double p0) {
+}  // extern "C"
+
+namespace {
+
+// v4 writer. `resumable` stores the particle ids and split baselines in the
+// records' id and E0 fields instead of the driver's slot index and constant E0.
+kpt_status write_snapshot(const char* path, const kpt_particles_host* h, uint64_t count, uint64_t capacity,
+                          double charge, double split_ratio, double p0, bool resumable) {
     if (!path || !h) return KPT_ERR_ARG;
+    if (resumable && count > 0 && (!h->id || !h->e0)) return KPT_ERR_ARG;
     if (count > 0 && (!h->x || !h->y || !h->p || !h->mu || !h->weight || !h->status || !h->split_level))
         return KPT_ERR_ARG;
     std::ofstream out(path, std::ios::binary);
@@ -1076,7 +1083,7 @@ kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h
     const double e0 = std::sqrt(1.0 + p0 * p0) - 1.0;
     const int32_t sort_key = 0;
     for (uint64_t i = 0; i < count; ++i) {
-        put(&i, 8);
+        put(resumable ? static_cast<const void*>(&h->id[i]) : static_cast<const void*>(&i), 8);
         put(&h->status[i], 4);
         put(&h->split_level[i], 4);
         put(&sort_key, 4);
@@ -1087,10 +1094,19 @@ kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h
         put(&h->p[i], 8);
         put(&h->mu[i], 8);
         put(&h->weight[i], 8);
-        put(&e0, 8);
+        put(resumable ? &h->e0[i] : &e0, 8);
     }
     out.write(buf.data(), static_cast<std::streamsize>(buf.size()));
     return out ? KPT_OK : KPT_ERR_IO;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_snapshot_write_host(const char* path, const kpt_particles_host* h, uint64_t count,
+                                   uint64_t capacity, double charge, double split_ratio, double p0) {
+    return write_snapshot(path, h, count, capacity, charge, split_ratio, p0, false);
 }
 
 kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint64_t* count_io, double charge,
@@ -1158,6 +1174,22 @@ kpt_status kpt_snapshot_read_host(const char* path, kpt_particles_host* h, uint6
     return KPT_OK;
 }
 
+kpt_status kpt_snapshot_write_resumable(kpt_ctx* ctx, const char* path, double charge, double split_ratio,
+                                        double p0) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    const uint64_t n = ctx->count;
+    std::vector<double> x(n), y(n), p(n), mu(n), w(n), e0(n);
+    std::vector<int32_t> status(n), level(n);
+    std::vector<uint64_t> id(n);
+    kpt_particles_host h{x.data(), y.data(), p.data(), nullptr, nullptr, w.data(), mu.data(),
+                         status.data(), level.data(), id.data(), e0.data()};
+    kpt_status st = kpt_particles_download(ctx, &h, n);
+    if (st != KPT_OK) return st;
+    st = write_snapshot(path, &h, n, ctx->capacity, charge, split_ratio, p0, true);
+    if (st != KPT_OK) return fail(ctx, st, "Failed to write particle snapshot.");
+    return KPT_OK;
+}
+
 kpt_status kpt_snapshot_write(kpt_ctx* ctx, const char* path, double charge, double split_ratio, double p0) {
     if (!ctx || !path) return KPT_ERR_ARG;
     const uint64_t n = ctx->count;
@@ -1187,8 +1219,9 @@ kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, doub
     std::vector<double> x(count), y(count), p(count), t(count, restart_time), dt(count, dt_min), w(count), mu(count),
         e0(count);
     std::vector<int32_t> status(count), level(count);
+    std::vector<uint64_t> id(count);
     kpt_particles_host h{x.data(), y.data(), p.data(), t.data(), dt.data(), w.data(), mu.data(),
-                         status.data(), level.data(), nullptr, e0.data()};
+                         status.data(), level.data(), id.data(), e0.data()};
     st = kpt_snapshot_read_host(path, &h, &count, charge, split_ratio, err, sizeof(err));
     if (st != KPT_OK) return fail(ctx, st, err);
     return kpt_particles_upload(ctx, &h, count);

@@ -310,6 +310,7 @@ EXPORTS = {
     "kpt_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32)]),
     "kpt_snapshot_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double]),
+    "kpt_snapshot_write_resumable": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double]),
     "kpt_snapshot_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double,
                                     C.c_double]),
     "kpt_write_histogram_csv": (C.c_int, [C.c_char_p, C.c_int32, C.c_double, C.c_double,
@@ -629,8 +630,9 @@ class Context:
         self._check(self.lib.kpt_histogram_xy(self.h, bx, by, x_lo, x_hi, y_lo, y_hi, _dp(out)))
         return out.reshape(by, bx)
 
-    def snapshot_write(self, path, charge, split_ratio, p0):
-        self._check(self.lib.kpt_snapshot_write(self.h, path.encode(), charge, split_ratio, p0))
+    def snapshot_write(self, path, charge, split_ratio, p0, resumable=False):
+        fn = self.lib.kpt_snapshot_write_resumable if resumable else self.lib.kpt_snapshot_write
+        self._check(fn(self.h, path.encode(), charge, split_ratio, p0))
 
     def snapshot_read(self, path, charge, split_ratio, restart_time, dt_min):
         self._check(self.lib.kpt_snapshot_read(self.h, path.encode(), charge, split_ratio, restart_time, dt_min))
