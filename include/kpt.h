@@ -265,6 +265,15 @@ kpt_status kpt_set_seed(kpt_ctx* ctx, uint64_t seed);
  * read one compact-field binary (include/AthenaFieldReader.hpp:274-360) and build
  * the packed coefficient field of `model` in `slot`. */
 kpt_status kpt_field_load_compact(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model);
+/* The same reader with stretched (non-uniform, strictly ascending) faces accepted
+ * when allow_stretched != 0; the reference's reader rejects them. On a stretched
+ * axis value k sits at face k (the driver's node convention, whose value k sits at
+ * x0 + k dx on a uniform axis), the packed derivatives use the three-point
+ * formula at those coordinates (include/FieldOperator.hpp:85-105), and the dt
+ * limiter's half cell width is that of the particle's face interval.
+ * kpt_field_upload accepts stretched faces as well. */
+kpt_status kpt_field_load_compact_ex(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model,
+                                     int32_t allow_stretched);
 
 /* Same as kpt_field_load_compact, from host arrays in the compact-field order
  * (Bx, By, Bz, Vx, Vy, Vz each [ny][nx], x fastest). */

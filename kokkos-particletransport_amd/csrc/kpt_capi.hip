@@ -40,6 +40,9 @@ struct FieldSlot {
     double* foc = nullptr;
     size_t foc_cap = 0;
     bool foc_valid = false;
+    // stretched axes: face coordinates and bin table on the device
+    double* face_dev[2] = {nullptr, nullptr};
+    int32_t* lut_dev[2] = {nullptr, nullptr};
 };
 
 }  // namespace
@@ -112,6 +115,21 @@ void dev_free(void* p) {
     if (p) (void)hipFree(p);
 }
 
+// The spacing test of validate_uniform_face_edges (include/AthenaFieldReader.hpp:
+// 167-193): all intervals equal the mean interval to 1024 ulp of the scale.
+bool faces_uniform(const double* edges, size_t count) {
+    const double span = edges[count - 1] - edges[0];
+    const double step = span / static_cast<double>(count - 1);
+    for (size_t i = 1; i < count; ++i) {
+        const double cur = edges[i] - edges[i - 1];
+        const double mag = std::max({std::fabs(span), std::fabs(step), std::fabs(cur), 1.0});
+        if (std::fabs(cur - step) > 1024.0 * std::numeric_limits<double>::epsilon() * mag) return false;
+    }
+    return true;
+}
+
+// A stretched axis (non-uniform faces) spans exactly [e_0, e_n]; a uniform one
+// keeps the analytic bounds start + k step.
 kpt::GridDesc make_grid_desc(int32_t nx, int32_t ny, const double* xe, const double* ye) {
     kpt::GridDesc g{};
     g.nx = nx;
@@ -119,15 +137,39 @@ kpt::GridDesc make_grid_desc(int32_t nx, int32_t ny, const double* xe, const dou
     const double* e[2] = {xe, ye};
     const int32_t n[2] = {nx, ny};
     for (int d = 0; d < 2; ++d) {
+        const bool stretched = n[d] > 1 && !faces_uniform(e[d], static_cast<size_t>(n[d]) + 1);
         g.start[d] = e[d][0];
         g.step[d] = (e[d][n[d]] - e[d][0]) / static_cast<double>(n[d]);
         g.inv_step[d] = 1.0 / g.step[d];
-        g.phys_lo[d] = g.start[d] + 0.0 * g.step[d];
-        g.phys_hi[d] = g.start[d] + static_cast<double>(n[d]) * g.step[d];
+        g.phys_lo[d] = stretched ? e[d][0] : g.start[d] + 0.0 * g.step[d];
+        g.phys_hi[d] = stretched ? e[d][n[d]] : g.start[d] + static_cast<double>(n[d]) * g.step[d];
         g.width[d] = g.phys_hi[d] - g.phys_lo[d];
         g.inv_width[d] = g.width[d] != 0.0 ? 1.0 / g.width[d] : 0.0;
+        g.face[d] = nullptr;
+        g.lut[d] = nullptr;
+        g.lut_n[d] = 0;
+        g.lut_steps[d] = 0;
+        g.lut_scale[d] = 0.0;
     }
     return g;
+}
+
+// Bin table of a stretched axis: 4 n bins of equal width over [e_0, e_n), bin
+// b holding the largest k with e_k <= the bin's lower end, and the most forward
+// moves a query in one bin can need.
+void stretched_table(const double* e, int n, std::vector<int32_t>& lut, int32_t& steps, double& scale) {
+    const int m = 4 * n;
+    const double w = e[n] - e[0];
+    lut.assign(static_cast<size_t>(m) + 1, 0);
+    for (int b = 0; b <= m; ++b) {
+        const double lo = e[0] + w * static_cast<double>(b) / m;
+        const int k = static_cast<int>(std::upper_bound(e, e + n + 1, lo) - e) - 1;
+        lut[b] = std::min(std::max(k, 0), n - 1);
+    }
+    steps = 1;
+    for (int b = 0; b < m; ++b) steps = std::max(steps, lut[b + 1] - lut[b] + 2);
+    lut.resize(static_cast<size_t>(m));
+    scale = static_cast<double>(m) / w;
 }
 
 // validate_face_edges / validate_uniform_face_edges
@@ -293,8 +335,8 @@ kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, in
         return fail(ctx, KPT_ERR_ARG, "invalid coefficient model (16 or 21).");
     if (nx <= 0 || ny <= 0) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: Athena binary dimensions must be positive.");
     std::string err;
-    if (!check_edges(xe, static_cast<size_t>(nx) + 1, "x", true, err) ||
-        !check_edges(ye, static_cast<size_t>(ny) + 1, "y", true, err))
+    if (!check_edges(xe, static_cast<size_t>(nx) + 1, "x", false, err) ||
+        !check_edges(ye, static_cast<size_t>(ny) + 1, "y", false, err))
         return fail(ctx, KPT_ERR_ARG, err);
     f.valid = false;
     kpt_status st = ensure_field_buffers(ctx, f, nx, ny, model);
@@ -305,6 +347,29 @@ kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, in
                                     ctx->stream));
     }
     f.g = make_grid_desc(nx, ny, xe, ye);
+    const double* faces[2] = {xe, ye};
+    const int32_t n[2] = {nx, ny};
+    for (int d = 0; d < 2; ++d) {
+        dev_free(f.face_dev[d]);
+        dev_free(f.lut_dev[d]);
+        f.face_dev[d] = nullptr;
+        f.lut_dev[d] = nullptr;
+        if (n[d] <= 1 || faces_uniform(faces[d], static_cast<size_t>(n[d]) + 1)) continue;
+        std::vector<int32_t> lut;
+        int32_t steps = 0;
+        double scale = 0.0;
+        stretched_table(faces[d], n[d], lut, steps, scale);
+        KPT_HIP(ctx, dev_alloc(&f.face_dev[d], static_cast<size_t>(n[d]) + 1));
+        KPT_HIP(ctx, dev_alloc(&f.lut_dev[d], lut.size()));
+        KPT_HIP(ctx, hipMemcpy(f.face_dev[d], faces[d], (static_cast<size_t>(n[d]) + 1) * sizeof(double),
+                               hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(f.lut_dev[d], lut.data(), lut.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        f.g.face[d] = f.face_dev[d];
+        f.g.lut[d] = f.lut_dev[d];
+        f.g.lut_n[d] = static_cast<int32_t>(lut.size());
+        f.g.lut_steps[d] = steps;
+        f.g.lut_scale[d] = scale;
+    }
     f.model = model;
     KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -435,6 +500,10 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.c32);
         dev_free(f.gen);
         dev_free(f.foc);
+        for (int d = 0; d < 2; ++d) {
+            dev_free(f.face_dev[d]);
+            dev_free(f.lut_dev[d]);
+        }
     }
     dev_free(ctx->pair32);
     dev_free(ctx->mlp_params);
@@ -515,8 +584,10 @@ kpt_status kpt_make_device_settings(const kpt_host_settings* h, int32_t nx, int3
     o->y_width = o->y_upper - o->y_lower;
     o->dx = o->x_width / static_cast<double>(nx);
     o->dy = o->y_width / static_cast<double>(ny);
-    o->interpolation_shift_x = 0.5 * o->dx;
-    o->interpolation_shift_y = 0.5 * o->dy;
+    // the driver's node convention: a value sits half a cell before the
+    // cell-centred sample point; on a stretched axis values sit at the faces
+    o->interpolation_shift_x = nx > 1 && !faces_uniform(xe, static_cast<size_t>(nx) + 1) ? 0.0 : 0.5 * o->dx;
+    o->interpolation_shift_y = ny > 1 && !faces_uniform(ye, static_cast<size_t>(ny) + 1) ? 0.0 : 0.5 * o->dy;
     o->mhd_dt = h->mhd_dt;
     o->p0 = h->p0;
     o->pmax = h->pmax;
@@ -571,11 +642,16 @@ kpt_status kpt_field_upload(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t n
 }
 
 kpt_status kpt_field_load_compact(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model) {
+    return kpt_field_load_compact_ex(ctx, path, slot, model, 0);
+}
+
+kpt_status kpt_field_load_compact_ex(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model,
+                                     int32_t allow_stretched) {
     if (!ctx || !path) return KPT_ERR_ARG;
     int32_t nx = 0, ny = 0;
     std::vector<double> xe, ye, comps;
     std::string err;
-    if (!read_compact(path, nx, ny, xe, ye, comps, false, err)) return fail(ctx, KPT_ERR_IO, err);
+    if (!read_compact(path, nx, ny, xe, ye, comps, allow_stretched != 0, err)) return fail(ctx, KPT_ERR_IO, err);
     const size_t cells = static_cast<size_t>(nx) * ny;
     const double* planes[6];
     for (int c = 0; c < 6; ++c) planes[c] = comps.data() + c * cells;
@@ -1597,6 +1673,8 @@ kpt_status kpt_generic_parker_build(kpt_ctx* ctx, const kpt_generic_parker_param
     if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
     FieldSlot& f = ctx->slot[slot];
     if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    if (f.g.face[0] || f.g.face[1])
+        return fail(ctx, KPT_ERR_ARG, "the generic solvers run on a uniform grid; this frame has stretched faces.");
     (void)hipSetDevice(ctx->device);
     const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_PARKER_COMPONENTS;
     if (elems > f.gen_cap) {
@@ -1880,6 +1958,8 @@ kpt_status kpt_generic_focus_build(kpt_ctx* ctx, const kpt_generic_focus_params*
     if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
     FieldSlot& f = ctx->slot[slot];
     if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
+    if (f.g.face[0] || f.g.face[1])
+        return fail(ctx, KPT_ERR_ARG, "the generic solvers run on a uniform grid; this frame has stretched faces.");
     (void)hipSetDevice(ctx->device);
     const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_FOCUS_COMPONENTS;
     if (elems > f.foc_cap) {
@@ -2283,6 +2363,11 @@ kpt_status kpt_ensemble_field_upload(kpt_ensemble* e, int32_t run, int32_t model
     for (const double* plane : {bx, by, bz, vx, vy, vz})
         for (size_t k = 0; k < cells; ++k)
             if (!std::isfinite(plane[k])) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: non-finite field value.");
+    for (const double* ed : {xe, ye}) {
+        const int n = ed == xe ? nx : ny;
+        if (n > 1 && !faces_uniform(ed, static_cast<size_t>(n) + 1))
+            return fail(ctx, KPT_ERR_ARG, "ensemble runs use uniform grids.");
+    }
     const double* planes[6] = {bx, by, bz, vx, vy, vz};
     (void)hipSetDevice(ctx->device);
     e->dirty = true;

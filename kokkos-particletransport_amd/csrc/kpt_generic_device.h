@@ -10,6 +10,7 @@
 #include <cstdint>
 
 #include "kpt_device_common.h"
+#include "kpt_grid_device.h"
 #include "kpt_internal.h"
 
 namespace kpt {
@@ -24,11 +25,6 @@ __device__ __forceinline__ double cell_coord(const GridDesc& g, int d, int k) {
     return 0.5 * ((g.start[d] + k * g.step[d]) + (g.start[d] + (k + 1.0) * g.step[d]));
 }
 
-// centered_three_point_derivative (include/ParkerCoefficient.hpp:215-233).
-__device__ __forceinline__ double three_point(double xl, double xc, double xu, double fl, double fc, double fu) {
-    const double lw = xc - xl, uw = xu - xc, tw = xu - xl;
-    return (-uw / (lw * tw)) * fl + ((uw - lw) / (lw * uw)) * fc + (lw / (uw * tw)) * fu;
-}
 
 struct RawView {
     const double* raw;  // 6 planes of nx * ny (Bx, By, Bz, Vx, Vy, Vz), x fastest

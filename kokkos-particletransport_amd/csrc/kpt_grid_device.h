@@ -39,6 +39,51 @@ __device__ __forceinline__ bool axis_stencil(double q, const GridDesc& g, int d,
     return true;
 }
 
+// centered_three_point_derivative (include/ParkerCoefficient.hpp:215-233).
+__device__ __forceinline__ double three_point(double xl, double xc, double xu, double fl, double fc, double fu) {
+    const double lw = xc - xl, uw = xu - xc, tw = xu - xl;
+    return (-uw / (lw * tw)) * fl + ((uw - lw) / (lw * uw)) * fc + (lw / (uw * tw)) * fu;
+}
+
+// Stencil of the driver's node convention. Uniform axis: value k sits at
+// start + k step, reached through the +step/2 interpolation shift and the
+// cell-centred inverse map (axis_stencil). Stretched axis (no counterpart in the
+// driver, whose reader rejects non-uniform faces): value k sits at face k and q
+// (the shift is 0 there) falls in the face interval [face k, face k+1), found
+// through the bin table and at most lut_steps forward moves. `h` receives the
+// width of the interval (the uniform step on a uniform axis).
+__device__ __forceinline__ bool node_stencil(double q, const GridDesc& g, int d, int n, int& i0, double& w,
+                                             double& h) {
+    if (g.face[d] == nullptr) {
+        h = g.step[d];
+        return axis_stencil(q, g, d, n, i0, w);
+    }
+    const double qw = grid_wrap(q, g.phys_lo[d], g.phys_hi[d], g.width[d], g.inv_width[d]);
+    if (!(qw == qw)) return false;
+    const double* e = g.face[d];
+    const double bf = (qw - g.start[d]) * g.lut_scale[d];
+    const int b = bf < 0.0 ? 0 : (bf >= static_cast<double>(g.lut_n[d] - 1) ? g.lut_n[d] - 1 : static_cast<int>(bf));
+    int k = g.lut[d][b];
+    if (k > 0 && qw < e[k]) --k;
+    for (int s = 0; s < g.lut_steps[d]; ++s)
+        if (k + 1 < n && qw >= e[k + 1]) ++k;
+    const double el = e[k], eu = e[k + 1];
+    h = eu - el;
+    double ww = (qw - el) / h;
+    ww = ww < 0.0 ? 0.0 : (ww > 1.0 ? 1.0 : ww);
+    i0 = k;
+    w = ww;
+    return true;
+}
+
+// Coordinate of stored node k (k in [-2, n + 1]) along a stretched axis: the
+// face of its periodic image shifted by whole periods.
+__device__ __forceinline__ double stretched_node(const GridDesc& g, int d, int n, int k) {
+    const int m = ((k % n) + n) % n;
+    const int period = (k - m) / n;
+    return g.face[d][m] + static_cast<double>(period) * g.width[d];
+}
+
 // The cell-centred stencil of the generic solvers (no interpolation shift).
 __device__ __forceinline__ bool axis_stencil_cc(double q, const GridDesc& g, int d, int n, int& i0, double& w) {
     return axis_stencil(q, g, d, n, i0, w);

@@ -21,15 +21,24 @@ inline uint64_t coefficient_plane(int nx, int ny) {
     return static_cast<uint64_t>(nx + 2) * static_cast<uint64_t>(ny + 2);
 }
 
-// Uniform cell-centred grid of a loaded frame (AnalyticCoordinateGrid with
-// UniformSpacing, include/AnalyticCoordinateGrid.hpp:159-210, and the bounds
-// cache of include/MappedCoordinateGrid.hpp:117-145).
+// Grid of a loaded frame: uniform cell-centred spacing (AnalyticCoordinateGrid
+// with UniformSpacing, include/AnalyticCoordinateGrid.hpp:159-210, and the
+// bounds cache of include/MappedCoordinateGrid.hpp:117-145). An axis with
+// non-uniform faces is "stretched": face[d] points to its n + 1 face coordinates
+// on the device and lut[d] to a table of lut_n[d] equal bins over the axis, bin
+// b holding the face interval that contains the bin's lower end; lut_steps[d]
+// bounds the intervals a bin can span. Null pointers mark a uniform axis.
 struct GridDesc {
     int32_t nx, ny;
     double start[2];
     double step[2];
     double inv_step[2];
     double phys_lo[2], phys_hi[2], width[2], inv_width[2];
+    const double* face[2];
+    const int32_t* lut[2];
+    int32_t lut_n[2];
+    int32_t lut_steps[2];
+    double lut_scale[2];
 };
 
 struct ParticleArrays {

@@ -80,19 +80,30 @@ __global__ void __launch_bounds__(kBlock) pack_kernel(const double* __restrict__
     };
     const double hx = g.nx > 1 ? cell(0, i + 1) - cell(0, i - 1) : 1.0;
     const double hy = g.ny > 1 ? cell(1, j + 1) - cell(1, j - 1) : 1.0;
+    // Derivative along an axis from the values at offsets -1, 0, +1: the
+    // driver's central difference on a uniform axis, the reference's three-point
+    // formula at the node coordinates on a stretched one.
+    auto along = [&](int d, double fl, double fc, double fu) -> double {
+        const int n = d == 0 ? g.nx : g.ny;
+        if (n <= 1) return 0.0;
+        if (g.face[d] == nullptr) return (fu - fl) / (d == 0 ? hx : hy);
+        const int c = d == 0 ? i : j;
+        return three_point(stretched_node(g, d, n, c - 1), stretched_node(g, d, n, c), stretched_node(g, d, n, c + 1),
+                           fl, fc, fu);
+    };
     // comp: 0..2 B, 3..5 V
     auto ddx = [&](int comp) -> double {
-        return g.nx > 1 ? (val(comp, i + 1, j) - val(comp, i - 1, j)) / hx : 0.0;
+        return along(0, val(comp, i - 1, j), val(comp, i, j), val(comp, i + 1, j));
     };
     auto ddy = [&](int comp) -> double {
-        return g.ny > 1 ? (val(comp, i, j + 1) - val(comp, i, j - 1)) / hy : 0.0;
+        return along(1, val(comp, i, j - 1), val(comp, i, j), val(comp, i, j + 1));
     };
     const double bx = val(0, i, j), by = val(1, i, j), bz = val(2, i, j);
     double c[24];
 #pragma unroll
     for (int k = 0; k < 24; ++k) c[k] = 0.0;
-    const double dbm_dx = g.nx > 1 ? (bmag(i + 1, j) - bmag(i - 1, j)) / hx : 0.0;
-    const double dbm_dy = g.ny > 1 ? (bmag(i, j + 1) - bmag(i, j - 1)) / hy : 0.0;
+    const double dbm_dx = along(0, bmag(i - 1, j), bmag(i, j), bmag(i + 1, j));
+    const double dbm_dy = along(1, bmag(i, j - 1), bmag(i, j), bmag(i, j + 1));
     if (model == KPT_MODEL_PARKER16) {
         c[0] = val(3, i, j);
         c[1] = val(4, i, j);
@@ -400,11 +411,11 @@ __device__ __forceinline__ const StepConst<float>& step_const<float>(const PushA
 // [dt_min, dt_max] clamp. The momentum candidate is formed in double.
 template <bool FOCUSED, typename RT>
 __device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p, const kpt_device_settings& s,
-                                               const StepConst<RT>& k) {
+                                               const StepConst<RT>& k, RT half_dx, RT half_dy) {
     if (FOCUSED) {
         const RT ds = T.skperp > RT(0) ? T.skperp : T.skpara;
         if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0 && T.dmu_dt != RT(0) && ds > RT(0)) {
-            RT d = rmin(sqr(rdiv(k.half_dx, ds)), sqr(rdiv(k.half_dy, ds)));
+            RT d = rmin(sqr(rdiv(half_dx, ds)), sqr(rdiv(half_dy, ds)));
             d = rmin(d, sqr(rdiv(ds, T.dx_dt)));
             d = rmin(d, sqr(rdiv(ds, T.dy_dt)));
             d = rmin(d, rdiv(RT(0.1), rabs(T.dmu_dt)));
@@ -415,7 +426,7 @@ __device__ __forceinline__ double candidate_dt(const StepTerms<RT>& T, double p,
     }
     if (T.dx_dt != RT(0) && T.dy_dt != RT(0) && T.dp_dt != 0.0) {
         const RT as = T.skperp > RT(0) ? T.skperp : T.skpara;
-        RT d = rmin(sqr(rdiv(k.half_dx, T.skpara)), sqr(rdiv(k.half_dy, T.skpara)));
+        RT d = rmin(sqr(rdiv(half_dx, T.skpara)), sqr(rdiv(half_dy, T.skpara)));
         d = rmin(d, sqr(rdiv(as, T.dx_dt)));
         d = rmin(d, sqr(rdiv(as, T.dy_dt)));
         return fmin(static_cast<double>(d), 0.1 * p / fabs(T.dp_dt));
@@ -508,7 +519,9 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
 // One lane of the push: particle `idx` of the set in `a` (lanes past a.count
 // or with status != 0 only join the wave reduction of the counters). Every lane
 // of a wavefront calls it together.
-template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
+// STR: the frame has a stretched axis (node_stencil); the uniform kernels keep
+// the plain stencil.
+template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR = false>
 __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) {
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr bool FP32 = sizeof(FT) == 4;
@@ -547,8 +560,20 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
 
             int i0, j0;
             double wx, wy;
-            if (!axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) ||
-                !axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy)) {
+            FT half_dx = step_const<FT>(a).half_dx, half_dy = step_const<FT>(a).half_dy;
+            if constexpr (STR) {
+                // half cell widths of the dt limiter: the local face interval on a
+                // stretched axis
+                double hx, hy;
+                if (!node_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx, hx) ||
+                    !node_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy, hy)) {
+                    status = 2;
+                    break;
+                }
+                if (a.g.face[0]) half_dx = static_cast<FT>(0.5 * hx);
+                if (a.g.face[1]) half_dy = static_cast<FT>(0.5 * hy);
+            } else if (!axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) ||
+                       !axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy)) {
                 status = 2;
                 break;
             }
@@ -604,7 +629,7 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
             }
 
             if (s.dt_min < s.dt_max) {
-                dt = candidate_dt<FOCUSED>(T, p, s, step_const<FT>(a));
+                dt = candidate_dt<FOCUSED>(T, p, s, step_const<FT>(a), half_dx, half_dy);
                 if (dt < s.dt_min) dt = s.dt_min;
                 if (dt > s.dt_max) dt = s.dt_max;
             } else {
@@ -725,11 +750,11 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
     }
 }
 
-template <int MODEL, bool FOCUSED, typename FT, int CLOSURE>
+template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
     push_kernel(PushArgs a) {
-    push_lane<MODEL, FOCUSED, FT, CLOSURE>(a, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
 }
 
 // Closure from the network outputs at pitch cosine mu (|mu| in the hull):
@@ -767,6 +792,7 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // One lane of the learned-closure push (the network image already staged in
 // LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
 // together: the sub-step loop and the network evaluations are wave-uniform.
+template <bool STR = false>
 __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& im, const float b40, const float b41,
                                               const int64_t idx) {
     const MlpScale ms = a.mlp_scale;
@@ -804,17 +830,28 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
         StepTerms<float> T{};
         float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lxi = 0.0f;
         bool h0 = false;
+        float half_dx = K.half_dx, half_dy = K.half_dy;
         if (run) {
             ++step_guard;
             x = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
             y = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
             int i0, j0;
-            double wx, wy;
-            if (!axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) ||
-                !axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy)) {
+            double wx, wy, hx = 0.0, hy = 0.0;
+            bool ok_stencil;
+            if constexpr (STR)
+                ok_stencil = node_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx, hx) &&
+                             node_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy, hy);
+            else
+                ok_stencil = axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) &&
+                             axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy);
+            if (!ok_stencil) {
                 status = 2;
                 run = false;
             } else {
+                if constexpr (STR) {
+                    if (a.g.face[0]) half_dx = static_cast<float>(0.5 * hx);
+                    if (a.g.face[1]) half_dy = static_cast<float>(0.5 * hy);
+                }
                 const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
                 double uw = 0.0;
                 if (blend) {
@@ -866,7 +903,7 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
             }
             if (run) {
                 if (s.dt_min < s.dt_max) {
-                    dt = candidate_dt<true>(T, p, s, K);
+                    dt = candidate_dt<true>(T, p, s, K, half_dx, half_dy);
                     if (dt < s.dt_min) dt = s.dt_min;
                     if (dt > s.dt_max) dt = s.dt_max;
                 } else {
@@ -991,12 +1028,13 @@ __device__ __forceinline__ MlpLds mlp_stage(const void* image, uint4* smem, floa
     return mlp_lds_view(smem);
 }
 
+template <bool STR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
     push_kernel_mlp(PushArgs a) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     float b40, b41;
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
-    push_lane_mlp(a, im, b40, b41, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+    push_lane_mlp<STR>(a, im, b40, b41, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1408,36 +1446,43 @@ hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out,
     return hipGetLastError();
 }
 
-hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream) {
-    if (a.count == 0) return hipSuccess;
+template <bool STR>
+hipError_t launch_push_t(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream) {
     const dim3 grid(grid_for(a.count)), block(kBlock);
     constexpr int R = KPT_CLOSURE_REDUCED, Q = KPT_CLOSURE_QLT;
     if (model == KPT_MODEL_PARKER16) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<16, false, float, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<16, false, float, R, STR>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<16, false, double, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<16, false, double, R, STR>), grid, block, 0, stream, a);
     } else if (focused && closure == KPT_CLOSURE_MLP) {
         if (!fp32 || a.mlp_image == nullptr) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(push_kernel_mlp, grid, block, 0, stream, a);
+        hipLaunchKernelGGL(push_kernel_mlp<STR>, grid, block, 0, stream, a);
     } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, true, float, Q>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, float, Q, STR>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<21, true, double, Q>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, double, Q, STR>), grid, block, 0, stream, a);
     } else if (focused) {
         if (closure != R) return hipErrorInvalidValue;
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, true, float, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, float, R, STR>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<21, true, double, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, double, R, STR>), grid, block, 0, stream, a);
     } else {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, false, float, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, float, R, STR>), grid, block, 0, stream, a);
         else
-            hipLaunchKernelGGL((push_kernel<21, false, double, R>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, double, R, STR>), grid, block, 0, stream, a);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_push(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream) {
+    if (a.count == 0) return hipSuccess;
+    if (a.g.face[0] != nullptr || a.g.face[1] != nullptr)
+        return launch_push_t<true>(a, model, focused, fp32, closure, stream);
+    return launch_push_t<false>(a, model, focused, fp32, closure, stream);
 }
 
 hipError_t launch_ensemble_advance(EnsembleRun* runs, int n_runs, EnsembleLoop* loop, unsigned int* queues,

@@ -15,6 +15,7 @@
 #include <cstdint>
 
 #include "kpt_internal.h"
+#include "kpt_grid_device.h"
 
 namespace kpt {
 namespace {
@@ -39,13 +40,10 @@ __global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint
             const int n[2] = {g.nx, g.ny};
             int idx[2];
             for (int d = 0; d < 2; ++d) {
-                double w = q[d] - floor((q[d] - g.phys_lo[d]) * g.inv_width[d]) * g.width[d];
-                if (w >= g.phys_hi[d]) w -= g.width[d];
-                if (w < g.phys_lo[d]) w += g.width[d];
-                double l = floor((w - g.start[d]) * g.inv_step[d] - 0.5);
-                l = l == l ? l : -1.0;  // NaN positions sort first; the push marks them
-                l = l < -1.0 ? -1.0 : (l > n[d] - 1 ? n[d] - 1 : l);
-                idx[d] = static_cast<int>(l);
+                int i0;
+                double w, h;
+                // NaN positions sort first; the push marks them
+                idx[d] = node_stencil(q[d], g, d, n[d], i0, w, h) ? i0 : -1;
             }
             key = static_cast<uint32_t>((idx[1] + 1) * (g.nx + 2) + (idx[0] + 1));
         }

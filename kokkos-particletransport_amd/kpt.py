@@ -279,6 +279,7 @@ EXPORTS = {
                                            C.POINTER(DeviceSettings)]),
     "kpt_set_seed": (C.c_int, [C.c_void_p, C.c_uint64]),
     "kpt_field_load_compact": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
+    "kpt_field_load_compact_ex": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
     "kpt_field_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
                          + [C.POINTER(C.c_double)] * 8),
     "kpt_field_advance": (C.c_int, [C.c_void_p]),
@@ -580,8 +581,8 @@ class Context:
         self._check(self.lib.kpt_field_upload(self.h, slot, model, len(xe) - 1, len(ye) - 1,
                                               _dp(xe), _dp(ye), *[_dp(a) for a in ps]))
 
-    def field_load_compact(self, path, slot, model):
-        self._check(self.lib.kpt_field_load_compact(self.h, path.encode(), slot, model))
+    def field_load_compact(self, path, slot, model, allow_stretched=False):
+        self._check(self.lib.kpt_field_load_compact_ex(self.h, path.encode(), slot, model, int(allow_stretched)))
 
     def field_advance(self):
         self._check(self.lib.kpt_field_advance(self.h))

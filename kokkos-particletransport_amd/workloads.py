@@ -34,16 +34,30 @@ def harris_field(nx=256, ny=128, x_range=(-1.0, 1.0), y_range=(-0.5, 0.5), width
     return xe, ye, [Bx, By, Bz, Vx, Vy, Vz]
 
 
+def stretched_edges(n, lo=-0.5, hi=0.5, ratio=10.0):
+    """n + 1 faces refined toward the middle of [lo, hi]: y = mid + half atanh(a s) /
+    atanh(a) with s uniform on [-1, 1] and a^2 = 1 - 1/ratio, so the spacing grows
+    smoothly from the middle to the ends by `ratio` (the grid coordinate s is a
+    tanh of y)."""
+    a = np.sqrt(1.0 - 1.0 / ratio)
+    s = np.linspace(-1.0, 1.0, n + 1)
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    e = mid + half * np.arctanh(a * s) / np.arctanh(a)
+    e[0], e[-1] = lo, hi
+    return e
+
+
 def island_field(nx=1024, ny=512, x_range=(-1.0, 1.0), y_range=(-0.5, 0.5), width=0.02,
-                 by_amp=0.05, bz=0.1, inflow=0.1, noise=0.01, seed=7, modes=4):
+                 by_amp=0.05, bz=0.1, inflow=0.1, noise=0.01, seed=7, modes=4, y_edges=None):
     """Harris sheet with an island chain By = a sum_k sin(k pi x + phi_k)/k exp(-(y/w)^2)
-    and 1% Gaussian velocity noise (seeded)."""
+    and 1% Gaussian velocity noise (seeded). Values sit at the nodes x0 + i dx (and,
+    with stretched `y_edges`, at the y faces)."""
     rng = np.random.default_rng(seed)
     phases = rng.uniform(0.0, 2.0 * np.pi, size=modes)
     xe = np.linspace(x_range[0], x_range[1], nx + 1)
-    ye = np.linspace(y_range[0], y_range[1], ny + 1)
+    ye = np.linspace(y_range[0], y_range[1], ny + 1) if y_edges is None else np.asarray(y_edges, dtype=np.float64)
     x = xe[0] + np.arange(nx) * (xe[-1] - xe[0]) / nx
-    y = ye[0] + np.arange(ny) * (ye[-1] - ye[0]) / ny
+    y = ye[0] + np.arange(ny) * (ye[-1] - ye[0]) / ny if y_edges is None else ye[:-1]
     X, Y = np.meshgrid(x, y)
     Bx = np.tanh(Y / width)
     By = np.zeros_like(X)

@@ -24,6 +24,7 @@
 //                      mu clamp; otherwise shares the Parker branch structure.
 //   split / histogram / summary / inject   invariants (weight conservation,
 //                      totals) and the reference's tracked histogram CSV edges.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -87,6 +88,15 @@ struct OGrid {
     double start[2];  // x0 = first face coordinate
     double step[2];   // uniform face spacing (AthenaFieldReader.hpp:167-193)
     double phys_lo[2], phys_hi[2], width[2], inv_width[2];
+    // stretched axis (non-uniform faces; the driver's reader rejects these):
+    // value k sits at face k, extended periodically
+    bool stretched[2] = {false, false};
+    std::vector<double> faces[2];
+
+    double node(int d, int k) const {
+        const int m = ((k % n[d]) + n[d]) % n[d];
+        return faces[d][m] + static_cast<double>((k - m) / n[d]) * width[d];
+    }
 
     // AnalyticCoordinateGrid face / cell coordinate (:159-176).
     double face(int d, double i) const { return start[d] + i * step[d]; }
@@ -117,6 +127,18 @@ struct OField {
     double& at(int i, int j, int c) { return data[flat(i, j) * comps + c]; }
 };
 
+// validate_uniform_face_edges' spacing test (AthenaFieldReader.hpp:167-193).
+bool faces_uniform(const double* e, int count) {
+    const double span = e[count - 1] - e[0];
+    const double step = span / static_cast<double>(count - 1);
+    for (int i = 1; i < count; ++i) {
+        const double cur = e[i] - e[i - 1];
+        const double mag = std::max({std::fabs(span), std::fabs(step), std::fabs(cur), 1.0});
+        if (std::fabs(cur - step) > 1024.0 * std::numeric_limits<double>::epsilon() * mag) return false;
+    }
+    return true;
+}
+
 void make_grid(OGrid& g, int nx, int ny, const double* xe, const double* ye) {
     g.n[0] = nx;
     g.n[1] = ny;
@@ -124,12 +146,32 @@ void make_grid(OGrid& g, int nx, int ny, const double* xe, const double* ye) {
     for (int d = 0; d < 2; ++d) {
         g.start[d] = e[d][0];
         g.step[d] = (e[d][g.n[d]] - e[d][0]) / static_cast<double>(g.n[d]);
-        // refresh_bounds_cache_on_host (MappedCoordinateGrid.hpp:117-145)
-        g.phys_lo[d] = g.face(d, 0.0);
-        g.phys_hi[d] = g.face(d, static_cast<double>(g.n[d]));
+        g.stretched[d] = g.n[d] > 1 && !faces_uniform(e[d], g.n[d] + 1);
+        if (g.stretched[d]) g.faces[d].assign(e[d], e[d] + g.n[d] + 1);
+        // refresh_bounds_cache_on_host (MappedCoordinateGrid.hpp:117-145); a
+        // stretched axis spans exactly [e_0, e_n]
+        g.phys_lo[d] = g.stretched[d] ? e[d][0] : g.face(d, 0.0);
+        g.phys_hi[d] = g.stretched[d] ? e[d][g.n[d]] : g.face(d, static_cast<double>(g.n[d]));
         g.width[d] = g.phys_hi[d] - g.phys_lo[d];
         g.inv_width[d] = g.width[d] != 0.0 ? 1.0 / g.width[d] : 0.0;
     }
+}
+
+// Face interval k (e_k <= q < e_{k+1}) of a wrapped coordinate on a stretched
+// axis, by bisection like StoredCoordinateGrid::grid_index
+// (include/StoredCoordinateGrid.hpp:128-173), with its weight and width.
+void stretched_interval(const OGrid& g, int d, double q, int& k, double& w, double& h) {
+    const std::vector<double>& e = g.faces[d];
+    int lo = 0, hi = g.n[d] - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (e[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    k = lo;
+    h = e[k + 1] - e[k];
+    w = (q - e[k]) / h;
+    w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
 }
 
 // Field::fill_ghost_cells, periodic in both axes (Field.hpp:274-316).
@@ -147,6 +189,19 @@ void fill_ghosts_periodic(OField& f) {
     }
 }
 
+double centered_three_point(double xl, double xc, double xu, double fl, double fc, double fu);
+
+// Derivative at node k of a stretched axis: the three-point formula at the node
+// coordinates (FieldOperator.hpp:85-105); at the outermost ghost layer the
+// one-sided second-order formula over the two inner neighbours.
+template <typename F>
+double stretched_derivative(const OGrid& g, int dim, int k, F val) {
+    const int lower = g.lower_ghosted(dim), upper = g.upper_ghosted(dim);
+    if (k <= lower) return (-3.0 * val(lower) + 4.0 * val(lower + 1) - val(lower + 2)) / (g.node(dim, lower + 2) - g.node(dim, lower));
+    if (k >= upper) return (3.0 * val(upper) - 4.0 * val(upper - 1) + val(upper - 2)) / (g.node(dim, upper) - g.node(dim, upper - 2));
+    return centered_three_point(g.node(dim, k - 1), g.node(dim, k), g.node(dim, k + 1), val(k - 1), val(k), val(k + 1));
+}
+
 // derivative_component (src/main.cpp:714-770): central difference on the ghosted
 // domain, one-sided second order at the outermost ghost layer.
 double derivative(const OField& f, int i, int j, int dim, int comp) {
@@ -159,6 +214,7 @@ double derivative(const OField& f, int i, int j, int dim, int comp) {
         a[dim] = k;
         return f.at(a[0], a[1], comp);
     };
+    if (f.g.stretched[dim]) return stretched_derivative(f.g, dim, idx[dim], val);
     if (idx[dim] <= lower) {
         const double x0 = f.g.cell(dim, lower);
         const double x2 = f.g.cell(dim, lower + 2);
@@ -190,6 +246,7 @@ double derivative_bmag(const OField& b, int i, int j, int dim) {
         a[dim] = k;
         return bmag_at(b, a[0], a[1]);
     };
+    if (b.g.stretched[dim]) return stretched_derivative(b.g, dim, idx[dim], val);
     if (idx[dim] <= lower) {
         const double x0 = b.g.cell(dim, lower);
         const double x2 = b.g.cell(dim, lower + 2);
@@ -270,6 +327,12 @@ void sample(const OField& f, const double q_in[2], double* result) {
     int lo[2], hi[2];
     double w[2];
     for (int d = 0; d < 2; ++d) {
+        if (f.g.stretched[d]) {
+            double h;
+            stretched_interval(f.g, d, q[d], lo[d], w[d], h);
+            hi[d] = lo[d] + 1;
+            continue;
+        }
         const int lb = f.g.lower_ghosted(d);
         const int ub = f.g.upper_ghosted(d);
         const double logical = f.g.cell_grid_index(d, q[d]);
@@ -443,6 +506,16 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
         const double q[2] = {x + S.interpolation_shift_x, y + S.interpolation_shift_y};
         const double rt = (t - interval_start) / S.mhd_dt;
         sample_time(lower, upper, q, rt, S.time_interpolation, c);
+        // half cell widths of the dt limiter: the settings' uniform ones, the
+        // particle's face interval on a stretched axis
+        double half_dx = 0.5 * S.dx, half_dy = 0.5 * S.dy;
+        for (int d = 0; d < 2; ++d) {
+            if (!lower.g.stretched[d]) continue;
+            int k;
+            double w, h;
+            stretched_interval(lower.g, d, lower.g.wrap(d, q[d]), k, w, h);
+            (d == 0 ? half_dx : half_dy) = 0.5 * h;
+        }
 
         double vx, vy, bx, by, bz, b, dvx_dx, dvx_dy = 0, dvy_dx = 0, dvy_dy, dvz_dx = 0,
                                           dvz_dy = 0, dbx_dx, dbx_dy, dby_dx, dby_dy, dbz_dx,
@@ -593,7 +666,7 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
         if (focused_transport) {
             const double ds = skperp > 0.0 ? skperp : skpara;
             if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0 && dmu_dt != 0.0 && ds > 0.0) {
-                dt = min_positive(sq(0.5 * S.dx / ds), sq(0.5 * S.dy / ds));
+                dt = min_positive(sq(half_dx / ds), sq(half_dy / ds));
                 dt = min_positive(dt, sq(ds / dx_dt));
                 dt = min_positive(dt, sq(ds / dy_dt));
                 dt = min_positive(dt, 0.1 * p / std::fabs(dp_dt));
@@ -604,7 +677,7 @@ uint64_t move_one(const OField& lower, const OField& upper, const kpt_device_set
             }
         } else if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
             const double as = skperp > 0.0 ? skperp : skpara;
-            dt = min_positive(sq(0.5 * S.dx / skpara), sq(0.5 * S.dy / skpara));
+            dt = min_positive(sq(half_dx / skpara), sq(half_dy / skpara));
             dt = min_positive(dt, sq(as / dx_dt));
             dt = min_positive(dt, sq(as / dy_dt));
             dt = min_positive(dt, 0.1 * p / std::fabs(dp_dt));
