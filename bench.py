@@ -5,7 +5,9 @@ One bench step = one 0.1 MHD frame interval of the whole particle set, with the
 inputs resident in HBM.
 
 Workload `focused` (default; the headline): focused transport on the
-1024 x 512 island-chain Harris field (two time-blended frames), 8 388 608
+1024 x 512 island-chain Harris field (two time-blended frames; y faces
+tanh-stretched toward y = 0 with spacing ratio 10, or uniform with
+--grid uniform), 8 388 608
 particles with mu ~ U[-1, 1] and p log-uniform on [1, 1e3], slab-QLT
 pitch-angle scattering (gamma_k = 5/3, sigma2 = 0.1, Lc = 0.02), fixed
 dt = 5e-5 (2000 sub-steps per interval). The measured path is the FP32 bulk
@@ -14,8 +16,9 @@ the GPU before the timed region) or with the exact formula (--closure qlt). The
 same particles are then pushed by the FP32 exact-QLT push (when the measured
 path is the learned one) and by the plain FP64 exact-QLT push, for the ratios.
 
-Workload `parker`: Parker transport at production shape, 4 194 304 particles,
-the driver's default physics with adaptive dt, FP64 or FP32.
+Workload `parker`: Parker transport at production shape (the same field and
+grid), 4 194 304 particles, the driver's default physics with adaptive dt, FP64
+or FP32.
 
 Workload `ensemble`: 64 independent runs, each on its own seeded 512 x 256
 island-chain field variant (guide field Bz ~ U[0, 0.5], inflow ~ U[0.05, 0.15],
@@ -56,9 +59,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def step_bytes(model, fp32, frames=2):
-    """Algorithmic bytes of one sub-step: the coefficient stencil the mover reads."""
-    return frames * 4 * model * (4 if fp32 else 8)
+def step_bytes(model, fp32, frames=2, stretched=False):
+    """Algorithmic bytes of one sub-step: the coefficient stencil the mover reads,
+    plus, on a stretched axis, the bin-table entry and the two faces of the
+    particle's interval."""
+    return frames * 4 * model * (4 if fp32 else 8) + (4 + 2 * 8 if stretched else 0)
 
 
 def state_bytes(model):
@@ -283,6 +288,10 @@ def main():
     ap.add_argument("--workload", choices=["focused", "parker", "ensemble"], default="focused")
     ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's size")
+    ap.add_argument("--grid", choices=["stretched", "uniform"], default="stretched",
+                    help="focused / parker: y faces of the field")
+    ap.add_argument("--stretch-ratio", type=float, default=10.0,
+                    help="stretched grid: largest over smallest y spacing")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None,
@@ -313,8 +322,10 @@ def main():
     prec = args.precision or ("fp32" if focused else "fp64")
     fp32 = prec == "fp32"
     model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
-    xe, ye, f0 = W.island_field(args.nx, args.ny, seed=7)
-    _, _, f1 = W.island_field(args.nx, args.ny, seed=8)
+    stretched = args.grid == "stretched"
+    y_edges = W.stretched_edges(args.ny, ratio=args.stretch_ratio) if stretched else None
+    xe, ye, f0 = W.island_field(args.nx, args.ny, seed=7, y_edges=y_edges)
+    _, _, f1 = W.island_field(args.nx, args.ny, seed=8, y_edges=y_edges)
     if focused:
         n = args.particles or W.C4["particles"]
         closure = kpt.CLOSURE_MLP if args.closure == "mlp" else kpt.CLOSURE_QLT
@@ -329,7 +340,7 @@ def main():
         s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
         start = None
     s.push_chunks = args.chunks
-    log(f"bench: {args.workload} {args.nx}x{args.ny}, {n} particles, {prec}"
+    log(f"bench: {args.workload} {args.nx}x{args.ny} {args.grid}, {n} particles, {prec}"
         + (f", closure {args.closure}" if focused else ""))
 
     fp64_leg = fp32_leg = mlp_report = None
@@ -374,15 +385,17 @@ def main():
     value = steps / elapsed
     launches = sum(a[0] for a in active)
     avg_launch_s = float(np.sum(kernel_ms)) / 1e3 / launches
-    per_step = step_bytes(model, fp32)
+    per_step = step_bytes(model, fp32, stretched=stretched)
     launch_bytes = steps / launches * per_step + float(np.mean([a[1] for a in active])) * state_bytes(model)
     achieved = launch_bytes / avg_launch_s / 1e9
-    workload = (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}_island_{n}" if focused else
-                f"parker_{prec}_{args.nx}x{args.ny}_island_{n}")
+    grid_tag = "_stretched" if stretched else ""
+    workload = (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}" if focused else
+                f"parker_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}")
     traffic = profiled_traffic(workload)
     cfg = {
         "workload": workload,
-        "field": f"{args.nx}x{args.ny} uniform grid, 2 time-blended frames",
+        "field": (f"{args.nx}x{args.ny}, " + (f"y faces tanh-stretched toward y = 0 (spacing ratio {args.stretch_ratio:g})"
+                                              if stretched else "uniform grid") + ", 2 time-blended frames"),
         "particles": n,
         "interval": s.mhd_dt,
         "sub_steps": int(steps),

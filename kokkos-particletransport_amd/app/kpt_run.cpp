@@ -62,6 +62,7 @@ struct RunSettings {
     uint32_t mlp_train_steps{8000};
     int device{0};
     bool resumable_snapshots{false};
+    bool allow_stretched_fields{false};
 
     int interval_count() const { return end_frame - start_frame; }
     bool walltime_limit_enabled() const { return walltime_limit_seconds > 0.0; }
@@ -222,6 +223,8 @@ void print_help() {
         << "  --mlp-train-steps N           Adam steps of the learned closure (default 8000)\n"
         << "  --resumable-snapshots         snapshots carry particle ids and split baselines,\n"
         << "                                so a restart continues the run bit for bit\n"
+        << "  --allow-stretched-fields      accept non-uniform field faces (value k at face k,\n"
+        << "                                three-point derivatives, local cell widths in dt)\n"
         << "  --device N                    GPU ordinal (default 0)\n";
 }
 
@@ -313,6 +316,8 @@ RunSettings parse(int argc, char** argv) {
             s.time_interpolation = false;
         } else if (a == "--resumable-snapshots") {
             s.resumable_snapshots = true;
+        } else if (a == "--allow-stretched-fields") {
+            s.allow_stretched_fields = true;
         } else if (a == "--precision") {
             const std::string p = val();
             if (p == "fp64") s.precision = KPT_PREC_FP64;
@@ -400,18 +405,19 @@ struct HostField {
     std::vector<double> xe, ye, planes[6];
 };
 
-HostField read_field(const std::string& path) {
+HostField read_field(const std::string& path, bool allow_stretched) {
     HostField f;
     char err[512] = {0};
+    const int32_t nonuniform = allow_stretched ? 1 : 0;
     if (kpt_compact_field_read(path.c_str(), &f.nx, &f.ny, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, nullptr, 0, err, sizeof(err)) != KPT_OK)
+                               nullptr, nullptr, nonuniform, err, sizeof(err)) != KPT_OK)
         throw std::runtime_error(err);
     f.xe.resize(f.nx + 1);
     f.ye.resize(f.ny + 1);
     for (auto& p : f.planes) p.resize(static_cast<size_t>(f.nx) * f.ny);
     if (kpt_compact_field_read(path.c_str(), &f.nx, &f.ny, f.xe.data(), f.ye.data(), f.planes[0].data(),
                                f.planes[1].data(), f.planes[2].data(), f.planes[3].data(), f.planes[4].data(),
-                               f.planes[5].data(), 0, err, sizeof(err)) != KPT_OK)
+                               f.planes[5].data(), nonuniform, err, sizeof(err)) != KPT_OK)
         throw std::runtime_error(err);
     return f;
 }
@@ -463,8 +469,9 @@ int run(const RunSettings& rs) {
     const int model = rs.transport == Transport::Parker ? KPT_MODEL_PARKER16 : KPT_MODEL_FOCUSED21;
 
     auto read_start = Clock::now();
-    HostField lower = read_field(frame_path(rs.field_dir, "field", rs.start_frame, ".bin"));
-    HostField upper = read_field(frame_path(rs.field_dir, "field", rs.start_frame + 1, ".bin"));
+    const bool stretched = rs.allow_stretched_fields;
+    HostField lower = read_field(frame_path(rs.field_dir, "field", rs.start_frame, ".bin"), stretched);
+    HostField upper = read_field(frame_path(rs.field_dir, "field", rs.start_frame + 1, ".bin"), stretched);
     upload(ctx, KPT_SLOT_LOWER, model, lower);
     upload(ctx, KPT_SLOT_UPPER, model, upper);
     const double initial_read_seconds = since(read_start);
@@ -549,7 +556,8 @@ int run(const RunSettings& rs) {
         if (frame > rs.start_frame) {
             const auto t = Clock::now();
             ctx.check(kpt_field_advance(ctx.c), "field advance");
-            upload(ctx, KPT_SLOT_UPPER, model, read_field(frame_path(rs.field_dir, "field", frame + 1, ".bin")));
+            upload(ctx, KPT_SLOT_UPPER, model,
+                   read_field(frame_path(rs.field_dir, "field", frame + 1, ".bin"), stretched));
             timing.read_seconds = since(t);
         }
         const double t0 = static_cast<double>(frame) * rs.host.mhd_dt;

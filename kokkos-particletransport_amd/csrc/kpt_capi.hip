@@ -42,7 +42,7 @@ struct FieldSlot {
     bool foc_valid = false;
     // stretched axes: face coordinates and bin table on the device
     double* face_dev[2] = {nullptr, nullptr};
-    int32_t* lut_dev[2] = {nullptr, nullptr};
+    kpt::FaceBin* bins_dev[2] = {nullptr, nullptr};
 };
 
 }  // namespace
@@ -146,30 +146,45 @@ kpt::GridDesc make_grid_desc(int32_t nx, int32_t ny, const double* xe, const dou
         g.width[d] = g.phys_hi[d] - g.phys_lo[d];
         g.inv_width[d] = g.width[d] != 0.0 ? 1.0 / g.width[d] : 0.0;
         g.face[d] = nullptr;
-        g.lut[d] = nullptr;
-        g.lut_n[d] = 0;
-        g.lut_steps[d] = 0;
-        g.lut_scale[d] = 0.0;
+        g.bins[d] = nullptr;
+        g.bins_n[d] = 0;
+        g.bins_scale[d] = 0.0;
     }
     return g;
 }
 
-// Bin table of a stretched axis: 4 n bins of equal width over [e_0, e_n), bin
-// b holding the largest k with e_k <= the bin's lower end, and the most forward
-// moves a query in one bin can need.
-void stretched_table(const double* e, int n, std::vector<int32_t>& lut, int32_t& steps, double& scale) {
-    const int m = 4 * n;
+// Bin table of a stretched axis: m equal bins over [e_0, e_n). Bin b records the
+// interval k that contains its lower end less a guard band g and the faces e_k,
+// e_{k+1}, e_{k+2}. m starts at the axis length over the narrowest interval and
+// doubles until every bin, widened by g on both sides, lies in [e_k, e_{k+2})
+// (in [e_k, e_n) for the last two intervals, as queries stay below e_n):
+// g (1e-9 of the axis) exceeds by far the rounding of the device's bin index, so
+// a query always finds its interval among k and k + 1. Fails past 2^22 bins.
+bool stretched_table(const double* e, int n, std::vector<kpt::FaceBin>& bins, double& scale) {
     const double w = e[n] - e[0];
-    lut.assign(static_cast<size_t>(m) + 1, 0);
-    for (int b = 0; b <= m; ++b) {
-        const double lo = e[0] + w * static_cast<double>(b) / m;
-        const int k = static_cast<int>(std::upper_bound(e, e + n + 1, lo) - e) - 1;
-        lut[b] = std::min(std::max(k, 0), n - 1);
+    const double g = 1e-9 * w;
+    double hmin = w;
+    for (int k = 0; k < n; ++k) hmin = std::min(hmin, e[k + 1] - e[k]);
+    const auto face = [&](int k) { return e[std::min(k, n)]; };
+    for (double m0 = std::max(static_cast<double>(n), std::ceil(w / hmin)); m0 <= static_cast<double>(1 << 22);
+         m0 *= 2.0) {
+        const int m = static_cast<int>(m0);
+        bins.assign(static_cast<size_t>(m), kpt::FaceBin{});
+        bool ok = true;
+        for (int b = 0; b < m && ok; ++b) {
+            const double lo = e[0] + w * static_cast<double>(b) / m - g;
+            const double hi = e[0] + w * static_cast<double>(b + 1) / m + g;
+            int k = static_cast<int>(std::upper_bound(e, e + n + 1, lo) - e) - 1;
+            k = std::min(std::max(k, 0), n - 1);
+            bins[b] = kpt::FaceBin{face(k), face(k + 1), face(k + 2), k, 0};
+            ok = (k == 0 || e[k] <= lo) && (k + 2 >= n || hi < e[k + 2]);  // queries lie in [e_0, e_n)
+        }
+        if (ok) {
+            scale = static_cast<double>(m) / w;
+            return true;
+        }
     }
-    steps = 1;
-    for (int b = 0; b < m; ++b) steps = std::max(steps, lut[b + 1] - lut[b] + 2);
-    lut.resize(static_cast<size_t>(m));
-    scale = static_cast<double>(m) / w;
+    return false;
 }
 
 // validate_face_edges / validate_uniform_face_edges
@@ -351,24 +366,26 @@ kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, in
     const int32_t n[2] = {nx, ny};
     for (int d = 0; d < 2; ++d) {
         dev_free(f.face_dev[d]);
-        dev_free(f.lut_dev[d]);
+        dev_free(f.bins_dev[d]);
         f.face_dev[d] = nullptr;
-        f.lut_dev[d] = nullptr;
+        f.bins_dev[d] = nullptr;
         if (n[d] <= 1 || faces_uniform(faces[d], static_cast<size_t>(n[d]) + 1)) continue;
-        std::vector<int32_t> lut;
-        int32_t steps = 0;
+        std::vector<kpt::FaceBin> bins;
         double scale = 0.0;
-        stretched_table(faces[d], n[d], lut, steps, scale);
+        if (!stretched_table(faces[d], n[d], bins, scale))
+            return fail(ctx, KPT_ERR_ARG, std::string(d == 0 ? "x" : "y") +
+                                              " faces are too strongly graded: the narrowest interval is below "
+                                              "2^-22 of the axis.");
         KPT_HIP(ctx, dev_alloc(&f.face_dev[d], static_cast<size_t>(n[d]) + 1));
-        KPT_HIP(ctx, dev_alloc(&f.lut_dev[d], lut.size()));
+        KPT_HIP(ctx, dev_alloc(&f.bins_dev[d], bins.size()));
         KPT_HIP(ctx, hipMemcpy(f.face_dev[d], faces[d], (static_cast<size_t>(n[d]) + 1) * sizeof(double),
                                hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(f.lut_dev[d], lut.data(), lut.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        KPT_HIP(ctx, hipMemcpy(f.bins_dev[d], bins.data(), bins.size() * sizeof(kpt::FaceBin),
+                               hipMemcpyHostToDevice));
         f.g.face[d] = f.face_dev[d];
-        f.g.lut[d] = f.lut_dev[d];
-        f.g.lut_n[d] = static_cast<int32_t>(lut.size());
-        f.g.lut_steps[d] = steps;
-        f.g.lut_scale[d] = scale;
+        f.g.bins[d] = f.bins_dev[d];
+        f.g.bins_n[d] = static_cast<int32_t>(bins.size());
+        f.g.bins_scale[d] = scale;
     }
     f.model = model;
     KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
@@ -502,7 +519,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.foc);
         for (int d = 0; d < 2; ++d) {
             dev_free(f.face_dev[d]);
-            dev_free(f.lut_dev[d]);
+            dev_free(f.bins_dev[d]);
         }
     }
     dev_free(ctx->pair32);

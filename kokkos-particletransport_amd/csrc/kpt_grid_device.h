@@ -49,9 +49,9 @@ __device__ __forceinline__ double three_point(double xl, double xc, double xu, d
 // start + k step, reached through the +step/2 interpolation shift and the
 // cell-centred inverse map (axis_stencil). Stretched axis (no counterpart in the
 // driver, whose reader rejects non-uniform faces): value k sits at face k and q
-// (the shift is 0 there) falls in the face interval [face k, face k+1), found
-// through the bin table and at most lut_steps forward moves. `h` receives the
-// width of the interval (the uniform step on a uniform axis).
+// (the shift is 0 there) falls in the face interval [face k, face k+1), read
+// from one bin record (interval k or k + 1 of the bin). `h` receives the width
+// of the interval (the uniform step on a uniform axis).
 __device__ __forceinline__ bool node_stencil(double q, const GridDesc& g, int d, int n, int& i0, double& w,
                                              double& h) {
     if (g.face[d] == nullptr) {
@@ -60,14 +60,16 @@ __device__ __forceinline__ bool node_stencil(double q, const GridDesc& g, int d,
     }
     const double qw = grid_wrap(q, g.phys_lo[d], g.phys_hi[d], g.width[d], g.inv_width[d]);
     if (!(qw == qw)) return false;
-    const double* e = g.face[d];
-    const double bf = (qw - g.start[d]) * g.lut_scale[d];
-    const int b = bf < 0.0 ? 0 : (bf >= static_cast<double>(g.lut_n[d] - 1) ? g.lut_n[d] - 1 : static_cast<int>(bf));
-    int k = g.lut[d][b];
-    if (k > 0 && qw < e[k]) --k;
-    for (int s = 0; s < g.lut_steps[d]; ++s)
-        if (k + 1 < n && qw >= e[k + 1]) ++k;
-    const double el = e[k], eu = e[k + 1];
+    const double bf = (qw - g.start[d]) * g.bins_scale[d];
+    const int b = bf < 0.0 ? 0 : (bf >= static_cast<double>(g.bins_n[d] - 1) ? g.bins_n[d] - 1 : static_cast<int>(bf));
+    const FaceBin fb = g.bins[d][b];
+    int k = fb.k;
+    double el = fb.lo, eu = fb.mid;
+    if (qw >= fb.mid && k + 1 < n) {
+        ++k;
+        el = fb.mid;
+        eu = fb.hi;
+    }
     h = eu - el;
     double ww = (qw - el) / h;
     ww = ww < 0.0 ? 0.0 : (ww > 1.0 ? 1.0 : ww);

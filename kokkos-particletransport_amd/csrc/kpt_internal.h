@@ -21,13 +21,22 @@ inline uint64_t coefficient_plane(int nx, int ny) {
     return static_cast<uint64_t>(nx + 2) * static_cast<uint64_t>(ny + 2);
 }
 
+// One bin of a stretched axis' lookup table, read with one 32-byte load: the
+// face interval k that contains the bin's lower end (less a guard band) and the
+// faces e_k, e_{k+1}, e_{k+2} (the last two repeat e_n past the end). The host
+// sizes the bins so that every query that rounds into the bin lies in [e_k,
+// e_{k+2}): its interval is k, or k + 1 when it reaches e_{k+1}.
+struct alignas(16) FaceBin {
+    double lo, mid, hi;
+    int32_t k, pad;
+};
+
 // Grid of a loaded frame: uniform cell-centred spacing (AnalyticCoordinateGrid
 // with UniformSpacing, include/AnalyticCoordinateGrid.hpp:159-210, and the
 // bounds cache of include/MappedCoordinateGrid.hpp:117-145). An axis with
 // non-uniform faces is "stretched": face[d] points to its n + 1 face coordinates
-// on the device and lut[d] to a table of lut_n[d] equal bins over the axis, bin
-// b holding the face interval that contains the bin's lower end; lut_steps[d]
-// bounds the intervals a bin can span. Null pointers mark a uniform axis.
+// on the device and bins[d] to bins_n[d] equal bins over [e_0, e_n), with
+// bins_scale[d] bins per unit length. Null pointers mark a uniform axis.
 struct GridDesc {
     int32_t nx, ny;
     double start[2];
@@ -35,10 +44,9 @@ struct GridDesc {
     double inv_step[2];
     double phys_lo[2], phys_hi[2], width[2], inv_width[2];
     const double* face[2];
-    const int32_t* lut[2];
-    int32_t lut_n[2];
-    int32_t lut_steps[2];
-    double lut_scale[2];
+    const FaceBin* bins[2];
+    int32_t bins_n[2];
+    double bins_scale[2];
 };
 
 struct ParticleArrays {

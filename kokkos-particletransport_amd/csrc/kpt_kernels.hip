@@ -9,6 +9,7 @@
 //    adjacent nodes, each pair one contiguous 2 x 128 B run for FP64 Parker.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -750,11 +751,23 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
     }
 }
 
+// Particle tile of a push workgroup. Workgroups are dispatched to the 8 XCDs
+// round-robin (workgroup b to XCD b % 8). Runs of `run` consecutive tiles go to
+// the XCDs in turn, so each XCD's L2 sees `run` spatially adjacent tiles of the
+// cell-sorted particles at a time; run = 1 is the plain order. The grid is a
+// multiple of 8 run and the map a bijection; ids past the last tile find no
+// particles.
+__device__ __forceinline__ int64_t push_first_particle(uint32_t run) {
+    const uint32_t b = blockIdx.x, i = b / 8, x = b % 8;
+    const uint64_t t = static_cast<uint64_t>(i / run) * (8ull * run) + static_cast<uint64_t>(x) * run + i % run;
+    return static_cast<int64_t>(t) * kBlock + threadIdx.x;
+}
+
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
-    push_kernel(PushArgs a) {
-    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+    push_kernel(PushArgs a, uint32_t run) {
+    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, push_first_particle(run));
 }
 
 // Closure from the network outputs at pitch cosine mu (|mu| in the hull):
@@ -1030,11 +1043,11 @@ __device__ __forceinline__ MlpLds mlp_stage(const void* image, uint4* smem, floa
 
 template <bool STR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
-    push_kernel_mlp(PushArgs a) {
+    push_kernel_mlp(PushArgs a, uint32_t run) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     float b40, b41;
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
-    push_lane_mlp<STR>(a, im, b40, b41, blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x);
+    push_lane_mlp<STR>(a, im, b40, b41, push_first_particle(run));
 }
 
 // ---------------------------------------------------------------------------
@@ -1448,32 +1461,40 @@ hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out,
 
 template <bool STR>
 hipError_t launch_push_t(const PushArgs& a, int model, bool focused, bool fp32, int closure, hipStream_t stream) {
-    const dim3 grid(grid_for(a.count)), block(kBlock);
+    // 256 runs, 32 per XCD: each XCD gets 32 bands spread over the sorted set,
+    // which balances work that varies with position (adaptive dt), and each band
+    // is long enough for vertical neighbours to share its L2. On 4 M (Parker) and
+    // 8 M (focused) particles of the 1024 x 512 field this measured +26% (Parker
+    // FP64) and +6% (focused FP64) on the stretched grid against the plain order,
+    // and within 1% on the uniform grid.
+    const uint32_t tiles = grid_for(a.count);
+    const uint32_t run = std::min(128u, std::max(1u, tiles / 256));
+    const dim3 grid((tiles + 8 * run - 1) / (8 * run) * (8 * run)), block(kBlock);
     constexpr int R = KPT_CLOSURE_REDUCED, Q = KPT_CLOSURE_QLT;
     if (model == KPT_MODEL_PARKER16) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<16, false, float, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<16, false, float, R, STR>), grid, block, 0, stream, a, run);
         else
-            hipLaunchKernelGGL((push_kernel<16, false, double, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<16, false, double, R, STR>), grid, block, 0, stream, a, run);
     } else if (focused && closure == KPT_CLOSURE_MLP) {
         if (!fp32 || a.mlp_image == nullptr) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(push_kernel_mlp<STR>, grid, block, 0, stream, a);
+        hipLaunchKernelGGL(push_kernel_mlp<STR>, grid, block, 0, stream, a, run);
     } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, true, float, Q, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, float, Q, STR>), grid, block, 0, stream, a, run);
         else
-            hipLaunchKernelGGL((push_kernel<21, true, double, Q, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, double, Q, STR>), grid, block, 0, stream, a, run);
     } else if (focused) {
         if (closure != R) return hipErrorInvalidValue;
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, true, float, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, float, R, STR>), grid, block, 0, stream, a, run);
         else
-            hipLaunchKernelGGL((push_kernel<21, true, double, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, true, double, R, STR>), grid, block, 0, stream, a, run);
     } else {
         if (fp32)
-            hipLaunchKernelGGL((push_kernel<21, false, float, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, float, R, STR>), grid, block, 0, stream, a, run);
         else
-            hipLaunchKernelGGL((push_kernel<21, false, double, R, STR>), grid, block, 0, stream, a);
+            hipLaunchKernelGGL((push_kernel<21, false, double, R, STR>), grid, block, 0, stream, a, run);
     }
     return hipGetLastError();
 }

@@ -145,3 +145,49 @@ def test_resumable_restart_continues_bit_for_bit(app, tmp_path):
     snap = kpt.snapshot_read_host(str(a / "particles_00004.bin"), h.charge, h.split_ratio)
     # the id field carries the particle ids (the cell sort moved them off their slots)
     assert not np.array_equal(snap["id"], np.arange(len(snap["id"]), dtype=np.uint64))
+
+
+@pytest.mark.gpu
+def test_stretched_fields_need_the_flag_and_match_library_loop(app, tmp_path):
+    """Compact fields with tanh-stretched y faces: refused with the reference
+    reader's message by default; with --allow-stretched-fields the run's particles
+    equal a loop driven through the library on the same faces."""
+    fields = tmp_path / "fields"
+    fields.mkdir()
+    ye = W.stretched_edges(32)
+    for f in range(3):
+        xe, ye, planes = W.island_field(64, 32, width=0.05, seed=3 + f, y_edges=ye)
+        planes[2] = planes[2] + 0.1
+        kpt.compact_field_write(str(fields / f"field{f:05d}.bin"), xe, ye, planes)
+    ppf, cap, seed = 2048, 8192, 5
+    common = ["--profile", "smoke", "--field-dir", str(fields), "--particles-per-frame", str(ppf),
+              "--capacity", str(cap), "--seed", str(seed), "--frames", "2"]
+    r = _run(*common, "--output-dir", str(tmp_path / "refused"))
+    assert r.returncode == 1 and "uniformly spaced" in r.stderr, r.stderr
+    out = tmp_path / "out"
+    r = _run(*common, "--output-dir", str(out), "--allow-stretched-fields")
+    assert r.returncode == 0, r.stderr
+
+    h = kpt.default_host_settings()
+    xe, ye, _ = kpt.compact_field_read(str(fields / "field00000.bin"), allow_nonuniform=True)
+    s = kpt.make_device_settings(h, xe, ye)
+    assert s.interpolation_shift_y == 0.0
+    s.push_chunks = 8
+    with kpt.Context(cap, seed=seed) as c:
+        for frame in range(2):
+            if frame == 0:
+                c.field_upload(kpt.SLOT_LOWER, 16, *kpt.compact_field_read(str(fields / "field00000.bin"),
+                                                                           allow_nonuniform=True))
+            else:
+                c.field_advance()
+            c.field_upload(kpt.SLOT_UPPER, 16, *kpt.compact_field_read(str(fields / f"field{frame + 1:05d}.bin"),
+                                                                       allow_nonuniform=True))
+            c.inject(s, ppf, frame * s.mhd_dt, frame)
+            c.push_interval(s, frame * s.mhd_dt, frame)
+            c.split(s)
+        want = c.download()
+    snap = kpt.snapshot_read_host(str(out / "particles_00002.bin"), h.charge, h.split_ratio)
+    n = len(want["x"])
+    assert n > ppf
+    for k in ("x", "y", "p", "weight", "status", "split_level"):
+        np.testing.assert_array_equal(snap[k][:n], want[k], err_msg=k)
