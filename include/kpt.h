@@ -345,7 +345,9 @@ kpt_status kpt_histogram_xy(kpt_ctx* ctx, int32_t bins_x, int32_t bins_y, double
 /* ParticleSystem::sort_by_key + compact_active_particles
  * (include/ParticleSystem.hpp:715-793): reorder the stored particles by the
  * lower-left node of their interpolation stencil on the lower frame's grid
- * (stable), inactive particles last. With compact != 0 the inactive particles
+ * (stable), inactive particles last. Nodes are numbered in blocks of 16 x 16
+ * (blocks row-major, nodes row-major inside a block), which keeps the particles
+ * that run together on one XCD in a compact patch. With compact != 0 the inactive particles
  * are dropped and the count shrinks to the active count. Slot order changes, so
  * the injected-noise mode (which is keyed by slot) rejects this call. */
 kpt_status kpt_sort_particles(kpt_ctx* ctx, const kpt_device_settings* s, int32_t compact,

@@ -213,7 +213,8 @@ def test_c1_statistics_against_oracle():
 
 
 def test_sort_is_a_permutation_and_does_not_change_physics():
-    """Cell sort: same particle multiset, keys non-decreasing, inactive last,
+    """Cell sort: same particle multiset, keys (stencil node numbered in 16 x 16
+    node blocks) non-decreasing, inactive last,
     compaction drops them; the Philox stream is keyed by particle id, so the
     push of the sorted set equals the push of the unsorted set particle by
     particle (matched by id), bit for bit."""
@@ -241,7 +242,8 @@ def test_sort_is_a_permutation_and_does_not_change_physics():
                 qy = np.where(qy >= s.y_upper, qy - s.y_width, qy)
                 ix = np.floor((qx - s.x_lower) / s.dx - 0.5).astype(int)
                 iy = np.floor((qy - s.y_lower) / s.dy - 0.5).astype(int)
-                key = (iy + 1) * 66 + (ix + 1)  # 64 x 32 grid: 66 stored nodes per row
+                I, J = ix + 1, iy + 1  # 64 x 32 grid: 66 x 34 stored nodes, 5 blocks of 16 per block row
+                key = ((J // 16) * 5 + I // 16) * 256 + (J % 16) * 16 + I % 16
                 assert np.mean(np.diff(key) >= 0) > 0.999
             st = c.push_interval(s, 0.0, 5)
             got = c.download()
