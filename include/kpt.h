@@ -275,6 +275,21 @@ kpt_status kpt_field_load_compact(kpt_ctx* ctx, const char* path, int32_t slot, 
 kpt_status kpt_field_load_compact_ex(kpt_ctx* ctx, const char* path, int32_t slot, int32_t model,
                                      int32_t allow_stretched);
 
+/* Background load of the next frame, for the frame loop of run_transport_loop
+ * (src/main.cpp:2305-2313: lower = upper, then read frame + 1 into upper). The
+ * reference reads each frame synchronously between intervals; here a host
+ * thread reads and validates `path` (as kpt_field_load_compact_ex) into pinned
+ * memory, then copies and packs it into a third, staging slot on a second
+ * stream while the caller's pushes run. One stage at a time. */
+kpt_status kpt_field_stage_compact(kpt_ctx* ctx, const char* path, int32_t model, int32_t allow_stretched);
+/* Finish the stage and rotate the slots: lower <- upper, upper <- staged, the
+ * same frames as kpt_field_advance followed by kpt_field_load_compact_ex into
+ * the upper slot. Device work issued afterwards is ordered after the staged
+ * copy and pack. `read_seconds` (may be NULL) receives the host time of the
+ * file read. A read or validation error of the stage is returned here, with
+ * the slots unchanged. */
+kpt_status kpt_field_advance_staged(kpt_ctx* ctx, double* read_seconds);
+
 /* Same as kpt_field_load_compact, from host arrays in the compact-field order
  * (Bx, By, Bz, Vx, Vy, Vz each [ny][nx], x fastest). */
 kpt_status kpt_field_upload(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, int32_t ny,

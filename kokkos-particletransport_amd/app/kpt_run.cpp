@@ -63,6 +63,7 @@ struct RunSettings {
     int device{0};
     bool resumable_snapshots{false};
     bool allow_stretched_fields{false};
+    bool stream_fields{true};
 
     int interval_count() const { return end_frame - start_frame; }
     bool walltime_limit_enabled() const { return walltime_limit_seconds > 0.0; }
@@ -225,6 +226,8 @@ void print_help() {
         << "                                so a restart continues the run bit for bit\n"
         << "  --allow-stretched-fields      accept non-uniform field faces (value k at face k,\n"
         << "                                three-point derivatives, local cell widths in dt)\n"
+        << "  --synchronous-field-reads     read each frame between intervals instead of\n"
+        << "                                staging the next one during the push\n"
         << "  --device N                    GPU ordinal (default 0)\n";
 }
 
@@ -318,6 +321,8 @@ RunSettings parse(int argc, char** argv) {
             s.resumable_snapshots = true;
         } else if (a == "--allow-stretched-fields") {
             s.allow_stretched_fields = true;
+        } else if (a == "--synchronous-field-reads") {
+            s.stream_fields = false;
         } else if (a == "--precision") {
             const std::string p = val();
             if (p == "fp64") s.precision = KPT_PREC_FP64;
@@ -542,9 +547,18 @@ int run(const RunSettings& rs) {
     header << "  kpara0=" << rs.host.kpara0 << " kperp/kpara=" << rs.host.kperp_over_kpara << '\n'
            << "  particle_v0=" << rs.host.particle_v0 << " duu0=" << rs.host.duu0 << '\n'
            << "  device=" << kpt_version() << " precision=" << (rs.precision == KPT_PREC_FP32 ? "fp32" : "fp64")
-           << " push_chunks=" << rs.push_chunks << '\n';
+           << " push_chunks=" << rs.push_chunks << '\n'
+           << "  field_reads=" << (rs.stream_fields ? "staged during the push" : "synchronous") << '\n';
     log.block(header.str());
 
+    // The frame after the upper one loads in the background during each push.
+    const auto stage = [&](int f) {
+        if (rs.stream_fields && f <= rs.end_frame)
+            ctx.check(kpt_field_stage_compact(ctx.c, frame_path(rs.field_dir, "field", f, ".bin").c_str(), model,
+                                              stretched ? 1 : 0),
+                      "field stage");
+    };
+    stage(rs.start_frame + 2);
     const auto run_start = Clock::now();
     bool stopped = false;
     int last = rs.start_frame;
@@ -555,9 +569,14 @@ int run(const RunSettings& rs) {
         timing.read_seconds = frame == rs.start_frame ? initial_read_seconds : 0.0;
         if (frame > rs.start_frame) {
             const auto t = Clock::now();
-            ctx.check(kpt_field_advance(ctx.c), "field advance");
-            upload(ctx, KPT_SLOT_UPPER, model,
-                   read_field(frame_path(rs.field_dir, "field", frame + 1, ".bin"), stretched));
+            if (rs.stream_fields) {
+                ctx.check(kpt_field_advance_staged(ctx.c, nullptr), "field advance");
+                stage(frame + 2);
+            } else {
+                ctx.check(kpt_field_advance(ctx.c), "field advance");
+                upload(ctx, KPT_SLOT_UPPER, model,
+                       read_field(frame_path(rs.field_dir, "field", frame + 1, ".bin"), stretched));
+            }
             timing.read_seconds = since(t);
         }
         const double t0 = static_cast<double>(frame) * rs.host.mhd_dt;

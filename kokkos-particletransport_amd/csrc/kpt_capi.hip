@@ -13,7 +13,10 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <chrono>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kpt_internal.h"
@@ -43,6 +46,29 @@ struct FieldSlot {
     // stretched axes: face coordinates and bin table on the device
     double* face_dev[2] = {nullptr, nullptr};
     kpt::FaceBin* bins_dev[2] = {nullptr, nullptr};
+    size_t face_cap[2] = {0, 0};
+    size_t bins_cap[2] = {0, 0};
+    // host copies of the tables, alive until the next fill of this slot (the
+    // asynchronous copies read them)
+    std::vector<double> host_face[2];
+    std::vector<kpt::FaceBin> host_bins[2];
+};
+
+// Background load of the next frame (kpt_field_stage_compact): a host thread
+// reads and validates the file into pinned memory, then copies it to `slot` and
+// packs it on `stream`, while the caller's pushes run on the context's stream.
+struct FieldStage {
+    FieldSlot slot;
+    std::thread worker;
+    bool busy = false;              // a stage was started and not yet finished
+    hipStream_t stream = nullptr;   // default priority, like the context's stream
+    hipEvent_t ready = nullptr;     // context-stream work queued before the stage
+    hipEvent_t done = nullptr;      // the staged frame is packed
+    double* pinned = nullptr;       // six planes, page-locked
+    size_t pinned_cap = 0;          // doubles
+    kpt_status code = KPT_OK;
+    std::string err;
+    double read_seconds = 0.0;
 };
 
 }  // namespace
@@ -57,6 +83,7 @@ struct kpt_ctx {
     uint64_t next_root = 0;
     kpt::ParticleArrays P{};
     FieldSlot slot[2];
+    FieldStage stage;
     uint64_t field_versions = 0;
     // FP32 frame-interleaved pair of the lower and upper frames (built by the
     // FP32 time-blended push when either slot changed)
@@ -259,14 +286,17 @@ kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
 kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_done, bool compact,
                          uint64_t* active);
 
-kpt_status ensure_field_buffers(kpt_ctx* ctx, FieldSlot& f, int32_t nx, int32_t ny, int model) {
+// Device buffers of a slot for an nx x ny field of `model` components; reused
+// when large enough, so a run of equal frames allocates once.
+hipError_t alloc_field_buffers(FieldSlot& f, int32_t nx, int32_t ny, int model) {
     const size_t raw = static_cast<size_t>(nx) * ny * 6;
     const size_t nodes = kpt::coefficient_plane(nx, ny) * static_cast<size_t>(model);
+    hipError_t e = hipSuccess;
     if (raw > f.raw_cap) {
         dev_free(f.raw);
         f.raw = nullptr;
         f.raw_cap = 0;
-        KPT_HIP(ctx, dev_alloc(&f.raw, raw));
+        if ((e = dev_alloc(&f.raw, raw)) != hipSuccess) return e;
         f.raw_cap = raw;
     }
     if (nodes > f.nodes_cap) {
@@ -275,17 +305,19 @@ kpt_status ensure_field_buffers(kpt_ctx* ctx, FieldSlot& f, int32_t nx, int32_t 
         f.c64 = nullptr;
         f.c32 = nullptr;
         f.nodes_cap = 0;
-        KPT_HIP(ctx, dev_alloc(&f.c64, nodes));
-        KPT_HIP(ctx, dev_alloc(&f.c32, nodes));
+        if ((e = dev_alloc(&f.c64, nodes)) != hipSuccess) return e;
+        if ((e = dev_alloc(&f.c32, nodes)) != hipSuccess) return e;
         f.nodes_cap = nodes;
     }
-    return KPT_OK;
+    return hipSuccess;
 }
 
-// Compact-field parser shared by kpt_compact_field_read and kpt_field_load_compact.
-bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double>& xe,
-                  std::vector<double>& ye, std::vector<double>& comps, bool allow_nonuniform,
-                  std::string& err) {
+// Compact-field parser shared by kpt_compact_field_read, kpt_field_load_compact
+// and the staging thread. `planes(cells)` returns room for the six planes of
+// cells values each (nullptr: out of memory).
+bool read_compact_into(const char* path, int32_t& nx, int32_t& ny, std::vector<double>& xe,
+                       std::vector<double>& ye, const std::function<double*(size_t)>& planes,
+                       bool allow_nonuniform, std::string& err) {
     std::ifstream in(path, std::ios::binary);
     if (!in) {
         err = "AthenaFieldReader: failed to open Athena binary file.";
@@ -320,10 +352,14 @@ bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double
     }
     if (!check_edges(xe.data(), xe.size(), "x", !allow_nonuniform, err)) return false;
     if (!check_edges(ye.data(), ye.size(), "y", !allow_nonuniform, err)) return false;
-    comps.resize(cells * 6);
+    double* comps = planes(cells);
+    if (comps == nullptr) {
+        err = "AthenaFieldReader: out of host memory for the field planes.";
+        return false;
+    }
     static const char* names[6] = {"Bx", "By", "Bz", "Vx", "Vy", "Vz"};
     for (int c = 0; c < 6; ++c) {
-        double* dst = comps.data() + static_cast<size_t>(c) * cells;
+        double* dst = comps + static_cast<size_t>(c) * cells;
         in.read(reinterpret_cast<char*>(dst), static_cast<std::streamsize>(cells * 8));
         if (!in) {
             err = std::string("AthenaFieldReader: failed to read ") + names[c] + ".";
@@ -343,56 +379,100 @@ bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double
     return true;
 }
 
-// Raw planes and packed FP64 / FP32 frames of one field into `f`.
-kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
-                       const double* ye, const double* const planes[6]) {
-    if (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21)
-        return fail(ctx, KPT_ERR_ARG, "invalid coefficient model (16 or 21).");
-    if (nx <= 0 || ny <= 0) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: Athena binary dimensions must be positive.");
-    std::string err;
+bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double>& xe,
+                  std::vector<double>& ye, std::vector<double>& comps, bool allow_nonuniform,
+                  std::string& err) {
+    return read_compact_into(
+        path, nx, ny, xe, ye,
+        [&](size_t cells) {
+            comps.resize(cells * 6);
+            return comps.data();
+        },
+        allow_nonuniform, err);
+}
+
+// Raw planes, packed FP64 / FP32 frames and stretched-axis tables of one field
+// into `f`, by work queued on `stream` (not waited for). Touches no context
+// state, so the staging thread uses it too. On failure returns the status and
+// sets `err`.
+kpt_status fill_slot(FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe, const double* ye,
+                     const double* const planes[6], hipStream_t stream, std::string& err) {
+    if (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21) {
+        err = "invalid coefficient model (16 or 21).";
+        return KPT_ERR_ARG;
+    }
+    if (nx <= 0 || ny <= 0) {
+        err = "AthenaFieldReader: Athena binary dimensions must be positive.";
+        return KPT_ERR_ARG;
+    }
     if (!check_edges(xe, static_cast<size_t>(nx) + 1, "x", false, err) ||
         !check_edges(ye, static_cast<size_t>(ny) + 1, "y", false, err))
-        return fail(ctx, KPT_ERR_ARG, err);
+        return KPT_ERR_ARG;
     f.valid = false;
-    kpt_status st = ensure_field_buffers(ctx, f, nx, ny, model);
-    if (st != KPT_OK) return st;
+    const auto hip_err = [&](hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return KPT_ERR_DEVICE;
+    };
+    hipError_t e = alloc_field_buffers(f, nx, ny, model);
+    if (e != hipSuccess) return hip_err(e, "field buffers");
     const size_t cells = static_cast<size_t>(nx) * ny;
     for (int c = 0; c < 6; ++c) {
-        KPT_HIP(ctx, hipMemcpyAsync(f.raw + c * cells, planes[c], cells * sizeof(double), hipMemcpyHostToDevice,
-                                    ctx->stream));
+        e = hipMemcpyAsync(f.raw + c * cells, planes[c], cells * sizeof(double), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_err(e, "field upload");
     }
     f.g = make_grid_desc(nx, ny, xe, ye);
     const double* faces[2] = {xe, ye};
     const int32_t n[2] = {nx, ny};
     for (int d = 0; d < 2; ++d) {
-        dev_free(f.face_dev[d]);
-        dev_free(f.bins_dev[d]);
-        f.face_dev[d] = nullptr;
-        f.bins_dev[d] = nullptr;
         if (n[d] <= 1 || faces_uniform(faces[d], static_cast<size_t>(n[d]) + 1)) continue;
-        std::vector<kpt::FaceBin> bins;
+        std::vector<kpt::FaceBin>& bins = f.host_bins[d];
         double scale = 0.0;
-        if (!stretched_table(faces[d], n[d], bins, scale))
-            return fail(ctx, KPT_ERR_ARG, std::string(d == 0 ? "x" : "y") +
-                                              " faces are too strongly graded: the narrowest interval is below "
-                                              "2^-22 of the axis.");
-        KPT_HIP(ctx, dev_alloc(&f.face_dev[d], static_cast<size_t>(n[d]) + 1));
-        KPT_HIP(ctx, dev_alloc(&f.bins_dev[d], bins.size()));
-        KPT_HIP(ctx, hipMemcpy(f.face_dev[d], faces[d], (static_cast<size_t>(n[d]) + 1) * sizeof(double),
-                               hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(f.bins_dev[d], bins.data(), bins.size() * sizeof(kpt::FaceBin),
-                               hipMemcpyHostToDevice));
+        if (!stretched_table(faces[d], n[d], bins, scale)) {
+            err = std::string(d == 0 ? "x" : "y") +
+                  " faces are too strongly graded: the narrowest interval is below 2^-22 of the axis.";
+            return KPT_ERR_ARG;
+        }
+        const size_t nf = static_cast<size_t>(n[d]) + 1;
+        if (nf > f.face_cap[d]) {
+            dev_free(f.face_dev[d]);
+            f.face_dev[d] = nullptr;
+            f.face_cap[d] = 0;
+            if ((e = dev_alloc(&f.face_dev[d], nf)) != hipSuccess) return hip_err(e, "face table");
+            f.face_cap[d] = nf;
+        }
+        if (bins.size() > f.bins_cap[d]) {
+            dev_free(f.bins_dev[d]);
+            f.bins_dev[d] = nullptr;
+            f.bins_cap[d] = 0;
+            if ((e = dev_alloc(&f.bins_dev[d], bins.size())) != hipSuccess) return hip_err(e, "bin table");
+            f.bins_cap[d] = bins.size();
+        }
+        f.host_face[d].assign(faces[d], faces[d] + nf);
+        if ((e = hipMemcpyAsync(f.face_dev[d], f.host_face[d].data(), nf * sizeof(double), hipMemcpyHostToDevice,
+                                stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(f.bins_dev[d], bins.data(), bins.size() * sizeof(kpt::FaceBin),
+                                hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hip_err(e, "stretched-axis tables");
         f.g.face[d] = f.face_dev[d];
         f.g.bins[d] = f.bins_dev[d];
         f.g.bins_n[d] = static_cast<int32_t>(bins.size());
         f.g.bins_scale[d] = scale;
     }
     f.model = model;
-    KPT_HIP(ctx, kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    f.version = ++ctx->field_versions;
+    if ((e = kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, stream)) != hipSuccess) return hip_err(e, "pack");
     f.gen_valid = false;
     f.foc_valid = false;
+    return KPT_OK;
+}
+
+// fill_slot on the context's stream, waited for: the slot is ready on return.
+kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
+                       const double* ye, const double* const planes[6]) {
+    std::string err;
+    const kpt_status st = fill_slot(f, model, nx, ny, xe, ye, planes, ctx->stream, err);
+    if (st != KPT_OK) return fail(ctx, st, err);
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    f.version = ++ctx->field_versions;
     f.valid = true;
     return KPT_OK;
 }
@@ -401,6 +481,65 @@ kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, i
                         const double* xe, const double* ye, const double* const planes[6]) {
     if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
     return upload_into(ctx, ctx->slot[slot], model, nx, ny, xe, ye, planes);
+}
+
+// Staging thread: read and validate the file into pinned memory, then copy and
+// pack it into the staging slot on the staging stream. Reports through the
+// stage's code and message only (the caller's thread owns the context).
+void stage_run(kpt_ctx* ctx, std::string path, int32_t model, bool allow_stretched) {
+    FieldStage& S = ctx->stage;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipSetDevice(ctx->device);
+    // the previous stage's copies out of the pinned buffer have finished
+    if (e == hipSuccess) e = hipEventSynchronize(S.done);
+    if (e != hipSuccess) {
+        S.code = KPT_ERR_DEVICE;
+        S.err = std::string("field stage: ") + hipGetErrorString(e);
+        return;
+    }
+    int32_t nx = 0, ny = 0;
+    std::vector<double> xe, ye;
+    std::string err;
+    const bool ok = read_compact_into(
+        path.c_str(), nx, ny, xe, ye,
+        [&](size_t cells) -> double* {
+            if (cells * 6 > S.pinned_cap) {
+                if (S.pinned) (void)hipHostFree(S.pinned);
+                S.pinned = nullptr;
+                S.pinned_cap = 0;
+                if (hipHostMalloc(reinterpret_cast<void**>(&S.pinned), cells * 6 * sizeof(double)) != hipSuccess) {
+                    S.pinned = nullptr;
+                    return nullptr;
+                }
+                S.pinned_cap = cells * 6;
+            }
+            return S.pinned;
+        },
+        allow_stretched, err);
+    S.read_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!ok) {
+        S.code = KPT_ERR_IO;
+        S.err = err;
+        return;
+    }
+    // the slot's buffers held the frame that work queued before the stage reads
+    if ((e = hipStreamWaitEvent(S.stream, S.ready, 0)) != hipSuccess) {
+        S.code = KPT_ERR_DEVICE;
+        S.err = std::string("field stage: ") + hipGetErrorString(e);
+        return;
+    }
+    const size_t cells = static_cast<size_t>(nx) * ny;
+    const double* planes[6];
+    for (int c = 0; c < 6; ++c) planes[c] = S.pinned + c * cells;
+    S.code = fill_slot(S.slot, model, nx, ny, xe.data(), ye.data(), planes, S.stream, err);
+    if (S.code != KPT_OK) {
+        S.err = err;
+        return;
+    }
+    if ((e = hipEventRecord(S.done, S.stream)) != hipSuccess) {
+        S.code = KPT_ERR_DEVICE;
+        S.err = std::string("field stage: ") + hipGetErrorString(e);
+    }
 }
 
 }  // namespace
@@ -500,6 +639,9 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
 kpt_status kpt_destroy(kpt_ctx* ctx) {
     if (!ctx) return KPT_OK;
     (void)hipSetDevice(ctx->device);
+    FieldStage& S = ctx->stage;
+    if (S.busy) S.worker.join();
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     kpt::ParticleArrays& P = ctx->P;
     for (void* p : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p),
@@ -511,7 +653,8 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
                     static_cast<void*>(ctx->hist_out), ctx->summary_partials,
                     static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
         dev_free(p);
-    for (FieldSlot& f : ctx->slot) {
+    for (FieldSlot* fp : {&ctx->slot[0], &ctx->slot[1], &S.slot}) {
+        FieldSlot& f = *fp;
         dev_free(f.raw);
         dev_free(f.c64);
         dev_free(f.c32);
@@ -539,6 +682,10 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(ctx->sort.vals[k]);
     }
     dev_free(ctx->sort.temp);
+    if (S.pinned) (void)hipHostFree(S.pinned);
+    if (S.ready) (void)hipEventDestroy(S.ready);
+    if (S.done) (void)hipEventDestroy(S.done);
+    if (S.stream) (void)hipStreamDestroy(S.stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -681,6 +828,53 @@ kpt_status kpt_field_advance(kpt_ctx* ctx) {
     if (!ctx->slot[KPT_SLOT_UPPER].valid) return fail(ctx, KPT_ERR_STATE, "upper field slot is empty.");
     std::swap(ctx->slot[KPT_SLOT_LOWER], ctx->slot[KPT_SLOT_UPPER]);
     ctx->slot[KPT_SLOT_UPPER].valid = false;
+    return KPT_OK;
+}
+
+kpt_status kpt_field_stage_compact(kpt_ctx* ctx, const char* path, int32_t model, int32_t allow_stretched) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    if (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21)
+        return fail(ctx, KPT_ERR_ARG, "invalid coefficient model (16 or 21).");
+    FieldStage& S = ctx->stage;
+    if (S.busy)
+        return fail(ctx, KPT_ERR_STATE, "a field stage is in flight: finish it with kpt_field_advance_staged.");
+    (void)hipSetDevice(ctx->device);
+    if (!S.stream) {
+        KPT_HIP(ctx, hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+        KPT_HIP(ctx, hipEventCreateWithFlags(&S.ready, hipEventDisableTiming));
+        KPT_HIP(ctx, hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+    }
+    KPT_HIP(ctx, hipEventRecord(S.ready, ctx->stream));
+    S.code = KPT_OK;
+    S.err.clear();
+    S.read_seconds = 0.0;
+    S.slot.valid = false;
+    try {
+        S.worker = std::thread(stage_run, ctx, std::string(path), model, allow_stretched != 0);
+    } catch (const std::exception& ex) {
+        return fail(ctx, KPT_ERR_STATE, std::string("field stage thread: ") + ex.what());
+    }
+    S.busy = true;
+    return KPT_OK;
+}
+
+kpt_status kpt_field_advance_staged(kpt_ctx* ctx, double* read_seconds) {
+    if (!ctx) return KPT_ERR_ARG;
+    FieldStage& S = ctx->stage;
+    if (!S.busy) return fail(ctx, KPT_ERR_STATE, "no field stage in flight.");
+    S.worker.join();
+    S.busy = false;
+    if (read_seconds) *read_seconds = S.read_seconds;
+    if (S.code != KPT_OK) return fail(ctx, S.code, S.err);
+    if (!ctx->slot[KPT_SLOT_UPPER].valid) return fail(ctx, KPT_ERR_STATE, "upper field slot is empty.");
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, hipStreamWaitEvent(ctx->stream, S.done, 0));
+    std::swap(ctx->slot[KPT_SLOT_LOWER], ctx->slot[KPT_SLOT_UPPER]);
+    std::swap(ctx->slot[KPT_SLOT_UPPER], S.slot);
+    FieldSlot& up = ctx->slot[KPT_SLOT_UPPER];
+    up.version = ++ctx->field_versions;
+    up.valid = true;
+    S.slot.valid = false;
     return KPT_OK;
 }
 

@@ -283,6 +283,8 @@ EXPORTS = {
     "kpt_field_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
                          + [C.POINTER(C.c_double)] * 8),
     "kpt_field_advance": (C.c_int, [C.c_void_p]),
+    "kpt_field_stage_compact": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
+    "kpt_field_advance_staged": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "kpt_field_download_coefficients": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                                   C.c_size_t]),
     "kpt_compact_field_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -586,6 +588,16 @@ class Context:
 
     def field_advance(self):
         self._check(self.lib.kpt_field_advance(self.h))
+
+    def field_stage_compact(self, path, model, allow_stretched=False):
+        """Start loading a compact-field file into the staging slot in the background."""
+        self._check(self.lib.kpt_field_stage_compact(self.h, path.encode(), model, int(allow_stretched)))
+
+    def field_advance_staged(self):
+        """Finish the stage and rotate lower <- upper <- staged; returns the file read seconds."""
+        t = C.c_double(0.0)
+        self._check(self.lib.kpt_field_advance_staged(self.h, C.byref(t)))
+        return t.value
 
     def field_coefficients(self, slot, nx, ny, model):
         out = np.empty((ny + 2) * (nx + 2) * model)

@@ -191,3 +191,25 @@ def test_stretched_fields_need_the_flag_and_match_library_loop(app, tmp_path):
     assert n > ppf
     for k in ("x", "y", "p", "weight", "status", "split_level"):
         np.testing.assert_array_equal(snap[k][:n], want[k], err_msg=k)
+
+
+@pytest.mark.gpu
+def test_staged_field_reads_equal_synchronous_reads(app, tmp_path):
+    """The default frame loop stages the next field during each push; the run's
+    snapshots and histograms equal those of --synchronous-field-reads byte for
+    byte."""
+    fields = tmp_path / "fields"
+    fields.mkdir()
+    _write_fields(str(fields), 6)
+    base = ["--profile", "smoke", "--field-dir", str(fields), "--particles-per-frame", "1500", "--capacity", "20000",
+            "--seed", "3", "--frames", "5", "--particle-snapshot-interval", "2", "--histogram-interval", "2"]
+    a, b = tmp_path / "staged", tmp_path / "sync"
+    r = _run(*base, "--output-dir", str(a))
+    assert r.returncode == 0, r.stderr
+    assert "field_reads=staged" in (a / "run.log").read_text()
+    r = _run(*base, "--output-dir", str(b), "--synchronous-field-reads")
+    assert r.returncode == 0, r.stderr
+    names = sorted(p.name for p in a.iterdir() if p.suffix in (".bin", ".csv") and p.name != "summary.csv")
+    assert "particles_00005.bin" in names and "momentum_histogram_00004.csv" in names
+    for name in names:
+        assert (a / name).read_bytes() == (b / name).read_bytes(), name
