@@ -437,7 +437,7 @@ void upload(Ctx& ctx, int slot, int model, const HostField& f) {
 struct Log {
     std::ofstream f;
     void block(const std::string& b) {
-        std::cout << b;
+        std::cout << b << std::flush;  // progress stays visible when stdout is a file
         if (f) {
             f << b;
             f.flush();

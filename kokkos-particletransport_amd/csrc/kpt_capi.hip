@@ -55,8 +55,9 @@ struct FieldSlot {
 };
 
 // Background load of the next frame (kpt_field_stage_compact): a host thread
-// reads and validates the file into pinned memory, then copies it to `slot` and
-// packs it on `stream`, while the caller's pushes run on the context's stream.
+// reads and validates the file into pinned memory and copies it to `slot` on
+// `stream`, while the caller's pushes run on the context's stream; the advance
+// packs it on the context's stream.
 struct FieldStage {
     FieldSlot slot;
     std::thread worker;
@@ -391,12 +392,12 @@ bool read_compact(const char* path, int32_t& nx, int32_t& ny, std::vector<double
         allow_nonuniform, err);
 }
 
-// Raw planes, packed FP64 / FP32 frames and stretched-axis tables of one field
-// into `f`, by work queued on `stream` (not waited for). Touches no context
-// state, so the staging thread uses it too. On failure returns the status and
-// sets `err`.
+// Raw planes, stretched-axis tables and (with `pack`) the packed FP64 / FP32
+// frames of one field into `f`, by work queued on `stream` (not waited for).
+// Touches no context state, so the staging thread uses it too. On failure
+// returns the status and sets `err`.
 kpt_status fill_slot(FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe, const double* ye,
-                     const double* const planes[6], hipStream_t stream, std::string& err) {
+                     const double* const planes[6], hipStream_t stream, bool pack, std::string& err) {
     if (model != KPT_MODEL_PARKER16 && model != KPT_MODEL_FOCUSED21) {
         err = "invalid coefficient model (16 or 21).";
         return KPT_ERR_ARG;
@@ -459,7 +460,8 @@ kpt_status fill_slot(FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const 
         f.g.bins_scale[d] = scale;
     }
     f.model = model;
-    if ((e = kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, stream)) != hipSuccess) return hip_err(e, "pack");
+    if (pack && (e = kpt::launch_pack(f.raw, f.g, model, f.c64, f.c32, stream)) != hipSuccess)
+        return hip_err(e, "pack");
     f.gen_valid = false;
     f.foc_valid = false;
     return KPT_OK;
@@ -469,7 +471,7 @@ kpt_status fill_slot(FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const 
 kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
                        const double* ye, const double* const planes[6]) {
     std::string err;
-    const kpt_status st = fill_slot(f, model, nx, ny, xe, ye, planes, ctx->stream, err);
+    const kpt_status st = fill_slot(f, model, nx, ny, xe, ye, planes, ctx->stream, true, err);
     if (st != KPT_OK) return fail(ctx, st, err);
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     f.version = ++ctx->field_versions;
@@ -483,8 +485,8 @@ kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, i
     return upload_into(ctx, ctx->slot[slot], model, nx, ny, xe, ye, planes);
 }
 
-// Staging thread: read and validate the file into pinned memory, then copy and
-// pack it into the staging slot on the staging stream. Reports through the
+// Staging thread: read and validate the file into pinned memory, then copy it
+// into the staging slot on the staging stream. Reports through the
 // stage's code and message only (the caller's thread owns the context).
 void stage_run(kpt_ctx* ctx, std::string path, int32_t model, bool allow_stretched) {
     FieldStage& S = ctx->stage;
@@ -531,7 +533,9 @@ void stage_run(kpt_ctx* ctx, std::string path, int32_t model, bool allow_stretch
     const size_t cells = static_cast<size_t>(nx) * ny;
     const double* planes[6];
     for (int c = 0; c < 6; ++c) planes[c] = S.pinned + c * cells;
-    S.code = fill_slot(S.slot, model, nx, ny, xe.data(), ye.data(), planes, S.stream, err);
+    // the pack waits for the advance: run beside a push, its 130 MB of frame
+    // writes (1024 x 512, 21 components) evict the push's frames from the caches
+    S.code = fill_slot(S.slot, model, nx, ny, xe.data(), ye.data(), planes, S.stream, false, err);
     if (S.code != KPT_OK) {
         S.err = err;
         return;
@@ -869,6 +873,7 @@ kpt_status kpt_field_advance_staged(kpt_ctx* ctx, double* read_seconds) {
     if (!ctx->slot[KPT_SLOT_UPPER].valid) return fail(ctx, KPT_ERR_STATE, "upper field slot is empty.");
     (void)hipSetDevice(ctx->device);
     KPT_HIP(ctx, hipStreamWaitEvent(ctx->stream, S.done, 0));
+    KPT_HIP(ctx, kpt::launch_pack(S.slot.raw, S.slot.g, S.slot.model, S.slot.c64, S.slot.c32, ctx->stream));
     std::swap(ctx->slot[KPT_SLOT_LOWER], ctx->slot[KPT_SLOT_UPPER]);
     std::swap(ctx->slot[KPT_SLOT_UPPER], S.slot);
     FieldSlot& up = ctx->slot[KPT_SLOT_UPPER];

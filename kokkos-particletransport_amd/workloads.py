@@ -48,12 +48,17 @@ def stretched_edges(n, lo=-0.5, hi=0.5, ratio=10.0):
 
 
 def island_field(nx=1024, ny=512, x_range=(-1.0, 1.0), y_range=(-0.5, 0.5), width=0.02,
-                 by_amp=0.05, bz=0.1, inflow=0.1, noise=0.01, seed=7, modes=4, y_edges=None):
+                 by_amp=0.05, bz=0.1, inflow=0.1, noise=0.01, seed=7, modes=4, y_edges=None, frame=0):
     """Harris sheet with an island chain By = a sum_k sin(k pi x + phi_k)/k exp(-(y/w)^2)
     and 1% Gaussian velocity noise (seeded). Values sit at the nodes x0 + i dx (and,
-    with stretched `y_edges`, at the y faces)."""
+    with stretched `y_edges`, at the y faces). `frame` > 0 gives a frame of a time
+    series: the island phases drift by 0.02 k per frame and the noise is drawn
+    afresh."""
     rng = np.random.default_rng(seed)
     phases = rng.uniform(0.0, 2.0 * np.pi, size=modes)
+    if frame:
+        phases = phases + 0.02 * np.arange(1, modes + 1) * frame
+        rng = np.random.default_rng([seed, frame])
     xe = np.linspace(x_range[0], x_range[1], nx + 1)
     ye = np.linspace(y_range[0], y_range[1], ny + 1) if y_edges is None else np.asarray(y_edges, dtype=np.float64)
     x = xe[0] + np.arange(nx) * (xe[-1] - xe[0]) / nx

@@ -185,3 +185,29 @@ def test_make_device_settings_matches_driver(kpt_mod):
     assert s.pmin_split == 2.0 * 0.1 and s.mu_max == 0.99
     assert s.pitch_angle_scattering_h0 == 0.2 and s.max_particle_steps_per_interval == 200000
     assert s.kpara0 == 0.00743592 and s.kperp_over_kpara == 0.01 and s.charge == -1.0
+
+
+def test_make_fields_writes_a_frame_series(kpt_mod, tmp_path):
+    """make_fields.py: compact files the driver's reader accepts (stretched ones
+    only with the non-uniform flag), frames that differ by the drifting island
+    phases, and links for frames past --distinct."""
+    import make_fields
+    import workloads as W
+
+    out = tmp_path / "series"
+    make_fields.main([str(out), "--frames", "5", "--nx", "32", "--ny", "16", "--distinct", "3"])
+    names = sorted(p.name for p in out.iterdir())
+    assert names == [f"field{f:05d}.bin" for f in range(5)]
+    assert os.path.islink(out / "field00003.bin") and os.readlink(out / "field00003.bin") == "field00000.bin"
+    assert os.path.getsize(out / "field00001.bin") == 8 + 8 * (32 + 16 + 2) + 48 * 32 * 16
+    with pytest.raises(kpt.KptError, match="uniformly spaced"):
+        kpt.compact_field_read(str(out / "field00000.bin"))
+    f0 = kpt.compact_field_read(str(out / "field00000.bin"), allow_nonuniform=True)
+    f1 = kpt.compact_field_read(str(out / "field00001.bin"), allow_nonuniform=True)
+    np.testing.assert_array_equal(f0[1], W.stretched_edges(16))
+    np.testing.assert_array_equal(f0[2][0], f1[2][0])       # Bx = tanh(y/w) does not drift
+    assert not np.array_equal(f0[2][1], f1[2][1])            # the island chain does
+    xe, ye, planes = W.island_field(32, 16, y_edges=W.stretched_edges(16), frame=0)
+    np.testing.assert_array_equal(f0[2][1], planes[1])
+    make_fields.main([str(tmp_path / "u"), "--frames", "1", "--nx", "8", "--ny", "4", "--grid", "uniform"])
+    kpt.compact_field_read(str(tmp_path / "u" / "field00000.bin"))
