@@ -623,6 +623,108 @@ kpt_status kpt_apply_boundaries(kpt_ctx* ctx, int32_t slot, int32_t periodic_mas
 kpt_status kpt_particle_kinematics(kpt_ctx* ctx, const kpt_particle_properties* props, double* gamma,
                                    double* speed, double* kinetic_energy, uint64_t count);
 
+/* ---- generic particle generator (include/ParticleGenerator.hpp) ------------------
+ * Injection and recycling over the cells of a field slot. The slot's raw planes
+ * (Bx, By, Bz, Vx, Vy, Vz) are taken as cell-centred values of its grid, as the
+ * generic solvers do; cell areas are the products of the cell widths (the
+ * Cartesian Jacobian is 1). Each particle draws from its own Philox stream keyed
+ * by its id, a stream tag, the attempt and a per-context generator call counter,
+ * so results do not depend on the launch shape; the reference draws from its
+ * Kokkos random pool. New particles take fresh root ids. */
+
+/* FieldPredicateComparison (include/ParticleGenerator.hpp:117-124). */
+enum {
+    KPT_CMP_GREATER = 0,
+    KPT_CMP_GREATER_EQUAL = 1,
+    KPT_CMP_LESS = 2,
+    KPT_CMP_LESS_EQUAL = 3,
+    KPT_CMP_ABS_GREATER = 4,
+    KPT_CMP_ABS_LESS = 5
+};
+/* ParticleRecycleSelection (include/ParticleGenerator.hpp:129-133) over the
+ * statuses of ParticleStatus (include/ParticleSystem.hpp:35-39): 0 active,
+ * 1 escaped, 2 inactive. */
+enum { KPT_RECYCLE_ESCAPED = 0, KPT_RECYCLE_INACTIVE = 1, KPT_RECYCLE_ANY = 2 };
+
+/* ParticleInjectionKinematics (include/ParticleGenerator.hpp:67-74), plus the
+ * time and first step the driver's push reads. */
+typedef struct kpt_injection_kinematics {
+    double momentum;  /* |p|, finite and >= 0 */
+    double mu;        /* finite */
+    double weight;    /* finite and > 0 */
+    int32_t status;   /* 0, 1 or 2 */
+    int32_t pad;
+    double time;      /* t of the placed particles */
+    double step_dt;   /* their stored step (the push's first adaptive guess) */
+} kpt_injection_kinematics;
+
+/* AxisAlignedParticleRegion (include/ParticleGenerator.hpp:17-60): finite and
+ * ascending bounds; membership is inclusive, draws are uniform in it. */
+typedef struct kpt_region {
+    double x_lo, x_hi, y_lo, y_hi;
+} kpt_region;
+
+/* FieldThresholdParticlePredicate (include/ParticleGenerator.hpp:254-358):
+ * raw component `component` (0..5) of the containing cell compared with
+ * `threshold`; false outside the physical domain. */
+typedef struct kpt_field_predicate {
+    int32_t component;
+    int32_t comparison; /* KPT_CMP_* */
+    double threshold;
+} kpt_field_predicate;
+
+typedef struct kpt_candidates kpt_candidates;
+
+/* make_field_particle_candidate_set(_in_region) (include/ParticleGenerator.hpp:
+ * 658-784): the cells of `slot` whose centre lies in `region` (NULL: the
+ * physical domain) and whose value passes `pred`, in x-fastest order, with
+ * cumulative cell areas. An empty set is valid; injecting from it is not. */
+kpt_status kpt_candidates_from_field(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
+                                     const kpt_field_predicate* pred, kpt_candidates** out);
+/* make_grid_particle_candidate_set (include/ParticleGenerator.hpp:786-845): every
+ * physical cell of the slot's grid with cumulative areas. */
+kpt_status kpt_candidates_from_grid(kpt_ctx* ctx, int32_t slot, kpt_candidates** out);
+/* candidate_count() and the total weight. */
+kpt_status kpt_candidates_info(const kpt_candidates* c, uint64_t* count, double* total_weight);
+/* The accepted cell indices (i + nx j) and cumulative weights; either output
+ * may be NULL, each holds `count` values. */
+kpt_status kpt_candidates_download(kpt_ctx* ctx, const kpt_candidates* c, uint32_t* cells, double* cumulative);
+kpt_status kpt_candidates_destroy(kpt_candidates* c);
+
+/* inject_particles_from_field_candidates / _from_grid_candidates
+ * (include/ParticleGenerator.hpp:957-1117): append n particles, each at a
+ * candidate drawn by weight (sample_cumulative_weight_index, :436-460): its cell
+ * centre (field candidates) or a uniform point in its cell (grid candidates).
+ * e0 is the kinetic energy of `momentum` under `props` (NULL: defaults).
+ * ParticleInjectionPlan's capacity check (:846-870) applies. */
+kpt_status kpt_inject_from_candidates(kpt_ctx* ctx, const kpt_candidates* c, uint64_t n,
+                                      const kpt_injection_kinematics* k, const kpt_particle_properties* props,
+                                      uint64_t* injected);
+/* inject_particles_uniform(_if) (include/ParticleGenerator.hpp:875-955,
+ * :1122-1132): append n particles uniform in `region` (NULL: the slot's physical
+ * domain), each redrawn up to max_attempts times while `pred` (NULL: always
+ * true) fails; those that never pass keep their last draw with status 2
+ * (inactive) and are counted in `rejected`. */
+kpt_status kpt_inject_uniform_if(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
+                                 const kpt_field_predicate* pred, int32_t max_attempts, uint64_t n,
+                                 const kpt_injection_kinematics* k, const kpt_particle_properties* props,
+                                 uint64_t* injected, uint64_t* rejected);
+/* recycle_particles_uniform(_if) (include/ParticleGenerator.hpp:1176-1262):
+ * every stored particle whose status matches `selection` (KPT_RECYCLE_*) is
+ * placed anew as kpt_inject_uniform_if places one, keeping its slot and id. */
+kpt_status kpt_recycle_uniform_if(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
+                                  const kpt_field_predicate* pred, int32_t max_attempts, int32_t selection,
+                                  const kpt_injection_kinematics* k, const kpt_particle_properties* props,
+                                  uint64_t* recycled, uint64_t* rejected);
+/* recycle_particles_from_grid_candidates / _from_field_candidates
+ * (include/ParticleGenerator.hpp:1314-1436): the same from a candidate set. */
+kpt_status kpt_recycle_from_candidates(kpt_ctx* ctx, const kpt_candidates* c, int32_t selection,
+                                       const kpt_injection_kinematics* k, const kpt_particle_properties* props,
+                                       uint64_t* recycled);
+/* The generator call counter: 0 after kpt_create, +1 per inject or recycle call
+ * above; it is the third Philox counter word of the next call's draws. */
+kpt_status kpt_generator_calls(const kpt_ctx* ctx, uint32_t* calls);
+
 /* ---- learned pitch-angle closure (KPT_CLOSURE_MLP) ------------------------------
  * No counterpart in the reference. A small network stands in for the resonance
  * part of the slab-QLT D_mumu of LegencyModel::evaluate_pitch_angle_scattering

@@ -234,6 +234,63 @@ hipError_t launch_hist_xy(const ParticleArrays& P, uint64_t count, int bx, int b
                           hipStream_t stream);
 // Cell sort (kpt_sort.hip): keys/values double buffers of `capacity` u32 each
 // plus radix-sort temporary storage of sort_temp_bytes(capacity).
+// ---- generic particle generator (kpt_inject.hip) ----
+// Philox stream tags of the generator: counter word 1 is tag | attempt.
+constexpr uint32_t kTagInject = 0x494E0000u;   // "IN"
+constexpr uint32_t kTagRecycle = 0x52430000u;  // "RC"
+
+// The field slot a generator call reads: its grid, raw planes (cell-centred
+// Bx, By, Bz, Vx, Vy, Vz), the region (x_lo, x_hi, y_lo, y_hi) and the
+// field-threshold predicate.
+struct GeneratorSource {
+    GridDesc g;
+    const double* raw;
+    double region[4];
+    int32_t use_predicate;
+    int32_t component;
+    int32_t comparison;
+    int32_t max_attempts;
+    double threshold;
+};
+
+// A built candidate set on the device: accepted cells (nullptr: every cell,
+// candidate index = cell index) and cumulative cell areas; kind 0 places
+// particles at cell centres (field candidates), 1 uniformly in the cell (grid).
+struct CandidateView {
+    GridDesc g;
+    const uint32_t* cells;
+    const double* cumulative;
+    uint64_t count;
+    double total;
+    int32_t kind;
+};
+
+// Scratch of one candidate build over `cells` cells.
+struct CandidateBuild {
+    uint32_t* flags;
+    uint32_t* offsets;
+    double* area;
+    uint32_t* out_cells;
+    double* out_area;
+    void* temp;
+    size_t temp_bytes;
+};
+
+hipError_t candidate_scratch_bytes(uint64_t cells, size_t* bytes);
+hipError_t launch_candidates(const GeneratorSource& src, int use_predicate, CandidateBuild& w, hipStream_t stream);
+hipError_t launch_cumulative(const double* area, double* cumulative, uint64_t count, void* temp, size_t temp_bytes,
+                             hipStream_t stream);
+hipError_t launch_inject_candidates(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
+                                    const CandidateView& c, const kpt_injection_kinematics& k, double e0,
+                                    uint32_t call, const uint32_t key[2], hipStream_t stream);
+hipError_t launch_inject_uniform(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
+                                 const GeneratorSource& src, const kpt_injection_kinematics& k, double e0,
+                                 uint32_t call, const uint32_t key[2], unsigned long long* rejected,
+                                 hipStream_t stream);
+hipError_t launch_recycle(const ParticleArrays& P, uint64_t count, int selection, int mode, const CandidateView& c,
+                          const GeneratorSource& src, const kpt_injection_kinematics& k, double e0, uint32_t call,
+                          const uint32_t key[2], unsigned long long* counters, hipStream_t stream);
+
 struct SortScratch {
     uint32_t* keys[2];
     uint32_t* vals[2];

@@ -193,6 +193,32 @@ class EnergySplitPolicy(C.Structure):
     _fields_ = [("energy_split_ratio", C.c_double), ("minimum_child_weight", C.c_double)]
 
 
+# FieldPredicateComparison and ParticleRecycleSelection (include/ParticleGenerator.hpp:117-133)
+CMP_GREATER, CMP_GREATER_EQUAL, CMP_LESS, CMP_LESS_EQUAL, CMP_ABS_GREATER, CMP_ABS_LESS = range(6)
+RECYCLE_ESCAPED, RECYCLE_INACTIVE, RECYCLE_ANY = range(3)
+
+
+class InjectionKinematics(C.Structure):
+    """kpt_injection_kinematics (ParticleInjectionKinematics, include/ParticleGenerator.hpp:67-74,
+    plus the time and stored step the driver's push reads)."""
+    _fields_ = [("momentum", C.c_double), ("mu", C.c_double), ("weight", C.c_double), ("status", C.c_int32),
+                ("pad", C.c_int32), ("time", C.c_double), ("step_dt", C.c_double)]
+
+
+class Region(C.Structure):
+    """kpt_region (AxisAlignedParticleRegion, include/ParticleGenerator.hpp:17-60)."""
+    _fields_ = [("x_lo", C.c_double), ("x_hi", C.c_double), ("y_lo", C.c_double), ("y_hi", C.c_double)]
+
+
+class FieldPredicate(C.Structure):
+    """kpt_field_predicate (FieldThresholdParticlePredicate, include/ParticleGenerator.hpp:254-358)."""
+    _fields_ = [("component", C.c_int32), ("comparison", C.c_int32), ("threshold", C.c_double)]
+
+
+def injection_kinematics(momentum=1.0, mu=0.0, weight=1.0, status=0, time=0.0, step_dt=0.0) -> InjectionKinematics:
+    return InjectionKinematics(momentum, mu, weight, status, 0, time, step_dt)
+
+
 class SplitStatistics(C.Structure):
     """kpt_split_statistics (ParticleSplitStatistics, include/ParticleSystem.hpp:268-275)."""
     _fields_ = [("active_count", C.c_uint64), ("split_count", C.c_uint64),
@@ -367,6 +393,24 @@ EXPORTS = {
                                           C.POINTER(EnergySplitPolicy), C.POINTER(SplitStatistics)]),
     "kpt_reset_split_baselines": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties)]),
     "kpt_apply_boundaries": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint64)]),
+    "kpt_candidates_from_field": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Region), C.POINTER(FieldPredicate),
+                                            C.POINTER(C.c_void_p)]),
+    "kpt_candidates_from_grid": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "kpt_candidates_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "kpt_candidates_download": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "kpt_candidates_destroy": (C.c_int, [C.c_void_p]),
+    "kpt_inject_from_candidates": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(InjectionKinematics),
+                                             C.POINTER(ParticleProperties), C.POINTER(C.c_uint64)]),
+    "kpt_inject_uniform_if": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Region), C.POINTER(FieldPredicate), C.c_int32,
+                                        C.c_uint64, C.POINTER(InjectionKinematics), C.POINTER(ParticleProperties),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "kpt_recycle_uniform_if": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Region), C.POINTER(FieldPredicate),
+                                         C.c_int32, C.c_int32, C.POINTER(InjectionKinematics),
+                                         C.POINTER(ParticleProperties), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64)]),
+    "kpt_recycle_from_candidates": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(InjectionKinematics),
+                                              C.POINTER(ParticleProperties), C.POINTER(C.c_uint64)]),
+    "kpt_generator_calls": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "kpt_particle_kinematics": (C.c_int, [C.c_void_p, C.POINTER(ParticleProperties), C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]),
     "kpt_mlp_default_train_opts": (C.c_int, [C.POINTER(MlpTrainOpts)]),
@@ -399,6 +443,35 @@ class KptError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"kpt error {code}: {msg}")
         self.code = code
+
+
+class Candidates:
+    """A kpt_candidates set (FieldParticleCandidateSet / GridParticleCandidateSet)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        n, t = C.c_uint64(), C.c_double()
+        _check_static(ctx.lib.kpt_candidates_info(h, C.byref(n), C.byref(t)))
+        self.count, self.total = n.value, t.value
+
+    def download(self):
+        """(cell indices i + nx j, cumulative weights)"""
+        cells = np.empty(self.count, dtype=np.uint32)
+        cum = np.empty(self.count)
+        self.ctx._check(self.ctx.lib.kpt_candidates_download(
+            self.ctx.h, self.h, cells.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(cum)))
+        return cells, cum
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.kpt_candidates_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def default_host_settings() -> HostSettings:
@@ -724,6 +797,55 @@ class Context:
     def apply_boundaries(self, slot, periodic_mask) -> int:
         out = C.c_uint64()
         self._check(self.lib.kpt_apply_boundaries(self.h, slot, periodic_mask, C.byref(out)))
+        return out.value
+
+    # ---- generic particle generator (include/ParticleGenerator.hpp) ------------
+    def candidates_from_field(self, slot, pred, region=None) -> "Candidates":
+        """Field-threshold candidate set; pred = (component, comparison, threshold)."""
+        h = C.c_void_p()
+        fp = FieldPredicate(*pred)
+        self._check(self.lib.kpt_candidates_from_field(self.h, slot, C.byref(Region(*region)) if region else None,
+                                                       C.byref(fp), C.byref(h)))
+        return Candidates(self, h)
+
+    def candidates_from_grid(self, slot) -> "Candidates":
+        h = C.c_void_p()
+        self._check(self.lib.kpt_candidates_from_grid(self.h, slot, C.byref(h)))
+        return Candidates(self, h)
+
+    def inject_from_candidates(self, cands, n, kin, props=None) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.kpt_inject_from_candidates(self.h, cands.h, n, C.byref(kin),
+                                                        C.byref(props) if props else None, C.byref(out)))
+        return out.value
+
+    def inject_uniform_if(self, slot, n, kin, region=None, pred=None, max_attempts=64, props=None):
+        """(injected, rejected)"""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.kpt_inject_uniform_if(self.h, slot, C.byref(Region(*region)) if region else None,
+                                                   C.byref(FieldPredicate(*pred)) if pred else None, max_attempts,
+                                                   n, C.byref(kin), C.byref(props) if props else None,
+                                                   C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def recycle_uniform_if(self, slot, kin, selection, region=None, pred=None, max_attempts=64, props=None):
+        """(recycled, rejected)"""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.kpt_recycle_uniform_if(self.h, slot, C.byref(Region(*region)) if region else None,
+                                                    C.byref(FieldPredicate(*pred)) if pred else None, max_attempts,
+                                                    selection, C.byref(kin), C.byref(props) if props else None,
+                                                    C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def recycle_from_candidates(self, cands, kin, selection, props=None) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.kpt_recycle_from_candidates(self.h, cands.h, selection, C.byref(kin),
+                                                         C.byref(props) if props else None, C.byref(out)))
+        return out.value
+
+    def generator_calls(self) -> int:
+        out = C.c_uint32()
+        self._check(self.lib.kpt_generator_calls(self.h, C.byref(out)))
         return out.value
 
     def kinematics(self, props: "ParticleProperties", count=None):

@@ -1880,3 +1880,307 @@ double kpto_generic_focus_step(const void* field, const kpt_generic_focus_params
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Generic particle generator (include/ParticleGenerator.hpp): candidate sets over
+// the cells of a field (field threshold in a region, :463-784; every cell,
+// :545-845), injection from them (:957-1117), uniform injection with rejection
+// against a field-threshold predicate (:875-955) and recycling (:1176-1470).
+// Raw planes are cell-centred values; cell areas are products of cell widths.
+// Draws: Philox with counter (id, tag | attempt, call, id >> 32) and the seed
+// as key. Coordinates follow the library's arithmetic: explicit fma for the
+// analytic faces and draws, an unfused product for the analytic upper bound.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kGenTagInject = 0x494E0000u;
+constexpr uint32_t kGenTagRecycle = 0x52430000u;
+
+struct GenGrid {
+    int n[2];
+    double start[2], step[2], inv_step[2], lo[2], hi[2];
+    bool stretched[2];
+    const double* e[2];
+};
+
+double unfused_bound(double start, int n, double step) {
+    double t = static_cast<double>(n) * step;
+    asm volatile("" : "+x"(t));  // keeps the product apart from the sum
+    return start + t;
+}
+
+GenGrid gen_grid(int nx, int ny, const double* xe, const double* ye) {
+    GenGrid g{};
+    g.n[0] = nx;
+    g.n[1] = ny;
+    g.e[0] = xe;
+    g.e[1] = ye;
+    for (int d = 0; d < 2; ++d) {
+        const double* e = g.e[d];
+        g.start[d] = e[0];
+        g.step[d] = (e[g.n[d]] - e[0]) / static_cast<double>(g.n[d]);
+        g.inv_step[d] = 1.0 / g.step[d];
+        g.stretched[d] = g.n[d] > 1 && !faces_uniform(e, g.n[d] + 1);
+        g.lo[d] = g.stretched[d] ? e[0] : g.start[d];
+        g.hi[d] = g.stretched[d] ? e[g.n[d]] : unfused_bound(g.start[d], g.n[d], g.step[d]);
+    }
+    return g;
+}
+
+double gen_face(const GenGrid& g, int d, int k) {
+    return g.stretched[d] ? g.e[d][k] : std::fma(static_cast<double>(k), g.step[d], g.start[d]);
+}
+
+double gen_centre(const GenGrid& g, int d, int k) {
+    return g.stretched[d] ? 0.5 * (g.e[d][k] + g.e[d][k + 1])
+                          : std::fma(static_cast<double>(k) + 0.5, g.step[d], g.start[d]);
+}
+
+// grid.cell_index on the physical domain: the interval e_k <= q < e_(k+1),
+// clamped to the cells (q = e_n falls in the last one).
+int gen_cell_of(const GenGrid& g, int d, double q) {
+    const int n = g.n[d];
+    if (!g.stretched[d]) {
+        const int k = static_cast<int>(std::floor((q - g.start[d]) * g.inv_step[d]));
+        return std::min(std::max(k, 0), n - 1);
+    }
+    const double* e = g.e[d];
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (e[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+double gen_area(const GenGrid& g, int i, int j) {
+    const double a = (gen_face(g, 0, i + 1) - gen_face(g, 0, i)) * (gen_face(g, 1, j + 1) - gen_face(g, 1, j));
+    return std::isfinite(a) && a > 0.0 ? a : 0.0;
+}
+
+bool gen_compare(double v, double thr, int cmp) {
+    switch (cmp) {
+        case KPT_CMP_GREATER: return v > thr;
+        case KPT_CMP_GREATER_EQUAL: return v >= thr;
+        case KPT_CMP_LESS: return v < thr;
+        case KPT_CMP_LESS_EQUAL: return v <= thr;
+        case KPT_CMP_ABS_GREATER: return std::fabs(v) > thr;
+        case KPT_CMP_ABS_LESS: return std::fabs(v) < thr;
+    }
+    return false;
+}
+
+struct GenSource {
+    GenGrid g;
+    const double* raw;
+    double region[4];
+    int use_pred, comp, cmp, max_attempts;
+    double thr;
+};
+
+GenSource gen_source(const double* raw, int nx, int ny, const double* xe, const double* ye, const double* region,
+                     int use_pred, int comp, int cmp, double thr, int max_attempts) {
+    GenSource s{};
+    s.g = gen_grid(nx, ny, xe, ye);
+    s.raw = raw;
+    for (int q = 0; q < 4; ++q) s.region[q] = region ? region[q] : (q < 2 ? (q == 0 ? s.g.lo[0] : s.g.hi[0])
+                                                                         : (q == 2 ? s.g.lo[1] : s.g.hi[1]));
+    s.use_pred = use_pred;
+    s.comp = comp;
+    s.cmp = cmp;
+    s.thr = thr;
+    s.max_attempts = max_attempts;
+    return s;
+}
+
+bool gen_predicate(const GenSource& s, double x, double y) {
+    const GenGrid& g = s.g;
+    if (!(x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1])) return false;
+    const int i = gen_cell_of(g, 0, x), j = gen_cell_of(g, 1, y);
+    const size_t cells = static_cast<size_t>(g.n[0]) * g.n[1];
+    return gen_compare(s.raw[static_cast<size_t>(s.comp) * cells + static_cast<size_t>(j) * g.n[0] + i], s.thr,
+                       s.cmp);
+}
+
+void gen_draw(uint64_t id, uint32_t tag, uint32_t attempt, uint32_t call, uint64_t seed, uint32_t r[4]) {
+    const uint32_t ctr[4] = {static_cast<uint32_t>(id), tag | attempt, call, static_cast<uint32_t>(id >> 32)};
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    philox10(ctr, key, r);
+}
+
+bool gen_uniform_if(const GenSource& s, uint64_t id, uint32_t tag, uint32_t call, uint64_t seed, double& x,
+                    double& y) {
+    for (int a = 0; a < s.max_attempts; ++a) {
+        uint32_t r[4];
+        gen_draw(id, tag, static_cast<uint32_t>(a), call, seed, r);
+        x = std::fma(s.region[1] - s.region[0], u01(r[0]), s.region[0]);
+        y = std::fma(s.region[3] - s.region[2], u01(r[1]), s.region[2]);
+        if (!s.use_pred || gen_predicate(s, x, y)) return true;
+    }
+    return false;
+}
+
+// sample_cumulative_weight_index (include/ParticleGenerator.hpp:436-460).
+uint64_t gen_sample_index(const double* cum, uint64_t count, double total, double u) {
+    const double target = total * u;
+    uint64_t left = 0, right = count - 1;
+    while (left < right) {
+        const uint64_t mid = left + (right - left) / 2;
+        if (target < cum[mid]) right = mid;
+        else left = mid + 1;
+    }
+    return left;
+}
+
+void gen_candidate_position(int kind, const uint32_t* cells, const double* cum, uint64_t count, double total,
+                            const GenGrid& g, const uint32_t r[4], double& x, double& y) {
+    const uint64_t idx = gen_sample_index(cum, count, total, u01(r[0]));
+    const uint64_t cell = cells ? cells[idx] : idx;
+    const int i = static_cast<int>(cell % g.n[0]), j = static_cast<int>(cell / g.n[0]);
+    if (kind == 0) {
+        x = gen_centre(g, 0, i);
+        y = gen_centre(g, 1, j);
+    } else {
+        const double x0 = gen_face(g, 0, i), x1 = gen_face(g, 0, i + 1);
+        const double y0 = gen_face(g, 1, j), y1 = gen_face(g, 1, j + 1);
+        x = std::fma(x1 - x0, u01(r[1]), x0);
+        y = std::fma(y1 - y0, u01(r[2]), y0);
+    }
+}
+
+void gen_place(kpt_particles_host* P, uint64_t i, double x, double y, const kpt_injection_kinematics* k, double e0,
+               int status) {
+    P->x[i] = x;
+    P->y[i] = y;
+    P->p[i] = k->momentum;
+    P->mu[i] = k->mu;
+    P->weight[i] = k->weight;
+    P->t[i] = k->time;
+    P->step_dt[i] = k->step_dt;
+    P->status[i] = status;
+    P->split_level[i] = 0;
+    if (P->e0) P->e0[i] = e0;
+}
+
+bool gen_selected(int status, int selection) {
+    switch (selection) {
+        case KPT_RECYCLE_ESCAPED: return status == 1;
+        case KPT_RECYCLE_INACTIVE: return status == 2;
+        case KPT_RECYCLE_ANY: return status == 1 || status == 2;
+    }
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+// make_field_particle_candidate_set_in_region (include/ParticleGenerator.hpp:
+// 658-769): accepted cells in x-fastest order with their cumulative areas (a
+// sequential sum). region may be NULL (the physical domain). Returns the count.
+uint64_t kpto_candidates_field(const double* raw, int32_t nx, int32_t ny, const double* xe, const double* ye,
+                               const double* region, int32_t comp, int32_t cmp, double thr, uint32_t* cells,
+                               double* cumulative, double* total) {
+    const GenSource s = gen_source(raw, nx, ny, xe, ye, region, 1, comp, cmp, thr, 1);
+    const size_t n = static_cast<size_t>(nx) * ny;
+    uint64_t count = 0;
+    double sum = 0.0;
+    for (size_t c = 0; c < n; ++c) {
+        const int i = static_cast<int>(c % nx), j = static_cast<int>(c / nx);
+        const double cx = gen_centre(s.g, 0, i), cy = gen_centre(s.g, 1, j);
+        if (!(cx >= s.region[0] && cx <= s.region[1] && cy >= s.region[2] && cy <= s.region[3])) continue;
+        if (!gen_compare(raw[static_cast<size_t>(comp) * n + c], thr, cmp)) continue;
+        sum += gen_area(s.g, i, j);
+        cells[count] = static_cast<uint32_t>(c);
+        cumulative[count] = sum;
+        ++count;
+    }
+    *total = sum;
+    return count;
+}
+
+// make_grid_particle_candidate_set (include/ParticleGenerator.hpp:786-845).
+void kpto_candidates_grid(int32_t nx, int32_t ny, const double* xe, const double* ye, double* cumulative,
+                          double* total) {
+    const GenGrid g = gen_grid(nx, ny, xe, ye);
+    double sum = 0.0;
+    for (int64_t c = 0; c < static_cast<int64_t>(nx) * ny; ++c) {
+        sum += gen_area(g, static_cast<int>(c % nx), static_cast<int>(c / nx));
+        cumulative[c] = sum;
+    }
+    *total = sum;
+}
+
+// inject_particles_from_field_candidates / _from_grid_candidates
+// (include/ParticleGenerator.hpp:957-1117) into slots first .. first + n - 1;
+// kind 0 places at cell centres, 1 uniformly in the cell. cells may be NULL
+// (every cell).
+void kpto_generator_inject_candidates(int32_t kind, const uint32_t* cells, const double* cumulative, uint64_t count,
+                                      double total, int32_t nx, int32_t ny, const double* xe, const double* ye,
+                                      uint64_t n, const kpt_injection_kinematics* k, double e0, uint64_t seed,
+                                      uint32_t call, uint64_t first_root, kpt_particles_host* P, uint64_t first) {
+    const GenGrid g = gen_grid(nx, ny, xe, ye);
+    for (uint64_t m = 0; m < n; ++m) {
+        const uint64_t root = first_root + m;
+        uint32_t r[4];
+        gen_draw(root, kGenTagInject, 0, call, seed, r);
+        double x, y;
+        gen_candidate_position(kind, cells, cumulative, count, total, g, r, x, y);
+        gen_place(P, first + m, x, y, k, e0, k->status);
+        if (P->id) P->id[first + m] = root;
+    }
+}
+
+// inject_particles_uniform_if (include/ParticleGenerator.hpp:875-955). region
+// may be NULL (the physical domain). Returns the rejected count.
+uint64_t kpto_generator_inject_uniform(const double* raw, int32_t nx, int32_t ny, const double* xe, const double* ye,
+                                       const double* region, int32_t use_pred, int32_t comp, int32_t cmp, double thr,
+                                       int32_t max_attempts, uint64_t n, const kpt_injection_kinematics* k, double e0,
+                                       uint64_t seed, uint32_t call, uint64_t first_root, kpt_particles_host* P,
+                                       uint64_t first) {
+    const GenSource s = gen_source(raw, nx, ny, xe, ye, region, use_pred, comp, cmp, thr, max_attempts);
+    uint64_t rejected = 0;
+    for (uint64_t m = 0; m < n; ++m) {
+        const uint64_t root = first_root + m;
+        double x = 0.0, y = 0.0;
+        const bool ok = gen_uniform_if(s, root, kGenTagInject, call, seed, x, y);
+        gen_place(P, first + m, x, y, k, e0, ok ? k->status : 2);
+        if (P->id) P->id[first + m] = root;
+        if (!ok) ++rejected;
+    }
+    return rejected;
+}
+
+// recycle_particles_uniform_if / _from_*_candidates (include/ParticleGenerator.hpp:
+// 1176-1470): mode 0 from the candidate set, 1 uniform with the predicate. ids
+// are kept. Returns the recycled count; *rejected those left inactive.
+uint64_t kpto_generator_recycle(int32_t mode, int32_t kind, const uint32_t* cells, const double* cumulative,
+                                uint64_t count, double total, const double* raw, int32_t nx, int32_t ny,
+                                const double* xe, const double* ye, const double* region, int32_t use_pred,
+                                int32_t comp, int32_t cmp, double thr, int32_t max_attempts, int32_t selection,
+                                const kpt_injection_kinematics* k, double e0, uint64_t seed, uint32_t call,
+                                kpt_particles_host* P, uint64_t stored, uint64_t* rejected) {
+    const GenSource s = gen_source(raw, nx, ny, xe, ye, region, use_pred, comp, cmp, thr, max_attempts);
+    uint64_t recycled = 0, rej = 0;
+    for (uint64_t i = 0; i < stored; ++i) {
+        if (!gen_selected(P->status[i], selection)) continue;
+        ++recycled;
+        const uint64_t id = P->id[i];
+        double x = 0.0, y = 0.0;
+        if (mode == 0) {
+            uint32_t r[4];
+            gen_draw(id, kGenTagRecycle, 0, call, seed, r);
+            gen_candidate_position(kind, cells, cumulative, count, total, s.g, r, x, y);
+            gen_place(P, i, x, y, k, e0, k->status);
+        } else {
+            const bool ok = gen_uniform_if(s, id, kGenTagRecycle, call, seed, x, y);
+            gen_place(P, i, x, y, k, e0, ok ? k->status : 2);
+            if (!ok) ++rej;
+        }
+    }
+    if (rejected) *rejected = rej;
+    return recycled;
+}
+
+}  // extern "C"

@@ -74,6 +74,23 @@ def load():
             "kpto_reset_split_baselines": (None, [P(kpt.ParticleProperties), P(kpt.ParticlesHost), C.c_uint64]),
             "kpto_apply_boundaries": (C.c_uint64, [C.c_int32, C.c_int32, dp, dp, C.c_int32,
                                                    P(kpt.ParticlesHost), C.c_uint64]),
+            "kpto_candidates_field": (C.c_uint64, [dp, C.c_int32, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32,
+                                                   C.c_double, P(C.c_uint32), dp, dp]),
+            "kpto_candidates_grid": (None, [C.c_int32, C.c_int32, dp, dp, dp, dp]),
+            "kpto_generator_inject_candidates": (None, [C.c_int32, P(C.c_uint32), dp, C.c_uint64, C.c_double,
+                                                        C.c_int32, C.c_int32, dp, dp, C.c_uint64,
+                                                        P(kpt.InjectionKinematics), C.c_double, C.c_uint64,
+                                                        C.c_uint32, C.c_uint64, P(kpt.ParticlesHost), C.c_uint64]),
+            "kpto_generator_inject_uniform": (C.c_uint64, [dp, C.c_int32, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32,
+                                                           C.c_int32, C.c_double, C.c_int32, C.c_uint64,
+                                                           P(kpt.InjectionKinematics), C.c_double, C.c_uint64,
+                                                           C.c_uint32, C.c_uint64, P(kpt.ParticlesHost),
+                                                           C.c_uint64]),
+            "kpto_generator_recycle": (C.c_uint64, [C.c_int32, C.c_int32, P(C.c_uint32), dp, C.c_uint64, C.c_double,
+                                                    dp, C.c_int32, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32,
+                                                    C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                                    P(kpt.InjectionKinematics), C.c_double, C.c_uint64, C.c_uint32,
+                                                    P(kpt.ParticlesHost), C.c_uint64, P(C.c_uint64)]),
             "kpto_generic_default_params": (None, [P(kpt.GenericParkerParams)]),
             "kpto_generic_focus_default_params": (None, [P(kpt.GenericFocusParams)]),
             "kpto_generic_focus_create": (C.c_void_p, [C.c_int32, C.c_int32, dp, dp, P(dp), dp,
@@ -215,6 +232,86 @@ def apply_boundaries(x_edges, y_edges, mask, parts, count):
     ye = np.ascontiguousarray(y_edges, dtype=np.float64)
     return load().kpto_apply_boundaries(len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye), mask,
                                         C.byref(kpt.particles_struct(parts)), count)
+
+
+# ---- generic particle generator (include/ParticleGenerator.hpp) ---------------
+def _raw(planes):
+    return np.ascontiguousarray(np.stack([np.asarray(p, dtype=np.float64) for p in planes]))
+
+
+def _u32p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _region(region):
+    return None if region is None else np.ascontiguousarray(region, dtype=np.float64)
+
+
+def candidates_field(planes, x_edges, y_edges, pred, region=None):
+    """(cells, cumulative weights, total) of the field-threshold candidate set;
+    pred = (component, comparison, threshold)."""
+    raw = _raw(planes)
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    n = (len(xe) - 1) * (len(ye) - 1)
+    cells, cum, tot = np.empty(n, dtype=np.uint32), np.empty(n), C.c_double()
+    reg = _region(region)
+    m = load().kpto_candidates_field(_dp(raw), len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye),
+                                     None if reg is None else _dp(reg), pred[0], pred[1], pred[2], _u32p(cells),
+                                     _dp(cum), C.byref(tot))
+    return cells[:m].copy(), cum[:m].copy(), tot.value
+
+
+def candidates_grid(x_edges, y_edges):
+    """(cumulative cell areas, total) over every cell, x fastest."""
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    cum, tot = np.empty((len(xe) - 1) * (len(ye) - 1)), C.c_double()
+    load().kpto_candidates_grid(len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye), _dp(cum), C.byref(tot))
+    return cum, tot.value
+
+
+def generator_inject_candidates(kind, cells, cum, total, x_edges, y_edges, n, kin, e0, seed, call, first_root,
+                                parts, first):
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    cum = np.ascontiguousarray(cum, dtype=np.float64)
+    cells = None if cells is None else np.ascontiguousarray(cells, dtype=np.uint32)
+    load().kpto_generator_inject_candidates(kind, _u32p(cells), _dp(cum), len(cum), total, len(xe) - 1,
+                                            len(ye) - 1, _dp(xe), _dp(ye), n, C.byref(kin), e0, seed, call,
+                                            first_root, C.byref(kpt.particles_struct(parts)), first)
+
+
+def generator_inject_uniform(planes, x_edges, y_edges, region, pred, max_attempts, n, kin, e0, seed, call,
+                             first_root, parts, first):
+    raw = _raw(planes)
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    reg = _region(region)
+    p = pred or (0, 0, 0.0)
+    return load().kpto_generator_inject_uniform(_dp(raw), len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye),
+                                                None if reg is None else _dp(reg), int(pred is not None), p[0], p[1],
+                                                p[2], max_attempts, n, C.byref(kin), e0, seed, call, first_root,
+                                                C.byref(kpt.particles_struct(parts)), first)
+
+
+def generator_recycle(mode, kin, e0, seed, call, parts, stored, selection, x_edges, y_edges, kind=0, cells=None,
+                      cum=None, total=0.0, planes=None, region=None, pred=None, max_attempts=64):
+    """(recycled, rejected); mode 0 from the candidate set, 1 uniform with the predicate."""
+    xe = np.ascontiguousarray(x_edges, dtype=np.float64)
+    ye = np.ascontiguousarray(y_edges, dtype=np.float64)
+    raw = _raw(planes) if planes is not None else np.zeros(1)
+    cum = np.ascontiguousarray(cum if cum is not None else np.zeros(1), dtype=np.float64)
+    cells = None if cells is None else np.ascontiguousarray(cells, dtype=np.uint32)
+    reg = _region(region)
+    p = pred or (0, 0, 0.0)
+    rej = C.c_uint64()
+    rec = load().kpto_generator_recycle(mode, kind, _u32p(cells), _dp(cum), len(cum) if cum is not None else 0,
+                                        total, _dp(raw), len(xe) - 1, len(ye) - 1, _dp(xe), _dp(ye),
+                                        None if reg is None else _dp(reg), int(pred is not None), p[0], p[1], p[2],
+                                        max_attempts, selection, C.byref(kin), e0, seed, call,
+                                        C.byref(kpt.particles_struct(parts)), stored, C.byref(rej))
+    return rec, rej.value
 
 
 def histogram_logp(parts, count, bins, pmin, pmax):
