@@ -189,10 +189,22 @@ def run_ensemble(args):
             e.sort()
             chosen, table = e.autotune(0, window=0.2)
             log(f"  autotune: chosen {chosen}; tried {table}")
+            every = max(1, min(args.sort_every, frames))
+
+            def loop(frame):
+                """A cell sort of every run, then `every` intervals from the graph, until `frames`
+                intervals are done. Returns (per-run stats summed, kernel ms)."""
+                tot, kms = None, 0.0
+                for k in range(0, frames, every):
+                    e.sort()
+                    st, ms = e.push(frame + k, frames=min(every, frames - k), graph=True)
+                    kms += ms
+                    tot = st if tot is None else [{f: a[f] + b[f] for f in a} for a, b in zip(tot, st)]
+                return tot, kms
+
             frame = 0
             for _ in range(args.warmup):
-                e.sort()
-                st, ms = e.push(frame, frames=frames, graph=True)
+                st, ms = loop(frame)
                 log(f"  warmup frames {frame}..{frame + frames - 1}: {sum(x['particle_steps'] for x in st)} steps, "
                     f"{ms:.1f} ms")
                 frame += frames
@@ -201,8 +213,7 @@ def run_ensemble(args):
             bytes_alg = 0.0
             t0 = time.perf_counter()
             for _ in range(args.steps):
-                e.sort()  # part of the step: the cell order of every run, then the frame loop
-                st, ms = e.push(frame, frames=frames, graph=True)
+                st, ms = loop(frame)  # the cell sorts are part of the step
                 frame += frames
                 kms += ms
                 for spec, x in zip(specs, st):
@@ -244,6 +255,7 @@ def run_ensemble(args):
             "tuning": chosen,
             "autotune_table": table,
             "graph": True,
+            "sort_every_intervals": every,
             "mlp_training": {k: report[k] for k in ("steps", "train_ms", "val_rms_rel_d")},
         },
         "roofline": {
@@ -287,6 +299,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["focused", "parker", "ensemble"], default="focused")
     ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
+    ap.add_argument("--sort-every", type=int, default=1,
+                    help="ensemble: intervals between batched cell sorts of every run")
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's size")
     ap.add_argument("--grid", choices=["stretched", "uniform"], default="stretched",
                     help="focused / parker: y faces of the field")

@@ -2288,7 +2288,8 @@ struct kpt_ensemble {
     bool backup_allocated = false;
     kpt::ParticleArrays P2{};  // cell sort target, allocated on first use
     bool p2_allocated = false;
-    kpt::SortScratch sort{};
+    kpt::SortScratch sort{};       // sized for all slots (one batched sort)
+    kpt::SortRun* d_sort_runs = nullptr;
     kpt::EnsembleRun* d_runs = nullptr;
     unsigned long long* d_counters = nullptr;  // 4 per run
     uint2* d_tiles = nullptr;
@@ -2555,6 +2556,7 @@ kpt_status kpt_ensemble_destroy(kpt_ensemble* e) {
         dev_free(e->sort.vals[k]);
     }
     dev_free(e->sort.temp);
+    dev_free(e->d_sort_runs);
     for (FieldSlot& f : e->field) {
         dev_free(f.raw);
         dev_free(f.c64);
@@ -2842,24 +2844,49 @@ extern "C" kpt_status kpt_ensemble_sort(kpt_ensemble* e, uint64_t* active) {
         if (!e->set[r]) return fail(ctx, KPT_ERR_STATE, "ensemble run " + std::to_string(r) + ": settings are not set.");
     }
     if (!e->p2_allocated) {
-        const uint64_t maxcap = *std::max_element(e->cap.begin(), e->cap.end());
+        const uint64_t n = std::max<uint64_t>(e->total_cap, 1);
         KPT_HIP(ctx, alloc_arrays(e->P2, e->total_cap));
         for (int k = 0; k < 2; ++k) {
-            KPT_HIP(ctx, dev_alloc(&e->sort.keys[k], std::max<uint64_t>(maxcap, 1)));
-            KPT_HIP(ctx, dev_alloc(&e->sort.vals[k], std::max<uint64_t>(maxcap, 1)));
+            KPT_HIP(ctx, dev_alloc(&e->sort.keys[k], n));
+            KPT_HIP(ctx, dev_alloc(&e->sort.vals[k], n));
         }
-        e->sort.temp_bytes = kpt::sort_temp_bytes(std::max<uint64_t>(maxcap, 1));
+        e->sort.temp_bytes = kpt::sort_temp_bytes(n);
         KPT_HIP(ctx, hipMalloc(&e->sort.temp, std::max<size_t>(e->sort.temp_bytes, 1)));
+        KPT_HIP(ctx, dev_alloc(&e->d_sort_runs, static_cast<size_t>(e->runs)));
         e->p2_allocated = true;
     }
     unsigned long long* act_dev = ctx->counters + 4;
     KPT_HIP(ctx, hipMemsetAsync(act_dev, 0, sizeof(unsigned long long), ctx->stream));
-    for (int r = 0; r < e->runs; ++r) {
-        if (e->count[r] == 0) continue;
-        const kpt_device_settings& s = e->s[r];
-        KPT_HIP(ctx, kpt::launch_sort(arrays_at(e->P, e->off[r]), arrays_at(e->P2, e->off[r]), e->count[r],
-                                      e->field[r].g, s.interpolation_shift_x, s.interpolation_shift_y,
-                                      std::numeric_limits<double>::infinity(), e->sort, act_dev, ctx->stream));
+    // One batched sort when (run, node key) fits 32 bits: key bits of the largest
+    // grid plus the two classes after its nodes, above them the run index.
+    uint32_t limit = 0;
+    for (int r = 0; r < e->runs; ++r) limit = std::max(limit, kpt::sort_key_limit(e->field[r].g));
+    uint32_t run_shift = 1;
+    while (run_shift < 32 && (1ull << run_shift) < static_cast<uint64_t>(limit) + 2) ++run_shift;
+    uint32_t run_bits = 0;
+    while ((1ull << run_bits) < static_cast<uint64_t>(e->runs)) ++run_bits;
+    if (run_shift + run_bits <= 32 && e->total_cap <= 0x7fffffffull) {
+        std::vector<kpt::SortRun> sr(static_cast<size_t>(e->runs));
+        for (int r = 0; r < e->runs; ++r) {
+            sr[r].g = e->field[r].g;
+            sr[r].shift_x = e->s[r].interpolation_shift_x;
+            sr[r].shift_y = e->s[r].interpolation_shift_y;
+            sr[r].off = e->off[r];
+            sr[r].count = e->count[r];
+        }
+        KPT_HIP(ctx, hipMemcpyAsync(e->d_sort_runs, sr.data(), sr.size() * sizeof(kpt::SortRun),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_ensemble_sort(e->d_sort_runs, e->runs, e->total_cap, run_shift, e->P, e->P2,
+                                               e->sort, act_dev, ctx->stream));
+        KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `sr` is read by the copy
+    } else {
+        for (int r = 0; r < e->runs; ++r) {
+            if (e->count[r] == 0) continue;
+            const kpt_device_settings& s = e->s[r];
+            KPT_HIP(ctx, kpt::launch_sort(arrays_at(e->P, e->off[r]), arrays_at(e->P2, e->off[r]), e->count[r],
+                                          e->field[r].g, s.interpolation_shift_x, s.interpolation_shift_y,
+                                          std::numeric_limits<double>::infinity(), e->sort, act_dev, ctx->stream));
+        }
     }
     unsigned long long act = 0;
     KPT_HIP(ctx, hipMemcpyAsync(&act, act_dev, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));

@@ -291,6 +291,14 @@ hipError_t launch_recycle(const ParticleArrays& P, uint64_t count, int selection
                           const GeneratorSource& src, const kpt_injection_kinematics& k, double e0, uint32_t call,
                           const uint32_t key[2], unsigned long long* counters, hipStream_t stream);
 
+// One run of a batched ensemble sort: its grid and interpolation shifts, its
+// slot range start and its stored count.
+struct SortRun {
+    GridDesc g;
+    double shift_x, shift_y;
+    uint64_t off, count;
+};
+
 struct SortScratch {
     uint32_t* keys[2];
     uint32_t* vals[2];
@@ -298,6 +306,13 @@ struct SortScratch {
     size_t temp_bytes;
 };
 size_t sort_temp_bytes(uint64_t capacity);
+// Node keys of a grid lie below this (16 x 16 blocks over the stored nodes).
+uint32_t sort_key_limit(const GridDesc& g);
+// Cell sort of every run of a batched ensemble in one key pass, one radix sort
+// of all `total` slots and one gather (src -> dst); `runs` is a device array.
+hipError_t launch_ensemble_sort(const SortRun* runs, int n_runs, uint64_t total, uint32_t run_shift,
+                                const ParticleArrays& src, const ParticleArrays& dst, SortScratch& w,
+                                unsigned long long* active, hipStream_t stream);
 // Writes the sorted particle set into `dst`; `active` (device) accumulates the
 // number of status-0 particles.
 // Particles with t >= t_done (finished their interval) sort after the active

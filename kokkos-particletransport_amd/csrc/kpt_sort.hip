@@ -27,6 +27,24 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kSortBlock = 16;  // nodes per side of a key block
 
+// Blocked node key of an active particle at (x, y): the stencil's lower-left node
+// (I, J) over the stored nodes, in 16 x 16 blocks.
+__device__ __forceinline__ uint32_t node_key(const GridDesc& g, double x, double y, double shift_x, double shift_y) {
+    const double q[2] = {x + shift_x, y + shift_y};
+    const int n[2] = {g.nx, g.ny};
+    int idx[2];
+    for (int d = 0; d < 2; ++d) {
+        int i0;
+        double w, h;
+        // NaN positions sort first; the push marks them
+        idx[d] = node_stencil(q[d], g, d, n[d], i0, w, h) ? i0 : -1;
+    }
+    const uint32_t I = static_cast<uint32_t>(idx[0] + 1), J = static_cast<uint32_t>(idx[1] + 1);
+    const uint32_t blocks_x = (static_cast<uint32_t>(g.nx) + 2 + kSortBlock - 1) / kSortBlock;
+    return (((J / kSortBlock) * blocks_x + I / kSortBlock) * kSortBlock + J % kSortBlock) * kSortBlock +
+           I % kSortBlock;
+}
+
 __global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint64_t count, GridDesc g,
                                                           double shift_x, double shift_y, double t_done,
                                                           uint32_t inactive_key,
@@ -41,19 +59,7 @@ __global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint
             key = inactive_key - 1;  // finished this interval: after the moving ones
         } else if (P.status[i] == 0) {
             act = true;
-            const double q[2] = {P.x[i] + shift_x, P.y[i] + shift_y};
-            const int n[2] = {g.nx, g.ny};
-            int idx[2];
-            for (int d = 0; d < 2; ++d) {
-                int i0;
-                double w, h;
-                // NaN positions sort first; the push marks them
-                idx[d] = node_stencil(q[d], g, d, n[d], i0, w, h) ? i0 : -1;
-            }
-            const uint32_t I = static_cast<uint32_t>(idx[0] + 1), J = static_cast<uint32_t>(idx[1] + 1);
-            const uint32_t blocks_x = (static_cast<uint32_t>(g.nx) + 2 + kSortBlock - 1) / kSortBlock;
-            key = (((J / kSortBlock) * blocks_x + I / kSortBlock) * kSortBlock + J % kSortBlock) * kSortBlock +
-                  I % kSortBlock;
+            key = node_key(g, P.x[i], P.y[i], shift_x, shift_y);
         }
         keys[i] = key;
         vals[i] = static_cast<uint32_t>(i);
@@ -87,7 +93,68 @@ int key_bits(uint32_t max_key) {
     return b;
 }
 
+// Keys of all slots of a batched ensemble: run r's slots [off, off + cap) get
+// (r << run_shift) | node key for active particles, | (2^run_shift - 2) for
+// inactive ones and | (2^run_shift - 1) for slots past the run's count. A stable
+// sort of all slots then keeps every run inside its own slot range and orders
+// it exactly as the run's own cell sort would.
+__global__ void __launch_bounds__(kBlock) ensemble_key_kernel(const SortRun* __restrict__ runs, int n_runs,
+                                                              uint64_t total, ParticleArrays P, uint32_t run_shift,
+                                                              uint32_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ vals,
+                                                              unsigned long long* __restrict__ active) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    bool act = false;
+    if (i < total) {
+        int lo = 0, hi = n_runs - 1;  // last run with off <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (runs[mid].off <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const SortRun& r = runs[lo];
+        const uint32_t top = (1u << run_shift) - 1u;
+        uint32_t key = top;
+        if (i - r.off < r.count) {
+            if (P.status[i] == 0) {
+                act = true;
+                key = node_key(r.g, P.x[i], P.y[i], r.shift_x, r.shift_y);
+            } else {
+                key = top - 1u;
+            }
+        }
+        keys[i] = (static_cast<uint32_t>(lo) << run_shift) | key;
+        vals[i] = static_cast<uint32_t>(i);
+    }
+    const uint64_t m = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(active, static_cast<unsigned long long>(__popcll(m)));
+}
+
 }  // namespace
+
+uint32_t sort_key_limit(const GridDesc& g) {
+    const uint32_t bx = (static_cast<uint32_t>(g.nx) + 2 + kSortBlock - 1) / kSortBlock;
+    const uint32_t by = (static_cast<uint32_t>(g.ny) + 2 + kSortBlock - 1) / kSortBlock;
+    return bx * by * kSortBlock * kSortBlock;
+}
+
+hipError_t launch_ensemble_sort(const SortRun* runs, int n_runs, uint64_t total, uint32_t run_shift,
+                                const ParticleArrays& src, const ParticleArrays& dst, SortScratch& w,
+                                unsigned long long* active, hipStream_t stream) {
+    if (total == 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>((total + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(ensemble_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, runs, n_runs, total, src,
+                       run_shift, w.keys[0], w.vals[0], active);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipcub::DoubleBuffer<uint32_t> k(w.keys[0], w.keys[1]), v(w.vals[0], w.vals[1]);
+    size_t bytes = w.temp_bytes;
+    const uint32_t top = static_cast<uint32_t>(n_runs - 1) << run_shift | ((1u << run_shift) - 1u);
+    e = hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, k, v, static_cast<int>(total), 0, key_bits(top), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, dst, total, v.Current());
+    return hipGetLastError();
+}
 
 size_t sort_temp_bytes(uint64_t capacity) {
     size_t bytes = 0;
