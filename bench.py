@@ -92,12 +92,16 @@ def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0):
         steps, _ = kpt_oracle.push_interval(lo, hi, s64, 0.0, 0, 7, parts, threads=threads)
         return steps, time.perf_counter() - t0
 
+    # grow the sample until it takes about target_seconds (small samples scale
+    # poorly over the threads, so one extrapolation undershoots)
     m = 1024
     steps, dt = run(m)
-    rate = steps / max(dt, 1e-9)
-    per_particle = steps / m
-    m = int(min(len(start["x"]), max(m, target_seconds * rate / max(per_particle, 1.0))))
-    steps, dt = run(m)
+    for _ in range(4):
+        if dt >= 0.6 * target_seconds or m >= len(start["x"]):
+            break
+        per_particle_s = dt / m
+        m = int(min(len(start["x"]), max(2 * m, target_seconds / max(per_particle_s, 1e-12))))
+        steps, dt = run(m)
     what = ("src/main.cpp:1082-1327" if model == kpt.MODEL_PARKER16 else
             "src/main.cpp:1332-1755 with the slab-QLT closure of include/LegencyModel.hpp:539-689 "
             "and the Picard mu update of include/FocusSolver.hpp:700-719")
