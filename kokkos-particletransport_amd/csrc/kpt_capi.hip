@@ -4,7 +4,10 @@
 // FP32, noise table, scratch for split / histogram / summary), validates
 // arguments the way the reference driver does (src/main.cpp:187-240,
 // include/AthenaFieldReader.hpp:86-270) and launches the kernels of
-// kpt_kernels.hip on the context's stream. Every call is synchronous.
+// kpt_kernels.hip on the context's stream. Every call is synchronous. The
+// batched ensemble and the generic particle generator have their own files
+// (kpt_ensemble_capi.hip, kpt_generator_capi.hip); kpt_capi_internal.h holds
+// the context type they share.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,134 +22,17 @@
 #include <thread>
 #include <vector>
 
-#include "kpt_internal.h"
+#include "kpt_capi_internal.h"
 
 namespace {
 
 thread_local std::string g_create_error;
 
-struct FieldSlot {
-    bool valid = false;
-    int model = 0;
-    kpt::GridDesc g{};
-    double* raw = nullptr;  // 6 planes of nx*ny (Bx, By, Bz, Vx, Vy, Vz)
-    double* c64 = nullptr;
-    float* c32 = nullptr;
-    size_t nodes_cap = 0;
-    size_t raw_cap = 0;
-    uint64_t version = 0;  // changes with every upload into the slot
-    // generic Parker basis built from raw (kpt_generic_parker_build)
-    double* gen = nullptr;
-    size_t gen_cap = 0;
-    bool gen_valid = false;
-    // generic focused basis built from raw (kpt_generic_focus_build)
-    double* foc = nullptr;
-    size_t foc_cap = 0;
-    bool foc_valid = false;
-    // stretched axes: face coordinates and bin table on the device
-    double* face_dev[2] = {nullptr, nullptr};
-    kpt::FaceBin* bins_dev[2] = {nullptr, nullptr};
-    size_t face_cap[2] = {0, 0};
-    size_t bins_cap[2] = {0, 0};
-    // host copies of the tables, alive until the next fill of this slot (the
-    // asynchronous copies read them)
-    std::vector<double> host_face[2];
-    std::vector<kpt::FaceBin> host_bins[2];
-};
-
-// Background load of the next frame (kpt_field_stage_compact): a host thread
-// reads and validates the file into pinned memory and copies it to `slot` on
-// `stream`, while the caller's pushes run on the context's stream; the advance
-// packs it on the context's stream.
-struct FieldStage {
-    FieldSlot slot;
-    std::thread worker;
-    bool busy = false;              // a stage was started and not yet finished
-    hipStream_t stream = nullptr;   // default priority, like the context's stream
-    hipEvent_t ready = nullptr;     // context-stream work queued before the stage
-    hipEvent_t done = nullptr;      // the staged frame is packed
-    double* pinned = nullptr;       // six planes, page-locked
-    size_t pinned_cap = 0;          // doubles
-    kpt_status code = KPT_OK;
-    std::string err;
-    double read_seconds = 0.0;
-};
-
 }  // namespace
-
-struct kpt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t capacity = 0;
-    uint64_t count = 0;
-    uint64_t seed = 0;
-    uint64_t next_root = 0;
-    uint32_t generator_calls = 0;  // Philox counter word of the generic generator's draws
-    kpt::ParticleArrays P{};
-    FieldSlot slot[2];
-    FieldStage stage;
-    uint64_t field_versions = 0;
-    // FP32 frame-interleaved pair of the lower and upper frames (built by the
-    // FP32 time-blended push when either slot changed)
-    float* pair32 = nullptr;
-    size_t pair32_cap = 0;
-    uint64_t pair_versions[2] = {0, 0};
-    // learned closure: FP32 parameters (device and host copies), hull, MFMA image
-    bool mlp_valid = false;
-    kpt_mlp_hull mlp_hull{};
-    float* mlp_params = nullptr;
-    void* mlp_image = nullptr;
-    std::vector<float> mlp_host;
-    double* noise = nullptr;
-    uint64_t noise_particles = 0;
-    uint32_t noise_steps = 0;
-    unsigned long long* counters = nullptr;  // 8 x u64
-    uint32_t* split_scratch = nullptr;
-    size_t split_scratch_cap = 0;
-    double* hist_partials = nullptr;
-    double* hist_out = nullptr;
-    void* summary_partials = nullptr;
-    kpt_summary* summary_out = nullptr;
-    // cell sort: second particle set (allocated on first sort) and scratch
-    kpt::ParticleArrays P2{};
-    bool p2_allocated = false;
-    kpt::SortScratch sort{};
-    std::string err;
-    char device_name[64] = {0};
-};
-
-namespace {
-
-kpt_status fail(kpt_ctx* ctx, kpt_status code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-kpt_status hip_fail(kpt_ctx* ctx, hipError_t e, const char* what) {
-    return fail(ctx, KPT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define KPT_HIP(ctx, call)                                  \
-    do {                                                    \
-        const hipError_t e_ = (call);                       \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
-    } while (0)
-
-template <typename T>
-hipError_t dev_alloc(T** p, size_t elems) {
-    *p = nullptr;
-    if (elems == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void**>(p), elems * sizeof(T));
-}
-
-void dev_free(void* p) {
-    if (p) (void)hipFree(p);
-}
 
 // The spacing test of validate_uniform_face_edges (include/AthenaFieldReader.hpp:
 // 167-193): all intervals equal the mean interval to 1024 ulp of the scale.
-bool faces_uniform(const double* edges, size_t count) {
+bool kpt_capi::faces_uniform(const double* edges, size_t count) {
     const double span = edges[count - 1] - edges[0];
     const double step = span / static_cast<double>(count - 1);
     for (size_t i = 1; i < count; ++i) {
@@ -156,6 +42,9 @@ bool faces_uniform(const double* edges, size_t count) {
     }
     return true;
 }
+
+namespace {
+
 
 // A stretched axis (non-uniform faces) spans exactly [e_0, e_n]; a uniform one
 // keeps the analytic bounds start + k step.
@@ -255,7 +144,9 @@ bool check_edges(const double* edges, size_t count, const char* axis, bool requi
     return true;
 }
 
-kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
+}  // namespace
+
+kpt_status kpt_capi::check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
     if (!s) return fail(ctx, KPT_ERR_ARG, "settings pointer is null.");
     const bool ok = std::isfinite(s->mhd_dt) && s->mhd_dt > 0.0 && std::isfinite(s->p0) && s->p0 > 0.0 &&
                     std::isfinite(s->dt_min) && s->dt_min > 0.0 && std::isfinite(s->dt_max) &&
@@ -284,6 +175,9 @@ kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s) {
     }
     return KPT_OK;
 }
+
+namespace {
+
 
 kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_done, bool compact,
                          uint64_t* active);
@@ -468,8 +362,10 @@ kpt_status fill_slot(FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const 
     return KPT_OK;
 }
 
+}  // namespace
+
 // fill_slot on the context's stream, waited for: the slot is ready on return.
-kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
+kpt_status kpt_capi::upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
                        const double* ye, const double* const planes[6]) {
     std::string err;
     const kpt_status st = fill_slot(f, model, nx, ny, xe, ye, planes, ctx->stream, true, err);
@@ -479,6 +375,9 @@ kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, in
     f.valid = true;
     return KPT_OK;
 }
+
+namespace {
+
 
 kpt_status upload_field(kpt_ctx* ctx, int32_t slot, int32_t model, int32_t nx, int32_t ny,
                         const double* xe, const double* ye, const double* const planes[6]) {
@@ -1976,10 +1875,8 @@ kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params
 
 // ---- ParticleSystem lifecycle -------------------------------------------------
 
-namespace {
-
 // ParticleProperties::validate (include/ParticleSystem.hpp:155-165).
-kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr) {
+kpt_status kpt_capi::check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr) {
     if (!pr) return fail(ctx, KPT_ERR_ARG, "particle properties pointer is null.");
     if (!std::isfinite(pr->rest_mass) || pr->rest_mass <= 0.0 || !std::isfinite(pr->charge) ||
         !std::isfinite(pr->speed_of_light) || pr->speed_of_light <= 0.0 || !std::isfinite(pr->energy_scale_erg) ||
@@ -1989,6 +1886,9 @@ kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr) {
                     "charge must be finite, and the energy scale must be positive and finite.");
     return KPT_OK;
 }
+
+namespace {
+
 
 }  // namespace
 
@@ -2272,987 +2172,3 @@ kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* 
 
 }  // extern "C"
 
-// ---- batched ensemble -----------------------------------------------------------
-
-struct kpt_ensemble {
-    kpt_ctx* ctx = nullptr;
-    int runs = 0;
-    int cu_count = 256;
-    std::vector<uint64_t> cap, off, count, next_root, seed;
-    std::vector<kpt_device_settings> s;
-    std::vector<char> set;
-    std::vector<FieldSlot> field;
-    uint64_t total_cap = 0;
-    kpt::ParticleArrays P{};
-    kpt::ParticleArrays B{};  // autotune backup, allocated on first use
-    bool backup_allocated = false;
-    kpt::ParticleArrays P2{};  // cell sort target, allocated on first use
-    bool p2_allocated = false;
-    kpt::SortScratch sort{};       // sized for all slots (one batched sort)
-    kpt::SortRun* d_sort_runs = nullptr;
-    kpt::EnsembleRun* d_runs = nullptr;
-    unsigned long long* d_counters = nullptr;  // 4 per run
-    uint2* d_tiles = nullptr;
-    size_t tiles_cap = 0;
-    uint32_t* d_gstart = nullptr;  // 9 entries
-    unsigned int* d_queues = nullptr;  // 8 entries
-    kpt::EnsembleLoop* d_loop = nullptr;
-    bool dirty = true;       // run table or tiles need a rebuild
-    bool has_mlp = false;
-    uint32_t n_tiles = 0;
-    kpt_ensemble_tuning tune{0, 8, 0.0};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    // kernel arguments the captured graph was recorded with
-    const void* g_tiles = nullptr;
-    int g_groups = 0, g_grid = 0, g_mlp = -1;
-};
-
-namespace {
-
-kpt::ParticleArrays arrays_at(const kpt::ParticleArrays& P, uint64_t o) {
-    kpt::ParticleArrays v = P;
-    v.x += o;
-    v.y += o;
-    v.p += o;
-    v.t += o;
-    v.step_dt += o;
-    v.weight += o;
-    v.mu += o;
-    v.status += o;
-    v.split_level += o;
-    v.id += o;
-    v.e0 += o;
-    v.substep += o;
-    return v;
-}
-
-hipError_t alloc_arrays(kpt::ParticleArrays& P, uint64_t n) {
-    hipError_t e;
-    if ((e = dev_alloc(&P.x, n)) != hipSuccess || (e = dev_alloc(&P.y, n)) != hipSuccess ||
-        (e = dev_alloc(&P.p, n)) != hipSuccess || (e = dev_alloc(&P.t, n)) != hipSuccess ||
-        (e = dev_alloc(&P.step_dt, n)) != hipSuccess || (e = dev_alloc(&P.weight, n)) != hipSuccess ||
-        (e = dev_alloc(&P.mu, n)) != hipSuccess || (e = dev_alloc(&P.status, n)) != hipSuccess ||
-        (e = dev_alloc(&P.split_level, n)) != hipSuccess || (e = dev_alloc(&P.id, n)) != hipSuccess ||
-        (e = dev_alloc(&P.e0, n)) != hipSuccess || (e = dev_alloc(&P.substep, n)) != hipSuccess)
-        return e;
-    return hipSuccess;
-}
-
-void free_arrays(kpt::ParticleArrays& P) {
-    for (void* q : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p), static_cast<void*>(P.t),
-                    static_cast<void*>(P.step_dt), static_cast<void*>(P.weight), static_cast<void*>(P.mu),
-                    static_cast<void*>(P.status), static_cast<void*>(P.split_level), static_cast<void*>(P.id),
-                    static_cast<void*>(P.e0), static_cast<void*>(P.substep)})
-        dev_free(q);
-    P = kpt::ParticleArrays{};
-}
-
-hipError_t copy_arrays(const kpt::ParticleArrays& dst, const kpt::ParticleArrays& src, uint64_t n, hipStream_t st) {
-    hipError_t e = hipSuccess;
-    auto cp = [&](void* d, const void* s, size_t bytes) {
-        if (e == hipSuccess && bytes) e = hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st);
-    };
-    cp(dst.x, src.x, n * 8);
-    cp(dst.y, src.y, n * 8);
-    cp(dst.p, src.p, n * 8);
-    cp(dst.t, src.t, n * 8);
-    cp(dst.step_dt, src.step_dt, n * 8);
-    cp(dst.weight, src.weight, n * 8);
-    cp(dst.mu, src.mu, n * 8);
-    cp(dst.status, src.status, n * 4);
-    cp(dst.split_level, src.split_level, n * 4);
-    cp(dst.id, src.id, n * 8);
-    cp(dst.e0, src.e0, n * 8);
-    cp(dst.substep, src.substep, n * 4);
-    return e;
-}
-
-kpt_status ens_run_ok(kpt_ensemble* e, int32_t run) {
-    if (!e) return KPT_ERR_ARG;
-    if (run < 0 || run >= e->runs) return fail(e->ctx, KPT_ERR_ARG, "ensemble run index out of range.");
-    return KPT_OK;
-}
-
-void drop_graph(kpt_ensemble* e) {
-    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->graph_exec = nullptr;
-    e->graph = nullptr;
-}
-
-// Run table and tile lists from the runs' fields, settings and counts.
-kpt_status ens_sync(kpt_ensemble* e) {
-    if (!e->dirty) return KPT_OK;
-    kpt_ctx* ctx = e->ctx;
-    std::vector<kpt::EnsembleRun> table(static_cast<size_t>(e->runs));
-    e->has_mlp = false;
-    for (int r = 0; r < e->runs; ++r) {
-        const std::string tag = "ensemble run " + std::to_string(r) + ": ";
-        if (!e->set[r]) return fail(ctx, KPT_ERR_STATE, tag + "settings are not set.");
-        const FieldSlot& f = e->field[r];
-        if (!f.valid) return fail(ctx, KPT_ERR_STATE, tag + "field is not loaded.");
-        const kpt_device_settings& s = e->s[r];
-        kpt::EnsembleRun& R = table[r];
-        kpt::PushArgs& a = R.a;
-        if (f.model == KPT_MODEL_PARKER16) {
-            if (s.focused_transport != 0) return fail(ctx, KPT_ERR_ARG, tag + "focused transport needs a FOCUSED21 field.");
-            R.kind = kpt::kEnsParker32;
-        } else {
-            if (s.focused_transport == 0 || (s.closure != KPT_CLOSURE_QLT && s.closure != KPT_CLOSURE_MLP))
-                return fail(ctx, KPT_ERR_ARG,
-                            tag + "a FOCUSED21 run must be focused transport with the QLT or learned closure.");
-            R.kind = s.closure == KPT_CLOSURE_MLP ? kpt::kEnsFocusedMlp32 : kpt::kEnsFocusedQlt32;
-        }
-        a.P = arrays_at(e->P, e->off[r]);
-        a.count = e->count[r];
-        a.field_lo = a.field_hi = f.c32;
-        a.field_pair = nullptr;
-        a.field_lo64 = a.field_hi64 = f.c64;
-        a.plane = kpt::coefficient_plane(f.g.nx, f.g.ny);
-        a.g = f.g;
-        a.s = s;
-        a.t0 = a.t_end = a.t_stop = 0.0;  // set on the device per interval
-        a.first_chunk = 1;
-        a.key[0] = static_cast<uint32_t>(e->seed[r]);
-        a.key[1] = static_cast<uint32_t>(e->seed[r] >> 32);
-        a.noise = nullptr;
-        a.noise_steps = 0;
-        a.qlt_norm = s.qlt_spectral_index * std::sin(M_PI / s.qlt_spectral_index) / M_PI;
-        a.k64 = kpt::make_step_const<double>(s, a.qlt_norm);
-        a.k32 = kpt::make_step_const<float>(s, a.qlt_norm);
-        a.mlp_image = nullptr;
-        a.mlp_on = 0;
-        if (R.kind == kpt::kEnsFocusedMlp32) {
-            if (!ctx->mlp_valid)
-                return fail(ctx, KPT_ERR_STATE, tag + "no pitch-angle network is loaded (kpt_mlp_train / kpt_mlp_set).");
-            const kpt_mlp_hull& h = ctx->mlp_hull;
-            a.mlp_image = ctx->mlp_image;
-            a.mlp_scale = kpt::mlp_scale(h);
-            a.mlp_on = (s.qlt_spectral_index >= h.gamma_lo && s.qlt_spectral_index <= h.gamma_hi && s.qlt_h0 == h.h0)
-                           ? 1 : 0;
-        }
-        if (R.kind == kpt::kEnsFocusedMlp32) e->has_mlp = true;
-        a.counters = e->d_counters + 4 * static_cast<size_t>(r);
-        R.pad = 0;
-    }
-    // tiles of 256 particles; groups: one queue, or one per XCD with the runs
-    // assigned longest-first to the least-loaded queue
-    const int G = e->tune.groups;
-    std::vector<std::vector<uint2>> per(G);
-    std::vector<int> order(e->runs);
-    for (int r = 0; r < e->runs; ++r) order[r] = r;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->count[a] > e->count[b]; });
-    std::vector<uint64_t> load(G, 0);
-    for (int r : order) {
-        const uint64_t nt = (e->count[r] + 255) / 256;
-        if (nt == 0) continue;
-        int gmin = 0;
-        for (int g = 1; g < G; ++g)
-            if (load[g] < load[gmin]) gmin = g;
-        load[gmin] += nt;
-        for (uint64_t k = 0; k < nt; ++k) per[gmin].push_back(make_uint2(static_cast<uint32_t>(r), static_cast<uint32_t>(256 * k)));
-    }
-    std::vector<uint2> tiles;
-    std::vector<uint32_t> gstart(9, 0);
-    for (int g = 0; g < G; ++g) {
-        gstart[g] = static_cast<uint32_t>(tiles.size());
-        tiles.insert(tiles.end(), per[g].begin(), per[g].end());
-    }
-    for (int g = G; g <= 8; ++g) gstart[g] = static_cast<uint32_t>(tiles.size());
-    e->n_tiles = static_cast<uint32_t>(tiles.size());
-    if (tiles.size() > e->tiles_cap) {
-        drop_graph(e);
-        dev_free(e->d_tiles);
-        e->d_tiles = nullptr;
-        e->tiles_cap = 0;
-        KPT_HIP(ctx, dev_alloc(&e->d_tiles, std::max<size_t>(tiles.size(), 1)));
-        e->tiles_cap = std::max<size_t>(tiles.size(), 1);
-    }
-    if (!tiles.empty())
-        KPT_HIP(ctx, hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    KPT_HIP(ctx, hipMemcpy(e->d_gstart, gstart.data(), 9 * sizeof(uint32_t), hipMemcpyHostToDevice));
-    KPT_HIP(ctx, hipMemcpy(e->d_runs, table.data(), table.size() * sizeof(kpt::EnsembleRun), hipMemcpyHostToDevice));
-    e->dirty = false;
-    return KPT_OK;
-}
-
-int ens_grid(const kpt_ensemble* e) { return e->tune.grid_blocks > 0 ? e->tune.grid_blocks : 2 * e->cu_count; }
-
-// One interval of every run: advance the run table, then the persistent push.
-hipError_t ens_launch(kpt_ensemble* e) {
-    kpt_ctx* ctx = e->ctx;
-    hipError_t st = kpt::launch_ensemble_advance(e->d_runs, e->runs, e->d_loop, e->d_queues, e->tune.groups, ctx->stream);
-    if (st != hipSuccess) return st;
-    return kpt::launch_ensemble(e->d_runs, e->d_tiles, e->d_gstart, e->tune.groups, e->d_queues, ens_grid(e),
-                                ctx->mlp_image, e->has_mlp ? 1 : 0, ctx->stream);
-}
-
-kpt_status ens_set_loop(kpt_ensemble* e, uint32_t first_frame, double window) {
-    kpt::EnsembleLoop L{first_frame, 0u, window};
-    KPT_HIP(e->ctx, hipMemcpy(e->d_loop, &L, sizeof(L), hipMemcpyHostToDevice));
-    return KPT_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-kpt_status kpt_ensemble_create(kpt_ctx* ctx, int32_t runs, const uint64_t* capacities, kpt_ensemble** out) {
-    if (!ctx || !out || !capacities) return KPT_ERR_ARG;
-    *out = nullptr;
-    if (runs <= 0 || runs > 65536) return fail(ctx, KPT_ERR_ARG, "ensemble run count must be in [1, 65536].");
-    (void)hipSetDevice(ctx->device);
-    kpt_ensemble* e = new kpt_ensemble();
-    e->ctx = ctx;
-    e->runs = runs;
-    int cu = 0;
-    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cu > 0)
-        e->cu_count = cu;
-    e->cap.assign(capacities, capacities + runs);
-    e->off.resize(runs);
-    e->count.assign(runs, 0);
-    e->next_root.assign(runs, 0);
-    e->seed.assign(runs, 0);
-    e->s.resize(runs);
-    e->set.assign(runs, 0);
-    e->field.resize(runs);
-    uint64_t total = 0;
-    for (int r = 0; r < runs; ++r) {
-        if (e->cap[r] > (1ull << 32)) {
-            delete e;
-            return fail(ctx, KPT_ERR_ARG, "ensemble run capacity must be at most 2^32.");
-        }
-        e->off[r] = total;
-        total += e->cap[r];
-    }
-    e->total_cap = total;
-    hipError_t er = alloc_arrays(e->P, total);
-    if (er == hipSuccess) er = dev_alloc(&e->d_runs, static_cast<size_t>(runs));
-    if (er == hipSuccess) er = dev_alloc(&e->d_counters, 4 * static_cast<size_t>(runs));
-    if (er == hipSuccess) er = dev_alloc(&e->d_gstart, 9);
-    if (er == hipSuccess) er = dev_alloc(&e->d_queues, 8);
-    if (er == hipSuccess) er = dev_alloc(&e->d_loop, 1);
-    if (er == hipSuccess && total) er = hipMemset(e->P.e0, 0, total * 8);
-    if (er == hipSuccess && total) er = hipMemset(e->P.substep, 0, total * 4);
-    if (er != hipSuccess) {
-        kpt_ensemble_destroy(e);
-        return hip_fail(ctx, er, "ensemble allocation");
-    }
-    *out = e;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_destroy(kpt_ensemble* e) {
-    if (!e) return KPT_OK;
-    (void)hipSetDevice(e->ctx->device);
-    (void)hipStreamSynchronize(e->ctx->stream);
-    drop_graph(e);
-    free_arrays(e->P);
-    if (e->backup_allocated) free_arrays(e->B);
-    if (e->p2_allocated) free_arrays(e->P2);
-    for (int k = 0; k < 2; ++k) {
-        dev_free(e->sort.keys[k]);
-        dev_free(e->sort.vals[k]);
-    }
-    dev_free(e->sort.temp);
-    dev_free(e->d_sort_runs);
-    for (FieldSlot& f : e->field) {
-        dev_free(f.raw);
-        dev_free(f.c64);
-        dev_free(f.c32);
-        dev_free(f.gen);
-        dev_free(f.foc);
-    }
-    for (void* q : {static_cast<void*>(e->d_runs), static_cast<void*>(e->d_counters), static_cast<void*>(e->d_tiles),
-                    static_cast<void*>(e->d_gstart), static_cast<void*>(e->d_queues), static_cast<void*>(e->d_loop)})
-        dev_free(q);
-    delete e;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_field_upload(kpt_ensemble* e, int32_t run, int32_t model, int32_t nx, int32_t ny,
-                                     const double* xe, const double* ye, const double* bx, const double* by,
-                                     const double* bz, const double* vx, const double* vy, const double* vz) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    kpt_ctx* ctx = e->ctx;
-    if (!xe || !ye || !bx || !by || !bz || !vx || !vy || !vz) return fail(ctx, KPT_ERR_ARG, "field array pointer is null.");
-    const size_t cells = static_cast<size_t>(std::max(nx, 0)) * std::max(ny, 0);
-    for (const double* plane : {bx, by, bz, vx, vy, vz})
-        for (size_t k = 0; k < cells; ++k)
-            if (!std::isfinite(plane[k])) return fail(ctx, KPT_ERR_ARG, "AthenaFieldReader: non-finite field value.");
-    for (const double* ed : {xe, ye}) {
-        const int n = ed == xe ? nx : ny;
-        if (n > 1 && !faces_uniform(ed, static_cast<size_t>(n) + 1))
-            return fail(ctx, KPT_ERR_ARG, "ensemble runs use uniform grids.");
-    }
-    const double* planes[6] = {bx, by, bz, vx, vy, vz};
-    (void)hipSetDevice(ctx->device);
-    e->dirty = true;
-    return upload_into(ctx, e->field[run], model, nx, ny, xe, ye, planes);
-}
-
-kpt_status kpt_ensemble_set_run(kpt_ensemble* e, int32_t run, const kpt_device_settings* s, uint64_t seed) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    st = check_settings(e->ctx, s);
-    if (st != KPT_OK) return st;
-    if (s->precision != KPT_PREC_FP32) return fail(e->ctx, KPT_ERR_ARG, "ensemble runs use the FP32 push.");
-    e->s[run] = *s;
-    e->s[run].time_interpolation = 0;  // one frame per run
-    e->s[run].push_chunks = 1;
-    e->seed[run] = seed;
-    e->set[run] = 1;
-    e->dirty = true;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_particles_upload(kpt_ensemble* e, int32_t run, const kpt_particles_host* h, uint64_t count) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    kpt_ctx* ctx = e->ctx;
-    if (!h) return KPT_ERR_ARG;
-    if (count > e->cap[run]) return fail(ctx, KPT_ERR_CAPACITY, "particle count exceeds the run's capacity.");
-    if (count > 0 && (!h->x || !h->y || !h->p || !h->t || !h->step_dt || !h->weight || !h->mu || !h->status ||
-                      !h->split_level))
-        return fail(ctx, KPT_ERR_ARG, "particle array pointer is null.");
-    (void)hipSetDevice(ctx->device);
-    const kpt::ParticleArrays P = arrays_at(e->P, e->off[run]);
-    uint64_t max_root = 0;
-    if (count > 0) {
-        KPT_HIP(ctx, hipMemcpy(P.x, h->x, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.y, h->y, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.p, h->p, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.t, h->t, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.step_dt, h->step_dt, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.weight, h->weight, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.mu, h->mu, count * 8, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.status, h->status, count * 4, hipMemcpyHostToDevice));
-        KPT_HIP(ctx, hipMemcpy(P.split_level, h->split_level, count * 4, hipMemcpyHostToDevice));
-        if (h->e0)
-            KPT_HIP(ctx, hipMemcpy(P.e0, h->e0, count * 8, hipMemcpyHostToDevice));
-        else
-            KPT_HIP(ctx, hipMemset(P.e0, 0, count * 8));
-        std::vector<uint64_t> ids(count);
-        for (uint64_t i = 0; i < count; ++i) ids[i] = h->id ? h->id[i] : i;
-        for (uint64_t i = 0; i < count; ++i) max_root = std::max<uint64_t>(max_root, (ids[i] & 0xFFFFFFFFull) + 1);
-        KPT_HIP(ctx, hipMemcpy(P.id, ids.data(), count * 8, hipMemcpyHostToDevice));
-    }
-    e->count[run] = count;
-    e->next_root[run] = max_root;
-    e->dirty = true;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_particles_download(kpt_ensemble* e, int32_t run, kpt_particles_host* h, uint64_t count) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    kpt_ctx* ctx = e->ctx;
-    if (!h) return KPT_ERR_ARG;
-    if (count > e->count[run]) return fail(ctx, KPT_ERR_ARG, "download count exceeds stored particles.");
-    if (count == 0) return KPT_OK;
-    (void)hipSetDevice(ctx->device);
-    const kpt::ParticleArrays P = arrays_at(e->P, e->off[run]);
-    if (h->x) KPT_HIP(ctx, hipMemcpy(h->x, P.x, count * 8, hipMemcpyDeviceToHost));
-    if (h->y) KPT_HIP(ctx, hipMemcpy(h->y, P.y, count * 8, hipMemcpyDeviceToHost));
-    if (h->p) KPT_HIP(ctx, hipMemcpy(h->p, P.p, count * 8, hipMemcpyDeviceToHost));
-    if (h->t) KPT_HIP(ctx, hipMemcpy(h->t, P.t, count * 8, hipMemcpyDeviceToHost));
-    if (h->step_dt) KPT_HIP(ctx, hipMemcpy(h->step_dt, P.step_dt, count * 8, hipMemcpyDeviceToHost));
-    if (h->weight) KPT_HIP(ctx, hipMemcpy(h->weight, P.weight, count * 8, hipMemcpyDeviceToHost));
-    if (h->mu) KPT_HIP(ctx, hipMemcpy(h->mu, P.mu, count * 8, hipMemcpyDeviceToHost));
-    if (h->status) KPT_HIP(ctx, hipMemcpy(h->status, P.status, count * 4, hipMemcpyDeviceToHost));
-    if (h->split_level) KPT_HIP(ctx, hipMemcpy(h->split_level, P.split_level, count * 4, hipMemcpyDeviceToHost));
-    if (h->id) KPT_HIP(ctx, hipMemcpy(h->id, P.id, count * 8, hipMemcpyDeviceToHost));
-    if (h->e0) KPT_HIP(ctx, hipMemcpy(h->e0, P.e0, count * 8, hipMemcpyDeviceToHost));
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_count(kpt_ensemble* e, int32_t run, uint64_t* count) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    if (!count) return KPT_ERR_ARG;
-    *count = e->count[run];
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_inject(kpt_ensemble* e, int32_t run, uint64_t n, double t0, uint32_t frame,
-                               uint64_t* injected) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    kpt_ctx* ctx = e->ctx;
-    if (!e->set[run]) return fail(ctx, KPT_ERR_STATE, "ensemble run settings are not set.");
-    if (injected) *injected = 0;
-    if (e->count[run] >= e->cap[run]) return KPT_OK;
-    const uint64_t m = std::min(n, e->cap[run] - e->count[run]);
-    (void)hipSetDevice(ctx->device);
-    const uint32_t key[2] = {static_cast<uint32_t>(e->seed[run]), static_cast<uint32_t>(e->seed[run] >> 32)};
-    const kpt_inject_opts o{};
-    KPT_HIP(ctx, kpt::launch_inject(arrays_at(e->P, e->off[run]), e->count[run], m, e->next_root[run], e->s[run], o, t0,
-                                    frame, key, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    e->count[run] += m;
-    e->next_root[run] += m;
-    e->dirty = true;
-    if (injected) *injected = m;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_set_tuning(kpt_ensemble* e, const kpt_ensemble_tuning* t) {
-    if (!e || !t) return KPT_ERR_ARG;
-    if (t->groups != 1 && t->groups != 8) return fail(e->ctx, KPT_ERR_ARG, "ensemble queue groups must be 1 or 8.");
-    if (t->grid_blocks < 0 || t->grid_blocks > (1 << 20))
-        return fail(e->ctx, KPT_ERR_ARG, "ensemble grid must be in [0, 2^20] workgroups.");
-    if (t->groups != e->tune.groups) e->dirty = true;
-    e->tune.grid_blocks = t->grid_blocks;
-    e->tune.groups = t->groups;
-    e->tune.ms = t->ms;
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_push(kpt_ensemble* e, uint32_t first_frame, uint32_t frames, int32_t use_graph,
-                             kpt_ensemble_run_stats* per_run, double* kernel_ms) {
-    if (!e) return KPT_ERR_ARG;
-    kpt_ctx* ctx = e->ctx;
-    (void)hipSetDevice(ctx->device);
-    kpt_status st = ens_sync(e);
-    if (st != KPT_OK) return st;
-    KPT_HIP(ctx, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(unsigned long long) * e->runs, ctx->stream));
-    st = ens_set_loop(e, first_frame, 1.0);
-    if (st != KPT_OK) return st;
-    if (use_graph && frames > 0) {
-        const int grid = ens_grid(e), mlp = e->has_mlp ? 1 : 0;
-        if (!e->graph_exec || e->g_tiles != e->d_tiles || e->g_groups != e->tune.groups || e->g_grid != grid ||
-            e->g_mlp != mlp) {
-            drop_graph(e);
-            KPT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t le = ens_launch(e);
-            hipGraph_t g = nullptr;
-            const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-            if (le != hipSuccess || ce != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                return hip_fail(ctx, le != hipSuccess ? le : ce, "ensemble graph capture");
-            }
-            e->graph = g;
-            KPT_HIP(ctx, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
-            e->g_tiles = e->d_tiles;
-            e->g_groups = e->tune.groups;
-            e->g_grid = grid;
-            e->g_mlp = mlp;
-        }
-    }
-    KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (uint32_t k = 0; k < frames; ++k) {
-        if (use_graph)
-            KPT_HIP(ctx, hipGraphLaunch(e->graph_exec, ctx->stream));
-        else
-            KPT_HIP(ctx, ens_launch(e));
-    }
-    KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    std::vector<unsigned long long> c(4 * static_cast<size_t>(e->runs));
-    KPT_HIP(ctx, hipMemcpyAsync(c.data(), e->d_counters, c.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0.0f;
-    KPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    if (kernel_ms) *kernel_ms = ms;
-    if (per_run)
-        for (int r = 0; r < e->runs; ++r) {
-            per_run[r].particle_steps = c[4 * r];
-            per_run[r].failures = c[4 * r + 1];
-            per_run[r].active_in = c[4 * r + 2];
-        }
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_autotune(kpt_ensemble* e, uint32_t frame, double window, kpt_ensemble_tuning* chosen,
-                                 kpt_ensemble_tuning* table, int32_t table_len, int32_t* tried) {
-    if (!e) return KPT_ERR_ARG;
-    kpt_ctx* ctx = e->ctx;
-    if (!(window > 0.0 && window <= 1.0)) return fail(ctx, KPT_ERR_ARG, "autotune window must be in (0, 1].");
-    (void)hipSetDevice(ctx->device);
-    kpt_status st = ens_sync(e);
-    if (st != KPT_OK) return st;
-    if (!e->backup_allocated) {
-        KPT_HIP(ctx, alloc_arrays(e->B, e->total_cap));
-        e->backup_allocated = true;
-    }
-    KPT_HIP(ctx, copy_arrays(e->B, e->P, e->total_cap, ctx->stream));
-    const kpt_ensemble_tuning saved = e->tune;
-    std::vector<kpt_ensemble_tuning> cands;
-    for (int groups : {8, 1})
-        for (int mult : {1, 2, 4}) cands.push_back(kpt_ensemble_tuning{mult * e->cu_count, groups, 0.0});
-    int best = -1;
-    for (size_t c = 0; c <= cands.size(); ++c) {
-        // candidate 0 runs twice: the first pass warms the caches and is not kept
-        const size_t ci = c == 0 ? 0 : c - 1;
-        st = kpt_ensemble_set_tuning(e, &cands[ci]);
-        if (st == KPT_OK) st = ens_sync(e);
-        if (st == KPT_OK) st = ens_set_loop(e, frame, window);
-        if (st != KPT_OK) break;
-        KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        KPT_HIP(ctx, ens_launch(e));
-        KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        float ms = 0.0f;
-        KPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        KPT_HIP(ctx, copy_arrays(e->P, e->B, e->total_cap, ctx->stream));
-        if (c == 0) continue;
-        cands[ci].ms = ms;
-        if (best < 0 || ms < cands[best].ms) best = static_cast<int>(ci);
-    }
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (st != KPT_OK) {
-        (void)kpt_ensemble_set_tuning(e, &saved);
-        return st;
-    }
-    st = kpt_ensemble_set_tuning(e, &cands[best]);
-    if (st != KPT_OK) return st;
-    if (chosen) *chosen = cands[best];
-    const int n = std::min<int>(static_cast<int>(cands.size()), std::max(table_len, 0));
-    for (int k = 0; table && k < n; ++k) table[k] = cands[k];
-    if (tried) *tried = static_cast<int32_t>(cands.size());
-    return KPT_OK;
-}
-
-kpt_status kpt_ensemble_histogram_logp(kpt_ensemble* e, int32_t run, int32_t bins, double pmin, double pmax,
-                                       double* out) {
-    kpt_status st = ens_run_ok(e, run);
-    if (st != KPT_OK) return st;
-    kpt_ctx* ctx = e->ctx;
-    if (!out) return KPT_ERR_ARG;
-    if (bins <= 0 || bins > 4096) return fail(ctx, KPT_ERR_ARG, "histogram bins must be in [1, 4096].");
-    if (!(pmin > 0.0) || !(pmax > pmin) || !std::isfinite(pmax))
-        return fail(ctx, KPT_ERR_ARG, "histogram range must satisfy 0 < pmin < pmax.");
-    const double log_pmin = std::log(pmin);
-    const double inv_dlogp = static_cast<double>(bins) / (std::log(pmax) - log_pmin);
-    (void)hipSetDevice(ctx->device);
-    KPT_HIP(ctx, kpt::launch_hist_logp(arrays_at(e->P, e->off[run]), e->count[run], bins, log_pmin, inv_dlogp, pmin,
-                                       pmax, ctx->hist_partials, ctx->hist_out, ctx->stream));
-    KPT_HIP(ctx, hipMemcpyAsync(out, ctx->hist_out, bins * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return KPT_OK;
-}
-
-}  // extern "C"
-
-extern "C" kpt_status kpt_ensemble_sort(kpt_ensemble* e, uint64_t* active) {
-    if (!e) return KPT_ERR_ARG;
-    kpt_ctx* ctx = e->ctx;
-    (void)hipSetDevice(ctx->device);
-    for (int r = 0; r < e->runs; ++r) {
-        if (!e->field[r].valid) return fail(ctx, KPT_ERR_STATE, "ensemble run " + std::to_string(r) + ": field is not loaded.");
-        if (!e->set[r]) return fail(ctx, KPT_ERR_STATE, "ensemble run " + std::to_string(r) + ": settings are not set.");
-    }
-    if (!e->p2_allocated) {
-        const uint64_t n = std::max<uint64_t>(e->total_cap, 1);
-        KPT_HIP(ctx, alloc_arrays(e->P2, e->total_cap));
-        for (int k = 0; k < 2; ++k) {
-            KPT_HIP(ctx, dev_alloc(&e->sort.keys[k], n));
-            KPT_HIP(ctx, dev_alloc(&e->sort.vals[k], n));
-        }
-        e->sort.temp_bytes = kpt::sort_temp_bytes(n);
-        KPT_HIP(ctx, hipMalloc(&e->sort.temp, std::max<size_t>(e->sort.temp_bytes, 1)));
-        KPT_HIP(ctx, dev_alloc(&e->d_sort_runs, static_cast<size_t>(e->runs)));
-        e->p2_allocated = true;
-    }
-    unsigned long long* act_dev = ctx->counters + 4;
-    KPT_HIP(ctx, hipMemsetAsync(act_dev, 0, sizeof(unsigned long long), ctx->stream));
-    // One batched sort when (run, node key) fits 32 bits: key bits of the largest
-    // grid plus the two classes after its nodes, above them the run index.
-    uint32_t limit = 0;
-    for (int r = 0; r < e->runs; ++r) limit = std::max(limit, kpt::sort_key_limit(e->field[r].g));
-    uint32_t run_shift = 1;
-    while (run_shift < 32 && (1ull << run_shift) < static_cast<uint64_t>(limit) + 2) ++run_shift;
-    uint32_t run_bits = 0;
-    while ((1ull << run_bits) < static_cast<uint64_t>(e->runs)) ++run_bits;
-    if (run_shift + run_bits <= 32 && e->total_cap <= 0x7fffffffull) {
-        std::vector<kpt::SortRun> sr(static_cast<size_t>(e->runs));
-        for (int r = 0; r < e->runs; ++r) {
-            sr[r].g = e->field[r].g;
-            sr[r].shift_x = e->s[r].interpolation_shift_x;
-            sr[r].shift_y = e->s[r].interpolation_shift_y;
-            sr[r].off = e->off[r];
-            sr[r].count = e->count[r];
-        }
-        KPT_HIP(ctx, hipMemcpyAsync(e->d_sort_runs, sr.data(), sr.size() * sizeof(kpt::SortRun),
-                                    hipMemcpyHostToDevice, ctx->stream));
-        KPT_HIP(ctx, kpt::launch_ensemble_sort(e->d_sort_runs, e->runs, e->total_cap, run_shift, e->P, e->P2,
-                                               e->sort, act_dev, ctx->stream));
-        KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `sr` is read by the copy
-    } else {
-        for (int r = 0; r < e->runs; ++r) {
-            if (e->count[r] == 0) continue;
-            const kpt_device_settings& s = e->s[r];
-            KPT_HIP(ctx, kpt::launch_sort(arrays_at(e->P, e->off[r]), arrays_at(e->P2, e->off[r]), e->count[r],
-                                          e->field[r].g, s.interpolation_shift_x, s.interpolation_shift_y,
-                                          std::numeric_limits<double>::infinity(), e->sort, act_dev, ctx->stream));
-        }
-    }
-    unsigned long long act = 0;
-    KPT_HIP(ctx, hipMemcpyAsync(&act, act_dev, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::swap(e->P, e->P2);
-    e->dirty = true;  // the run table points into the other particle set now
-    if (active) *active = act;
-    return KPT_OK;
-}
-
-// ---- generic particle generator (include/ParticleGenerator.hpp) -----------------
-
-struct kpt_candidates {
-    int device = 0;
-    kpt::CandidateView view{};
-    uint32_t* cells = nullptr;    // accepted cells (field candidates)
-    double* cumulative = nullptr;
-    double* face[2] = {nullptr, nullptr};         // copies of the slot's stretched tables,
-    kpt::FaceBin* bins[2] = {nullptr, nullptr};   // so the set outlives later uploads
-};
-
-namespace {
-
-kpt_status check_kinematics(kpt_ctx* ctx, const kpt_injection_kinematics* k) {
-    if (!k) return fail(ctx, KPT_ERR_ARG, "injection kinematics pointer is null.");
-    // validate_particle_injection_kinematics (include/ParticleGenerator.hpp:89-102)
-    if (!std::isfinite(k->mu) || !std::isfinite(k->weight) || k->weight <= 0.0)
-        return fail(ctx, KPT_ERR_ARG, "ParticleInjectionKinematics: mu must be finite and weight must be positive.");
-    if (!std::isfinite(k->momentum) || k->momentum < 0.0)
-        return fail(ctx, KPT_ERR_ARG,
-                    "ParticleInjectionKinematics: momentum magnitude must be finite and non-negative.");
-    if (k->status < 0 || k->status > 2) return fail(ctx, KPT_ERR_ARG, "particle status must be 0, 1 or 2.");
-    if (!std::isfinite(k->time) || !std::isfinite(k->step_dt) || k->step_dt < 0.0)
-        return fail(ctx, KPT_ERR_ARG, "injection time must be finite and the stored step finite and >= 0.");
-    return KPT_OK;
-}
-
-// e0 = ParticleProperties::kinetic_energy_from_momentum_magnitude (include/
-// ParticleSystem.hpp:170-193) of the injected momentum.
-kpt_status injection_energy(kpt_ctx* ctx, const kpt_particle_properties* props, double momentum, double& e0) {
-    kpt_particle_properties pr;
-    kpt_default_particle_properties(&pr);
-    if (props) pr = *props;
-    const kpt_status st = check_properties(ctx, &pr);
-    if (st != KPT_OK) return st;
-    const double nm = momentum / (pr.rest_mass * pr.speed_of_light);
-    e0 = (std::sqrt(1.0 + nm * nm) - 1.0) * pr.rest_mass * pr.speed_of_light * pr.speed_of_light;
-    return KPT_OK;
-}
-
-// The slot, region and predicate of a generator call.
-kpt_status make_source(kpt_ctx* ctx, int32_t slot, const kpt_region* region, const kpt_field_predicate* pred,
-                       int32_t max_attempts, kpt::GeneratorSource& src) {
-    if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
-    const FieldSlot& f = ctx->slot[slot];
-    if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
-    src = kpt::GeneratorSource{};
-    src.g = f.g;
-    src.raw = f.raw;
-    if (region) {
-        const double b[4] = {region->x_lo, region->x_hi, region->y_lo, region->y_hi};
-        for (int d = 0; d < 2; ++d)
-            if (!std::isfinite(b[2 * d]) || !std::isfinite(b[2 * d + 1]) || !(b[2 * d + 1] > b[2 * d]))
-                return fail(ctx, KPT_ERR_ARG, "AxisAlignedParticleRegion: bounds must be finite and ascending.");
-        for (int q = 0; q < 4; ++q) src.region[q] = b[q];
-    } else {
-        // make_particle_region_from_grid (include/ParticleGenerator.hpp:208-228)
-        src.region[0] = f.g.phys_lo[0];
-        src.region[1] = f.g.phys_hi[0];
-        src.region[2] = f.g.phys_lo[1];
-        src.region[3] = f.g.phys_hi[1];
-    }
-    if (pred) {
-        if (pred->component < 0 || pred->component > 5)
-            return fail(ctx, KPT_ERR_ARG, "Field particle injection: component index is out of range.");
-        if (pred->comparison < KPT_CMP_GREATER || pred->comparison > KPT_CMP_ABS_LESS)
-            return fail(ctx, KPT_ERR_ARG, "unknown field predicate comparison.");
-        if (!std::isfinite(pred->threshold)) return fail(ctx, KPT_ERR_ARG, "field predicate threshold must be finite.");
-        src.use_predicate = 1;
-        src.component = pred->component;
-        src.comparison = pred->comparison;
-        src.threshold = pred->threshold;
-    }
-    src.max_attempts = max_attempts;
-    return KPT_OK;
-}
-
-void destroy_candidates(kpt_candidates* c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    dev_free(c->cells);
-    dev_free(c->cumulative);
-    for (int d = 0; d < 2; ++d) {
-        dev_free(c->face[d]);
-        dev_free(c->bins[d]);
-    }
-    delete c;
-}
-
-// Candidate set over the cells of `src` (all of them without a predicate):
-// flags and areas, scans, and private copies of the stretched-axis tables.
-kpt_status build_candidates(kpt_ctx* ctx, const kpt::GeneratorSource& src, int field_kind, kpt_candidates** out) {
-    const uint64_t cells = static_cast<uint64_t>(src.g.nx) * src.g.ny;
-    if (cells > 0x7fffffffull) return fail(ctx, KPT_ERR_ARG, "field has too many cells for a candidate set.");
-    (void)hipSetDevice(ctx->device);
-    kpt_candidates* c = new kpt_candidates();
-    c->device = ctx->device;
-    kpt::CandidateBuild w{};
-    size_t scan_bytes = 0;
-    auto cleanup = [&](kpt_status st) {
-        dev_free(w.flags);
-        dev_free(w.offsets);
-        dev_free(w.area);
-        dev_free(w.out_area);
-        dev_free(w.temp);
-        if (st != KPT_OK) destroy_candidates(c);
-        return st;
-    };
-    hipError_t e = kpt::candidate_scratch_bytes(cells, &scan_bytes);
-    if (e == hipSuccess) e = dev_alloc(&w.flags, cells);
-    if (e == hipSuccess) e = dev_alloc(&w.offsets, cells);
-    if (e == hipSuccess) e = dev_alloc(&w.area, cells);
-    if (e == hipSuccess) e = dev_alloc(&w.out_area, cells);
-    if (e == hipSuccess) e = dev_alloc(&c->cells, cells);
-    if (e == hipSuccess) e = dev_alloc(&c->cumulative, cells);
-    if (e == hipSuccess) e = hipMalloc(&w.temp, std::max<size_t>(scan_bytes, 1));
-    if (e != hipSuccess) return cleanup(hip_fail(ctx, e, "candidate set buffers"));
-    w.temp_bytes = scan_bytes;
-    w.out_cells = c->cells;
-    kpt::GridDesc g = src.g;
-    for (int d = 0; d < 2; ++d) {
-        if (!src.g.face[d]) continue;
-        const size_t nf = static_cast<size_t>(d == 0 ? g.nx : g.ny) + 1;
-        if ((e = dev_alloc(&c->face[d], nf)) != hipSuccess ||
-            (e = dev_alloc(&c->bins[d], static_cast<size_t>(g.bins_n[d]))) != hipSuccess ||
-            (e = hipMemcpyAsync(c->face[d], src.g.face[d], nf * sizeof(double), hipMemcpyDeviceToDevice,
-                                ctx->stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(c->bins[d], src.g.bins[d], static_cast<size_t>(g.bins_n[d]) * sizeof(kpt::FaceBin),
-                                hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess)
-            return cleanup(hip_fail(ctx, e, "candidate set grid tables"));
-        g.face[d] = c->face[d];
-        g.bins[d] = c->bins[d];
-    }
-    kpt::GeneratorSource s2 = src;
-    s2.g = g;
-    uint64_t count = cells;
-    if (field_kind) {
-        if ((e = kpt::launch_candidates(s2, 1, w, ctx->stream)) != hipSuccess)
-            return cleanup(hip_fail(ctx, e, "candidate flags"));
-        uint32_t last[2] = {0, 0};
-        if ((e = hipMemcpyAsync(&last[0], w.offsets + cells - 1, 4, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(&last[1], w.flags + cells - 1, 4, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-            return cleanup(hip_fail(ctx, e, "candidate count"));
-        count = static_cast<uint64_t>(last[0]) + last[1];
-        if ((e = kpt::launch_cumulative(w.out_area, c->cumulative, count, w.temp, w.temp_bytes, ctx->stream)) !=
-            hipSuccess)
-            return cleanup(hip_fail(ctx, e, "candidate weights"));
-    } else {
-        if ((e = kpt::launch_candidates(s2, 0, w, ctx->stream)) != hipSuccess ||
-            (e = kpt::launch_cumulative(w.area, c->cumulative, count, w.temp, w.temp_bytes, ctx->stream)) !=
-                hipSuccess)
-            return cleanup(hip_fail(ctx, e, "grid candidate weights"));
-        dev_free(c->cells);
-        c->cells = nullptr;
-    }
-    double total = 0.0;
-    if (count > 0 &&
-        ((e = hipMemcpyAsync(&total, c->cumulative + count - 1, 8, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess))
-        return cleanup(hip_fail(ctx, e, "candidate total"));
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(hip_fail(ctx, e, "candidate set"));
-    if (count > 0 && (!std::isfinite(total) || total <= 0.0))
-        return cleanup(fail(ctx, KPT_ERR_ARG,
-                            field_kind ? "Field particle candidate set: total physical volume must be positive."
-                                       : "Grid particle candidate set: total physical volume must be positive."));
-    c->view.g = g;
-    c->view.cells = c->cells;
-    c->view.cumulative = c->cumulative;
-    c->view.count = count;
-    c->view.total = total;
-    c->view.kind = field_kind ? 0 : 1;
-    *out = c;
-    return cleanup(KPT_OK);
-}
-
-kpt_status read_counters(kpt_ctx* ctx, unsigned long long out[2]) {
-    KPT_HIP(ctx, hipMemcpyAsync(out, ctx->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return KPT_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-kpt_status kpt_candidates_from_field(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
-                                     const kpt_field_predicate* pred, kpt_candidates** out) {
-    if (!ctx || !pred || !out) return KPT_ERR_ARG;
-    *out = nullptr;
-    kpt::GeneratorSource src;
-    const kpt_status st = make_source(ctx, slot, region, pred, 1, src);
-    if (st != KPT_OK) return st;
-    return build_candidates(ctx, src, 1, out);
-}
-
-kpt_status kpt_candidates_from_grid(kpt_ctx* ctx, int32_t slot, kpt_candidates** out) {
-    if (!ctx || !out) return KPT_ERR_ARG;
-    *out = nullptr;
-    kpt::GeneratorSource src;
-    const kpt_status st = make_source(ctx, slot, nullptr, nullptr, 1, src);
-    if (st != KPT_OK) return st;
-    if (src.g.nx <= 0 || src.g.ny <= 0) return fail(ctx, KPT_ERR_ARG, "Grid particle candidate set: grid domain has no cells.");
-    return build_candidates(ctx, src, 0, out);
-}
-
-kpt_status kpt_candidates_info(const kpt_candidates* c, uint64_t* count, double* total_weight) {
-    if (!c) return KPT_ERR_ARG;
-    if (count) *count = c->view.count;
-    if (total_weight) *total_weight = c->view.total;
-    return KPT_OK;
-}
-
-kpt_status kpt_candidates_download(kpt_ctx* ctx, const kpt_candidates* c, uint32_t* cells, double* cumulative) {
-    if (!ctx || !c) return KPT_ERR_ARG;
-    const uint64_t n = c->view.count;
-    if (n == 0) return KPT_OK;
-    (void)hipSetDevice(c->device);
-    if (cells) {
-        if (c->cells) {
-            KPT_HIP(ctx, hipMemcpy(cells, c->cells, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        } else {
-            for (uint64_t k = 0; k < n; ++k) cells[k] = static_cast<uint32_t>(k);
-        }
-    }
-    if (cumulative) KPT_HIP(ctx, hipMemcpy(cumulative, c->cumulative, n * sizeof(double), hipMemcpyDeviceToHost));
-    return KPT_OK;
-}
-
-kpt_status kpt_candidates_destroy(kpt_candidates* c) {
-    destroy_candidates(c);
-    return KPT_OK;
-}
-
-kpt_status kpt_inject_from_candidates(kpt_ctx* ctx, const kpt_candidates* c, uint64_t n,
-                                      const kpt_injection_kinematics* k, const kpt_particle_properties* props,
-                                      uint64_t* injected) {
-    if (!ctx || !c) return KPT_ERR_ARG;
-    if (injected) *injected = 0;
-    kpt_status st = check_kinematics(ctx, k);
-    if (st != KPT_OK) return st;
-    double e0 = 0.0;
-    if ((st = injection_energy(ctx, props, k->momentum, e0)) != KPT_OK) return st;
-    // validate_particle_injection_plan (include/ParticleGenerator.hpp:846-870)
-    if (n > ctx->capacity - ctx->count)
-        return fail(ctx, KPT_ERR_ARG, "ParticleInjectionPlan: requested range exceeds capacity.");
-    if (c->view.count == 0)
-        return fail(ctx, KPT_ERR_STATE, c->view.kind == 0
-                                            ? "ParticleGenerator: field-conditioned injection has no candidate grid points."
-                                            : "ParticleGenerator: grid injection has no candidate cells.");
-    (void)hipSetDevice(ctx->device);
-    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
-    KPT_HIP(ctx, kpt::launch_inject_candidates(ctx->P, ctx->count, n, ctx->next_root, c->view, *k, e0,
-                                               ctx->generator_calls, key, ctx->stream));
-    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->generator_calls += 1;
-    ctx->count += n;
-    ctx->next_root += n;
-    if (injected) *injected = n;
-    return KPT_OK;
-}
-
-kpt_status kpt_inject_uniform_if(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
-                                 const kpt_field_predicate* pred, int32_t max_attempts, uint64_t n,
-                                 const kpt_injection_kinematics* k, const kpt_particle_properties* props,
-                                 uint64_t* injected, uint64_t* rejected) {
-    if (!ctx) return KPT_ERR_ARG;
-    if (injected) *injected = 0;
-    if (rejected) *rejected = 0;
-    kpt_status st = check_kinematics(ctx, k);
-    if (st != KPT_OK) return st;
-    double e0 = 0.0;
-    if ((st = injection_energy(ctx, props, k->momentum, e0)) != KPT_OK) return st;
-    if (max_attempts <= 0 || max_attempts > 65535)
-        return fail(ctx, KPT_ERR_ARG, "ParticleInjectionPlan: max_attempts must be positive (and below 65536).");
-    if (n > ctx->capacity - ctx->count)
-        return fail(ctx, KPT_ERR_ARG, "ParticleInjectionPlan: requested range exceeds capacity.");
-    kpt::GeneratorSource src;
-    if ((st = make_source(ctx, slot, region, pred, max_attempts, src)) != KPT_OK) return st;
-    (void)hipSetDevice(ctx->device);
-    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
-    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-    KPT_HIP(ctx, kpt::launch_inject_uniform(ctx->P, ctx->count, n, ctx->next_root, src, *k, e0, ctx->generator_calls,
-                                            key, ctx->counters, ctx->stream));
-    unsigned long long cnt[2] = {0, 0};
-    if ((st = read_counters(ctx, cnt)) != KPT_OK) return st;
-    ctx->generator_calls += 1;
-    ctx->count += n;
-    ctx->next_root += n;
-    if (injected) *injected = n;
-    if (rejected) *rejected = cnt[0];
-    return KPT_OK;
-}
-
-kpt_status kpt_recycle_uniform_if(kpt_ctx* ctx, int32_t slot, const kpt_region* region,
-                                  const kpt_field_predicate* pred, int32_t max_attempts, int32_t selection,
-                                  const kpt_injection_kinematics* k, const kpt_particle_properties* props,
-                                  uint64_t* recycled, uint64_t* rejected) {
-    if (!ctx) return KPT_ERR_ARG;
-    if (recycled) *recycled = 0;
-    if (rejected) *rejected = 0;
-    kpt_status st = check_kinematics(ctx, k);
-    if (st != KPT_OK) return st;
-    double e0 = 0.0;
-    if ((st = injection_energy(ctx, props, k->momentum, e0)) != KPT_OK) return st;
-    if (max_attempts <= 0 || max_attempts > 65535)
-        return fail(ctx, KPT_ERR_ARG, "ParticleGenerator: recycle max_attempts must be positive (and below 65536).");
-    if (selection < KPT_RECYCLE_ESCAPED || selection > KPT_RECYCLE_ANY)
-        return fail(ctx, KPT_ERR_ARG, "unknown recycle selection.");
-    kpt::GeneratorSource src;
-    if ((st = make_source(ctx, slot, region, pred, max_attempts, src)) != KPT_OK) return st;
-    (void)hipSetDevice(ctx->device);
-    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
-    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-    KPT_HIP(ctx, kpt::launch_recycle(ctx->P, ctx->count, selection, 1, kpt::CandidateView{}, src, *k, e0,
-                                     ctx->generator_calls, key, ctx->counters, ctx->stream));
-    unsigned long long cnt[2] = {0, 0};
-    if ((st = read_counters(ctx, cnt)) != KPT_OK) return st;
-    ctx->generator_calls += 1;
-    if (recycled) *recycled = cnt[0];
-    if (rejected) *rejected = cnt[1];
-    return KPT_OK;
-}
-
-kpt_status kpt_recycle_from_candidates(kpt_ctx* ctx, const kpt_candidates* c, int32_t selection,
-                                       const kpt_injection_kinematics* k, const kpt_particle_properties* props,
-                                       uint64_t* recycled) {
-    if (!ctx || !c) return KPT_ERR_ARG;
-    if (recycled) *recycled = 0;
-    kpt_status st = check_kinematics(ctx, k);
-    if (st != KPT_OK) return st;
-    double e0 = 0.0;
-    if ((st = injection_energy(ctx, props, k->momentum, e0)) != KPT_OK) return st;
-    if (selection < KPT_RECYCLE_ESCAPED || selection > KPT_RECYCLE_ANY)
-        return fail(ctx, KPT_ERR_ARG, "unknown recycle selection.");
-    if (c->view.count == 0)
-        return fail(ctx, KPT_ERR_STATE, c->view.kind == 0
-                                            ? "ParticleGenerator: field-conditioned recycling has no candidate grid points."
-                                            : "ParticleGenerator: grid recycling has no candidate cells.");
-    (void)hipSetDevice(ctx->device);
-    const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
-    KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-    kpt::GeneratorSource none{};
-    KPT_HIP(ctx, kpt::launch_recycle(ctx->P, ctx->count, selection, 0, c->view, none, *k, e0, ctx->generator_calls,
-                                     key, ctx->counters, ctx->stream));
-    unsigned long long cnt[2] = {0, 0};
-    if ((st = read_counters(ctx, cnt)) != KPT_OK) return st;
-    ctx->generator_calls += 1;
-    if (recycled) *recycled = cnt[0];
-    return KPT_OK;
-}
-
-kpt_status kpt_generator_calls(const kpt_ctx* ctx, uint32_t* calls) {
-    if (!ctx || !calls) return KPT_ERR_ARG;
-    *calls = ctx->generator_calls;
-    return KPT_OK;
-}
-
-}  // extern "C"
