@@ -305,7 +305,18 @@ hipError_t launch_mlp_infer(const void* image, const float* x, uint64_t n, float
 
 // Image = kFrags BF16 fragments | kBiasTiles FP32 accumulator initialisers |
 // b4[2] (FP32, read by the kernels from global memory).
+//
+// The image carries the network with the SiLU's exp2 scale folded in: every
+// hidden pre-activation is stored as z' = -log2(e) z, so that
+// silu(z) = -ln(2) z' / (1 + 2^z'). Layer 1 (weights and bias column) and the
+// biases of layers 2 and 3 are scaled by -log2(e); the hidden activations are
+// kept as h' = z' / (1 + 2^z') = -log2(e) silu(z), so the weights of layers 2 and
+// 3 are unchanged (the factors cancel) and those of layer 4 are scaled by
+// -ln(2). Each activation then costs one multiply fewer. Scales are applied in
+// FP32 before the BF16 rounding.
 void mlp_build_image(const float* p, void* image) {
+    constexpr float kPre = -1.4426950408889634f;  // -log2(e)
+    constexpr float kOut = -0.6931471805599453f;  // -ln(2)
     uint16_t* frag = static_cast<uint16_t*>(image);
     float* bias = reinterpret_cast<float*>(static_cast<char*>(image) + kMlpFragBytes);
     auto put = [&](int f, int lane, int i, float v) { frag[(static_cast<size_t>(f) * 64 + lane) * 8 + i] = to_bf16(v); };
@@ -318,7 +329,7 @@ void mlp_build_image(const float* p, void* image) {
                 float v = 0.0f;
                 if (k < kMlpIn) v = p[kOffW1 + kMlpIn * m + k];
                 else if (k == kMlpIn) v = p[kOffB1 + m];
-                put(kFragA1 + t, lane, i, v);
+                put(kFragA1 + t, lane, i, v * kPre);
                 for (int s = 0; s < 4; ++s) {
                     const int feat = mlp_k_feature(s, k);
                     put(kFragA2 + 4 * t + s, lane, i, p[kOffW2 + kMlpH * m + feat]);
@@ -327,14 +338,14 @@ void mlp_build_image(const float* p, void* image) {
             }
             for (int s = 0; s < 4; ++s) {
                 const int feat = mlp_k_feature(s, k);
-                put(kFragA4 + s, lane, i, row < kMlpOut ? p[kOffW4 + kMlpH * row + feat] : 0.0f);
+                put(kFragA4 + s, lane, i, row < kMlpOut ? p[kOffW4 + kMlpH * row + feat] * kOut : 0.0f);
             }
         }
         for (int t = 0; t < 2; ++t)
             for (int j = 0; j < 16; ++j) {
                 const int m = mlp_acc_row(t, j, half);
-                bias[(static_cast<size_t>(0 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB2 + m];
-                bias[(static_cast<size_t>(1 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB3 + m];
+                bias[(static_cast<size_t>(0 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB2 + m] * kPre;
+                bias[(static_cast<size_t>(1 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB3 + m] * kPre;
             }
     }
     float* b4 = reinterpret_cast<float*>(static_cast<char*>(image) + kMlpLdsBytes);

@@ -33,17 +33,19 @@ __device__ __forceinline__ void mlp_load_lds(const void* image, void* lds) {
     for (unsigned i = threadIdx.x; i < kMlpLdsBytes / 16; i += blockDim.x) dst[i] = src[i];
 }
 
-// SiLU with the native exp2 and reciprocal.
-__device__ __forceinline__ float silu_native(float z) {
-    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+// SiLU on the image's scaled pre-activation z' = -log2(e) z (see
+// mlp_build_image): returns z' / (1 + 2^z') = -log2(e) silu(z), with the native
+// exp2 and reciprocal.
+__device__ __forceinline__ float silu_scaled(float zs) {
+    return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
 }
 
 // Accumulators of one tile -> the B operands of K steps 2t and 2t + 1.
 __device__ __forceinline__ void silu_pack(const f32x16& acc, bf16x8& k0, bf16x8& k1) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        k0[i] = static_cast<__bf16>(silu_native(acc[i]));
-        k1[i] = static_cast<__bf16>(silu_native(acc[8 + i]));
+        k0[i] = static_cast<__bf16>(silu_scaled(acc[i]));
+        k1[i] = static_cast<__bf16>(silu_scaled(acc[8 + i]));
     }
 }
 

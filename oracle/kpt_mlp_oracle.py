@@ -119,16 +119,30 @@ def bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+FOLD_PRE = np.float32(-1.4426950408889634)   # -log2(e)
+FOLD_OUT = np.float32(-0.6931471805599453)   # -ln(2)
+
+
+def _scaled(w, c):
+    """FP32 product with the fold constant (as the image builder forms it)."""
+    return np.asarray(w, dtype=np.float32) * c
+
+
 def forward_bf16(p, x):
     """The wavefront MFMA evaluation restated: BF16 weights, biases of layer 1
     in BF16 (they ride in the K column), BF16 inputs and hidden activations,
-    FP32-or-better accumulation, FP32 biases of layers 2-4."""
+    FP32-or-better accumulation, FP32 biases of layers 2-4. The image folds the
+    SiLU's exp2 scale into the network: pre-activations are z' = -log2(e) z,
+    activations h' = z' / (1 + 2^z') = -log2(e) silu(z), so layer 1 and the
+    biases of layers 2 and 3 carry -log2(e), layers 2 and 3 keep their weights
+    and layer 4 carries -ln(2); in exact arithmetic the function is unchanged."""
     w1, b1, w2, b2, w3, b3, w4, b4 = unpack(p)
+    act = lambda zs: bf16(zs / (1.0 + np.exp2(zs)))  # noqa: E731
     x = bf16(x)
-    z1 = x @ bf16(w1).T + bf16(b1)
-    h = bf16(silu(z1))
-    z2 = h @ bf16(w2).T + np.float32(b2)
-    h = bf16(silu(z2))
-    z3 = h @ bf16(w3).T + np.float32(b3)
-    h = bf16(silu(z3))
-    return h @ bf16(w4).T + np.float32(b4)
+    z1 = x @ bf16(_scaled(w1, FOLD_PRE)).T + bf16(_scaled(b1, FOLD_PRE))
+    h = act(z1)
+    z2 = h @ bf16(w2).T + _scaled(b2, FOLD_PRE)
+    h = act(z2)
+    z3 = h @ bf16(w3).T + _scaled(b3, FOLD_PRE)
+    h = act(z3)
+    return h @ bf16(_scaled(w4, FOLD_OUT)).T + np.float32(b4)
