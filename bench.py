@@ -183,6 +183,7 @@ def run_ensemble(args):
         log("training the D_mumu network on the GPU ...")
         o = kpt.mlp_default_train_opts()
         o.steps = args.mlp_steps
+        o.hull.activation = kpt.MLP_ACTIVATIONS[args.mlp_activation]
         report = ctx.mlp_train(o)
         with kpt.Ensemble(ctx, [spec["count"] for spec in specs]) as e:
             for r, (spec, s) in enumerate(zip(specs, settings)):
@@ -261,6 +262,7 @@ def run_ensemble(args):
             "graph": True,
             "sort_every_intervals": every,
             "mlp_training": {k: report[k] for k in ("steps", "train_ms", "val_rms_rel_d")},
+            "mlp_activation": args.mlp_activation,
         },
         "roofline": {
             "bound": "hbm",
@@ -316,6 +318,8 @@ def main():
                     help="default: fp32 for focused, fp64 for parker")
     ap.add_argument("--closure", choices=["qlt", "mlp"], default="mlp")
     ap.add_argument("--mlp-steps", type=int, default=8000, help="Adam steps of the network training")
+    ap.add_argument("--mlp-activation", choices=sorted(kpt.MLP_ACTIVATIONS), default="hard-swish",
+                    help="hidden-layer activation of the learned closure")
     ap.add_argument("--no-fp64-leg", action="store_true",
                     help="focused: skip the plain FP64 push of the same particles")
     ap.add_argument("--no-fp32-leg", action="store_true",
@@ -367,6 +371,7 @@ def main():
             log("training the D_mumu network on the GPU ...")
             o = kpt.mlp_default_train_opts()
             o.steps = args.mlp_steps
+            o.hull.activation = kpt.MLP_ACTIVATIONS[args.mlp_activation]
             t_train = time.perf_counter()
             mlp_report = ctx.mlp_train(o)
             mlp_report["wall_s"] = time.perf_counter() - t_train
@@ -425,6 +430,7 @@ def main():
         cfg["closure"] = args.closure
         cfg["dt"] = s.dt_min
         if mlp_report:
+            cfg["mlp_activation"] = args.mlp_activation
             cfg["mlp_training"] = {k: mlp_report[k] for k in ("steps", "train_ms", "final_loss",
                                                           "val_rms_lnr", "val_max_lnr",
                                                           "val_rms_rel_d", "val_rms_y1")}

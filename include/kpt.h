@@ -739,12 +739,22 @@ kpt_status kpt_generator_calls(const kpt_ctx* ctx, uint32_t* calls);
  * the exact formula. */
 enum { KPT_MLP_INPUTS = 3, KPT_MLP_HIDDEN = 64, KPT_MLP_OUTPUTS = 2, KPT_MLP_PARAMS = 8706 };
 
+/* Hidden-layer activation of the network. */
+enum {
+    KPT_MLP_SILU = 0,       /* z / (1 + e^-z)                                        */
+    KPT_MLP_HARD_SWISH = 1  /* z clamp(z / 6 + 1/2, 0, 1): no transcendental per unit */
+};
+
+/* The trained network's description: its input hull, output scaling and
+ * activation. */
 typedef struct kpt_mlp_hull {
     double mu_lo;                 /* |mu| >= mu_lo (and < 1)            default 0.02  */
     double log_xi_lo, log_xi_hi;  /* ln xi range                        ln 1e-7, 0    */
     double gamma_lo, gamma_hi;    /* gamma_k range                      1.5, 2        */
     double h0;                    /* resonance regularisation trained   1e-6          */
     double out_mean, out_scale;   /* ln R = out_mean + out_scale * y0 (set by training) */
+    int32_t activation;           /* KPT_MLP_SILU or KPT_MLP_HARD_SWISH (default)      */
+    int32_t reserved;             /* 0                                                  */
 } kpt_mlp_hull;
 
 typedef struct kpt_mlp_train_opts {

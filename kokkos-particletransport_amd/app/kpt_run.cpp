@@ -60,6 +60,7 @@ struct RunSettings {
     double qlt_correlation_length{0.02};
     double qlt_spectral_index{5.0 / 3.0};
     uint32_t mlp_train_steps{8000};
+    int32_t mlp_activation{KPT_MLP_HARD_SWISH};
     int device{0};
     bool resumable_snapshots{false};
     bool allow_stretched_fields{false};
@@ -222,6 +223,8 @@ void print_help() {
         << "  --qlt-sigma2 X --qlt-correlation-length X --qlt-spectral-index X\n"
         << "                                slab-QLT turbulence (default 0.1, 0.02, 5/3)\n"
         << "  --mlp-train-steps N           Adam steps of the learned closure (default 8000)\n"
+        << "  --mlp-activation silu|hard-swish\n"
+        << "                                hidden-layer activation of the learned closure (default hard-swish)\n"
         << "  --resumable-snapshots         snapshots carry particle ids and split baselines,\n"
         << "                                so a restart continues the run bit for bit\n"
         << "  --allow-stretched-fields      accept non-uniform field faces (value k at face k,\n"
@@ -344,6 +347,11 @@ RunSettings parse(int argc, char** argv) {
             s.qlt_spectral_index = std::stod(val());
         } else if (a == "--mlp-train-steps") {
             s.mlp_train_steps = static_cast<uint32_t>(std::stoul(val()));
+        } else if (a == "--mlp-activation") {
+            const std::string c = val();
+            if (c == "silu") s.mlp_activation = KPT_MLP_SILU;
+            else if (c == "hard-swish") s.mlp_activation = KPT_MLP_HARD_SWISH;
+            else throw std::runtime_error("Unknown network activation: " + c);
         } else if (a == "--device") {
             s.device = std::stoi(val());
         } else {
@@ -514,6 +522,7 @@ int run(const RunSettings& rs) {
         kpt_mlp_train_opts o;
         kpt_mlp_default_train_opts(&o);
         o.steps = rs.mlp_train_steps;
+        o.hull.activation = rs.mlp_activation;
         kpt_mlp_report r;
         ctx.check(kpt_mlp_train(ctx.c, &o, &r), "pitch-angle network training");
         std::ostringstream l;

@@ -997,12 +997,14 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.k32 = kpt::make_step_const<float>(*s, a.qlt_norm);
     a.mlp_image = nullptr;
     a.mlp_on = 0;
+    a.mlp_act = kpt::kActSilu;
     if (s->closure == KPT_CLOSURE_MLP) {
         if (!fp32) return fail(ctx, KPT_ERR_ARG, "the learned pitch-angle closure runs in the FP32 push.");
         if (!ctx->mlp_valid) return fail(ctx, KPT_ERR_STATE, "no pitch-angle network is loaded (kpt_mlp_train / kpt_mlp_set).");
         const kpt_mlp_hull& h = ctx->mlp_hull;
         a.mlp_image = ctx->mlp_image;
         a.mlp_scale = kpt::mlp_scale(h);
+        a.mlp_act = h.activation;
         a.mlp_on = (s->qlt_spectral_index >= h.gamma_lo && s->qlt_spectral_index <= h.gamma_hi &&
                     s->qlt_h0 == h.h0) ? 1 : 0;
     }
@@ -1427,7 +1429,8 @@ bool hull_ok(const kpt_mlp_hull& h) {
     return std::isfinite(h.mu_lo) && h.mu_lo > 0.0 && h.mu_lo < 1.0 && std::isfinite(h.log_xi_lo) &&
            std::isfinite(h.log_xi_hi) && h.log_xi_hi > h.log_xi_lo && std::isfinite(h.gamma_lo) &&
            std::isfinite(h.gamma_hi) && h.gamma_hi > h.gamma_lo && h.gamma_lo > 1.0 && std::isfinite(h.h0) &&
-           h.h0 > 0.0 && std::isfinite(h.out_mean) && std::isfinite(h.out_scale) && h.out_scale > 0.0;
+           h.h0 > 0.0 && std::isfinite(h.out_mean) && std::isfinite(h.out_scale) && h.out_scale > 0.0 &&
+           kpt::mlp_activation_ok(h.activation);
 }
 
 // Install FP32 parameters: device copy, host copy and the BF16 MFMA image.
@@ -1436,7 +1439,7 @@ kpt_status mlp_install(kpt_ctx* ctx, const float* params, const kpt_mlp_hull& h)
     if (!ctx->mlp_image) KPT_HIP(ctx, hipMalloc(&ctx->mlp_image, kpt::kMlpImageBytes));
     ctx->mlp_host.assign(params, params + kpt::kMlpParams);
     std::vector<unsigned char> image(kpt::kMlpImageBytes, 0);
-    kpt::mlp_build_image(params, image.data());
+    kpt::mlp_build_image(params, h.activation, image.data());
     KPT_HIP(ctx, hipMemcpy(ctx->mlp_params, params, kpt::kMlpParams * sizeof(float), hipMemcpyHostToDevice));
     KPT_HIP(ctx, hipMemcpy(ctx->mlp_image, image.data(), image.size(), hipMemcpyHostToDevice));
     ctx->mlp_hull = h;
@@ -1485,6 +1488,7 @@ kpt_status kpt_mlp_default_train_opts(kpt_mlp_train_opts* o) {
     o->hull.h0 = 1.0e-6;
     o->hull.out_mean = 0.0;
     o->hull.out_scale = 1.0;
+    o->hull.activation = KPT_MLP_HARD_SWISH;
     return KPT_OK;
 }
 
@@ -1531,8 +1535,9 @@ kpt_status kpt_mlp_gradient(kpt_ctx* ctx, const float* feats, const float* targe
     KPT_HIP(ctx, b.get(&lp, parts));
     KPT_HIP(ctx, hipMemcpyAsync(x, feats, n * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     KPT_HIP(ctx, hipMemcpyAsync(y, targets, n * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, x, n, z, yhat, ctx->stream));
-    KPT_HIP(ctx, kpt::launch_mlp_backward(ctx->mlp_params, x, z, yhat, y, n, dz, dy, gp, g, lp, ctx->stream));
+    const int act = ctx->mlp_hull.activation;
+    KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, act, x, n, z, yhat, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_backward(ctx->mlp_params, act, x, z, yhat, y, n, dz, dy, gp, g, lp, ctx->stream));
     std::vector<float> lh(parts);
     KPT_HIP(ctx, hipMemcpyAsync(grad, g, kpt::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipMemcpyAsync(lh.data(), lp, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1555,10 +1560,11 @@ kpt_status kpt_mlp_infer(kpt_ctx* ctx, const float* feats, uint64_t n, int32_t m
     KPT_HIP(ctx, b.get(&x, n * 3));
     KPT_HIP(ctx, b.get(&o, n * 2));
     KPT_HIP(ctx, hipMemcpyAsync(x, feats, n * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    const int act = ctx->mlp_hull.activation;
     if (mfma)
-        KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, x, n, o, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, act, x, n, o, ctx->stream));
     else
-        KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, x, n, nullptr, o, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_forward(ctx->mlp_params, act, x, n, nullptr, o, ctx->stream));
     KPT_HIP(ctx, hipMemcpyAsync(out, o, n * 2 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KPT_OK;
@@ -1635,9 +1641,9 @@ kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_r
         const float lr = static_cast<float>(opts->lr * 0.5 * (1.0 + std::cos(M_PI * it / opts->steps)));
         b1c *= 0.9;
         b2c *= 0.999;
-        KPT_HIP(ctx, kpt::launch_mlp_forward(params, X + 3 * off, B, z, yhat, ctx->stream));
-        KPT_HIP(ctx, kpt::launch_mlp_backward(params, X + 3 * off, z, yhat, Y + 2 * off, B, dz, dy, gp, g, lp,
-                                              ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_forward(params, h.activation, X + 3 * off, B, z, yhat, ctx->stream));
+        KPT_HIP(ctx, kpt::launch_mlp_backward(params, h.activation, X + 3 * off, z, yhat, Y + 2 * off, B, dz, dy, gp,
+                                              g, lp, ctx->stream));
         KPT_HIP(ctx, kpt::launch_mlp_adam(params, m, v, g, lr, static_cast<float>(1.0 - b1c),
                                           static_cast<float>(1.0 - b2c), ctx->stream));
     }
@@ -1662,7 +1668,7 @@ kpt_status kpt_mlp_train(kpt_ctx* ctx, const kpt_mlp_train_opts* opts, kpt_mlp_r
     KPT_HIP(ctx, b.get(&vo, V * 2));
     KPT_HIP(ctx, b.get(&vraw, V));
     KPT_HIP(ctx, kpt::launch_mlp_generate(vx, vy, vraw, V, opts->seed, N, h, ctx->stream));
-    KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, vx, V, vo, ctx->stream));
+    KPT_HIP(ctx, kpt::launch_mlp_infer(ctx->mlp_image, h.activation, vx, V, vo, ctx->stream));
     std::vector<double> vr(V);
     std::vector<float> ty(2 * V), to(2 * V);
     KPT_HIP(ctx, hipMemcpyAsync(vr.data(), vraw, V * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -163,12 +163,14 @@ kpt_status ens_sync(kpt_ensemble* e) {
         a.k32 = kpt::make_step_const<float>(s, a.qlt_norm);
         a.mlp_image = nullptr;
         a.mlp_on = 0;
+        a.mlp_act = kpt::kActSilu;
         if (R.kind == kpt::kEnsFocusedMlp32) {
             if (!ctx->mlp_valid)
                 return fail(ctx, KPT_ERR_STATE, tag + "no pitch-angle network is loaded (kpt_mlp_train / kpt_mlp_set).");
             const kpt_mlp_hull& h = ctx->mlp_hull;
             a.mlp_image = ctx->mlp_image;
             a.mlp_scale = kpt::mlp_scale(h);
+            a.mlp_act = h.activation;
             a.mlp_on = (s.qlt_spectral_index >= h.gamma_lo && s.qlt_spectral_index <= h.gamma_hi && s.qlt_h0 == h.h0)
                            ? 1 : 0;
         }

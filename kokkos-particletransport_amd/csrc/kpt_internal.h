@@ -134,11 +134,12 @@ struct PushArgs {
     double qlt_norm;      // gamma_k sin(pi / gamma_k) / pi (pitch_angle_spectrum_normalization)
     StepConst<double> k64;
     StepConst<float> k32;
-    // learned closure (KPT_CLOSURE_MLP): network image and feature scaling;
-    // mlp_on = 0 sends every evaluation to the exact formula
+    // learned closure (KPT_CLOSURE_MLP): network image, feature scaling and
+    // activation (KPT_MLP_*); mlp_on = 0 sends every evaluation to the exact formula
     const void* mlp_image;
     MlpScale mlp_scale;
     int32_t mlp_on;
+    int32_t mlp_act;
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
 };
 

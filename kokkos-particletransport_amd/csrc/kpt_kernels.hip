@@ -805,7 +805,7 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // One lane of the learned-closure push (the network image already staged in
 // LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
 // together: the sub-step loop and the network evaluations are wave-uniform.
-template <bool STR = false>
+template <bool STR = false, int ACT = kActSilu>
 __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& im, const float b40, const float b41,
                                               const int64_t idx) {
     const MlpScale ms = a.mlp_scale;
@@ -897,7 +897,7 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
             }
         }
         float y0 = 0.0f, y1 = 0.0f;
-        if (use_net) mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);  // use_net is uniform over the launch
+        if (use_net) mlp_eval_wave<ACT>(im, f0, f1, f2, b40, b41, y0, y1);  // use_net is uniform over the launch
 
         // phase 2: closure at mu0, step size, noise, transverse increments, first Picard pass
         float r3 = 0.0f, dtr = 0.0f, delta_x = 0.0f, delta_y = 0.0f;
@@ -976,7 +976,7 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
         // phases 3 and 4: the two Picard passes at the trial values
 #pragma unroll 1
         for (int it = 1; it < 3; ++it) {
-            if (use_net) mlp_eval_wave(im, f0, f1, f2, b40, b41, y0, y1);
+            if (use_net) mlp_eval_wave<ACT>(im, f0, f1, f2, b40, b41, y0, y1);
             bool hn = false;
             f0 = f1 = f2 = 0.0f;
             if (run) {
@@ -1041,13 +1041,13 @@ __device__ __forceinline__ MlpLds mlp_stage(const void* image, uint4* smem, floa
     return mlp_lds_view(smem);
 }
 
-template <bool STR>
+template <bool STR, int ACT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
     push_kernel_mlp(PushArgs a, uint32_t run) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     float b40, b41;
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
-    push_lane_mlp<STR>(a, im, b40, b41, push_first_particle(run));
+    push_lane_mlp<STR, ACT>(a, im, b40, b41, push_first_particle(run));
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,8 +1089,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
             push_lane<16, false, float, KPT_CLOSURE_REDUCED>(R.a, idx);
         else if (R.kind == kEnsFocusedQlt32)
             push_lane<21, true, float, KPT_CLOSURE_QLT>(R.a, idx);
+        else if (R.a.mlp_act == kActHardSwish)
+            push_lane_mlp<false, kActHardSwish>(R.a, im, b40, b41, idx);
         else
-            push_lane_mlp(R.a, im, b40, b41, idx);
+            push_lane_mlp<false, kActSilu>(R.a, im, b40, b41, idx);
     }
 }
 
@@ -1477,8 +1479,11 @@ hipError_t launch_push_t(const PushArgs& a, int model, bool focused, bool fp32, 
         else
             hipLaunchKernelGGL((push_kernel<16, false, double, R, STR>), grid, block, 0, stream, a, run);
     } else if (focused && closure == KPT_CLOSURE_MLP) {
-        if (!fp32 || a.mlp_image == nullptr) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(push_kernel_mlp<STR>, grid, block, 0, stream, a, run);
+        if (!fp32 || a.mlp_image == nullptr || !mlp_activation_ok(a.mlp_act)) return hipErrorInvalidValue;
+        if (a.mlp_act == kActHardSwish)
+            hipLaunchKernelGGL((push_kernel_mlp<STR, kActHardSwish>), grid, block, 0, stream, a, run);
+        else
+            hipLaunchKernelGGL((push_kernel_mlp<STR, kActSilu>), grid, block, 0, stream, a, run);
     } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
             hipLaunchKernelGGL((push_kernel<21, true, float, Q, STR>), grid, block, 0, stream, a, run);

@@ -8,7 +8,8 @@
 // (1 - mu^2) are exact; the network stands in for the resonance function R:
 //   inputs  f0 = ln|mu|, f1 = ln xi, f2 = gamma_k (affinely scaled to [-1, 1]),
 //   outputs y0 = (ln R - out_mean) / out_scale,  y1 = |mu| d ln R / d|mu|,
-// three hidden layers of 64 SiLU units. The five physical inputs of the closure
+// three hidden layers of 64 units (SiLU, or hard-swish z clamp(z / 6 + 1/2, 0, 1),
+// which needs no transcendental). The five physical inputs of the closure
 // (mu, ln p, ln sigma2, gamma_k, ln Lc) reduce exactly to (|mu|, ln xi, gamma_k):
 // sigma2 is a prefactor and p, Lc and |B| enter only through xi.
 // Then D = P (1 - mu^2) R and dD/dmu = P R ((1 - mu^2) sgn(mu) y1 / |mu| - 2 mu).
@@ -46,6 +47,11 @@ constexpr int kOffW4 = kOffB3 + kMlpH;
 constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpH;
 constexpr int kMlpParams = kOffB4 + kMlpOut;
 static_assert(kMlpParams == KPT_MLP_PARAMS, "parameter count");
+
+// Hidden-layer activations.
+constexpr int kActSilu = KPT_MLP_SILU;
+constexpr int kActHardSwish = KPT_MLP_HARD_SWISH;
+inline bool mlp_activation_ok(int32_t act) { return act == kActSilu || act == kActHardSwish; }
 
 // MFMA fragments: 16 bytes (8 bf16) per lane, 64 lanes per fragment.
 //   A1[t]      t = 0, 1         (layer 1, K = features, bias at k = 3)
@@ -111,17 +117,19 @@ __host__ __device__ inline void mlp_exact_target(double mu_abs, double log_xi, d
 hipError_t launch_mlp_generate(float* feats, float* targets, double* raw_lnr, uint64_t n, uint64_t seed,
                                uint64_t first, const kpt_mlp_hull& h, hipStream_t stream);
 hipError_t launch_mlp_normalize(float* targets, uint64_t n, float mean, float inv_scale, hipStream_t stream);
-hipError_t launch_mlp_forward(const float* params, const float* x, uint64_t n, float* z, float* yhat,
+hipError_t launch_mlp_forward(const float* params, int act, const float* x, uint64_t n, float* z, float* yhat,
                               hipStream_t stream);
-hipError_t launch_mlp_backward(const float* params, const float* x, const float* z, const float* yhat,
+hipError_t launch_mlp_backward(const float* params, int act, const float* x, const float* z, const float* yhat,
                                const float* y, uint64_t n, float* dz, float* dy, float* grad_partials, float* grad,
                                float* loss_partials, hipStream_t stream);
 hipError_t launch_mlp_adam(float* params, float* m, float* v, const float* grad, float lr, float b1c, float b2c,
                            hipStream_t stream);
-hipError_t launch_mlp_infer(const void* lds_image, const float* x, uint64_t n, float* out, hipStream_t stream);
+hipError_t launch_mlp_infer(const void* lds_image, int act, const float* x, uint64_t n, float* out,
+                            hipStream_t stream);
 size_t mlp_loss_partials(uint64_t n);
 size_t mlp_grad_partials(uint64_t n);  // floats of per-chunk gradient partial sums
-// Host: BF16 MFMA fragments and bias initialisers from FP32 parameters.
-void mlp_build_image(const float* params, void* image /* kMlpImageBytes */);
+// Host: BF16 MFMA fragments and bias initialisers from FP32 parameters of a
+// network with activation act.
+void mlp_build_image(const float* params, int act, void* image /* kMlpImageBytes */);
 
 }  // namespace kpt

@@ -26,6 +26,23 @@ __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.0f / (1.0f + expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
+// Hard-swish z u, u = clamp(z / 6 + 1/2, 0, 1); derivative u + z / 6 = 2u - 1/2
+// inside the ramp, 0 below it and 1 above it.
+__device__ __forceinline__ float hswish_derivative(float z) {
+    const float u = __builtin_fmaf(z, 1.0f / 6.0f, 0.5f);
+    return u <= 0.0f ? 0.0f : (u >= 1.0f ? 1.0f : 2.0f * u - 0.5f);
+}
+// Activation and derivative of the training kernels.
+template <int ACT>
+__device__ __forceinline__ float act_f(float z) {
+    if constexpr (ACT == kActHardSwish) return hard_swish(z);
+    else return silu(z);
+}
+template <int ACT>
+__device__ __forceinline__ float act_df(float z) {
+    if constexpr (ACT == kActHardSwish) return hswish_derivative(z);
+    else return dsilu(z);
+}
 
 // Training set: |mu| half uniform on [mu_lo, 1), half log-uniform; ln xi and
 // gamma_k uniform over the hull. Counter (index lo, index hi, 'MLP', 0).
@@ -57,6 +74,7 @@ __global__ void __launch_bounds__(kBlk) normalize_kernel(float* __restrict__ tar
     if (i < n) targets[2 * i] = (targets[2 * i] - mean) * inv_scale;
 }
 
+template <int ACT>
 __global__ void __launch_bounds__(kBlk) fwd_kernel(const float* __restrict__ params, const float* __restrict__ x,
                                                    uint64_t n, float* __restrict__ z, float* __restrict__ yhat) {
     __shared__ float w[kMlpParams];
@@ -71,7 +89,7 @@ __global__ void __launch_bounds__(kBlk) fwd_kernel(const float* __restrict__ par
     for (int j = 0; j < kMlpH; ++j) {
         const float v = w[kOffB1 + j] + w[kOffW1 + 3 * j] * x0 + w[kOffW1 + 3 * j + 1] * x1 + w[kOffW1 + 3 * j + 2] * x2;
         if (zb) zb[j] = v;
-        h[j] = silu(v);
+        h[j] = act_f<ACT>(v);
     }
 #pragma unroll
     for (int j = 0; j < kMlpH; ++j) {
@@ -79,7 +97,7 @@ __global__ void __launch_bounds__(kBlk) fwd_kernel(const float* __restrict__ par
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k) v += w[kOffW2 + kMlpH * j + k] * h[k];
         if (zb) zb[kMlpH + j] = v;
-        g[j] = silu(v);
+        g[j] = act_f<ACT>(v);
     }
 #pragma unroll
     for (int j = 0; j < kMlpH; ++j) {
@@ -87,7 +105,7 @@ __global__ void __launch_bounds__(kBlk) fwd_kernel(const float* __restrict__ par
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k) v += w[kOffW3 + kMlpH * j + k] * g[k];
         if (zb) zb[2 * kMlpH + j] = v;
-        h[j] = silu(v);
+        h[j] = act_f<ACT>(v);
     }
 #pragma unroll
     for (int o = 0; o < kMlpOut; ++o) {
@@ -99,6 +117,7 @@ __global__ void __launch_bounds__(kBlk) fwd_kernel(const float* __restrict__ par
 }
 
 // Loss L = 1/(2n) sum_b sum_o (yhat - y)^2; dL/dyhat = (yhat - y) / n.
+template <int ACT>
 __global__ void __launch_bounds__(kBlk) bwd_data_kernel(const float* __restrict__ params, const float* __restrict__ z,
                                                         const float* __restrict__ yhat, const float* __restrict__ y,
                                                         uint64_t n, float* __restrict__ dz, float* __restrict__ dy,
@@ -121,14 +140,14 @@ __global__ void __launch_bounds__(kBlk) bwd_data_kernel(const float* __restrict_
         float d3[kMlpH], d2[kMlpH];
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k)
-            d3[k] = (w[kOffW4 + k] * d0 + w[kOffW4 + kMlpH + k] * d1) * dsilu(zb[2 * kMlpH + k]);
+            d3[k] = (w[kOffW4 + k] * d0 + w[kOffW4 + kMlpH + k] * d1) * act_df<ACT>(zb[2 * kMlpH + k]);
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k) {
             db[2 * kMlpH + k] = d3[k];
             float v = 0.0f;
 #pragma unroll
             for (int j = 0; j < kMlpH; ++j) v += w[kOffW3 + kMlpH * j + k] * d3[j];
-            d2[k] = v * dsilu(zb[kMlpH + k]);
+            d2[k] = v * act_df<ACT>(zb[kMlpH + k]);
         }
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k) {
@@ -136,7 +155,7 @@ __global__ void __launch_bounds__(kBlk) bwd_data_kernel(const float* __restrict_
             float v = 0.0f;
 #pragma unroll
             for (int j = 0; j < kMlpH; ++j) v += w[kOffW2 + kMlpH * j + k] * d2[j];
-            db[k] = v * dsilu(zb[k]);
+            db[k] = v * act_df<ACT>(zb[k]);
         }
     }
     // fixed-order block reduction of the loss
@@ -155,6 +174,7 @@ __global__ void __launch_bounds__(kBlk) bwd_data_kernel(const float* __restrict_
 // chunk's samples in sample order; grad = sum over chunks in chunk order.
 constexpr uint64_t kGradChunk = 128;
 
+template <int ACT>
 __global__ void __launch_bounds__(kBlk) bwd_weight_kernel(const float* __restrict__ x, const float* __restrict__ z,
                                                           const float* __restrict__ dz, const float* __restrict__ dy,
                                                           uint64_t n_total, float* __restrict__ partial) {
@@ -176,19 +196,19 @@ __global__ void __launch_bounds__(kBlk) bwd_weight_kernel(const float* __restric
         for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + i];
     } else if (q < kOffB2) {
         const int i = (q - kOffW2) / kMlpH, j = (q - kOffW2) % kMlpH;
-        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + kMlpH + i] * silu(z[b * kZ + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + kMlpH + i] * act_f<ACT>(z[b * kZ + j]);
     } else if (q < kOffW3) {
         const int i = q - kOffB2;
         for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + kMlpH + i];
     } else if (q < kOffB3) {
         const int i = (q - kOffW3) / kMlpH, j = (q - kOffW3) % kMlpH;
-        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + 2 * kMlpH + i] * silu(z[b * kZ + kMlpH + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + 2 * kMlpH + i] * act_f<ACT>(z[b * kZ + kMlpH + j]);
     } else if (q < kOffW4) {
         const int i = q - kOffB3;
         for (uint64_t b = 0; b < m; ++b) acc += dz[b * kZ + 2 * kMlpH + i];
     } else if (q < kOffB4) {
         const int i = (q - kOffW4) / kMlpH, j = (q - kOffW4) % kMlpH;
-        for (uint64_t b = 0; b < m; ++b) acc += dy[2 * b + i] * silu(z[b * kZ + 2 * kMlpH + j]);
+        for (uint64_t b = 0; b < m; ++b) acc += dy[2 * b + i] * act_f<ACT>(z[b * kZ + 2 * kMlpH + j]);
     } else {
         const int i = q - kOffB4;
         for (uint64_t b = 0; b < m; ++b) acc += dy[2 * b + i];
@@ -218,6 +238,7 @@ __global__ void __launch_bounds__(kBlk) adam_kernel(float* __restrict__ p, float
 }
 
 // BF16 MFMA evaluation of n samples, one per lane; lanes past n evaluate zeros.
+template <int ACT>
 __global__ void __launch_bounds__(kBlk) infer_kernel(const void* __restrict__ image, const float* __restrict__ x,
                                                      uint64_t n, float* __restrict__ out) {
     __shared__ uint4 lds[kMlpLdsBytes / 16];
@@ -233,7 +254,7 @@ __global__ void __launch_bounds__(kBlk) infer_kernel(const void* __restrict__ im
     }
     const float* b4 = reinterpret_cast<const float*>(static_cast<const char*>(image) + kMlpLdsBytes);
     float o0, o1;
-    mlp_eval_wave(im, f0, f1, f2, b4[0], b4[1], o0, o1);
+    mlp_eval_wave<ACT>(im, f0, f1, f2, b4[0], b4[1], o0, o1);
     if (b < n) {
         out[2 * b] = o0;
         out[2 * b + 1] = o1;
@@ -266,25 +287,38 @@ hipError_t launch_mlp_normalize(float* targets, uint64_t n, float mean, float in
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_forward(const float* params, const float* x, uint64_t n, float* z, float* yhat,
+hipError_t launch_mlp_forward(const float* params, int act, const float* x, uint64_t n, float* z, float* yhat,
                               hipStream_t stream) {
+    if (!mlp_activation_ok(act)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fwd_kernel, dim3(blocks(n)), dim3(kBlk), 0, stream, params, x, n, z, yhat);
+    if (act == kActHardSwish)
+        hipLaunchKernelGGL(fwd_kernel<kActHardSwish>, dim3(blocks(n)), dim3(kBlk), 0, stream, params, x, n, z, yhat);
+    else
+        hipLaunchKernelGGL(fwd_kernel<kActSilu>, dim3(blocks(n)), dim3(kBlk), 0, stream, params, x, n, z, yhat);
     return hipGetLastError();
 }
 
 size_t mlp_loss_partials(uint64_t n) { return blocks(n); }
 size_t mlp_grad_partials(uint64_t n) { return ((n + kGradChunk - 1) / kGradChunk) * kMlpParams; }
 
-hipError_t launch_mlp_backward(const float* params, const float* x, const float* z, const float* yhat,
+hipError_t launch_mlp_backward(const float* params, int act, const float* x, const float* z, const float* yhat,
                                const float* y, uint64_t n, float* dz, float* dy, float* grad_partials, float* grad,
                                float* loss_partials, hipStream_t stream) {
+    if (!mlp_activation_ok(act)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     const uint32_t chunks = static_cast<uint32_t>((n + kGradChunk - 1) / kGradChunk);
-    hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks(n)), dim3(kBlk), 0, stream, params, z, yhat, y, n, dz, dy,
-                       loss_partials);
-    hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks(kMlpParams), chunks), dim3(kBlk), 0, stream, x, z, dz, dy, n,
-                       grad_partials);
+    const dim3 data_grid(blocks(n)), weight_grid(blocks(kMlpParams), chunks);
+    if (act == kActHardSwish) {
+        hipLaunchKernelGGL(bwd_data_kernel<kActHardSwish>, data_grid, dim3(kBlk), 0, stream, params, z, yhat, y, n, dz,
+                           dy, loss_partials);
+        hipLaunchKernelGGL(bwd_weight_kernel<kActHardSwish>, weight_grid, dim3(kBlk), 0, stream, x, z, dz, dy, n,
+                           grad_partials);
+    } else {
+        hipLaunchKernelGGL(bwd_data_kernel<kActSilu>, data_grid, dim3(kBlk), 0, stream, params, z, yhat, y, n, dz, dy,
+                           loss_partials);
+        hipLaunchKernelGGL(bwd_weight_kernel<kActSilu>, weight_grid, dim3(kBlk), 0, stream, x, z, dz, dy, n,
+                           grad_partials);
+    }
     hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks(kMlpParams)), dim3(kBlk), 0, stream, grad_partials, chunks,
                        grad);
     return hipGetLastError();
@@ -297,16 +331,22 @@ hipError_t launch_mlp_adam(float* params, float* m, float* v, const float* grad,
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_infer(const void* image, const float* x, uint64_t n, float* out, hipStream_t stream) {
+hipError_t launch_mlp_infer(const void* image, int act, const float* x, uint64_t n, float* out,
+                            hipStream_t stream) {
+    if (!mlp_activation_ok(act)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(infer_kernel, dim3(blocks(n)), dim3(kBlk), 0, stream, image, x, n, out);
+    if (act == kActHardSwish)
+        hipLaunchKernelGGL(infer_kernel<kActHardSwish>, dim3(blocks(n)), dim3(kBlk), 0, stream, image, x, n, out);
+    else
+        hipLaunchKernelGGL(infer_kernel<kActSilu>, dim3(blocks(n)), dim3(kBlk), 0, stream, image, x, n, out);
     return hipGetLastError();
 }
 
 // Image = kFrags BF16 fragments | kBiasTiles FP32 accumulator initialisers |
 // b4[2] (FP32, read by the kernels from global memory).
 //
-// The image carries the network with the SiLU's exp2 scale folded in: every
+// A hard-swish network is stored as trained. A SiLU network is stored with the
+// SiLU's exp2 scale folded in: every
 // hidden pre-activation is stored as z' = -log2(e) z, so that
 // silu(z) = -ln(2) z' / (1 + 2^z'). Layer 1 (weights and bias column) and the
 // biases of layers 2 and 3 are scaled by -log2(e); the hidden activations are
@@ -314,9 +354,10 @@ hipError_t launch_mlp_infer(const void* image, const float* x, uint64_t n, float
 // 3 are unchanged (the factors cancel) and those of layer 4 are scaled by
 // -ln(2). Each activation then costs one multiply fewer. Scales are applied in
 // FP32 before the BF16 rounding.
-void mlp_build_image(const float* p, void* image) {
-    constexpr float kPre = -1.4426950408889634f;  // -log2(e)
-    constexpr float kOut = -0.6931471805599453f;  // -ln(2)
+void mlp_build_image(const float* p, int act, void* image) {
+    const bool fold = act == kActSilu;
+    const float kPre = fold ? -1.4426950408889634f : 1.0f;  // -log2(e)
+    const float kOut = fold ? -0.6931471805599453f : 1.0f;  // -ln(2)
     uint16_t* frag = static_cast<uint16_t*>(image);
     float* bias = reinterpret_cast<float*>(static_cast<char*>(image) + kMlpFragBytes);
     auto put = [&](int f, int lane, int i, float v) { frag[(static_cast<size_t>(f) * 64 + lane) * 8 + i] = to_bf16(v); };

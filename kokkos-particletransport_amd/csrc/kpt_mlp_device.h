@@ -40,12 +40,24 @@ __device__ __forceinline__ float silu_scaled(float zs) {
     return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
 }
 
+// Hard-swish z clamp(z / 6 + 1/2, 0, 1): one FMA with a [0, 1] clamp and a multiply.
+__device__ __forceinline__ float hard_swish(float z) {
+    return z * __builtin_amdgcn_fmed3f(__builtin_fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
+}
+
+template <int ACT>
+__device__ __forceinline__ float mlp_activation(float z) {
+    if constexpr (ACT == kActHardSwish) return hard_swish(z);
+    else return silu_scaled(z);
+}
+
 // Accumulators of one tile -> the B operands of K steps 2t and 2t + 1.
-__device__ __forceinline__ void silu_pack(const f32x16& acc, bf16x8& k0, bf16x8& k1) {
+template <int ACT>
+__device__ __forceinline__ void act_pack(const f32x16& acc, bf16x8& k0, bf16x8& k1) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        k0[i] = static_cast<__bf16>(silu_scaled(acc[i]));
-        k1[i] = static_cast<__bf16>(silu_scaled(acc[8 + i]));
+        k0[i] = static_cast<__bf16>(mlp_activation<ACT>(acc[i]));
+        k1[i] = static_cast<__bf16>(mlp_activation<ACT>(acc[8 + i]));
     }
 }
 
@@ -53,14 +65,15 @@ __device__ __forceinline__ void silu_pack(const f32x16& acc, bf16x8& k0, bf16x8&
 
 // One 32-sample column block through the network. Output rows 0 and 1 land in
 // registers 0 and 1 of the lanes of half 0.
+template <int ACT>
 __device__ __forceinline__ void mlp_block(const MlpLds& im, int lane, const bf16x8& xin, float b40, float b41,
                                           float& o0, float& o1) {
     const f32x16 zero = {};
     f32x16 a0 = KPT_MFMA(im.frag[(kFragA1 + 0) * 64 + lane], xin, zero);
     f32x16 a1 = KPT_MFMA(im.frag[(kFragA1 + 1) * 64 + lane], xin, zero);
     bf16x8 h0, h1, h2, h3;
-    silu_pack(a0, h0, h1);
-    silu_pack(a1, h2, h3);
+    act_pack<ACT>(a0, h0, h1);
+    act_pack<ACT>(a1, h2, h3);
 #pragma unroll
     for (int layer = 0; layer < 2; ++layer) {
         const int fa = layer == 0 ? kFragA2 : kFragA3;
@@ -74,8 +87,8 @@ __device__ __forceinline__ void mlp_block(const MlpLds& im, int lane, const bf16
         a1 = KPT_MFMA(im.frag[(fa + 6) * 64 + lane], h2, a1);
         a0 = KPT_MFMA(im.frag[(fa + 3) * 64 + lane], h3, a0);
         a1 = KPT_MFMA(im.frag[(fa + 7) * 64 + lane], h3, a1);
-        silu_pack(a0, h0, h1);
-        silu_pack(a1, h2, h3);
+        act_pack<ACT>(a0, h0, h1);
+        act_pack<ACT>(a1, h2, h3);
     }
     f32x16 y = {};
     if (lane < 32) {
@@ -93,6 +106,7 @@ __device__ __forceinline__ void mlp_block(const MlpLds& im, int lane, const bf16
 // The 64 samples of a wavefront, one per lane (features f0..f2 already scaled).
 // Every lane of the wavefront must execute this call: the MFMAs and the swaps of
 // the two lane halves involve all 64 lanes.
+template <int ACT>
 __device__ __forceinline__ void mlp_eval_wave(const MlpLds& im, float f0, float f1, float f2, float b40, float b41,
                                               float& o0, float& o1) {
     int lane = static_cast<int>(threadIdx.x & 63u);
@@ -112,11 +126,11 @@ __device__ __forceinline__ void mlp_eval_wave(const MlpLds& im, float f0, float 
         x1[3] = static_cast<__bf16>(1.0f);
     }
     float p0, p1, q0, q1;
-    mlp_block(im, lane, x0, b40, b41, p0, p1);  // samples of lanes 0..31
+    mlp_block<ACT>(im, lane, x0, b40, b41, p0, p1);  // samples of lanes 0..31
 #ifndef KPT_MLP_INTERLEAVE_BLOCKS
     __builtin_amdgcn_sched_barrier(0);          // one block's working set at a time
 #endif
-    mlp_block(im, lane, x1, b40, b41, q0, q1);  // samples of lanes 32..63, results in lanes 0..31
+    mlp_block<ACT>(im, lane, x1, b40, b41, q0, q1);  // samples of lanes 32..63, results in lanes 0..31
     const float r0 = __shfl_xor(q0, 32), r1 = __shfl_xor(q1, 32);
     o0 = lane < 32 ? p0 : r0;
     o1 = lane < 32 ? p1 : r1;

@@ -230,11 +230,13 @@ class SplitStatistics(C.Structure):
 
 
 class MlpHull(C.Structure):
-    """kpt_mlp_hull: training hull and output scaling of the pitch-angle network."""
+    """kpt_mlp_hull: training hull, output scaling and activation of the
+    pitch-angle network."""
     _fields_ = [
         ("mu_lo", C.c_double), ("log_xi_lo", C.c_double), ("log_xi_hi", C.c_double),
         ("gamma_lo", C.c_double), ("gamma_hi", C.c_double), ("h0", C.c_double),
         ("out_mean", C.c_double), ("out_scale", C.c_double),
+        ("activation", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -260,6 +262,8 @@ class MlpReport(C.Structure):
 
 
 MLP_PARAMS = 8706
+MLP_SILU, MLP_HARD_SWISH = 0, 1
+MLP_ACTIVATIONS = {"silu": MLP_SILU, "hard-swish": MLP_HARD_SWISH}
 
 
 def _dp(a):

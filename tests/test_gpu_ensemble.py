@@ -18,8 +18,9 @@ KINDS = ["parker", "parker", "qlt", "qlt", "mlp", "mlp"]
 FRAME = 3
 
 
-def _hull():
+def _hull(act=kpt.MLP_SILU):
     h = kpt.MlpHull()
+    h.activation = act
     h.mu_lo, h.log_xi_lo, h.log_xi_hi = 0.02, -4.0, 4.0
     h.gamma_lo, h.gamma_hi, h.h0 = 1.5, 2.0, 1.0e-6
     h.out_mean, h.out_scale = -3.0, 1.0
@@ -55,29 +56,30 @@ def _build(c, specs):
     return e
 
 
-def _alone(spec):
+def _alone(spec, act=kpt.MLP_SILU):
     s = _settings(spec)
     parts = W.c5_particles(spec, s, frame=FRAME)
     model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
     with kpt.Context(max(spec["count"], 1), seed=spec["seed"]) as c:
         if spec["kind"] == "mlp":
-            c.mlp_set(_params(), _hull())
+            c.mlp_set(_params(), _hull(act))
         c.field_upload(kpt.SLOT_LOWER, model, spec["xe"], spec["ye"], spec["planes"])
         c.upload(parts)
         st = c.push_interval(s, FRAME * s.mhd_dt, FRAME)
         return c.download(), st
 
 
-def test_ensemble_runs_equal_standalone_pushes():
+@pytest.mark.parametrize("act", [kpt.MLP_SILU, kpt.MLP_HARD_SWISH])
+def test_ensemble_runs_equal_standalone_pushes(act):
     specs = _specs()
     with kpt.Context(16) as c:
-        c.mlp_set(_params(), _hull())
+        c.mlp_set(_params(), _hull(act))
         with _build(c, specs) as e:
             stats, ms = e.push(FRAME, frames=1)
             got = [e.download(r) for r in range(len(specs))]
     assert ms > 0.0
     for r, spec in enumerate(specs):
-        want, st = _alone(spec)
+        want, st = _alone(spec, act)
         assert stats[r]["particle_steps"] == st.particle_steps, r
         assert stats[r]["failures"] == st.failures, r
         assert stats[r]["active_in"] == st.active_in, r

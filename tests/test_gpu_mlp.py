@@ -18,6 +18,13 @@ from helpers import chi2_two_sample
 pytestmark = pytest.mark.gpu
 
 HULL0 = kpt.mlp_default_train_opts().hull
+ACTS = [kpt.MLP_SILU, kpt.MLP_HARD_SWISH]
+
+
+def _hull(act):
+    h = kpt.MlpHull.from_buffer_copy(HULL0)
+    h.activation = act
+    return h
 
 
 def _random_net(seed=0):
@@ -31,7 +38,8 @@ def _random_net(seed=0):
     return p.astype(np.float32)
 
 
-def test_training_gradient_matches_oracle():
+@pytest.mark.parametrize("act", ACTS)
+def test_training_gradient_matches_oracle(act):
     """Forward + backward training kernels against the float64 oracle gradient
     (itself pinned by finite differences) on a 517-sample batch."""
     p = _random_net(1)
@@ -39,9 +47,9 @@ def test_training_gradient_matches_oracle():
     x = rng.uniform(-1, 1, (517, 3)).astype(np.float32)
     y = rng.standard_normal((517, 2)).astype(np.float32)
     with kpt.Context(16) as c:
-        c.mlp_set(p, HULL0)
+        c.mlp_set(p, _hull(act))
         g, loss = c.mlp_gradient(x, y)
-    lref, gref = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+    lref, gref = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64), act)
     assert loss == pytest.approx(lref, rel=1e-5)
     scale = np.max(np.abs(gref))
     assert np.max(np.abs(g - gref)) < 2e-5 * scale, np.max(np.abs(g - gref)) / scale
@@ -50,18 +58,20 @@ def test_training_gradient_matches_oracle():
         assert np.max(np.abs(g[a:b])) > 1e-4 * scale
 
 
-def test_mfma_inference_matches_bf16_restatement():
+@pytest.mark.parametrize("act", ACTS)
+def test_mfma_inference_matches_bf16_restatement(act):
     """The wavefront BF16 MFMA evaluation (n not a multiple of 64) against the
     CPU restatement with BF16 weights and activations, and against the FP32
     forward kernel within BF16 rounding."""
     p = _random_net(3)
     x = np.random.default_rng(4).uniform(-1, 1, (4000, 3)).astype(np.float32)
     with kpt.Context(16) as c:
-        c.mlp_set(p, HULL0)
+        c.mlp_set(p, _hull(act))
         ym = c.mlp_infer(x, mfma=True)
         yf = c.mlp_infer(x, mfma=False)
-    _, y64 = M.forward(p.astype(np.float64), x)
-    yb = M.forward_bf16(p, x)
+        assert c.mlp_get()[1].activation == act
+    _, y64 = M.forward(p.astype(np.float64), x, act)
+    yb = M.forward_bf16(p, x, act)
     scale = max(1.0, float(np.max(np.abs(y64))))
     np.testing.assert_allclose(yf, y64, rtol=1e-5, atol=1e-5 * scale)
     assert np.max(np.abs(ym - yb)) < 1e-2 * scale
@@ -69,18 +79,32 @@ def test_mfma_inference_matches_bf16_restatement():
     assert np.max(np.abs(ym - y64)) < 3e-2 * scale
 
 
-@pytest.fixture(scope="module")
-def trained():
-    """One training run on the GPU (2^20 exact-QLT samples, 4000 Adam steps)."""
+@pytest.fixture(scope="module", params=ACTS, ids=["silu", "hard-swish"])
+def trained(request):
+    """One training run on the GPU per activation (2^20 exact-QLT samples, 4000
+    Adam steps)."""
     o = kpt.mlp_default_train_opts()
     o.samples = 1 << 20
     o.steps = 4000
+    o.hull.activation = request.param
     with kpt.Context(16) as c:
         t0 = time.perf_counter()
         rep = c.mlp_train(o)
         rep["wall_s"] = time.perf_counter() - t0
         params, hull = c.mlp_get()
     return params, hull, rep
+
+
+def test_invalid_activation_is_refused():
+    h = _hull(kpt.MLP_SILU)
+    h.activation = 2
+    with kpt.Context(16) as c:
+        with pytest.raises(kpt.KptError, match="invalid pitch-angle network hull"):
+            c.mlp_set(_random_net(0), h)
+        o = kpt.mlp_default_train_opts()
+        o.hull.activation = -1
+        with pytest.raises(kpt.KptError, match="invalid pitch-angle network hull"):
+            c.mlp_train(o)
 
 
 def test_training_reaches_exact_qlt(trained):
