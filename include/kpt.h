@@ -516,9 +516,10 @@ kpt_status kpt_ensemble_inject(kpt_ensemble* e, int32_t run, uint64_t n, double 
  * counters; kernel_ms the device time of the push launches. */
 kpt_status kpt_ensemble_push(kpt_ensemble* e, uint32_t first_frame, uint32_t frames, int32_t use_graph,
                              kpt_ensemble_run_stats* per_run, double* kernel_ms);
-/* Time the candidate launch shapes on the first `window` (0 < window <= 1) of the
- * interval starting at frame `frame`, keep the fastest, and leave the particles as
- * they were. `table` (may be NULL) receives up to `table_len` candidates tried. */
+/* Time the candidate launch shapes ({1, 2, 3, 4} workgroups per CU x {8, 1}
+ * queues) on the first `window` (0 < window <= 1) of the interval starting at
+ * frame `frame`, keep the fastest, and leave the particles as they were. `table`
+ * (may be NULL) receives up to `table_len` candidates tried. */
 kpt_status kpt_ensemble_autotune(kpt_ensemble* e, uint32_t frame, double window, kpt_ensemble_tuning* chosen,
                                  kpt_ensemble_tuning* table, int32_t table_len, int32_t* tried);
 kpt_status kpt_ensemble_set_tuning(kpt_ensemble* e, const kpt_ensemble_tuning* t);

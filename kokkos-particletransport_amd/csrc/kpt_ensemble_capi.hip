@@ -522,7 +522,7 @@ kpt_status kpt_ensemble_autotune(kpt_ensemble* e, uint32_t frame, double window,
     const kpt_ensemble_tuning saved = e->tune;
     std::vector<kpt_ensemble_tuning> cands;
     for (int groups : {8, 1})
-        for (int mult : {1, 2, 4}) cands.push_back(kpt_ensemble_tuning{mult * e->cu_count, groups, 0.0});
+        for (int mult : {1, 2, 3, 4}) cands.push_back(kpt_ensemble_tuning{mult * e->cu_count, groups, 0.0});
     int best = -1;
     for (size_t c = 0; c <= cands.size(); ++c) {
         // candidate 0 runs twice: the first pass warms the caches and is not kept

@@ -508,14 +508,21 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
 // interpolation and coefficient arithmetic type (x, y, t, p and mu are carried in
 // double in both). CLOSURE: pitch-angle closure of the focused branch.
 // ---------------------------------------------------------------------------
-// Waves per SIMD of the push: the unified VGPR/AGPR file holds 512 registers
-// per lane, so 2 waves get 256 each and keep every corner load of a sub-step in
-// flight; more waves trade loads in flight per wave for latency hiding.
+// Waves per SIMD of the push. The unified VGPR/AGPR file holds 512 registers
+// per lane: 2 waves get 256 each and keep every corner load of a sub-step in
+// flight without spills; 3 and 4 waves spill registers to scratch but hide more
+// gather latency. Measured per launch on the 1024 x 512 stretched island field
+// (8 M focused / 4 M Parker particles), against 2 waves:
+//   FP64 focused: 3 waves +63%, 4 waves +106%;   FP64 Parker: 3 waves -1%, 4 +47%;
+//   FP32 focused: 3 waves -20%, 4 waves -29%;    FP32 Parker: 3 waves -20%, 4 -16%.
 #ifndef KPT_PUSH_WAVES_FP64
 #define KPT_PUSH_WAVES_FP64 2
 #endif
-#ifndef KPT_PUSH_WAVES_FP32
-#define KPT_PUSH_WAVES_FP32 2
+#ifndef KPT_PUSH_WAVES_FP32_PARKER
+#define KPT_PUSH_WAVES_FP32_PARKER 3
+#endif
+#ifndef KPT_PUSH_WAVES_FP32_FOCUSED
+#define KPT_PUSH_WAVES_FP32_FOCUSED 4
 #endif
 // One lane of the push: particle `idx` of the set in `a` (lanes past a.count
 // or with status != 0 only join the wave reduction of the counters). Every lane
@@ -765,7 +772,9 @@ __device__ __forceinline__ int64_t push_first_particle(uint32_t run) {
 
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 ? KPT_PUSH_WAVES_FP32 : KPT_PUSH_WAVES_FP64)))
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 8 ? KPT_PUSH_WAVES_FP64
+                                                      : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED
+                                                                 : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
     push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, push_first_particle(run));
 }
@@ -799,8 +808,10 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // the initial mu and at the two Picard trial values), lanes that are finished
 // or failed with dummy features.
 // ---------------------------------------------------------------------------
+// Waves per SIMD: 3 (256 VGPRs at 2 waves, 168 with spills to scratch at 3);
+// measured per launch against 2 waves: 3 waves -23%, 4 waves -16%.
 #ifndef KPT_PUSH_WAVES_MLP
-#define KPT_PUSH_WAVES_MLP 2
+#define KPT_PUSH_WAVES_MLP 3
 #endif
 // One lane of the learned-closure push (the network image already staged in
 // LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
@@ -1059,7 +1070,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
 // tiles from the next queues. Every workgroup leaves once all queues are
 // exhausted.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
+#ifndef KPT_ENSEMBLE_WAVES
+#define KPT_ENSEMBLE_WAVES KPT_PUSH_WAVES_MLP
+#endif
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_ENSEMBLE_WAVES)))
     ensemble_kernel(const EnsembleRun* __restrict__ runs, const uint2* __restrict__ tiles,
                     const uint32_t* __restrict__ group_start, int groups, unsigned int* __restrict__ queues,
                     const void* __restrict__ mlp_image, int has_mlp) {

@@ -45,27 +45,45 @@ __device__ __forceinline__ uint32_t node_key(const GridDesc& g, double x, double
            I % kSortBlock;
 }
 
+// Adds the workgroup's count of active slots (one per lane, summed over the
+// grid-stride loop) to *active with one atomic: a single counter hit once per
+// wavefront serialises in the L2 (about 12 ns per atomic, 1.3 ms per 8 M slots).
+__device__ __forceinline__ void add_block_count(uint32_t c, unsigned long long* active) {
+    __shared__ uint32_t part[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+        if (t != 0) atomicAdd(active, static_cast<unsigned long long>(t));
+    }
+}
+
+// Workgroups of the key kernels: a grid-stride loop over at most this many.
+constexpr uint64_t kKeyGrid = 2048;
+
 __global__ void __launch_bounds__(kBlock) sort_key_kernel(ParticleArrays P, uint64_t count, GridDesc g,
                                                           double shift_x, double shift_y, double t_done,
                                                           uint32_t inactive_key,
                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                           unsigned long long* __restrict__ active) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    bool act = false;
-    if (i < count) {
+    uint32_t n_act = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
         uint32_t key = inactive_key;
         if (P.status[i] == 0 && P.t[i] >= t_done) {
-            act = true;
+            ++n_act;
             key = inactive_key - 1;  // finished this interval: after the moving ones
         } else if (P.status[i] == 0) {
-            act = true;
+            ++n_act;
             key = node_key(g, P.x[i], P.y[i], shift_x, shift_y);
         }
         keys[i] = key;
         vals[i] = static_cast<uint32_t>(i);
     }
-    const uint64_t m = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(active, static_cast<unsigned long long>(__popcll(m)));
+    add_block_count(n_act, active);
 }
 
 __global__ void __launch_bounds__(kBlock) gather_kernel(ParticleArrays src, ParticleArrays dst, uint64_t count,
@@ -103,31 +121,44 @@ __global__ void __launch_bounds__(kBlock) ensemble_key_kernel(const SortRun* __r
                                                               uint32_t* __restrict__ keys,
                                                               uint32_t* __restrict__ vals,
                                                               unsigned long long* __restrict__ active) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    bool act = false;
-    if (i < total) {
-        int lo = 0, hi = n_runs - 1;  // last run with off <= i
+    const uint32_t top = (1u << run_shift) - 1u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t n_act = 0;
+    for (uint64_t base = blockIdx.x * static_cast<uint64_t>(kBlock); base < total;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t i = base + threadIdx.x;
+        // A wavefront's 64 slots are contiguous: the run of its first slot is found
+        // once, with scalar loads, and lanes past the next run's start step forward
+        // (only a wavefront that straddles a run boundary has such lanes).
+        const uint64_t i0 = base + 64u * wave;
+        int lo = 0, hi = n_runs - 1;  // last run with off <= i0
         while (lo < hi) {
             const int mid = (lo + hi + 1) / 2;
-            if (runs[mid].off <= i) lo = mid;
+            if (runs[mid].off <= i0) lo = mid;
             else hi = mid - 1;
         }
-        const SortRun& r = runs[lo];
-        const uint32_t top = (1u << run_shift) - 1u;
-        uint32_t key = top;
-        if (i - r.off < r.count) {
-            if (P.status[i] == 0) {
-                act = true;
-                key = node_key(r.g, P.x[i], P.y[i], r.shift_x, r.shift_y);
-            } else {
-                key = top - 1u;
-            }
+        int rl = lo;
+        while (rl + 1 < n_runs && runs[rl + 1].off <= i) ++rl;
+        if (i < total) {
+            auto slot_key = [&](const SortRun& r) {
+                uint32_t key = top;
+                if (i - r.off < r.count) {
+                    if (P.status[i] == 0) {
+                        ++n_act;
+                        key = node_key(r.g, P.x[i], P.y[i], r.shift_x, r.shift_y);
+                    } else {
+                        key = top - 1u;
+                    }
+                }
+                return key;
+            };
+            // one run per wavefront (the common case): its record by scalar loads
+            const uint32_t key = __ballot(rl != lo) == 0 ? slot_key(runs[lo]) : slot_key(runs[rl]);
+            keys[i] = (static_cast<uint32_t>(rl) << run_shift) | key;
+            vals[i] = static_cast<uint32_t>(i);
         }
-        keys[i] = (static_cast<uint32_t>(lo) << run_shift) | key;
-        vals[i] = static_cast<uint32_t>(i);
     }
-    const uint64_t m = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(active, static_cast<unsigned long long>(__popcll(m)));
+    add_block_count(n_act, active);
 }
 
 }  // namespace
@@ -143,7 +174,8 @@ hipError_t launch_ensemble_sort(const SortRun* runs, int n_runs, uint64_t total,
                                 unsigned long long* active, hipStream_t stream) {
     if (total == 0) return hipSuccess;
     const unsigned blocks = static_cast<unsigned>((total + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(ensemble_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, runs, n_runs, total, src,
+    const unsigned key_blocks = static_cast<unsigned>(std::min<uint64_t>(blocks, kKeyGrid));
+    hipLaunchKernelGGL(ensemble_key_kernel, dim3(key_blocks), dim3(kBlock), 0, stream, runs, n_runs, total, src,
                        run_shift, w.keys[0], w.vals[0], active);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -173,7 +205,8 @@ hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uin
     const uint32_t blocks_y = (static_cast<uint32_t>(g.ny) + 2 + kSortBlock - 1) / kSortBlock;
     const uint32_t inactive_key = blocks_x * blocks_y * kSortBlock * kSortBlock + 1;
     const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(sort_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, count, g, shift_x, shift_y,
+    const unsigned key_blocks = static_cast<unsigned>(std::min<uint64_t>(blocks, kKeyGrid));
+    hipLaunchKernelGGL(sort_key_kernel, dim3(key_blocks), dim3(kBlock), 0, stream, src, count, g, shift_x, shift_y,
                        t_done, inactive_key, w.keys[0], w.vals[0], active);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -120,7 +120,7 @@ def test_autotune_restores_state_and_shapes_do_not_change_results():
         with _build(c, specs) as e:
             before = [e.download(r) for r in range(len(specs))]
             chosen, table = e.autotune(FRAME, window=0.5)
-            assert len(table) == 6 and all(t["ms"] > 0.0 for t in table)
+            assert len(table) == 8 and all(t["ms"] > 0.0 for t in table)
             assert chosen in table and chosen["ms"] == min(t["ms"] for t in table)
             for r in range(len(specs)):
                 now = e.download(r)
