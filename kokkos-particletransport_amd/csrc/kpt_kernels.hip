@@ -514,7 +514,8 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
 // gather latency. Measured per launch on the 1024 x 512 stretched island field
 // (8 M focused / 4 M Parker particles), against 2 waves:
 //   FP64 focused: 3 waves +63%, 4 waves +106%;   FP64 Parker: 3 waves -1%, 4 +47%;
-//   FP32 focused: 3 waves -20%, 4 waves -29%;    FP32 Parker: 3 waves -20%, 4 -16%.
+//   FP32 focused: 3 waves -20%, 4 waves -29%, 5 waves +1%;
+//   FP32 Parker: 3 waves -20%, 4 waves -16%.
 #ifndef KPT_PUSH_WAVES_FP64
 #define KPT_PUSH_WAVES_FP64 2
 #endif

@@ -31,7 +31,8 @@ def test_help_lists_reference_options(app):
     assert r.returncode == 0
     for opt in ("--profile", "--transport", "--restart-particle-snapshot", "--overwrite-output", "--frames",
                 "--rank-scale", "--histogram-interval", "--particle-snapshot-interval", "--walltime-hours",
-                "--walltime-reserve-minutes", "--no-split", "--no-time-interp", "--precision", "--closure"):
+                "--walltime-reserve-minutes", "--no-split", "--no-time-interp", "--precision", "--closure",
+                "--mlp-activation"):
         assert opt in r.stdout, opt
 
 
@@ -48,6 +49,7 @@ def test_help_lists_reference_options(app):
     (["--restart-particle-snapshot", "/tmp/snap.bin"],
      "Cannot infer restart frame from particle snapshot name; expected particles_XXXXX.bin."),
     (["--closure", "mlp", "--transport", "focused"], "the learned pitch-angle closure runs with --precision fp32"),
+    (["--mlp-activation", "tanh"], "Unknown network activation: tanh"),
 ])
 def test_validation_messages(app, args, msg):
     r = _run(*args)
