@@ -29,6 +29,13 @@ dt = mhd_dt / 50. One bench step is a cell sort of every run followed by a
 interval, the loop replayed from a hipGraph, the launch shape chosen by the
 autotuner first.
 
+Workload `generic`: the generic global-dt solvers (--solver parker: ParkerSolver,
+--solver focused: FocusTransportSolver) on the 1024 x 512 island field plus a
+0.2 guide field (uniform, periodic), 4 194 304 particles; one bench step is 20
+solver steps, each a global dt reduction over all particles and the stochastic
+step, FP64, on particles cell-sorted once before the steps. Its roofline prices both passes and takes the time per solver step
+from the host clock around the synchronous calls.
+
 Prints one JSON line. `value` is particle-steps/s over the timed steps (exact
 sub-step counts from the kernel). `roofline` prices the push kernel's algorithmic
 bytes (frames x 4 corners x components x word size per sub-step, plus the
@@ -53,6 +60,7 @@ import workloads as W  # noqa: E402
 
 METRIC = "particle-steps/s, FP32+MLP-D_μμ vs FP64 plain push, at matched spectrum χ² error"
 HBM_PEAK_GBS = 8000.0
+GENERIC_STEPS = 20  # solver steps per bench step of the generic workload
 
 
 def log(msg):
@@ -169,6 +177,128 @@ def profiled_traffic(workload):
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
                     return float(parts[2])
     return None
+
+
+def run_generic(args):
+    """The generic global-dt solvers (ParkerSolver / FocusTransportSolver): one
+    bench step is `GENERIC_STEPS` solver steps of every particle, each a global dt
+    reduction over all particles followed by the stochastic step."""
+    focus = args.solver == "focused"
+    n = args.particles or 4194304
+    xe, ye, planes = W.island_field(args.nx, args.ny, seed=7)
+    planes[2] = planes[2] + 0.2  # guide field: |B| > 0 everywhere (the solvers abort at nulls)
+    if focus:
+        p = kpt.generic_focus_default_params()
+        p.model.perpendicular_model, p.model.perpendicular_parallel_ratio = 1, 0.05
+        p.model.minimum_solar_wind_speed = 1e-3
+        p.model.sigma2, p.model.lslab, p.model.l2d, p.model.lc = 0.1, 0.05, 0.02, 0.02
+        p.maximum_pitch_angle_cosine = 0.995
+        comps = 27
+    else:
+        p = kpt.generic_default_params()
+        p.perpendicular_model, p.perpendicular_parallel_ratio = 1, 0.05
+        p.minimum_solar_wind_speed = 1e-3
+        p.sigma2, p.lslab, p.l2d = 0.3, 0.05, 0.02
+        comps = 18
+    rng = np.random.default_rng(3)
+    parts = kpt.empty_particles(n)
+    parts["x"][:] = rng.uniform(xe[0], xe[-1], n)
+    parts["y"][:] = rng.uniform(ye[0], ye[-1], n)
+    parts["p"][:] = np.exp(rng.uniform(np.log(0.5), np.log(50.0), n))
+    parts["mu"][:] = rng.uniform(-1, 1, n)
+    parts["id"][:] = np.arange(n)
+    k = GENERIC_STEPS
+    log(f"bench: generic {args.solver} solver, {args.nx}x{args.ny}, {n} particles, {k} global-dt steps per step")
+    with kpt.Context(n, seed=114514) as ctx:
+        ctx.field_upload(kpt.SLOT_LOWER, kpt.MODEL_PARKER16, xe, ye, planes)
+        if focus:
+            ctx.focus_build(p, kpt.SLOT_LOWER)
+        else:
+            ctx.generic_build(p, kpt.SLOT_LOWER)
+        ctx.upload(parts)
+        # ParticleSystem::sort_by_key before the steps, as a production loop does:
+        # the stencil gathers of a wavefront then share cache lines (particles move
+        # far less than a cell over the timed steps)
+        ctx.sort(kpt.make_device_settings(kpt.default_host_settings(), xe, ye))
+        step_fn = ctx.focus_step if focus else ctx.generic_step
+        step = 0
+        fails = 0
+
+        def one_bench_step():
+            nonlocal step, fails
+            for _ in range(k):
+                _, nf = step_fn(p, step)
+                fails += nf
+                step += 1
+
+        for w in range(args.warmup):
+            one_bench_step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            one_bench_step()
+        elapsed = time.perf_counter() - t0
+        active = int(np.sum(ctx.download()["status"] == 0))
+    # each solver step runs two passes over the particles, every pass reading the
+    # 4-corner stencil of the FP64 basis and the particle state
+    state_dt = 4 + 24 + (8 if focus else 0)                  # status, x, y, p (, mu)
+    state_step = state_dt + 8 + 24 + (8 if focus else 0)      # + id, written x, y, p (, mu)
+    per_step = 2 * 4 * comps * 8 + state_dt + state_step
+    steps_total = n * k * args.steps
+    value = steps_total / elapsed
+    secs_per_solver_step = elapsed / (k * args.steps)
+    achieved = n * per_step / secs_per_solver_step / 1e9
+    workload = f"generic_{args.solver}_{args.nx}x{args.ny}_island_{n}"
+    out = {
+        "metric": f"particle-steps/s, generic {args.solver} solver (global dt), FP64",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded island-chain Harris field plus a 0.2 guide field; uniform x, y, mu, "
+                "log-uniform p)",
+        "config": {"workload": workload, "solver": args.solver, "field": f"{args.nx}x{args.ny} uniform, periodic",
+                   "particles": n, "solver_steps_per_bench_step": k, "failures": int(fails),
+                   "active_after": active, "cell_sorted": True},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(workload),
+            "kernel": ("focus" if focus else "generic") + "_dt_kernel + " + ("focus" if focus else "generic")
+                      + "_step_kernel",
+            "timing": "host clock per synchronous solver step (both launches and the dt read-back)",
+            "ms_per_solver_step": secs_per_solver_step * 1e3,
+            "algorithmic_bytes_per_solver_step": n * per_step,
+            "bytes_per_particle_step": per_step,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import kpt_oracle
+
+        field = (kpt_oracle.GenericFocusField(xe, ye, planes, p) if focus
+                 else kpt_oracle.GenericField(xe, ye, planes, p))
+        m, done, t_cpu, s_cpu = 16384, 0, 0.0, 0
+        sample = {key: v[:m].copy() for key, v in parts.items()}
+        while t_cpu < 10.0 and s_cpu < 2000:
+            t1 = time.perf_counter()
+            field.step(sample, s_cpu, seed=1)
+            t_cpu += time.perf_counter() - t1
+            s_cpu += 1
+            done += m
+        out["cpu_baseline"] = {"value": done / t_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{m} of the bench particles, {s_cpu} global-dt steps ({t_cpu:.1f} s), "
+                                         "serial FP64 restatement of the solver's advance_stochastic"}
+    print(json.dumps(out), flush=True)
 
 
 def run_ensemble(args):
@@ -303,7 +433,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["focused", "parker", "ensemble"], default="focused")
+    ap.add_argument("--workload", choices=["focused", "parker", "ensemble", "generic"], default="focused")
+    ap.add_argument("--solver", choices=["parker", "focused"], default="parker",
+                    help="generic workload: ParkerSolver or FocusTransportSolver")
     ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
     ap.add_argument("--sort-every", type=int, default=1,
                     help="ensemble: intervals between batched cell sorts of every run")
@@ -339,6 +471,9 @@ def main():
 
     if args.workload == "ensemble":
         run_ensemble(args)
+        return
+    if args.workload == "generic":
+        run_generic(args)
         return
     focused = args.workload == "focused"
     prec = args.precision or ("fp32" if focused else "fp64")

@@ -16,6 +16,7 @@
 // ghost-filled intermediate fields on a periodic grid.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -279,14 +280,17 @@ __global__ void __launch_bounds__(kBlk) focus_dt_kernel(ParticleArrays P, uint64
                                                         const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                         kpt_generic_focus_params p,
                                                         unsigned long long* __restrict__ min_bits) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
     double m = 1.0e300;
-    if (i < count && P.status[i] == 0) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * kBlk) {
+        if (P.status[i] != 0) continue;
         FocTerms T;
-        if (focus_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], P.mu[i], T))
-            m = focus_local_dt(g, T);
+        if (focus_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], P.mu[i], T)) {
+            const double l = focus_local_dt(g, T);
+            m = l < m ? l : m;
+        }
     }
-    wave_min_to(m, min_bits);
+    block_min_to(m, min_bits);
 }
 
 __global__ void __launch_bounds__(kBlk) focus_step_kernel(ParticleArrays P, uint64_t count,
@@ -353,7 +357,8 @@ hipError_t launch_focus_pack(const double* raw, const double* dvdt, const GridDe
 hipError_t launch_focus_min_dt(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
                                const kpt_generic_focus_params& p, unsigned long long* min_bits, hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(focus_dt_kernel, dim3(blocks(count)), dim3(kBlk), 0, stream, P, count, f,
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(blocks(count), kMinDtGrid));
+    hipLaunchKernelGGL(focus_dt_kernel, dim3(grid), dim3(kBlk), 0, stream, P, count, f,
                        coefficient_plane(g.nx, g.ny), g, p, min_bits);
     return hipGetLastError();
 }

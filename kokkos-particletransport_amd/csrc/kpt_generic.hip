@@ -13,6 +13,7 @@
 // invariant direction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -143,14 +144,17 @@ __global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint
                                                           const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                           kpt_generic_parker_params p,
                                                           unsigned long long* __restrict__ min_bits) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
     double m = 1.0e300;
-    if (i < count && P.status[i] == 0) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * kBlk) {
+        if (P.status[i] != 0) continue;
         GenTerms T;
-        if (generic_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], T))
-            m = generic_local_dt(g, T);
+        if (generic_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], T)) {
+            const double l = generic_local_dt(g, T);
+            m = l < m ? l : m;
+        }
     }
-    wave_min_to(m, min_bits);
+    block_min_to(m, min_bits);
 }
 
 __global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, uint64_t count,
@@ -212,7 +216,8 @@ hipError_t launch_generic_min_dt(const ParticleArrays& P, uint64_t count, const 
                                  const kpt_generic_parker_params& p, unsigned long long* min_bits,
                                  hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(generic_dt_kernel, dim3(blocks(count)), dim3(kBlk), 0, stream, P, count, f,
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(blocks(count), kMinDtGrid));
+    hipLaunchKernelGGL(generic_dt_kernel, dim3(grid), dim3(kBlk), 0, stream, P, count, f,
                        coefficient_plane(g.nx, g.ny), g, p, min_bits);
     return hipGetLastError();
 }

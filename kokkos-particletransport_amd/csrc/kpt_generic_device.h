@@ -127,16 +127,27 @@ __device__ __forceinline__ void philox_normals4(uint64_t id, uint32_t tag, uint3
     }
 }
 
-// Wave-wide minimum of a positive double, then one atomicMin on its bit
-// pattern per wave: positive doubles order like their bits, so the result is
-// exact and independent of the order of the atomics.
-__device__ __forceinline__ void wave_min_to(double m, unsigned long long* min_bits) {
+// Workgroup-wide minimum of a positive double, then one atomicMin on its bit
+// pattern per workgroup: positive doubles order like their bits, so the result
+// is exact and independent of the order of the atomics. (One atomic per
+// wavefront on the single counter serialises in the L2.) Every thread of the
+// workgroup must call it.
+__device__ __forceinline__ void block_min_to(double m, unsigned long long* min_bits) {
+    __shared__ double part[16];  // up to 1024 threads
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const double o = __shfl_xor(m, off, 64);
         m = o < m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(min_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < (blockDim.x + 63) / 64; ++w) m = part[w] < m ? part[w] : m;
+        atomicMin(min_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
 }
+
+// Workgroups of the dt reductions: a grid-stride loop over at most this many.
+constexpr uint64_t kMinDtGrid = 2048;
 
 }  // namespace kpt
