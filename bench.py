@@ -452,6 +452,9 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=8000, help="Adam steps of the network training")
     ap.add_argument("--mlp-activation", choices=sorted(kpt.MLP_ACTIVATIONS), default="hard-swish",
                     help="hidden-layer activation of the learned closure")
+    ap.add_argument("--mlp-passes", type=int, choices=[1, 3], default=1,
+                    help="network evaluations per sub-step: 1 (the Picard passes extend the first "
+                         "evaluation) or 3 (one per Picard pass)")
     ap.add_argument("--no-fp64-leg", action="store_true",
                     help="focused: skip the plain FP64 push of the same particles")
     ap.add_argument("--no-fp32-leg", action="store_true",
@@ -488,6 +491,7 @@ def main():
         closure = kpt.CLOSURE_MLP if args.closure == "mlp" else kpt.CLOSURE_QLT
         s = W.c3_device_settings(xe, ye, closure=closure,
                                  precision=kpt.PREC_FP32 if fp32 else kpt.PREC_FP64)
+        s.mlp_passes = args.mlp_passes
         start = W.c3_particles(n, s, seed=11)
     else:
         n = args.particles or 4 * 1024 * 1024

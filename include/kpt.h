@@ -132,7 +132,13 @@ typedef struct kpt_device_settings {
      *    derivative taken from the loaded MLP inside its training hull
      *    (kpt_mlp_set) and from the exact QLT outside it. */
     int32_t closure;
-    int32_t reserved1;
+    /* KPT_CLOSURE_MLP: network evaluations per sub-step.
+     *  0 or 1: one, at the sub-step's initial mu; the two Picard passes take
+     *    ln(D / (1 - mu^2)) at their trial mu to first order in ln|mu| from that
+     *    evaluation, with the network's own slope output d ln R / d ln|mu|;
+     *  3: one at each of the three Picard passes.
+     * A trial mu outside the training hull takes the exact formula either way. */
+    int32_t mlp_passes;
     double qlt_sigma2;          /* delta B^2 / B0^2 */
     double qlt_correlation_length;
     double qlt_spectral_index;  /* gamma_k, PitchAngleScatteringParameters default 5/3 */

@@ -172,6 +172,10 @@ kpt_status kpt_capi::check_settings(kpt_ctx* ctx, const kpt_device_settings* s) 
                         "LegencyModel::PitchAngleScatteringParameters: coefficients and floors must be finite, "
                         "with spectral_index > 1, resonance_regularization_h0 > 0, sigma2 > 0, Lc > 0 and "
                         "0 < mu_max <= 1.");
+        if (s->closure == KPT_CLOSURE_MLP && s->mlp_passes != 0 && s->mlp_passes != 1 && s->mlp_passes != 3)
+            return fail(ctx, KPT_ERR_ARG,
+                        "mlp_passes must be 0 or 1 (one network evaluation per sub-step) or 3 (one per Picard "
+                        "pass).");
     }
     return KPT_OK;
 }
@@ -1444,6 +1448,7 @@ kpt_status mlp_install(kpt_ctx* ctx, const float* params, const kpt_mlp_hull& h)
     KPT_HIP(ctx, hipMemcpy(ctx->mlp_image, image.data(), image.size(), hipMemcpyHostToDevice));
     ctx->mlp_hull = h;
     ctx->mlp_valid = true;
+    ++ctx->mlp_version;
     return KPT_OK;
 }
 

@@ -82,8 +82,10 @@ struct kpt_ctx {
     float* pair32 = nullptr;
     size_t pair32_cap = 0;
     uint64_t pair_versions[2] = {0, 0};
-    // learned closure: FP32 parameters (device and host copies), hull, MFMA image
+    // learned closure: FP32 parameters (device and host copies), hull, MFMA image;
+    // mlp_version counts installs (ensembles rebuild their run tables on a change)
     bool mlp_valid = false;
+    uint64_t mlp_version = 0;
     kpt_mlp_hull mlp_hull{};
     float* mlp_params = nullptr;
     void* mlp_image = nullptr;

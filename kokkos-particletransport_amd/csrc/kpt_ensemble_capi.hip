@@ -38,6 +38,7 @@ struct kpt_ensemble {
     unsigned int* d_queues = nullptr;  // 8 entries
     kpt::EnsembleLoop* d_loop = nullptr;
     bool dirty = true;       // run table or tiles need a rebuild
+    uint64_t mlp_version = 0;  // the context's network install the run table holds
     bool has_mlp = false;
     uint32_t n_tiles = 0;
     kpt_ensemble_tuning tune{0, 8, 0.0};
@@ -123,8 +124,10 @@ void drop_graph(kpt_ensemble* e) {
 
 // Run table and tile lists from the runs' fields, settings and counts.
 kpt_status ens_sync(kpt_ensemble* e) {
-    if (!e->dirty) return KPT_OK;
     kpt_ctx* ctx = e->ctx;
+    // the table copies the network's activation, feature scaling and on/off flag:
+    // a kpt_mlp_set or kpt_mlp_train since the last build invalidates it
+    if (!e->dirty && e->mlp_version == ctx->mlp_version) return KPT_OK;
     std::vector<kpt::EnsembleRun> table(static_cast<size_t>(e->runs));
     e->has_mlp = false;
     for (int r = 0; r < e->runs; ++r) {
@@ -143,6 +146,9 @@ kpt_status ens_sync(kpt_ensemble* e) {
                 return fail(ctx, KPT_ERR_ARG,
                             tag + "a FOCUSED21 run must be focused transport with the QLT or learned closure.");
             R.kind = s.closure == KPT_CLOSURE_MLP ? kpt::kEnsFocusedMlp32 : kpt::kEnsFocusedQlt32;
+            if (s.closure == KPT_CLOSURE_MLP && s.mlp_passes == 3)
+                return fail(ctx, KPT_ERR_ARG,
+                            tag + "the batched ensemble evaluates the network once per sub-step (mlp_passes 0 or 1).");
         }
         a.P = arrays_at(e->P, e->off[r]);
         a.count = e->count[r];
@@ -216,6 +222,7 @@ kpt_status ens_sync(kpt_ensemble* e) {
     KPT_HIP(ctx, hipMemcpy(e->d_gstart, gstart.data(), 9 * sizeof(uint32_t), hipMemcpyHostToDevice));
     KPT_HIP(ctx, hipMemcpy(e->d_runs, table.data(), table.size() * sizeof(kpt::EnsembleRun), hipMemcpyHostToDevice));
     e->dirty = false;
+    e->mlp_version = ctx->mlp_version;
     return KPT_OK;
 }
 

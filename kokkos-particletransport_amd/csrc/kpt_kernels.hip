@@ -181,8 +181,9 @@ __device__ __forceinline__ double rabs(double v) { return fabs(v); }
 __device__ __forceinline__ float rabs(float v) { return fabsf(v); }
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
 __device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
-__device__ __forceinline__ double rsqrt_(double v) { return sqrt(v); }
-__device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_sqrtf(v); }
+// rt_sqrt is the square root (not its reciprocal) in the step's type.
+__device__ __forceinline__ double rt_sqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float rt_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ double rrcp(double v) { return 1.0 / v; }
 __device__ __forceinline__ float rrcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
@@ -296,10 +297,10 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
     const RT kpara = k.kpara0 * knorm;
     const RT kperp = kpara * k.kperp_ratio;
     if (!rfinite(kpara) || kpara <= RT(0) || !rfinite(kperp) || kperp < RT(0)) return false;
-    o.skpara = rsqrt_(RT(2) * kpara);
-    o.skperp = rsqrt_(RT(2) * kperp);
-    o.skpara_perp = MODEL == 16 ? rsqrt_(RT(2) * (kpara - kperp))
-                                : (kpara > kperp ? rsqrt_(RT(2) * (kpara - kperp)) : RT(0));
+    o.skpara = rt_sqrt(RT(2) * kpara);
+    o.skperp = rt_sqrt(RT(2) * kperp);
+    o.skpara_perp = MODEL == 16 ? rt_sqrt(RT(2) * (kpara - kperp))
+                                : (kpara > kperp ? rt_sqrt(RT(2) * (kpara - kperp)) : RT(0));
     const RT kpp = FOCUSED ? -kperp : kpara - kperp;
 
     RT dkdx = RT(0), dkdy = RT(0);
@@ -316,7 +317,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
                        kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - RT(2) * bx * by * db_dx * ib3);
     const RT dkxy_dy = kpp * dkdy * bx * by * ib2 +
                        kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - RT(2) * bx * by * db_dy * ib3);
-    const RT drift_den = rsqrt_(sqr(rdiv(k.drift_c1, p)) + sqr(rdiv(k.drift_c2, p * p)));
+    const RT drift_den = rt_sqrt(sqr(rdiv(k.drift_c1, p)) + sqr(rdiv(k.drift_c2, p * p)));
 
     o.bx = bx;
     o.by = by;
@@ -340,7 +341,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
         const RT vdy = vdp * (muf1 * (bz * db_dx) * ib2 + mu2 * (bz * kx - bx * kz) * ib3 +
                               muf1 * by * bdot_curvb * ib3);
         // QLT: relativistic speed p / gamma (m = c = 1)
-        const RT gamma_p = QLT ? rsqrt_(RT(1) + p * p) : RT(1);
+        const RT gamma_p = QLT ? rt_sqrt(RT(1) + p * p) : RT(1);
         const RT speed = QLT ? rdiv(p, gamma_p) : rdiv(k.v0 * p, k.p0);
         if (!rfinite(speed) || speed <= RT(0)) return false;
         const RT vpb = speed * mu * ib;
@@ -677,12 +678,12 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
                 r3 = draw_sqrt3<FT>(rb[3]);
             }
             const FT dtr = static_cast<FT>(dt);
-            const FT sdt = rsqrt_(dtr);
+            const FT sdt = rt_sqrt(dtr);
             FT delta_x, delta_y;
             if (FOCUSED) {
                 // src/main.cpp:1671-1702 (ran1, ran2, ignored, pitch)
                 const FT bxn = T.bx * T.ib, byn = T.by * T.ib, bzn = T.bz * T.ib;
-                const FT bxy = rsqrt_(bxn * bxn + byn * byn);
+                const FT bxy = rt_sqrt(bxn * bxn + byn * byn);
                 FT sx = FT(0), sy = FT(0);
                 if (bxy > FT(0)) {
                     const FT ibxy = rrcp(bxy);
@@ -709,14 +710,14 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
                                         FT(2) * rdiv(T.f_bv, T.f_speed)) +
                                    dD;
                         }
-                        const FT scale = D > FT(0) ? rsqrt_(FT(2) * D * dtr) : FT(0);
+                        const FT scale = D > FT(0) ? rt_sqrt(FT(2) * D * dtr) : FT(0);
                         trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3),
                                            s.mu_max);
                     }
                     pitch_mu = trial;
                 } else {
                     const double m = pitch_mu + static_cast<double>(T.dmu_dt * dtr) +
-                                     static_cast<double>(r3 * rsqrt_(FT(2) * T.duu) * sdt);
+                                     static_cast<double>(r3 * rt_sqrt(FT(2) * T.duu) * sdt);
                     pitch_mu = m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m);
                 }
             } else {
@@ -780,16 +781,34 @@ __global__ void __launch_bounds__(kBlock)
     push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, push_first_particle(run));
 }
 
-// Closure from the network outputs at pitch cosine mu (|mu| in the hull):
-// D = P (1 - mu^2) R, dD/dmu = P R ((1 - mu^2) sgn(mu) y1 / |mu| - 2 mu),
-// ln R = out_mean + out_scale y0.
-__device__ __forceinline__ void mlp_closure(float y0, float y1, float mu, float pref, const MlpScale& ms, float& D,
-                                            float& dD) {
+// Closure from the network outputs at pitch cosine mu (|mu| in the hull), with
+// log2 R = l2r = (out_mean + out_scale y0) log2(e) and y1 = d ln R / d ln|mu|:
+// D = P (1 - mu^2) R, dD/dmu = P R ((1 - mu^2) sgn(mu) y1 / |mu| - 2 mu).
+__device__ __forceinline__ float mlp_log2_r(float y0, const MlpScale& ms) {
+    return (ms.out_mean + ms.out_scale * y0) * 1.4426950408889634f;
+}
+__device__ __forceinline__ void mlp_closure_l2(float l2r, float y1, float mu, float pref, float& D, float& dD) {
     const float omm = 1.0f - mu * mu;
-    const float r = __builtin_amdgcn_exp2f((ms.out_mean + ms.out_scale * y0) * 1.4426950408889634f);
+    const float r = __builtin_amdgcn_exp2f(l2r);
     const float sgn = mu > 0.0f ? 1.0f : -1.0f;
     D = pref * omm * r;
     dD = pref * r * (omm * sgn * y1 * __builtin_amdgcn_rcpf(fabsf(mu)) - 2.0f * mu);
+}
+__device__ __forceinline__ void mlp_closure(float y0, float y1, float mu, float pref, const MlpScale& ms, float& D,
+                                            float& dD) {
+    mlp_closure_l2(mlp_log2_r(y0, ms), y1, mu, pref, D, dD);
+}
+
+// The closure at a Picard trial value m from the evaluation at the sub-step's
+// initial mu0: log2 R to first order in log2|mu| about mu0 with the network's
+// slope output there, log2 R(m) = log2 R(mu0) + y1 (log2|m| - log2|mu0|), and
+// the slope held at y1. R is close to a power of |mu| over one sub-step's pitch
+// change (|d y1 / d ln|mu|| <= gamma_k^2 / 4 for slab QLT), so the expansion
+// error is below the network's own.
+__device__ __forceinline__ void mlp_closure_extend(float l2r0, float y1, float l2mu0, float m, float pref, float& D,
+                                                   float& dD) {
+    const float l2m = __builtin_amdgcn_logf(fabsf(m));
+    mlp_closure_l2(l2r0 + y1 * (l2m - l2mu0), y1, m, pref, D, dD);
 }
 
 // One closure evaluation at mu: the network's outputs where the sample lies in
@@ -805,9 +824,12 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // ---------------------------------------------------------------------------
 // Focused FP32 push with the learned closure. Same mover as push_kernel<21,
 // true, float, QLT>, with the sub-step loop run wave-uniformly: every lane of
-// the wavefront takes part in the three network evaluations of a sub-step (D at
-// the initial mu and at the two Picard trial values), lanes that are finished
-// or failed with dummy features.
+// the wavefront takes part in the network evaluations of a sub-step, lanes that
+// are finished or failed with dummy features.
+// PASSES = 1: one evaluation per sub-step, at the initial mu; the two Picard
+// passes extend it to their trial values (mlp_closure_extend).
+// PASSES = 3: one evaluation at each Picard pass (the initial mu and the two
+// trial values).
 // ---------------------------------------------------------------------------
 // Waves per SIMD: 3 (256 VGPRs at 2 waves, 168 with spills to scratch at 3);
 // measured per launch against 2 waves: 3 waves -23%, 4 waves -16%.
@@ -817,9 +839,10 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // One lane of the learned-closure push (the network image already staged in
 // LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
 // together: the sub-step loop and the network evaluations are wave-uniform.
-template <bool STR = false, int ACT = kActSilu>
+template <bool STR = false, int ACT = kActSilu, int PASSES = 1>
 __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& im, const float b40, const float b41,
                                               const int64_t idx) {
+    static_assert(PASSES == 1 || PASSES == 3, "one or three network evaluations per sub-step");
     const MlpScale ms = a.mlp_scale;
     const StepConst<float>& K = a.k32;
     const kpt_device_settings& s = a.s;
@@ -853,7 +876,7 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
 
         // phase 1: sample, rates, features at the initial mu
         StepTerms<float> T{};
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lxi = 0.0f;
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lxi = 0.0f, l2mu0 = 0.0f;
         bool h0 = false;
         float half_dx = K.half_dx, half_dy = K.half_dy;
         if (run) {
@@ -904,7 +927,8 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
                     run = false;
                 } else {
                     lxi = __builtin_amdgcn_logf(T.q_xi) * 0.6931471805599453f;
-                    h0 = use_net && mlp_features(ms, fabsf(static_cast<float>(pitch_mu)), lxi, K.qlt_gk, f0, f1, f2);
+                    h0 = use_net &&
+                         mlp_features(ms, fabsf(static_cast<float>(pitch_mu)), lxi, K.qlt_gk, f0, f1, f2, l2mu0);
                 }
             }
         }
@@ -965,9 +989,9 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
                 }
                 if (run) {
                     dtr = static_cast<float>(dt);
-                    const float sdt = rsqrt_(dtr);
+                    const float sdt = rt_sqrt(dtr);
                     const float bxn = T.bx * T.ib, byn = T.by * T.ib, bzn = T.bz * T.ib;
-                    const float bxy = rsqrt_(bxn * bxn + byn * byn);
+                    const float bxy = rt_sqrt(bxn * bxn + byn * byn);
                     float sx = 0.0f, sy = 0.0f;
                     if (bxy > 0.0f) {
                         const float ibxy = rrcp(bxy);
@@ -976,37 +1000,65 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
                     }
                     delta_x = T.dx_dt * dtr + sx;
                     delta_y = T.dy_dt * dtr + sy;
-                    const float scale = D > 0.0f ? rsqrt_(2.0f * D * dtr) : 0.0f;
+                    const float scale = D > 0.0f ? rt_sqrt(2.0f * D * dtr) : 0.0f;
                     trial = reflect_mu(mu0 + static_cast<double>(dtr * T.dmu_dt) + static_cast<double>(scale * r3),
                                        s.mu_max);
-                    const float m1 = static_cast<float>(reflect_mu(trial, s.mu_max));
-                    h1 = use_net && mlp_features(ms, fabsf(m1), lxi, K.qlt_gk, f0, f1, f2);
+                    if constexpr (PASSES == 3) {
+                        const float m1 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                        float l2m;
+                        h1 = use_net && mlp_features(ms, fabsf(m1), lxi, K.qlt_gk, f0, f1, f2, l2m);
+                    }
                 }
             }
         }
 
-        // phases 3 and 4: the two Picard passes at the trial values
-#pragma unroll 1
-        for (int it = 1; it < 3; ++it) {
-            if (use_net) mlp_eval_wave<ACT>(im, f0, f1, f2, b40, b41, y0, y1);
-            bool hn = false;
-            f0 = f1 = f2 = 0.0f;
+        if constexpr (PASSES == 1) {
+            // phases 3 and 4: the two Picard passes, with the closure extended from mu0
             if (run) {
-                const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
-                float D, dD;
-                closure_at(h1, y0, y1, m, T, K, ms, D, dD);
-                const float rate = 0.5f * (1.0f - m * m) *
-                                       (T.f_speed * T.f_divb + m * T.f_divv - 3.0f * m * T.f_bb -
-                                        2.0f * rdiv(T.f_bv, T.f_speed)) +
-                                   dD;
-                const float scale = D > 0.0f ? rsqrt_(2.0f * D * dtr) : 0.0f;
-                trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3), s.mu_max);
-                if (it == 1) {
-                    const float m2 = static_cast<float>(reflect_mu(trial, s.mu_max));
-                    hn = use_net && mlp_features(ms, fabsf(m2), lxi, K.qlt_gk, f0, f1, f2);
+                const float l2r0 = mlp_log2_r(y0, ms);
+#pragma unroll 1
+                for (int it = 1; it < 3; ++it) {
+                    const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    float D, dD;
+                    if (h0 && mlp_mu_in_hull(ms, fabsf(m)))
+                        mlp_closure_extend(l2r0, y1, l2mu0, m, T.q_pref, D, dD);
+                    else
+                        qlt_scattering<float>(m, T.q_xi, T.q_xig, T.q_pref, K.qlt_gk, K.qlt_h0, D, dD);
+                    const float rate = 0.5f * (1.0f - m * m) *
+                                           (T.f_speed * T.f_divb + m * T.f_divv - 3.0f * m * T.f_bb -
+                                            2.0f * rdiv(T.f_bv, T.f_speed)) +
+                                       dD;
+                    const float scale = D > 0.0f ? rt_sqrt(2.0f * D * dtr) : 0.0f;
+                    trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3),
+                                       s.mu_max);
                 }
             }
-            h1 = hn;
+        } else {
+            // phases 3 and 4: the two Picard passes, each with its own network evaluation
+#pragma unroll 1
+            for (int it = 1; it < 3; ++it) {
+                if (use_net) mlp_eval_wave<ACT>(im, f0, f1, f2, b40, b41, y0, y1);
+                bool hn = false;
+                f0 = f1 = f2 = 0.0f;
+                if (run) {
+                    const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    float D, dD;
+                    closure_at(h1, y0, y1, m, T, K, ms, D, dD);
+                    const float rate = 0.5f * (1.0f - m * m) *
+                                           (T.f_speed * T.f_divb + m * T.f_divv - 3.0f * m * T.f_bb -
+                                            2.0f * rdiv(T.f_bv, T.f_speed)) +
+                                       dD;
+                    const float scale = D > 0.0f ? rt_sqrt(2.0f * D * dtr) : 0.0f;
+                    trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3),
+                                       s.mu_max);
+                    if (it == 1) {
+                        const float m2 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                        float l2m;
+                        hn = use_net && mlp_features(ms, fabsf(m2), lxi, K.qlt_gk, f0, f1, f2, l2m);
+                    }
+                }
+                h1 = hn;
+            }
         }
 
         if (run) {
@@ -1053,13 +1105,13 @@ __device__ __forceinline__ MlpLds mlp_stage(const void* image, uint4* smem, floa
     return mlp_lds_view(smem);
 }
 
-template <bool STR, int ACT>
+template <bool STR, int ACT, int PASSES>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_PUSH_WAVES_MLP)))
     push_kernel_mlp(PushArgs a, uint32_t run) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     float b40, b41;
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
-    push_lane_mlp<STR, ACT>(a, im, b40, b41, push_first_particle(run));
+    push_lane_mlp<STR, ACT, PASSES>(a, im, b40, b41, push_first_particle(run));
 }
 
 // ---------------------------------------------------------------------------
@@ -1105,9 +1157,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
         else if (R.kind == kEnsFocusedQlt32)
             push_lane<21, true, float, KPT_CLOSURE_QLT>(R.a, idx);
         else if (R.a.mlp_act == kActHardSwish)
-            push_lane_mlp<false, kActHardSwish>(R.a, im, b40, b41, idx);
+            push_lane_mlp<false, kActHardSwish, 1>(R.a, im, b40, b41, idx);
         else
-            push_lane_mlp<false, kActSilu>(R.a, im, b40, b41, idx);
+            push_lane_mlp<false, kActSilu, 1>(R.a, im, b40, b41, idx);
     }
 }
 
@@ -1495,10 +1547,18 @@ hipError_t launch_push_t(const PushArgs& a, int model, bool focused, bool fp32, 
             hipLaunchKernelGGL((push_kernel<16, false, double, R, STR>), grid, block, 0, stream, a, run);
     } else if (focused && closure == KPT_CLOSURE_MLP) {
         if (!fp32 || a.mlp_image == nullptr || !mlp_activation_ok(a.mlp_act)) return hipErrorInvalidValue;
-        if (a.mlp_act == kActHardSwish)
-            hipLaunchKernelGGL((push_kernel_mlp<STR, kActHardSwish>), grid, block, 0, stream, a, run);
-        else
-            hipLaunchKernelGGL((push_kernel_mlp<STR, kActSilu>), grid, block, 0, stream, a, run);
+        const bool three = a.s.mlp_passes == 3;
+        if (a.mlp_act == kActHardSwish) {
+            if (three)
+                hipLaunchKernelGGL((push_kernel_mlp<STR, kActHardSwish, 3>), grid, block, 0, stream, a, run);
+            else
+                hipLaunchKernelGGL((push_kernel_mlp<STR, kActHardSwish, 1>), grid, block, 0, stream, a, run);
+        } else {
+            if (three)
+                hipLaunchKernelGGL((push_kernel_mlp<STR, kActSilu, 3>), grid, block, 0, stream, a, run);
+            else
+                hipLaunchKernelGGL((push_kernel_mlp<STR, kActSilu, 1>), grid, block, 0, stream, a, run);
+        }
     } else if (focused && closure == KPT_CLOSURE_QLT) {
         if (fp32)
             hipLaunchKernelGGL((push_kernel<21, true, float, Q, STR>), grid, block, 0, stream, a, run);

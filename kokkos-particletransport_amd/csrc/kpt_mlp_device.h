@@ -139,15 +139,22 @@ __device__ __forceinline__ void mlp_eval_wave(const MlpLds& im, float f0, float 
 
 #undef KPT_MFMA
 
-// Scaled features and hull membership of one closure evaluation.
+// Scaled features and hull membership of one closure evaluation; l2mu returns
+// log2|mu|.
 __device__ __forceinline__ bool mlp_features(const MlpScale& m, float mu_abs, float log_xi, float gk, float& f0,
-                                             float& f1, float& f2) {
-    const float lmu = __builtin_amdgcn_logf(mu_abs) * 0.6931471805599453f;
+                                             float& f1, float& f2, float& l2mu) {
+    l2mu = __builtin_amdgcn_logf(mu_abs);
+    const float lmu = l2mu * 0.6931471805599453f;
     f0 = (lmu - m.c0) * m.s0;
     f1 = (log_xi - m.c1) * m.s1;
     f2 = (gk - m.c2) * m.s2;
     return mu_abs >= m.mu_lo && mu_abs < 1.0f && log_xi >= m.lxi_lo && log_xi <= m.lxi_hi && gk >= m.g_lo &&
            gk <= m.g_hi;
+}
+
+// The pitch-cosine part of the hull test (xi and gamma_k fixed within a sub-step).
+__device__ __forceinline__ bool mlp_mu_in_hull(const MlpScale& m, float mu_abs) {
+    return mu_abs >= m.mu_lo && mu_abs < 1.0f;
 }
 
 }  // namespace kpt

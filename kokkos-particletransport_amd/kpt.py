@@ -56,7 +56,7 @@ class DeviceSettings(C.Structure):
         ("pitch_angle_scattering_h0", C.c_double), ("split_ratio", C.c_double),
         ("pmin_split", C.c_double), ("mu_max", C.c_double),
         ("precision", C.c_int32), ("push_chunks", C.c_int32),
-        ("closure", C.c_int32), ("reserved1", C.c_int32),
+        ("closure", C.c_int32), ("mlp_passes", C.c_int32),
         ("qlt_sigma2", C.c_double), ("qlt_correlation_length", C.c_double),
         ("qlt_spectral_index", C.c_double), ("qlt_h0", C.c_double),
         ("fp64_null_b", C.c_double),
