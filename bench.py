@@ -79,9 +79,10 @@ def state_bytes(model):
     return 88 if model == kpt.MODEL_PARKER16 else 104
 
 
-def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0):
+def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0, seed=7):
     """Time the oracle (FP64 restatement of the reference push) on a bounded
-    sample of the bench's own particles, using this host's cores."""
+    sample of the bench's own particles, using this host's cores. Returns the
+    baseline record, the sample's start state and the oracle's end state."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import kpt_oracle
 
@@ -97,23 +98,23 @@ def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0):
     def run(m):
         parts = {k: v[:m].copy() for k, v in start.items()}
         t0 = time.perf_counter()
-        steps, _ = kpt_oracle.push_interval(lo, hi, s64, 0.0, 0, 7, parts, threads=threads)
-        return steps, time.perf_counter() - t0
+        steps, _ = kpt_oracle.push_interval(lo, hi, s64, 0.0, 0, seed, parts, threads=threads)
+        return steps, time.perf_counter() - t0, parts
 
     # grow the sample until it takes about target_seconds (small samples scale
     # poorly over the threads, so one extrapolation undershoots)
     m = 1024
-    steps, dt = run(m)
+    steps, dt, end = run(m)
     for _ in range(4):
         if dt >= 0.6 * target_seconds or m >= len(start["x"]):
             break
         per_particle_s = dt / m
         m = int(min(len(start["x"]), max(2 * m, target_seconds / max(per_particle_s, 1e-12))))
-        steps, dt = run(m)
+        steps, dt, end = run(m)
     what = ("src/main.cpp:1082-1327" if model == kpt.MODEL_PARKER16 else
             "src/main.cpp:1332-1755 with the slab-QLT closure of include/LegencyModel.hpp:539-689 "
             "and the Picard mu update of include/FocusSolver.hpp:700-719")
-    return {
+    rec = {
         "value": steps / dt,
         "unit": "particle-steps/s",
         "cores": threads,
@@ -121,6 +122,109 @@ def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0):
         "sample": f"{m} of the bench particles, one 0.1 interval ({steps} sub-steps, {dt:.1f} s), "
                   f"FP64 restatement of {what}, OpenMP",
     }
+    return rec, {k: v[:m].copy() for k, v in start.items()}, end
+
+
+# MI355X peaks of the compute-side roofline: 256 CUs x 4 SIMDs at 2.4 GHz; a
+# wave64 VALU instruction holds its SIMD for 4 cycles; unpacked FP32 and FP64 FMA
+# 78.6 TFLOP/s (packed FP32 is off in this build); dense BF16 MFMA 2.5 PFLOP/s.
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+FP32_PEAK = 78.6e12
+FP64_PEAK = 78.6e12
+BF16_MFMA_PEAK = 2.5e15
+
+
+def profiled_counters(workload):
+    """Per-launch counters of this workload's dominant kernel from the committed
+    rocprofv3 summaries (lines `counters_per_launch <workload> name=value ...`),
+    with the summary's file name, or ({}, None)."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) > 2 and parts[0] == "counters_per_launch" and parts[1] == workload:
+                    return {k: float(v) for k, v in (p.split("=", 1) for p in parts[2:])}, os.path.basename(path)
+    return {}, None
+
+
+def roofline_block(workload, kernel, launch_s, launches, alg_bytes):
+    """The roofline of the dominant kernel on every resource the profile
+    measures, with `bound` the one nearest its peak. Counters per launch come
+    from profiles/ (instruction and FLOP counts, HBM bytes); times are this run's
+    HIP-event launch averages."""
+    c, src = profiled_counters(workload)
+    traffic = profiled_traffic(workload)
+    hbm_alg = alg_bytes / launch_s / 1e9
+    res = {"hbm_algorithmic": {"achieved": hbm_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": hbm_alg / HBM_PEAK_GBS,
+                               "note": "algorithmic stencil bytes; cell-sorted gathers are served by L2 and MALL, "
+                                       "so this can exceed 1 and is not HBM evidence"}}
+    if traffic is not None:
+        g = traffic / launch_s / 1e9
+        res["hbm"] = {"achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g / HBM_PEAK_GBS}
+    if "SQ_INSTS_VALU" in c:
+        v = c["SQ_INSTS_VALU"] / launch_s
+        res["valu_issue"] = {"achieved": v / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
+                             "frac": v / VALU_ISSUE_PEAK}
+    if "FLOP_F32" in c:
+        v = c["FLOP_F32"] / launch_s
+        res["fp32"] = {"achieved": v / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": v / FP32_PEAK}
+    if "FLOP_F64" in c:
+        v = c["FLOP_F64"] / launch_s
+        res["fp64"] = {"achieved": v / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": v / FP64_PEAK}
+    if c.get("FLOP_BF16_MFMA", 0.0) > 0.0:
+        v = c["FLOP_BF16_MFMA"] / launch_s
+        res["bf16_mfma"] = {"achieved": v / 1e12, "peak": BF16_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+                            "frac": v / BF16_MFMA_PEAK}
+    measured = {k: r for k, r in res.items() if k != "hbm_algorithmic"}
+    bound = max(measured, key=lambda k: measured[k]["frac"]) if measured else "hbm_algorithmic"
+    top = res[bound]
+    out = {"bound": bound, "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"],
+           "frac": top["frac"], "traffic": traffic, "kernel": kernel, "kernel_ms_avg": launch_s * 1e3,
+           "launches": int(launches), "algorithmic_bytes_per_launch": alg_bytes, "resources": res}
+    if src:
+        out["counters"] = {"source": f"profiles/{src}", "per_launch": c,
+                           "profiled_kernel_ms": c.get("kernel_ms"),
+                           "stale": bool(c.get("kernel_ms") and abs(c["kernel_ms"] / (launch_s * 1e3) - 1) > 0.1)}
+    return out
+
+
+def accuracy_gate(xe, ye, f0, f1, model, s, net, sample, want, oracle_seed, other_seed=2718):
+    """The measured path on the CPU baseline's sample against the oracle's end
+    state: with the oracle's seed, the median per-particle differences; with an
+    independent seed, two-sample chi-square and KS p-values of the final
+    distributions and of the displacements (the metric's matched-spectrum
+    condition, passed when every p-value exceeds 0.01 divided by the number of
+    tests, a family-wise level of 0.01)."""
+    import accuracy as A
+
+    def push(seed):
+        with kpt.Context(len(sample["x"]), seed=seed) as c:
+            if net is not None:
+                c.mlp_set(*net)
+            c.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
+            c.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
+            c.upload(sample)
+            st = c.push_interval(s, 0.0, 0)
+            return A.sort_by_id(c.download()), st
+
+    want = A.sort_by_id(want)
+    start = A.sort_by_id(sample)
+    same, _ = push(oracle_seed)
+    other, st = push(other_seed)
+    if model == kpt.MODEL_FOCUSED21:
+        p = A.focused_pvalues(want, other, start, s)
+    else:
+        p = A.displacement_pvalues(want, other, start, s.x_width, s.y_width, ("dx", "dy", "dlnp"))
+    return {"particles": len(sample["x"]), "failures": int(st.failures),
+            "pathwise_median_same_seed": A.pathwise_medians(same, want, s.x_width, s.y_width),
+            "median_abs_dmu_oracle": float(np.median(np.abs(want["mu"] - start["mu"]))) if "mu" in want else None,
+            "pvalues_independent_seed": p, "min_pvalue": min(p.values()),
+            # family-wise level 0.01 over all the tests (Bonferroni)
+            "threshold": 0.01 / len(p),
+            "passed": bool(min(p.values()) > 0.01 / len(p) and st.failures == 0)}
 
 
 def copy_probe(mib=512, repeats=3):
@@ -287,7 +391,8 @@ def run_generic(args):
 
         field = (kpt_oracle.GenericFocusField(xe, ye, planes, p) if focus
                  else kpt_oracle.GenericField(xe, ye, planes, p))
-        m, done, t_cpu, s_cpu = 16384, 0, 0.0, 0
+        threads = kpt_oracle.max_threads()
+        m, done, t_cpu, s_cpu = 131072, 0, 0.0, 0
         sample = {key: v[:m].copy() for key, v in parts.items()}
         while t_cpu < 10.0 and s_cpu < 2000:
             t1 = time.perf_counter()
@@ -295,9 +400,10 @@ def run_generic(args):
             t_cpu += time.perf_counter() - t1
             s_cpu += 1
             done += m
-        out["cpu_baseline"] = {"value": done / t_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": done / t_cpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                                "sample": f"{m} of the bench particles, {s_cpu} global-dt steps ({t_cpu:.1f} s), "
-                                         "serial FP64 restatement of the solver's advance_stochastic"}
+                                         "FP64 restatement of the solver's compute_adaptive_time_step and "
+                                         "advance_stochastic, OpenMP over particles"}
     print(json.dumps(out), flush=True)
 
 
@@ -426,6 +532,159 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def _leg(ctx, s, start, warmup, steps):
+    ctx.upload(start)
+    st, f, el, km, act = timed_push(ctx, s, warmup, steps)
+    launches = sum(a[0] for a in act)
+    return {"value": st / el, "sub_steps": int(st), "failures": int(f),
+            "kernel_ms_avg": float(np.sum(km)) / launches, "launches": int(launches),
+            "active_in": float(np.mean([a[1] for a in act]))}
+
+
+def run_push(args):
+    """The focused (headline) and Parker push workloads (see the module docstring)."""
+    focused = args.workload == "focused"
+    prec = args.precision or ("fp32" if focused else "fp64")
+    fp32 = prec == "fp32"
+    model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
+    stretched = args.grid == "stretched"
+    y_edges = W.stretched_edges(args.ny, ratio=args.stretch_ratio) if stretched else None
+    xe, ye, f0 = W.island_field(args.nx, args.ny, seed=7, y_edges=y_edges)
+    _, _, f1 = W.island_field(args.nx, args.ny, seed=8, y_edges=y_edges)
+    learned = focused and args.closure == "mlp"
+    if focused:
+        n = args.particles or W.C4["particles"]
+        closure = kpt.CLOSURE_MLP if learned else kpt.CLOSURE_QLT
+        s = W.c3_device_settings(xe, ye, closure=closure,
+                                 precision=kpt.PREC_FP32 if fp32 else kpt.PREC_FP64)
+        s.mlp_passes = args.mlp_passes
+        start = W.c3_particles(n, s, seed=11)
+    else:
+        n = args.particles or 4 * 1024 * 1024
+        h = kpt.default_host_settings()
+        h.split_particles = 0
+        s = kpt.make_device_settings(h, xe, ye)
+        s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
+        start = None
+    s.push_chunks = args.chunks
+    log(f"bench: {args.workload} {args.nx}x{args.ny} {args.grid}, {n} particles, {prec}"
+        + (f", closure {args.closure}" if focused else ""))
+
+    legs, reports, nets = {}, {}, {}
+    with kpt.Context(n, seed=114514) as ctx:
+        ctx.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
+        ctx.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
+        if not focused:
+            opts = kpt.InjectOpts()
+            opts.fixed_time = 1
+            assert ctx.inject(s, n, 0.0, 0, opts) == n
+            start = ctx.download()
+        acts = [args.mlp_activation] if learned else []
+        if learned and not args.no_alt_activation_leg:
+            acts += [a for a in sorted(kpt.MLP_ACTIVATIONS) if a != args.mlp_activation]
+        for act in acts:
+            log(f"training the D_mumu network ({act}) on the GPU ...")
+            o = kpt.mlp_default_train_opts()
+            o.steps = args.mlp_steps
+            o.hull.activation = kpt.MLP_ACTIVATIONS[act]
+            t_train = time.perf_counter()
+            reports[act] = ctx.mlp_train(o)
+            reports[act]["wall_s"] = time.perf_counter() - t_train
+            nets[act] = ctx.mlp_get()
+            log(f"  {reports[act]}")
+        # the measured path first, then the comparison legs on the same particles
+        if learned:
+            for act in acts:
+                log(f"FP32 push with the learned closure ({act}) ...")
+                ctx.mlp_set(*nets[act])
+                legs["mlp_" + act] = _leg(ctx, s, start, args.warmup, args.steps)
+            ctx.mlp_set(*nets[args.mlp_activation])
+        else:
+            legs["measured"] = _leg(ctx, s, start, args.warmup, args.steps)
+        main_leg = legs["mlp_" + args.mlp_activation] if learned else legs["measured"]
+        summary = ctx.summary()
+        if learned and not args.no_fp32_leg:
+            log("FP32 exact-QLT push of the same particles ...")
+            s32 = s.copy()
+            s32.closure = kpt.CLOSURE_QLT
+            legs["fp32_exact_qlt"] = _leg(ctx, s32, start, args.warmup, args.steps)
+        if focused and not args.no_fp64_leg:
+            log("plain FP64 push of the same particles ...")
+            s64 = s.copy()
+            s64.precision = kpt.PREC_FP64
+            s64.closure = kpt.CLOSURE_QLT
+            legs["fp64_plain"] = _leg(ctx, s64, start, args.warmup, args.steps)
+
+    steps = main_leg["sub_steps"]
+    value = main_leg["value"]
+    launch_s = main_leg["kernel_ms_avg"] / 1e3
+    per_step = step_bytes(model, fp32, stretched=stretched)
+    launch_bytes = steps / main_leg["launches"] * per_step + main_leg["active_in"] * state_bytes(model)
+    grid_tag = "_stretched" if stretched else ""
+    act_tag = ("_" + args.mlp_activation.replace("-", "")) if learned else ""
+    workload = (f"focused_qlt_{args.closure}{act_tag}_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}" if focused
+                else f"parker_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}")
+    kernel = "push_kernel_mlp" if learned else "push_kernel"
+    cfg = {
+        "workload": workload,
+        "field": (f"{args.nx}x{args.ny}, " + (f"y faces tanh-stretched toward y = 0 (spacing ratio {args.stretch_ratio:g})"
+                                              if stretched else "uniform grid") + ", 2 time-blended frames"),
+        "particles": n,
+        "interval": s.mhd_dt,
+        "sub_steps": int(steps),
+        "failures": int(main_leg["failures"]),
+        "active_after": int(summary["active"]),
+        "push_chunks": args.chunks,
+    }
+    if focused:
+        cfg["closure"] = args.closure
+        cfg["dt"] = s.dt_min
+    if learned:
+        cfg["mlp_activation"] = args.mlp_activation
+        cfg["mlp_network"] = "3 -> 64 -> 64 -> 64 -> 2, BF16 weights, FP32 accumulation (MFMA 32x32x16)"
+        cfg["mlp_evaluations_per_sub_step"] = 1 if args.mlp_passes == 1 else 3
+        cfg["mlp_training"] = {a: {k: reports[a][k] for k in ("steps", "train_ms", "final_loss", "val_rms_lnr",
+                                                               "val_max_lnr", "val_rms_rel_d", "val_rms_y1")}
+                               for a in reports}
+    cfg["legs"] = legs
+    if "fp64_plain" in legs:
+        cfg["speedup_vs_fp64_plain"] = value / legs["fp64_plain"]["value"]
+        for key, leg in legs.items():
+            if key.startswith("mlp_"):
+                cfg[f"speedup_vs_fp64_plain_{key[4:]}"] = leg["value"] / legs["fp64_plain"]["value"]
+        if "fp32_exact_qlt" in legs:
+            cfg["speedup_vs_fp64_plain_fp32_exact_qlt"] = legs["fp32_exact_qlt"]["value"] / legs["fp64_plain"]["value"]
+    if "fp32_exact_qlt" in legs:
+        cfg["ratio_vs_fp32_exact_qlt"] = value / legs["fp32_exact_qlt"]["value"]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_leg["sub_steps"] / value / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if fp32 else "f64",
+        "data": "synthetic (seeded island-chain Harris field; "
+                + ("uniform x, y, mu, log-uniform p)" if focused else "uniform injection)"),
+        "config": cfg,
+        "roofline": roofline_block(workload, kernel, launch_s, main_leg["launches"], launch_bytes),
+    }
+    out["roofline"]["bytes_per_step"] = per_step
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        oracle_seed = 7
+        out["cpu_baseline"], sample, want = cpu_baseline(xe, ye, f0, f1, model, s, start, seed=oracle_seed)
+        log("accuracy of the measured path on the baseline's sample ...")
+        out["accuracy"] = accuracy_gate(xe, ye, f0, f1, model, s, nets.get(args.mlp_activation), sample, want,
+                                        oracle_seed)
+        cfg["matched_spectrum"] = out["accuracy"]["passed"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true",
@@ -450,8 +709,11 @@ def main():
                     help="default: fp32 for focused, fp64 for parker")
     ap.add_argument("--closure", choices=["qlt", "mlp"], default="mlp")
     ap.add_argument("--mlp-steps", type=int, default=8000, help="Adam steps of the network training")
-    ap.add_argument("--mlp-activation", choices=sorted(kpt.MLP_ACTIVATIONS), default="hard-swish",
-                    help="hidden-layer activation of the learned closure")
+    ap.add_argument("--mlp-activation", choices=sorted(kpt.MLP_ACTIVATIONS), default="silu",
+                    help="hidden-layer activation of the measured learned closure (the other one is "
+                         "pushed as a further leg)")
+    ap.add_argument("--no-alt-activation-leg", action="store_true",
+                    help="focused with --closure mlp: skip the leg with the other activation")
     ap.add_argument("--mlp-passes", type=int, choices=[1, 3], default=1,
                     help="network evaluations per sub-step: 1 (the Picard passes extend the first "
                          "evaluation) or 3 (one per Picard pass)")
@@ -478,140 +740,7 @@ def main():
     if args.workload == "generic":
         run_generic(args)
         return
-    focused = args.workload == "focused"
-    prec = args.precision or ("fp32" if focused else "fp64")
-    fp32 = prec == "fp32"
-    model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
-    stretched = args.grid == "stretched"
-    y_edges = W.stretched_edges(args.ny, ratio=args.stretch_ratio) if stretched else None
-    xe, ye, f0 = W.island_field(args.nx, args.ny, seed=7, y_edges=y_edges)
-    _, _, f1 = W.island_field(args.nx, args.ny, seed=8, y_edges=y_edges)
-    if focused:
-        n = args.particles or W.C4["particles"]
-        closure = kpt.CLOSURE_MLP if args.closure == "mlp" else kpt.CLOSURE_QLT
-        s = W.c3_device_settings(xe, ye, closure=closure,
-                                 precision=kpt.PREC_FP32 if fp32 else kpt.PREC_FP64)
-        s.mlp_passes = args.mlp_passes
-        start = W.c3_particles(n, s, seed=11)
-    else:
-        n = args.particles or 4 * 1024 * 1024
-        h = kpt.default_host_settings()
-        h.split_particles = 0
-        s = kpt.make_device_settings(h, xe, ye)
-        s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
-        start = None
-    s.push_chunks = args.chunks
-    log(f"bench: {args.workload} {args.nx}x{args.ny} {args.grid}, {n} particles, {prec}"
-        + (f", closure {args.closure}" if focused else ""))
-
-    fp64_leg = fp32_leg = mlp_report = None
-    with kpt.Context(n, seed=114514) as ctx:
-        if focused and args.closure == "mlp":
-            log("training the D_mumu network on the GPU ...")
-            o = kpt.mlp_default_train_opts()
-            o.steps = args.mlp_steps
-            o.hull.activation = kpt.MLP_ACTIVATIONS[args.mlp_activation]
-            t_train = time.perf_counter()
-            mlp_report = ctx.mlp_train(o)
-            mlp_report["wall_s"] = time.perf_counter() - t_train
-            log(f"  {mlp_report}")
-        ctx.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
-        ctx.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
-        if focused:
-            ctx.upload(start)
-        else:
-            opts = kpt.InjectOpts()
-            opts.fixed_time = 1
-            assert ctx.inject(s, n, 0.0, 0, opts) == n
-            start = ctx.download()
-        steps, failures, elapsed, kernel_ms, active = timed_push(ctx, s, args.warmup, args.steps)
-        summary = ctx.summary()
-        if focused and args.closure == "mlp" and not args.no_fp32_leg:
-            log("FP32 exact-QLT push of the same particles ...")
-            s32 = s.copy()
-            s32.closure = kpt.CLOSURE_QLT
-            ctx.upload(start)
-            st32, f32, el32, km32, _ = timed_push(ctx, s32, args.warmup, args.steps)
-            fp32_leg = {"value": st32 / el32, "sub_steps": int(st32), "failures": int(f32),
-                        "kernel_ms_avg": float(np.mean(km32))}
-        if focused and not args.no_fp64_leg:
-            log("plain FP64 push of the same particles ...")
-            s64 = s.copy()
-            s64.precision = kpt.PREC_FP64
-            s64.closure = kpt.CLOSURE_QLT
-            ctx.upload(start)
-            st64, f64, el64, km64, _ = timed_push(ctx, s64, args.warmup, args.steps)
-            fp64_leg = {"value": st64 / el64, "sub_steps": int(st64), "failures": int(f64),
-                        "kernel_ms_avg": float(np.mean(km64))}
-
-    value = steps / elapsed
-    launches = sum(a[0] for a in active)
-    avg_launch_s = float(np.sum(kernel_ms)) / 1e3 / launches
-    per_step = step_bytes(model, fp32, stretched=stretched)
-    launch_bytes = steps / launches * per_step + float(np.mean([a[1] for a in active])) * state_bytes(model)
-    achieved = launch_bytes / avg_launch_s / 1e9
-    grid_tag = "_stretched" if stretched else ""
-    workload = (f"focused_qlt_{args.closure}_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}" if focused else
-                f"parker_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}")
-    traffic = profiled_traffic(workload)
-    cfg = {
-        "workload": workload,
-        "field": (f"{args.nx}x{args.ny}, " + (f"y faces tanh-stretched toward y = 0 (spacing ratio {args.stretch_ratio:g})"
-                                              if stretched else "uniform grid") + ", 2 time-blended frames"),
-        "particles": n,
-        "interval": s.mhd_dt,
-        "sub_steps": int(steps),
-        "failures": int(failures),
-        "active_after": int(summary["active"]),
-        "push_chunks": args.chunks,
-    }
-    if focused:
-        cfg["closure"] = args.closure
-        cfg["dt"] = s.dt_min
-        if mlp_report:
-            cfg["mlp_activation"] = args.mlp_activation
-            cfg["mlp_training"] = {k: mlp_report[k] for k in ("steps", "train_ms", "final_loss",
-                                                          "val_rms_lnr", "val_max_lnr",
-                                                          "val_rms_rel_d", "val_rms_y1")}
-        if fp32_leg:
-            cfg["fp32_exact_qlt"] = fp32_leg
-            cfg["ratio_vs_fp32_exact_qlt"] = value / fp32_leg["value"]
-        if fp64_leg:
-            cfg["fp64_plain"] = fp64_leg
-            cfg["speedup_vs_fp64_plain"] = value / fp64_leg["value"]
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if fp32 else "f64",
-        "data": "synthetic (seeded island-chain Harris field; "
-                + ("uniform x, y, mu, log-uniform p)" if focused else "uniform injection)"),
-        "config": cfg,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "push_kernel_mlp" if (focused and args.closure == "mlp") else "push_kernel",
-            "kernel_ms_avg": avg_launch_s * 1e3,
-            "launches": int(launches),
-            "algorithmic_bytes_per_launch": launch_bytes,
-            "bytes_per_step": per_step,
-        },
-    }
-    if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(xe, ye, f0, f1, model, s, start)
-    print(json.dumps(out), flush=True)
+    run_push(args)
 
 
 if __name__ == "__main__":

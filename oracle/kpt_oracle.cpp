@@ -26,6 +26,7 @@
 //                      totals) and the reference's tracked histogram CSV edges.
 #include <algorithm>
 #include <cmath>
+#include <omp.h>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -1316,10 +1317,15 @@ void* kpto_generic_create(int32_t nx, int32_t ny, const double* xe, const double
 
 // compute_adaptive_time_step (ParkerSolver.hpp:1084-1130): courant * min, 0 when
 // no alive particle gives a finite bound.
+// OpenMP threads the generic solver loops use (OMP_NUM_THREADS or every core).
+int32_t kpto_max_threads() { return omp_get_max_threads(); }
+
 double kpto_generic_time_step(const void* field, const kpt_generic_parker_params* p, const kpt_particles_host* P,
                               uint64_t count) {
     const OField& F = *static_cast<const OField*>(field);
     double m = 1.0e300;
+    // particles are independent and the minimum is exact in any order
+#pragma omp parallel for schedule(static, 1024) reduction(min : m)
     for (uint64_t i = 0; i < count; ++i) {
         if (P->status[i] != 0) continue;
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
@@ -1343,6 +1349,7 @@ double kpto_generic_step(const void* field, const kpt_generic_parker_params* p, 
     if (!(dt > 0.0)) return 0.0;
     const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
     uint64_t fails = 0;
+#pragma omp parallel for schedule(static, 1024) reduction(+ : fails)
     for (uint64_t i = 0; i < count; ++i) {
         if (P->status[i] != 0) continue;
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
@@ -1808,6 +1815,7 @@ double kpto_generic_focus_time_step(const void* field, const kpt_generic_focus_p
                                     uint64_t count) {
     const OField& F = *static_cast<const OField*>(field);
     double m = 1.0e300;
+#pragma omp parallel for schedule(static, 1024) reduction(min : m)
     for (uint64_t i = 0; i < count; ++i) {
         if (P->status[i] != 0) continue;
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
@@ -1843,6 +1851,7 @@ double kpto_generic_focus_step(const void* field, const kpt_generic_focus_params
     if (!(dt > 0.0)) return 0.0;
     const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
     uint64_t fails = 0;
+#pragma omp parallel for schedule(static, 1024) reduction(+ : fails)
     for (uint64_t i = 0; i < count; ++i) {
         if (P->status[i] != 0) continue;
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};

@@ -109,6 +109,7 @@ def load():
             "kpto_generic_step": (C.c_double, [C.c_void_p, P(kpt.GenericParkerParams),
                                                P(kpt.ParticlesHost), C.c_uint64, C.c_uint32,
                                                C.c_uint64, dp, C.c_uint32, P(C.c_uint64)]),
+            "kpto_max_threads": (C.c_int32, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -428,3 +429,8 @@ class GenericFocusField(Field):
         dt = load().kpto_generic_focus_step(self.h, C.byref(self.params), C.byref(h), len(parts["x"]), step, seed,
                                             _dp(nz), ns, C.byref(fails))
         return dt, fails.value
+
+
+def max_threads() -> int:
+    """OpenMP threads of the generic solver loops (OMP_NUM_THREADS or every core)."""
+    return int(load().kpto_max_threads())

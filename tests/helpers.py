@@ -46,18 +46,4 @@ def rel_err(a, b, floor=1e-300):
     return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
 
 
-def chi2_two_sample(h1, h2, min_count=5):
-    """Two-sample chi-square on weighted histograms with unit weights.
-    Returns (statistic, dof, p-value) over bins where either count >= min_count."""
-    from scipy import stats
-
-    h1 = np.asarray(h1, dtype=np.float64).ravel()
-    h2 = np.asarray(h2, dtype=np.float64).ravel()
-    n1, n2 = h1.sum(), h2.sum()
-    mask = (h1 + h2) >= min_count
-    a, b = h1[mask], h2[mask]
-    k1 = np.sqrt(n2 / n1)
-    k2 = np.sqrt(n1 / n2)
-    stat = np.sum((k1 * a - k2 * b) ** 2 / (a + b))
-    dof = int(mask.sum()) - 1
-    return stat, dof, float(stats.chi2.sf(stat, max(dof, 1)))
+from accuracy import chi2_two_sample  # noqa: E402,F401  (shared with bench.py)
