@@ -195,6 +195,14 @@ typedef struct kpt_particles_host {
     uint64_t* id;
     double* e0;  /* initial_kinetic_energy: baseline of the energy-growth split;
                     NULL on upload stores 0 (an unset baseline) */
+    /* ParticleSystem::previous_position and previous_step_position
+     * (include/ParticleSystem.hpp:342-343): NULL on upload stores the current
+     * position (as ParticleSystem::set_particle does, :516-522). */
+    double* prev_x;
+    double* prev_y;
+    double* prev_step_x;
+    double* prev_step_y;
+    int32_t* sort_key; /* ParticleSystem::sort_key (:349); NULL on upload stores 0 */
 } kpt_particles_host;
 
 typedef struct kpt_push_stats {
@@ -472,6 +480,14 @@ kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_p
  * holds the four Gaussians (g_x, g_y, g_z, g_parallel) of each step. */
 kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
                                    uint32_t step, double* dt_used, uint64_t* failures);
+/* advance_deterministic (include/ParkerSolver.hpp:228-255): the same step with no
+ * stochastic increments, x += dt u, p = max(p + dt p_rate, 0).
+ * Both advance calls follow the reference's order: the adaptive dt (an error
+ * with the reference's message when it is not positive and finite), the periodic
+ * wrap of alive particles, capture_previous_positions of every stored particle,
+ * the update. */
+kpt_status kpt_generic_parker_advance_deterministic(kpt_ctx* ctx, const kpt_generic_parker_params* p,
+                                                    int32_t slot, double* dt_used, uint64_t* failures);
 
 /* ---- batched ensemble: many independent runs in one persistent launch ----------
  * No counterpart in the reference, which runs one configuration per process. An
@@ -580,6 +596,11 @@ kpt_status kpt_generic_focus_time_step(kpt_ctx* ctx, const kpt_generic_focus_par
  * speed, invalid coefficients) get status 2 and are counted in failures. */
 kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
                                   uint32_t step, double* dt_used, uint64_t* failures);
+/* advance_deterministic (include/FocusSolver.hpp:251-271): the same step with the
+ * perpendicular increment and the pitch-angle normal zero (the Picard passes
+ * then integrate the drift only); dt, wrap and capture as the Parker calls. */
+kpt_status kpt_generic_focus_advance_deterministic(kpt_ctx* ctx, const kpt_generic_focus_params* p,
+                                                   int32_t slot, double* dt_used, uint64_t* failures);
 
 /* ---- ParticleSystem lifecycle (include/ParticleSystem.hpp) ----------------------- */
 
@@ -629,6 +650,31 @@ kpt_status kpt_apply_boundaries(kpt_ctx* ctx, int32_t slot, int32_t periodic_mas
  * first `count` stored particles; any output may be NULL. */
 kpt_status kpt_particle_kinematics(kpt_ctx* ctx, const kpt_particle_properties* props, double* gamma,
                                    double* speed, double* kinetic_energy, uint64_t count);
+/* capture_previous_positions (include/ParticleSystem.hpp:623-628):
+ * previous_step_position = previous_position, previous_position = position, for
+ * every stored particle. */
+kpt_status kpt_capture_previous_positions(kpt_ctx* ctx);
+/* sort_by_key (include/ParticleSystem.hpp:727-760): every stored array permuted by
+ * the stored sort keys (kpt_particles_host.sort_key) in [min_key, max_key]; keys
+ * outside the range take its nearest end. Stable: equal keys keep their slot
+ * order (one admissible order of Kokkos::BinSort). */
+kpt_status kpt_sort_by_key(kpt_ctx* ctx, int32_t min_key, int32_t max_key);
+/* compact_active_particles (include/ParticleSystem.hpp:769-793): sort key 0 for
+ * alive particles and 1 for the others, sort_by_key(0, 1), and the stored count
+ * becomes the alive count (returned in `active`). */
+kpt_status kpt_compact_active_particles(kpt_ctx* ctx, uint64_t* active);
+/* write_binary_snapshot (include/ParticleSystem.hpp:936-1006): the version-4 file
+ * of the stored particles with their ids, sort keys, both previous positions and
+ * split baselines, and `species` (ParticleSpecies: 0 proton .. 3 custom), the
+ * properties and the splitting policy in the header. */
+kpt_status kpt_particle_system_write_snapshot(kpt_ctx* ctx, const char* path, const kpt_particle_properties* props,
+                                              int32_t species, const kpt_energy_split_policy* policy);
+/* read_binary_snapshot (include/ParticleSystem.hpp:1011-1145) with its checks and
+ * messages: the file's particles replace the stored set (time and step_dt 0), and
+ * its header's properties, species and policy are returned (any output may be
+ * NULL). A file with more particles than the capacity is refused. */
+kpt_status kpt_particle_system_read_snapshot(kpt_ctx* ctx, const char* path, kpt_particle_properties* props,
+                                             int32_t* species, kpt_energy_split_policy* policy);
 
 /* ---- generic particle generator (include/ParticleGenerator.hpp) ------------------
  * Injection and recycling over the cells of a field slot. The slot's raw planes

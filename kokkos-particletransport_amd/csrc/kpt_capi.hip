@@ -510,18 +510,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
     kpt::ParticleArrays& P = ctx->P;
-    if ((e = dev_alloc(&P.x, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.y, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.p, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.t, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.step_dt, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.weight, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.mu, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.status, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.split_level, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.id, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.e0, capacity)) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = dev_alloc(&P.substep, capacity)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = kpt::alloc_arrays(P, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
     ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
     if ((e = dev_alloc(&ctx->split_scratch, ctx->split_scratch_cap)) != hipSuccess) return bail(e, "hipMalloc");
@@ -540,6 +529,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     if ((e = hipMemset(P.split_level, 0, capacity * 4)) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(P.weight, 0, capacity * 8)) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(P.e0, 0, capacity * 8)) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(P.sort_key, 0, capacity * 4)) != hipSuccess) return bail(e, "hipMemset");
     *out = ctx;
     return KPT_OK;
 }
@@ -551,12 +541,8 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
     if (S.busy) S.worker.join();
     if (S.stream) (void)hipStreamSynchronize(S.stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    kpt::ParticleArrays& P = ctx->P;
-    for (void* p : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p),
-                    static_cast<void*>(P.t), static_cast<void*>(P.step_dt), static_cast<void*>(P.weight),
-                    static_cast<void*>(P.mu), static_cast<void*>(P.status), static_cast<void*>(P.split_level),
-                    static_cast<void*>(P.id), static_cast<void*>(P.e0), static_cast<void*>(P.substep),
-                    static_cast<void*>(ctx->counters),
+    kpt::free_arrays(ctx->P);
+    for (void* p : {static_cast<void*>(ctx->counters),
                     static_cast<void*>(ctx->split_scratch), static_cast<void*>(ctx->hist_partials),
                     static_cast<void*>(ctx->hist_out), ctx->summary_partials,
                     static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
@@ -576,15 +562,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
     dev_free(ctx->pair32);
     dev_free(ctx->mlp_params);
     dev_free(ctx->mlp_image);
-    if (ctx->p2_allocated) {
-        kpt::ParticleArrays& Q = ctx->P2;
-        for (void* p : {static_cast<void*>(Q.x), static_cast<void*>(Q.y), static_cast<void*>(Q.p),
-                        static_cast<void*>(Q.t), static_cast<void*>(Q.step_dt), static_cast<void*>(Q.weight),
-                        static_cast<void*>(Q.mu), static_cast<void*>(Q.status),
-                        static_cast<void*>(Q.split_level), static_cast<void*>(Q.id),
-                        static_cast<void*>(Q.e0), static_cast<void*>(Q.substep)})
-            dev_free(p);
-    }
+    if (ctx->p2_allocated) kpt::free_arrays(ctx->P2);
     for (int k = 0; k < 2; ++k) {
         dev_free(ctx->sort.keys[k]);
         dev_free(ctx->sort.vals[k]);
@@ -874,6 +852,7 @@ kpt_status kpt_particles_upload(kpt_ctx* ctx, const kpt_particles_host* h, uint6
             KPT_HIP(ctx, hipMemcpy(P.e0, h->e0, count * 8, hipMemcpyHostToDevice));
         else
             KPT_HIP(ctx, hipMemset(P.e0, 0, count * 8));
+        KPT_HIP(ctx, upload_system_arrays(P, h, count));
     }
     uint64_t max_root = 0;
     if (h->id) {
@@ -912,6 +891,7 @@ kpt_status kpt_particles_download(kpt_ctx* ctx, kpt_particles_host* h, uint64_t 
     if (h->split_level) KPT_HIP(ctx, hipMemcpy(h->split_level, P.split_level, count * 4, hipMemcpyDeviceToHost));
     if (h->id) KPT_HIP(ctx, hipMemcpy(h->id, P.id, count * 8, hipMemcpyDeviceToHost));
     if (h->e0) KPT_HIP(ctx, hipMemcpy(h->e0, P.e0, count * 8, hipMemcpyDeviceToHost));
+    KPT_HIP(ctx, download_system_arrays(h, P, count));
     return KPT_OK;
 }
 
@@ -1129,18 +1109,7 @@ kpt_status sort_in_place(kpt_ctx* ctx, const kpt_device_settings& s, double t_do
     if (!ctx->p2_allocated) {
         kpt::ParticleArrays& Q = ctx->P2;
         const uint64_t cap = ctx->capacity;
-        KPT_HIP(ctx, dev_alloc(&Q.x, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.y, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.p, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.t, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.step_dt, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.weight, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.mu, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.status, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.split_level, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.id, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.e0, cap));
-        KPT_HIP(ctx, dev_alloc(&Q.substep, cap));
+        KPT_HIP(ctx, kpt::alloc_arrays(Q, cap));
         for (int k = 0; k < 2; ++k) {
             KPT_HIP(ctx, dev_alloc(&ctx->sort.keys[k], cap));
             KPT_HIP(ctx, dev_alloc(&ctx->sort.vals[k], cap));
@@ -1423,6 +1392,231 @@ kpt_status kpt_snapshot_read(kpt_ctx* ctx, const char* path, double charge, doub
     st = kpt_snapshot_read_host(path, &h, &count, charge, split_ratio, err, sizeof(err));
     if (st != KPT_OK) return fail(ctx, st, err);
     return kpt_particles_upload(ctx, &h, count);
+}
+
+// ---- ParticleSystem storage -----------------------------------------------------
+
+kpt_status kpt_capture_previous_positions(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    KPT_HIP(ctx, kpt::launch_capture_previous(ctx->P, ctx->count, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+kpt_status sort_stored(kpt_ctx* ctx, int32_t min_key, int32_t max_key, bool compact, uint64_t* active) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (max_key < min_key) return fail(ctx, KPT_ERR_ARG, "ParticleSystem: invalid sort key range.");
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->p2_allocated) {
+        kpt::ParticleArrays& Q = ctx->P2;
+        const uint64_t cap = ctx->capacity;
+        KPT_HIP(ctx, kpt::alloc_arrays(Q, cap));
+        for (int k = 0; k < 2; ++k) {
+            KPT_HIP(ctx, dev_alloc(&ctx->sort.keys[k], cap));
+            KPT_HIP(ctx, dev_alloc(&ctx->sort.vals[k], cap));
+        }
+        ctx->sort.temp_bytes = kpt::sort_temp_bytes(cap);
+        KPT_HIP(ctx, hipMalloc(&ctx->sort.temp, std::max<size_t>(ctx->sort.temp_bytes, 1)));
+        std::vector<int32_t> two(static_cast<size_t>(std::min<uint64_t>(cap, 1u << 20)), 2);
+        for (uint64_t off = 0; off < cap; off += two.size()) {
+            const size_t n2 = static_cast<size_t>(std::min<uint64_t>(two.size(), cap - off));
+            KPT_HIP(ctx, hipMemcpy(Q.status + off, two.data(), n2 * 4, hipMemcpyHostToDevice));
+        }
+        ctx->p2_allocated = true;
+    }
+    unsigned long long* act_dev = ctx->counters + 4;
+    KPT_HIP(ctx, hipMemsetAsync(act_dev, 0, sizeof(unsigned long long), ctx->stream));
+    KPT_HIP(ctx, kpt::launch_sort_stored(ctx->P, ctx->P2, ctx->count, min_key, max_key, compact ? 1 : 0, ctx->sort,
+                                         act_dev, ctx->stream));
+    unsigned long long act = 0;
+    KPT_HIP(ctx, hipMemcpyAsync(&act, act_dev, sizeof(act), hipMemcpyDeviceToHost, ctx->stream));
+    KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::swap(ctx->P, ctx->P2);
+    if (compact && act < ctx->count) {
+        // the dropped slots return to the storage default (status 2)
+        std::vector<int32_t> two(static_cast<size_t>(ctx->count - act), 2);
+        KPT_HIP(ctx, hipMemcpy(ctx->P.status + act, two.data(), two.size() * 4, hipMemcpyHostToDevice));
+        ctx->count = act;
+    }
+    if (active) *active = act;
+    return KPT_OK;
+}
+
+// ParticleProperties::validate (include/ParticleSystem.hpp:155-165) and
+// ParticleSplittingPolicy::validate (:287-293) of a snapshot header.
+bool properties_ok(const kpt_particle_properties& pr) {
+    return std::isfinite(pr.rest_mass) && pr.rest_mass > 0.0 && std::isfinite(pr.charge) &&
+           std::isfinite(pr.speed_of_light) && pr.speed_of_light > 0.0 && std::isfinite(pr.energy_scale_erg) &&
+           pr.energy_scale_erg > 0.0;
+}
+bool policy_ok(const kpt_energy_split_policy& po) {
+    return std::isfinite(po.energy_split_ratio) && po.energy_split_ratio > 1.0 &&
+           std::isfinite(po.minimum_child_weight) && po.minimum_child_weight >= 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_sort_by_key(kpt_ctx* ctx, int32_t min_key, int32_t max_key) {
+    if (ctx && (ctx->count <= 1 || min_key == max_key)) return KPT_OK;  // nothing to order (:730-732)
+    return sort_stored(ctx, min_key, max_key, false, nullptr);
+}
+
+kpt_status kpt_compact_active_particles(kpt_ctx* ctx, uint64_t* active) {
+    if (!ctx) return KPT_ERR_ARG;
+    if (ctx->count == 0) {
+        if (active) *active = 0;
+        return KPT_OK;
+    }
+    return sort_stored(ctx, 0, 1, true, active);
+}
+
+kpt_status kpt_particle_system_write_snapshot(kpt_ctx* ctx, const char* path, const kpt_particle_properties* props,
+                                              int32_t species, const kpt_energy_split_policy* policy) {
+    if (!ctx || !path || !props || !policy) return KPT_ERR_ARG;
+    if (!properties_ok(*props) || species < 0 || species > 3)
+        return fail(ctx, KPT_ERR_ARG, "ParticleProperties: invalid particle properties or species.");
+    if (!policy_ok(*policy))
+        return fail(ctx, KPT_ERR_ARG,
+                    "ParticleSplittingPolicy: energy split ratio must be greater than one and minimum child weight "
+                    "must be finite and non-negative.");
+    const uint64_t n = ctx->count;
+    std::vector<double> x(n), y(n), p(n), mu(n), w(n), e0(n), px(n), py(n), psx(n), psy(n);
+    std::vector<int32_t> status(n), level(n), key(n);
+    std::vector<uint64_t> id(n);
+    kpt_particles_host h{x.data(),  y.data(),  p.data(),   nullptr,    nullptr,    w.data(),   mu.data(),
+                         status.data(), level.data(), id.data(), e0.data(), px.data(), py.data(), psx.data(),
+                         psy.data(), key.data()};
+    kpt_status st = kpt_particles_download(ctx, &h, n);
+    if (st != KPT_OK) return st;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to open binary particle output file.");
+    // write_binary_snapshot (include/ParticleSystem.hpp:936-1006): 88-byte header,
+    // 100-byte records
+    std::vector<char> buf;
+    buf.reserve(88 + static_cast<size_t>(n) * 100);
+    auto put = [&buf](const void* v, size_t bytes) {
+        const char* c = static_cast<const char*>(v);
+        buf.insert(buf.end(), c, c + bytes);
+    };
+    const char magic[8] = {'K', 'P', 'T', 'P', 'R', 'T', '\0', '\0'};
+    const uint32_t version = 4u, endian = 0x01020304u;
+    const int32_t dim = 2;
+    const uint64_t cap = ctx->capacity;
+    put(magic, 8);
+    put(&version, 4);
+    put(&endian, 4);
+    put(&dim, 4);
+    put(&species, 4);
+    put(&n, 8);
+    put(&cap, 8);
+    put(&props->rest_mass, 8);
+    put(&props->charge, 8);
+    put(&props->speed_of_light, 8);
+    put(&props->energy_scale_erg, 8);
+    put(&policy->energy_split_ratio, 8);
+    put(&policy->minimum_child_weight, 8);
+    for (uint64_t i = 0; i < n; ++i) {
+        put(&id[i], 8);
+        put(&status[i], 4);
+        put(&level[i], 4);
+        put(&key[i], 4);
+        put(&x[i], 8);
+        put(&y[i], 8);
+        put(&px[i], 8);
+        put(&py[i], 8);
+        put(&psx[i], 8);
+        put(&psy[i], 8);
+        put(&p[i], 8);
+        put(&mu[i], 8);
+        put(&w[i], 8);
+        put(&e0[i], 8);
+    }
+    out.write(buf.data(), static_cast<std::streamsize>(buf.size()));
+    if (!out) return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to write binary particle output file.");
+    return KPT_OK;
+}
+
+kpt_status kpt_particle_system_read_snapshot(kpt_ctx* ctx, const char* path, kpt_particle_properties* props,
+                                             int32_t* species, kpt_energy_split_policy* policy) {
+    if (!ctx || !path) return KPT_ERR_ARG;
+    // read_binary_snapshot (include/ParticleSystem.hpp:1011-1145) with its checks
+    // and messages; the set replaces the stored particles (time 0, step_dt 0)
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to open binary particle input file.");
+    auto get = [&in](void* v, size_t bytes) {
+        in.read(static_cast<char*>(v), static_cast<std::streamsize>(bytes));
+        return static_cast<bool>(in);
+    };
+    char magic[8];
+    const char expect[8] = {'K', 'P', 'T', 'P', 'R', 'T', '\0', '\0'};
+    if (!get(magic, 8) || std::memcmp(magic, expect, 8) != 0)
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: unrecognized particle binary magic.");
+    uint32_t version = 0, endian = 0;
+    int32_t dim = 0, sp = 0;
+    uint64_t count = 0, capacity = 0;
+    kpt_particle_properties pr{};
+    kpt_energy_split_policy po{};
+    if (!get(&version, 4) || !get(&endian, 4))
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to read binary particle scalar.");
+    if (version != 4u) return fail(ctx, KPT_ERR_IO, "ParticleSystem: only particle binary version 4 is restart-readable.");
+    if (endian != 0x01020304u)
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: particle binary endian does not match this host.");
+    if (!get(&dim, 4) || !get(&sp, 4) || !get(&count, 8) || !get(&capacity, 8))
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to read binary particle scalar.");
+    if (dim != 2) return fail(ctx, KPT_ERR_IO, "ParticleSystem: snapshot dimension does not match ParticleSystem.");
+    if (sp < 0 || sp > 3) return fail(ctx, KPT_ERR_IO, "ParticleSystem: snapshot contains an invalid particle species.");
+    if (!get(&pr.rest_mass, 8) || !get(&pr.charge, 8) || !get(&pr.speed_of_light, 8) || !get(&pr.energy_scale_erg, 8) ||
+        !get(&po.energy_split_ratio, 8) || !get(&po.minimum_child_weight, 8))
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to read binary particle scalar.");
+    if (!properties_ok(pr)) return fail(ctx, KPT_ERR_IO, "ParticleProperties: invalid particle properties.");
+    if (!policy_ok(po))
+        return fail(ctx, KPT_ERR_IO,
+                    "ParticleSplittingPolicy: energy split ratio must be greater than one and minimum child weight "
+                    "must be finite and non-negative.");
+    if (count > ctx->capacity)
+        return fail(ctx, KPT_ERR_CAPACITY, "ParticleSystem: snapshot stores " + std::to_string(count) +
+                                               " particles, more than the capacity " + std::to_string(ctx->capacity) +
+                                               ".");
+    const uint64_t n = count;
+    std::vector<double> x(n), y(n), p(n), t(n, 0.0), dt(n, 0.0), w(n), mu(n), e0(n), px(n), py(n), psx(n), psy(n);
+    std::vector<int32_t> status(n), level(n), key(n);
+    std::vector<uint64_t> id(n);
+    std::vector<char> rec(static_cast<size_t>(n) * 100);
+    if (n > 0 && !get(rec.data(), rec.size()))
+        return fail(ctx, KPT_ERR_IO, "ParticleSystem: failed to read binary particle scalar.");
+    for (uint64_t i = 0; i < n; ++i) {
+        const char* r = rec.data() + i * 100;
+        std::memcpy(&id[i], r + 0, 8);
+        std::memcpy(&status[i], r + 8, 4);
+        std::memcpy(&level[i], r + 12, 4);
+        std::memcpy(&key[i], r + 16, 4);
+        std::memcpy(&x[i], r + 20, 8);
+        std::memcpy(&y[i], r + 28, 8);
+        std::memcpy(&px[i], r + 36, 8);
+        std::memcpy(&py[i], r + 44, 8);
+        std::memcpy(&psx[i], r + 52, 8);
+        std::memcpy(&psy[i], r + 60, 8);
+        std::memcpy(&p[i], r + 68, 8);
+        std::memcpy(&mu[i], r + 76, 8);
+        std::memcpy(&w[i], r + 84, 8);
+        std::memcpy(&e0[i], r + 92, 8);
+    }
+    kpt_particles_host h{x.data(),  y.data(),  p.data(),   t.data(),   dt.data(),  w.data(),   mu.data(),
+                         status.data(), level.data(), id.data(), e0.data(), px.data(), py.data(), psx.data(),
+                         psy.data(), key.data()};
+    kpt_status st = kpt_particles_upload(ctx, &h, n);
+    if (st != KPT_OK) return st;
+    if (props) *props = pr;
+    if (species) *species = sp;
+    if (policy) *policy = po;
+    return KPT_OK;
 }
 
 }  // extern "C"
@@ -1854,32 +2048,55 @@ kpt_status kpt_generic_parker_time_step(kpt_ctx* ctx, const kpt_generic_parker_p
     return generic_dt(ctx, p, *f, dt);
 }
 
-kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot, uint32_t step,
-                                   double* dt_used, uint64_t* failures) {
+}  // extern "C"
+
+namespace {
+
+// ParkerSolver::advance_stochastic / advance_deterministic (include/
+// ParkerSolver.hpp:228-299): dt, then boundary wrap, capture of the previous
+// positions and the step in one kernel.
+kpt_status generic_advance(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot, uint32_t step,
+                           bool deterministic, double* dt_used, uint64_t* failures) {
     if (!ctx) return KPT_ERR_ARG;
     kpt_status st = check_generic(ctx, p);
     if (st != KPT_OK) return st;
     const FieldSlot* f;
     st = generic_slot(ctx, slot, &f);
     if (st != KPT_OK) return st;
-    if (ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
+    if (!deterministic && ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
         return fail(ctx, KPT_ERR_STATE, "noise table does not cover this step of every stored particle.");
     (void)hipSetDevice(ctx->device);
     double dt = 0.0;
     st = generic_dt(ctx, p, *f, &dt);
     if (st != KPT_OK) return st;
     if (!std::isfinite(dt) || dt <= 0.0)
-        return fail(ctx, KPT_ERR_STATE, "ParkerSolver: adaptive stochastic time step is not positive and finite.");
+        return fail(ctx, KPT_ERR_STATE, deterministic
+                                            ? "ParkerSolver: adaptive deterministic time step is not positive and finite."
+                                            : "ParkerSolver: adaptive stochastic time step is not positive and finite.");
     const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
     KPT_HIP(ctx, hipMemsetAsync(ctx->counters + 5, 0, sizeof(unsigned long long), ctx->stream));
     KPT_HIP(ctx, kpt::launch_generic_step(ctx->P, ctx->count, f->gen, f->g, *p, dt, step, key, ctx->noise,
-                                          ctx->noise_steps, ctx->counters + 5, ctx->stream));
+                                          ctx->noise_steps, deterministic ? 1 : 0, ctx->counters + 5, ctx->stream));
     unsigned long long nf = 0;
     KPT_HIP(ctx, hipMemcpyAsync(&nf, ctx->counters + 5, 8, hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (dt_used) *dt_used = dt;
     if (failures) *failures = nf;
     return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_generic_parker_step(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot, uint32_t step,
+                                   double* dt_used, uint64_t* failures) {
+    return generic_advance(ctx, p, slot, step, false, dt_used, failures);
+}
+
+kpt_status kpt_generic_parker_advance_deterministic(kpt_ctx* ctx, const kpt_generic_parker_params* p, int32_t slot,
+                                                    double* dt_used, uint64_t* failures) {
+    return generic_advance(ctx, p, slot, 0, true, dt_used, failures);
 }
 
 }  // extern "C"
@@ -2152,15 +2369,21 @@ kpt_status kpt_generic_focus_time_step(kpt_ctx* ctx, const kpt_generic_focus_par
     return focus_dt(ctx, p, *f, dt);
 }
 
-kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot, uint32_t step,
-                                  double* dt_used, uint64_t* failures) {
+}  // extern "C"
+
+namespace {
+
+// FocusTransportSolver::advance_stochastic / advance_deterministic (include/
+// FocusSolver.hpp:251-290).
+kpt_status focus_advance(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot, uint32_t step,
+                         bool deterministic, double* dt_used, uint64_t* failures) {
     if (!ctx) return KPT_ERR_ARG;
     kpt_status st = check_focus(ctx, p);
     if (st != KPT_OK) return st;
     const FieldSlot* f;
     st = focus_slot(ctx, slot, &f);
     if (st != KPT_OK) return st;
-    if (ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
+    if (!deterministic && ctx->noise && (ctx->noise_particles < ctx->count || step >= ctx->noise_steps))
         return fail(ctx, KPT_ERR_STATE, "noise table does not cover this step of every stored particle.");
     (void)hipSetDevice(ctx->device);
     double dt = 0.0;
@@ -2168,17 +2391,32 @@ kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* 
     if (st != KPT_OK) return st;
     if (!std::isfinite(dt) || dt <= 0.0)
         return fail(ctx, KPT_ERR_STATE,
-                    "FocusTransportSolver: adaptive stochastic time step is not positive and finite.");
+                    deterministic ? "FocusTransportSolver: adaptive deterministic time step is not positive and finite."
+                                  : "FocusTransportSolver: adaptive stochastic time step is not positive and finite.");
     const uint32_t key[2] = {static_cast<uint32_t>(ctx->seed), static_cast<uint32_t>(ctx->seed >> 32)};
     KPT_HIP(ctx, hipMemsetAsync(ctx->counters + 5, 0, sizeof(unsigned long long), ctx->stream));
     KPT_HIP(ctx, kpt::launch_focus_step(ctx->P, ctx->count, f->foc, f->g, *p, dt, step, key, ctx->noise,
-                                        ctx->noise_steps, ctx->counters + 5, ctx->stream));
+                                        ctx->noise_steps, deterministic ? 1 : 0, ctx->counters + 5, ctx->stream));
     unsigned long long nf = 0;
     KPT_HIP(ctx, hipMemcpyAsync(&nf, ctx->counters + 5, 8, hipMemcpyDeviceToHost, ctx->stream));
     KPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (dt_used) *dt_used = dt;
     if (failures) *failures = nf;
     return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kpt_status kpt_generic_focus_step(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot, uint32_t step,
+                                  double* dt_used, uint64_t* failures) {
+    return focus_advance(ctx, p, slot, step, false, dt_used, failures);
+}
+
+kpt_status kpt_generic_focus_advance_deterministic(kpt_ctx* ctx, const kpt_generic_focus_params* p, int32_t slot,
+                                                   double* dt_used, uint64_t* failures) {
+    return focus_advance(ctx, p, slot, 0, true, dt_used, failures);
 }
 
 }  // extern "C"

@@ -136,6 +136,37 @@ inline void dev_free(void* p) {
     if (p) (void)hipFree(p);
 }
 
+// The ParticleSystem arrays of n uploaded particles: previous positions default
+// to the uploaded position, sort keys to 0.
+inline hipError_t upload_system_arrays(const kpt::ParticleArrays& P, const kpt_particles_host* h, uint64_t n) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    auto put = [&](void* d, const void* src) {
+        if (e == hipSuccess) e = hipMemcpy(d, src, n * 8, hipMemcpyHostToDevice);
+    };
+    put(P.prev_x, h->prev_x ? h->prev_x : h->x);
+    put(P.prev_y, h->prev_y ? h->prev_y : h->y);
+    put(P.prev_step_x, h->prev_step_x ? h->prev_step_x : h->x);
+    put(P.prev_step_y, h->prev_step_y ? h->prev_step_y : h->y);
+    if (e == hipSuccess)
+        e = h->sort_key ? hipMemcpy(P.sort_key, h->sort_key, n * 4, hipMemcpyHostToDevice)
+                        : hipMemset(P.sort_key, 0, n * 4);
+    return e;
+}
+
+inline hipError_t download_system_arrays(kpt_particles_host* h, const kpt::ParticleArrays& P, uint64_t n) {
+    hipError_t e = hipSuccess;
+    auto get = [&](void* d, const void* src, size_t es) {
+        if (e == hipSuccess && d && n) e = hipMemcpy(d, src, n * es, hipMemcpyDeviceToHost);
+    };
+    get(h->prev_x, P.prev_x, 8);
+    get(h->prev_y, P.prev_y, 8);
+    get(h->prev_step_x, P.prev_step_x, 8);
+    get(h->prev_step_y, P.prev_step_y, 8);
+    get(h->sort_key, P.sort_key, 4);
+    return e;
+}
+
 // Helpers defined in kpt_capi.hip.
 kpt_status check_settings(kpt_ctx* ctx, const kpt_device_settings* s);
 kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr);

@@ -51,63 +51,10 @@ struct kpt_ensemble {
 
 namespace {
 
-kpt::ParticleArrays arrays_at(const kpt::ParticleArrays& P, uint64_t o) {
-    kpt::ParticleArrays v = P;
-    v.x += o;
-    v.y += o;
-    v.p += o;
-    v.t += o;
-    v.step_dt += o;
-    v.weight += o;
-    v.mu += o;
-    v.status += o;
-    v.split_level += o;
-    v.id += o;
-    v.e0 += o;
-    v.substep += o;
-    return v;
-}
-
-hipError_t alloc_arrays(kpt::ParticleArrays& P, uint64_t n) {
-    hipError_t e;
-    if ((e = dev_alloc(&P.x, n)) != hipSuccess || (e = dev_alloc(&P.y, n)) != hipSuccess ||
-        (e = dev_alloc(&P.p, n)) != hipSuccess || (e = dev_alloc(&P.t, n)) != hipSuccess ||
-        (e = dev_alloc(&P.step_dt, n)) != hipSuccess || (e = dev_alloc(&P.weight, n)) != hipSuccess ||
-        (e = dev_alloc(&P.mu, n)) != hipSuccess || (e = dev_alloc(&P.status, n)) != hipSuccess ||
-        (e = dev_alloc(&P.split_level, n)) != hipSuccess || (e = dev_alloc(&P.id, n)) != hipSuccess ||
-        (e = dev_alloc(&P.e0, n)) != hipSuccess || (e = dev_alloc(&P.substep, n)) != hipSuccess)
-        return e;
-    return hipSuccess;
-}
-
-void free_arrays(kpt::ParticleArrays& P) {
-    for (void* q : {static_cast<void*>(P.x), static_cast<void*>(P.y), static_cast<void*>(P.p), static_cast<void*>(P.t),
-                    static_cast<void*>(P.step_dt), static_cast<void*>(P.weight), static_cast<void*>(P.mu),
-                    static_cast<void*>(P.status), static_cast<void*>(P.split_level), static_cast<void*>(P.id),
-                    static_cast<void*>(P.e0), static_cast<void*>(P.substep)})
-        dev_free(q);
-    P = kpt::ParticleArrays{};
-}
-
-hipError_t copy_arrays(const kpt::ParticleArrays& dst, const kpt::ParticleArrays& src, uint64_t n, hipStream_t st) {
-    hipError_t e = hipSuccess;
-    auto cp = [&](void* d, const void* s, size_t bytes) {
-        if (e == hipSuccess && bytes) e = hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st);
-    };
-    cp(dst.x, src.x, n * 8);
-    cp(dst.y, src.y, n * 8);
-    cp(dst.p, src.p, n * 8);
-    cp(dst.t, src.t, n * 8);
-    cp(dst.step_dt, src.step_dt, n * 8);
-    cp(dst.weight, src.weight, n * 8);
-    cp(dst.mu, src.mu, n * 8);
-    cp(dst.status, src.status, n * 4);
-    cp(dst.split_level, src.split_level, n * 4);
-    cp(dst.id, src.id, n * 8);
-    cp(dst.e0, src.e0, n * 8);
-    cp(dst.substep, src.substep, n * 4);
-    return e;
-}
+using kpt::alloc_arrays;
+using kpt::arrays_at;
+using kpt::copy_arrays;
+using kpt::free_arrays;
 
 kpt_status ens_run_ok(kpt_ensemble* e, int32_t run) {
     if (!e) return KPT_ERR_ARG;
@@ -284,6 +231,7 @@ kpt_status kpt_ensemble_create(kpt_ctx* ctx, int32_t runs, const uint64_t* capac
     if (er == hipSuccess) er = dev_alloc(&e->d_loop, 1);
     if (er == hipSuccess && total) er = hipMemset(e->P.e0, 0, total * 8);
     if (er == hipSuccess && total) er = hipMemset(e->P.substep, 0, total * 4);
+    if (er == hipSuccess && total) er = hipMemset(e->P.sort_key, 0, total * 4);
     if (er != hipSuccess) {
         kpt_ensemble_destroy(e);
         return hip_fail(ctx, er, "ensemble allocation");
@@ -383,6 +331,7 @@ kpt_status kpt_ensemble_particles_upload(kpt_ensemble* e, int32_t run, const kpt
             KPT_HIP(ctx, hipMemcpy(P.e0, h->e0, count * 8, hipMemcpyHostToDevice));
         else
             KPT_HIP(ctx, hipMemset(P.e0, 0, count * 8));
+        KPT_HIP(ctx, upload_system_arrays(P, h, count));
         std::vector<uint64_t> ids(count);
         for (uint64_t i = 0; i < count; ++i) ids[i] = h->id ? h->id[i] : i;
         for (uint64_t i = 0; i < count; ++i) max_root = std::max<uint64_t>(max_root, (ids[i] & 0xFFFFFFFFull) + 1);
@@ -414,6 +363,7 @@ kpt_status kpt_ensemble_particles_download(kpt_ensemble* e, int32_t run, kpt_par
     if (h->split_level) KPT_HIP(ctx, hipMemcpy(h->split_level, P.split_level, count * 4, hipMemcpyDeviceToHost));
     if (h->id) KPT_HIP(ctx, hipMemcpy(h->id, P.id, count * 8, hipMemcpyDeviceToHost));
     if (h->e0) KPT_HIP(ctx, hipMemcpy(h->e0, P.e0, count * 8, hipMemcpyDeviceToHost));
+    KPT_HIP(ctx, download_system_arrays(h, P, count));
     return KPT_OK;
 }
 

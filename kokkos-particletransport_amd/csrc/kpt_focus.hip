@@ -297,23 +297,42 @@ __global__ void __launch_bounds__(kBlk) focus_step_kernel(ParticleArrays P, uint
                                                           const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                           kpt_generic_focus_params p, double dt, uint32_t step,
                                                           uint32_t k0, uint32_t k1, const double* __restrict__ noise,
-                                                          uint32_t noise_steps,
+                                                          uint32_t noise_steps, int deterministic,
                                                           unsigned long long* __restrict__ failures) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
     uint32_t fail = 0;
+    if (i < count) {
+        // apply_grid_boundary_conditions (periodic wrap of alive particles), then
+        // capture_previous_positions (include/ParticleSystem.hpp:623-628) for
+        // every stored slot, as the solver's advance does before its update
+        const bool alive = P.status[i] == 0;
+        double x = P.x[i], y = P.y[i];
+        if (alive) {
+            x = wrap_axis(g, 0, x);
+            y = wrap_axis(g, 1, y);
+            P.x[i] = x;
+            P.y[i] = y;
+        }
+        P.prev_step_x[i] = P.prev_x[i];
+        P.prev_step_y[i] = P.prev_y[i];
+        P.prev_x[i] = x;
+        P.prev_y[i] = y;
+    }
     if (i < count && P.status[i] == 0) {
-        const double x = wrap_axis(g, 0, P.x[i]), y = wrap_axis(g, 1, P.y[i]);
+        const double x = P.x[i], y = P.y[i];
         FocTerms T;
         if (!focus_terms(f, plane, g, p, x, y, P.p[i], P.mu[i], T)) {
             P.status[i] = 2;
             fail = 1;
         } else {
-            double n[4];
-            if (noise != nullptr) {
-                const double* row = noise + (i * noise_steps + step) * 4;
-                for (int d = 0; d < 4; ++d) n[d] = row[d];
-            } else {
-                philox_normals4(P.id[i], 0x464F4355u, step, k0, k1, n);
+            double n[4] = {0.0, 0.0, 0.0, 0.0};  // advance_deterministic: no increments
+            if (!deterministic) {
+                if (noise != nullptr) {
+                    const double* row = noise + (i * noise_steps + step) * 4;
+                    for (int d = 0; d < 4; ++d) n[d] = row[d];
+                } else {
+                    philox_normals4(P.id[i], 0x464F4355u, step, k0, k1, n);
+                }
             }
             // sample_perpendicular_diffusion_increment (include/FocusSolver.hpp:635-659)
             double par = 0.0;
@@ -365,11 +384,11 @@ hipError_t launch_focus_min_dt(const ParticleArrays& P, uint64_t count, const do
 
 hipError_t launch_focus_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
                              const kpt_generic_focus_params& p, double dt, uint32_t step, const uint32_t key[2],
-                             const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                             const double* noise, uint32_t noise_steps, int deterministic, unsigned long long* failures,
                              hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(focus_step_kernel, dim3(blocks(count)), dim3(kBlk), 0, stream, P, count, f,
-                       coefficient_plane(g.nx, g.ny), g, p, dt, step, key[0], key[1], noise, noise_steps, failures);
+                       coefficient_plane(g.nx, g.ny), g, p, dt, step, key[0], key[1], noise, noise_steps, deterministic, failures);
     return hipGetLastError();
 }
 

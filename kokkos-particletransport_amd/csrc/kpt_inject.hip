@@ -150,7 +150,12 @@ __device__ __forceinline__ void place(const ParticleArrays& P, uint64_t i, doubl
     P.step_dt[i] = k.step_dt;
     P.status[i] = status;
     P.split_level[i] = 0;
+    P.sort_key[i] = 0;
     P.e0[i] = e0;
+    // both previous positions start at the new position, the sort key at 0
+    // (include/ParticleGenerator.hpp:939-953)
+    P.prev_x[i] = P.prev_step_x[i] = x;
+    P.prev_y[i] = P.prev_step_y[i] = y;
 }
 
 // One particle's draw from a candidate set: the candidate by weight, then its

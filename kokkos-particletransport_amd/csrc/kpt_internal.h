@@ -62,7 +62,76 @@ struct ParticleArrays {
     uint64_t* id;
     double* e0;         // split baseline: initial kinetic energy
     uint32_t* substep;  // sub-steps taken in the current interval (push chunks only)
+    // ParticleSystem::previous_position and previous_step_position
+    // (include/ParticleSystem.hpp:342-343): the positions before the last and the
+    // last-but-one generic solver step (capture_previous_positions, :623-628)
+    double* prev_x;
+    double* prev_y;
+    double* prev_step_x;
+    double* prev_step_y;
+    int32_t* sort_key;  // ParticleSystem::sort_key (:349), the key of sort_by_key
 };
+
+// Every array of a particle set with its element size, for the host code that
+// allocates, frees, copies or offsets all of them.
+template <typename F>
+inline void for_each_array(ParticleArrays& P, F&& f) {
+    f(reinterpret_cast<void**>(&P.x), 8);
+    f(reinterpret_cast<void**>(&P.y), 8);
+    f(reinterpret_cast<void**>(&P.p), 8);
+    f(reinterpret_cast<void**>(&P.t), 8);
+    f(reinterpret_cast<void**>(&P.step_dt), 8);
+    f(reinterpret_cast<void**>(&P.weight), 8);
+    f(reinterpret_cast<void**>(&P.mu), 8);
+    f(reinterpret_cast<void**>(&P.status), 4);
+    f(reinterpret_cast<void**>(&P.split_level), 4);
+    f(reinterpret_cast<void**>(&P.id), 8);
+    f(reinterpret_cast<void**>(&P.e0), 8);
+    f(reinterpret_cast<void**>(&P.substep), 4);
+    f(reinterpret_cast<void**>(&P.prev_x), 8);
+    f(reinterpret_cast<void**>(&P.prev_y), 8);
+    f(reinterpret_cast<void**>(&P.prev_step_x), 8);
+    f(reinterpret_cast<void**>(&P.prev_step_y), 8);
+    f(reinterpret_cast<void**>(&P.sort_key), 4);
+}
+
+// The particle set starting at slot `o` of P.
+inline ParticleArrays arrays_at(const ParticleArrays& P, uint64_t o) {
+    ParticleArrays v = P;
+    for_each_array(v, [&](void** a, size_t es) { *a = static_cast<char*>(*a) + o * es; });
+    return v;
+}
+
+inline hipError_t alloc_arrays(ParticleArrays& P, uint64_t n) {
+    hipError_t e = hipSuccess;
+    for_each_array(P, [&](void** a, size_t es) {
+        *a = nullptr;
+        if (e == hipSuccess && n) e = hipMalloc(a, n * es);
+    });
+    return e;
+}
+
+inline void free_arrays(ParticleArrays& P) {
+    for_each_array(P, [](void** a, size_t) {
+        if (*a) (void)hipFree(*a);
+        *a = nullptr;
+    });
+}
+
+// Stream-ordered copy of n slots of every array.
+inline hipError_t copy_arrays(const ParticleArrays& dst, const ParticleArrays& src, uint64_t n, hipStream_t st) {
+    ParticleArrays d = dst, s = src;
+    void* sp[17];
+    int k = 0;
+    for_each_array(s, [&](void** a, size_t) { sp[k++] = *a; });
+    hipError_t e = hipSuccess;
+    k = 0;
+    for_each_array(d, [&](void** a, size_t es) {
+        if (e == hipSuccess && n) e = hipMemcpyAsync(*a, sp[k], n * es, hipMemcpyDeviceToDevice, st);
+        ++k;
+    });
+    return e;
+}
 
 // Constants of the sub-step arithmetic in its own type, formed once on the host
 // (kpt_push_interval) so that the float push reads 32-bit scalars directly.
@@ -172,7 +241,7 @@ hipError_t launch_generic_min_dt(const ParticleArrays& P, uint64_t count, const 
                                  hipStream_t stream);
 hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
                                const kpt_generic_parker_params& p, double dt, uint32_t step, const uint32_t key[2],
-                               const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                               const double* noise, uint32_t noise_steps, int deterministic, unsigned long long* failures,
                                hipStream_t stream);
 // out[2 e + 0] = lo[e], out[2 e + 1] = hi[e] for e < n
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream);
@@ -199,7 +268,7 @@ hipError_t launch_focus_min_dt(const ParticleArrays& P, uint64_t count, const do
                                const kpt_generic_focus_params& p, unsigned long long* min_bits, hipStream_t stream);
 hipError_t launch_focus_step(const ParticleArrays& P, uint64_t count, const double* f, const GridDesc& g,
                              const kpt_generic_focus_params& p, double dt, uint32_t step, const uint32_t key[2],
-                             const double* noise, uint32_t noise_steps, unsigned long long* failures,
+                             const double* noise, uint32_t noise_steps, int deterministic, unsigned long long* failures,
                              hipStream_t stream);
 // Ensemble: `tiles` lists (run, first particle) of every 256-particle tile,
 // grouped per work queue: group g owns tiles [group_start[g], group_start[g+1]).
@@ -322,6 +391,14 @@ hipError_t launch_ensemble_sort(const SortRun* runs, int n_runs, uint64_t total,
 hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,
                        double shift_x, double shift_y, double t_done, SortScratch& w,
                        unsigned long long* active, hipStream_t stream);
+
+// ParticleSystem::sort_by_key / compact_active_particles on the stored sort keys
+// (see stored_key_kernel); `active` (device) accumulates the alive count.
+hipError_t launch_sort_stored(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, int32_t min_key,
+                              int32_t max_key, int compact, SortScratch& w, unsigned long long* active,
+                              hipStream_t stream);
+// ParticleSystem::capture_previous_positions of the first `count` slots.
+hipError_t launch_capture_previous(const ParticleArrays& P, uint64_t count, hipStream_t stream);
 
 // Summary: `partials` holds summary_partial_bytes() bytes.
 size_t summary_partial_bytes();

@@ -1216,6 +1216,11 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(ParticleArrays P, uint64
     // split baseline in the driver's units (rest mass = c = 1), the E0 its
     // snapshots record (src/main.cpp:2160-2170)
     P.e0[i] = sqrt(1.0 + P.p[i] * P.p[i]) - 1.0;
+    // ParticleSystem state of a new particle (set_particle,
+    // include/ParticleSystem.hpp:516-522)
+    P.prev_x[i] = P.prev_step_x[i] = P.x[i];
+    P.prev_y[i] = P.prev_step_y[i] = P.y[i];
+    P.sort_key[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1316,6 +1321,11 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, u
             P.weight[child] = w;
             P.mu[child] = P.mu[i];
             P.e0[child] = P.e0[i];
+            P.prev_x[child] = P.prev_x[i];
+            P.prev_y[child] = P.prev_y[i];
+            P.prev_step_x[child] = P.prev_step_x[i];
+            P.prev_step_y[child] = P.prev_step_y[i];
+            P.sort_key[child] = P.sort_key[i];
             P.status[child] = 0;
             P.split_level[i] = new_level;
             P.split_level[child] = new_level;

@@ -108,6 +108,13 @@ __global__ void __launch_bounds__(kBlk) esplit_apply_kernel(ParticleArrays P, ui
             P.step_dt[child] = P.step_dt[i];
             P.mu[child] = P.mu[i];
             P.e0[child] = P.e0[i];
+            // split_particles_on_energy_growth copies both previous positions
+            // and the sort key (include/ParticleSystem.hpp:883-896)
+            P.prev_x[child] = P.prev_x[i];
+            P.prev_y[child] = P.prev_y[i];
+            P.prev_step_x[child] = P.prev_step_x[i];
+            P.prev_step_y[child] = P.prev_step_y[i];
+            P.sort_key[child] = P.sort_key[i];
             P.weight[i] = cw;
             P.weight[child] = cw;
             P.status[child] = 0;
@@ -181,7 +188,24 @@ __global__ void __launch_bounds__(kBlk) kinematics_kernel(const double* __restri
     if (ekin) ekin[i] = (g - 1.0) * pr.rest_mass * pr.speed_of_light * pr.speed_of_light;
 }
 
+// capture_previous_positions (include/ParticleSystem.hpp:623-628).
+__global__ void __launch_bounds__(kBlk) capture_kernel(ParticleArrays P, uint64_t count) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlk) + threadIdx.x;
+    if (i >= count) return;
+    P.prev_step_x[i] = P.prev_x[i];
+    P.prev_step_y[i] = P.prev_y[i];
+    P.prev_x[i] = P.x[i];
+    P.prev_y[i] = P.y[i];
+}
+
 }  // namespace
+
+hipError_t launch_capture_previous(const ParticleArrays& P, uint64_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(capture_kernel, dim3(static_cast<unsigned>((count + kBlk - 1) / kBlk)), dim3(kBlk), 0, stream,
+                       P, count);
+    return hipGetLastError();
+}
 
 size_t esplit_scratch_elems(uint64_t count) { return static_cast<size_t>(tiles_for(count)) + 1; }
 

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "kpt_device_common.h"
 #include "kpt_internal.h"
 #include "kpt_grid_device.h"
 
@@ -103,6 +104,11 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(ParticleArrays src, Part
     dst.e0[i] = src.e0[s];
     dst.id[i] = src.id[s];
     dst.substep[i] = src.substep[s];
+    dst.prev_x[i] = src.prev_x[s];
+    dst.prev_y[i] = src.prev_y[s];
+    dst.prev_step_x[i] = src.prev_step_x[s];
+    dst.prev_step_y[i] = src.prev_step_y[s];
+    dst.sort_key[i] = src.sort_key[s];
 }
 
 int key_bits(uint32_t max_key) {
@@ -163,6 +169,33 @@ __global__ void __launch_bounds__(kBlock) ensemble_key_kernel(const SortRun* __r
 
 }  // namespace
 
+// Keys of ParticleSystem::sort_by_key (include/ParticleSystem.hpp:727-760):
+// the stored key less min_key, clamped to [0, max_key - min_key]. With
+// `compact`, the key first becomes 0 for alive particles and 1 otherwise and is
+// stored (compact_active_particles, :769-793); `active` counts alive slots.
+__global__ void __launch_bounds__(kBlock) stored_key_kernel(ParticleArrays P, uint64_t count, int32_t min_key,
+                                                            int32_t max_key, int compact, uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals,
+                                                            unsigned long long* __restrict__ active) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    uint32_t alive = 0;
+    if (i < count) {
+        alive = P.status[i] == 0 ? 1u : 0u;
+        int32_t k;
+        if (compact) {
+            k = alive ? 0 : 1;
+            P.sort_key[i] = k;
+        } else {
+            k = P.sort_key[i];
+        }
+        k = k < min_key ? min_key : (k > max_key ? max_key : k);
+        keys[i] = static_cast<uint32_t>(static_cast<int64_t>(k) - min_key);
+        vals[i] = static_cast<uint32_t>(i);
+    }
+    const unsigned int a = wave_sum(alive);
+    if ((threadIdx.x & 63) == 0 && a) atomicAdd(active, static_cast<unsigned long long>(a));
+}
+
 uint32_t sort_key_limit(const GridDesc& g) {
     const uint32_t bx = (static_cast<uint32_t>(g.nx) + 2 + kSortBlock - 1) / kSortBlock;
     const uint32_t by = (static_cast<uint32_t>(g.ny) + 2 + kSortBlock - 1) / kSortBlock;
@@ -193,6 +226,24 @@ size_t sort_temp_bytes(uint64_t capacity) {
     hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, static_cast<int>(capacity), 0, 32);
     return bytes;
+}
+
+hipError_t launch_sort_stored(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, int32_t min_key,
+                              int32_t max_key, int compact, SortScratch& w, unsigned long long* active,
+                              hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(stored_key_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, count, min_key, max_key, compact,
+                       w.keys[0], w.vals[0], active);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipcub::DoubleBuffer<uint32_t> k(w.keys[0], w.keys[1]), v(w.vals[0], w.vals[1]);
+    size_t bytes = w.temp_bytes;
+    const uint32_t top = static_cast<uint32_t>(static_cast<int64_t>(max_key) - min_key);
+    e = hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, k, v, static_cast<int>(count), 0, key_bits(top), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(kBlock), 0, stream, src, dst, count, v.Current());
+    return hipGetLastError();
 }
 
 hipError_t launch_sort(const ParticleArrays& src, const ParticleArrays& dst, uint64_t count, const GridDesc& g,

@@ -32,6 +32,13 @@ PARTICLE_FIELDS = (
     ("step_dt", np.float64), ("weight", np.float64), ("mu", np.float64),
     ("status", np.int32), ("split_level", np.int32), ("id", np.uint64), ("e0", np.float64),
 )
+# ParticleSystem arrays (include/ParticleSystem.hpp:342-349): both previous
+# positions and the sort key. Optional in a particle dict: absent on upload, the
+# previous positions take the current one and the key 0.
+SYSTEM_FIELDS = (
+    ("prev_x", np.float64), ("prev_y", np.float64), ("prev_step_x", np.float64), ("prev_step_y", np.float64),
+    ("sort_key", np.int32),
+)
 
 
 class DeviceSettings(C.Structure):
@@ -94,6 +101,9 @@ class ParticlesHost(C.Structure):
         ("mu", C.POINTER(C.c_double)), ("status", C.POINTER(C.c_int32)),
         ("split_level", C.POINTER(C.c_int32)), ("id", C.POINTER(C.c_uint64)),
         ("e0", C.POINTER(C.c_double)),
+        ("prev_x", C.POINTER(C.c_double)), ("prev_y", C.POINTER(C.c_double)),
+        ("prev_step_x", C.POINTER(C.c_double)), ("prev_step_y", C.POINTER(C.c_double)),
+        ("sort_key", C.POINTER(C.c_int32)),
     ]
 
 
@@ -277,7 +287,7 @@ def _fp(a):
 def particles_struct(parts: dict) -> ParticlesHost:
     """ParticlesHost pointing into a dict of contiguous numpy arrays (None -> NULL)."""
     h = ParticlesHost()
-    for name, dt in PARTICLE_FIELDS:
+    for name, dt in PARTICLE_FIELDS + SYSTEM_FIELDS:
         a = parts.get(name)
         if a is None:
             continue
@@ -287,8 +297,10 @@ def particles_struct(parts: dict) -> ParticlesHost:
     return h
 
 
-def empty_particles(n: int) -> dict:
-    return {name: np.zeros(n, dtype=dt) for name, dt in PARTICLE_FIELDS}
+def empty_particles(n: int, system=False) -> dict:
+    """Zeroed particle dict; `system` adds the ParticleSystem arrays."""
+    fields = PARTICLE_FIELDS + (SYSTEM_FIELDS if system else ())
+    return {name: np.zeros(n, dtype=dt) for name, dt in fields}
 
 
 # ---------------------------------------------------------------------------
@@ -373,6 +385,17 @@ EXPORTS = {
                                               C.POINTER(C.c_double)]),
     "kpt_generic_focus_step": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32, C.c_uint32,
                                          C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_generic_focus_advance_deterministic": (C.c_int, [C.c_void_p, C.POINTER(GenericFocusParams), C.c_int32,
+                                                          C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_generic_parker_advance_deterministic": (C.c_int, [C.c_void_p, C.POINTER(GenericParkerParams), C.c_int32,
+                                                           C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kpt_capture_previous_positions": (C.c_int, [C.c_void_p]),
+    "kpt_sort_by_key": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "kpt_compact_active_particles": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kpt_particle_system_write_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(ParticleProperties),
+                                                     C.c_int32, C.POINTER(EnergySplitPolicy)]),
+    "kpt_particle_system_read_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(ParticleProperties),
+                                                    C.POINTER(C.c_int32), C.POINTER(EnergySplitPolicy)]),
     "kpt_ensemble_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
     "kpt_ensemble_destroy": (C.c_int, [C.c_void_p]),
     "kpt_ensemble_field_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
@@ -687,9 +710,10 @@ class Context:
         h = particles_struct(parts)
         self._check(self.lib.kpt_particles_upload(self.h, C.byref(h), n))
 
-    def download(self, count=None) -> dict:
+    def download(self, count=None, system=False) -> dict:
+        """The stored particles; `system` adds the previous positions and sort keys."""
         n = self.count if count is None else count
-        parts = empty_particles(n)
+        parts = empty_particles(n, system=system)
         h = particles_struct(parts)
         self._check(self.lib.kpt_particles_download(self.h, C.byref(h), n))
         return parts
@@ -789,7 +813,42 @@ class Context:
                                                     C.byref(nf)))
         return dt.value, nf.value
 
+    def generic_advance_deterministic(self, params: "GenericParkerParams", slot=0):
+        """ParkerSolver::advance_deterministic; returns (dt, failures)."""
+        dt, nf = C.c_double(), C.c_uint64()
+        self._check(self.lib.kpt_generic_parker_advance_deterministic(self.h, C.byref(params), slot, C.byref(dt),
+                                                                      C.byref(nf)))
+        return dt.value, nf.value
+
+    def focus_advance_deterministic(self, params: "GenericFocusParams", slot=0):
+        """FocusTransportSolver::advance_deterministic; returns (dt, failures)."""
+        dt, nf = C.c_double(), C.c_uint64()
+        self._check(self.lib.kpt_generic_focus_advance_deterministic(self.h, C.byref(params), slot, C.byref(dt),
+                                                                     C.byref(nf)))
+        return dt.value, nf.value
+
     # ---- ParticleSystem lifecycle ---------------------------------------------
+    def capture_previous_positions(self):
+        self._check(self.lib.kpt_capture_previous_positions(self.h))
+
+    def sort_by_key(self, min_key, max_key):
+        self._check(self.lib.kpt_sort_by_key(self.h, min_key, max_key))
+
+    def compact_active_particles(self) -> int:
+        act = C.c_uint64()
+        self._check(self.lib.kpt_compact_active_particles(self.h, C.byref(act)))
+        return act.value
+
+    def system_snapshot_write(self, path, props: "ParticleProperties", species, policy: "EnergySplitPolicy"):
+        self._check(self.lib.kpt_particle_system_write_snapshot(self.h, path.encode(), C.byref(props), species,
+                                                                C.byref(policy)))
+
+    def system_snapshot_read(self, path):
+        """Replace the stored set from a ParticleSystem snapshot; returns (properties, species, policy)."""
+        pr, po, sp = ParticleProperties(), EnergySplitPolicy(), C.c_int32()
+        self._check(self.lib.kpt_particle_system_read_snapshot(self.h, path.encode(), C.byref(pr), C.byref(sp),
+                                                               C.byref(po)))
+        return pr, sp.value, po
     def split_energy_growth(self, props: "ParticleProperties", policy: "EnergySplitPolicy") -> dict:
         st = SplitStatistics()
         self._check(self.lib.kpt_split_energy_growth(self.h, C.byref(props), C.byref(policy), C.byref(st)))

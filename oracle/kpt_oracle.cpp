@@ -1273,6 +1273,30 @@ void gen_normals(const double* table, uint32_t steps, uint64_t i, uint64_t id, u
 
 }  // namespace
 
+namespace {
+
+// apply_grid_boundary_conditions (periodic wrap of alive particles) followed by
+// capture_previous_positions (include/ParticleSystem.hpp:623-628) for every
+// stored slot, as ParkerSolver / FocusTransportSolver::advance_* do before their
+// update (include/ParkerSolver.hpp:265-282, include/FocusSolver.hpp:276-293);
+// the capture is skipped when the caller passes no previous-position arrays.
+void wrap_and_capture(const OGrid& g, kpt_particles_host* P, uint64_t count) {
+    const bool capture = P->prev_x && P->prev_y && P->prev_step_x && P->prev_step_y;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (P->status[i] == 0) {
+            P->x[i] = g.wrap(0, P->x[i]);
+            P->y[i] = g.wrap(1, P->y[i]);
+        }
+        if (!capture) continue;
+        P->prev_step_x[i] = P->prev_x[i];
+        P->prev_step_y[i] = P->prev_y[i];
+        P->prev_x[i] = P->x[i];
+        P->prev_y[i] = P->y[i];
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 void kpto_generic_default_params(kpt_generic_parker_params* p) {
@@ -1347,6 +1371,7 @@ double kpto_generic_step(const void* field, const kpt_generic_parker_params* p, 
     const double dt = kpto_generic_time_step(field, p, P, count);
     if (failures) *failures = 0;
     if (!(dt > 0.0)) return 0.0;
+    wrap_and_capture(F.g, P, count);
     const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
     uint64_t fails = 0;
 #pragma omp parallel for schedule(static, 1024) reduction(+ : fails)
@@ -1450,6 +1475,12 @@ uint64_t kpto_split_energy_growth(const kpt_particle_properties* pr, double rati
         P->split_level[i] += 1;
         P->split_level[child] = P->split_level[i];
         if (P->id) P->id[child] = first_root + rank;
+        // both previous positions and the sort key follow the parent (:883-896)
+        if (P->prev_x) P->prev_x[child] = P->prev_x[i];
+        if (P->prev_y) P->prev_y[child] = P->prev_y[i];
+        if (P->prev_step_x) P->prev_step_x[child] = P->prev_step_x[i];
+        if (P->prev_step_y) P->prev_step_y[child] = P->prev_step_y[i];
+        if (P->sort_key) P->sort_key[child] = P->sort_key[i];
         ++stats[1];
     }
     return count + stats[1];
@@ -1849,6 +1880,7 @@ double kpto_generic_focus_step(const void* field, const kpt_generic_focus_params
     const double dt = kpto_generic_focus_time_step(field, p, P, count);
     if (failures) *failures = 0;
     if (!(dt > 0.0)) return 0.0;
+    wrap_and_capture(F.g, P, count);
     const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
     uint64_t fails = 0;
 #pragma omp parallel for schedule(static, 1024) reduction(+ : fails)
