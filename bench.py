@@ -16,9 +16,17 @@ the GPU before the timed region) or with the exact formula (--closure qlt). The
 same particles are then pushed by the FP32 exact-QLT push (when the measured
 path is the learned one) and by the plain FP64 exact-QLT push, for the ratios.
 
-Workload `parker`: Parker transport at production shape (the same field and
-grid), 4 194 304 particles, the driver's default physics with adaptive dt, FP64
-or FP32.
+Workload `parker`: the production Parker loop (C2) on the same field and grid
+as a drifting frame series, the driver's default physics with adaptive dt, FP64
+or FP32. One bench step is one frame of the reference's loop: the upper frame is
+uploaded, 1600 x 16 = 25 600 particles are injected, every stored particle is
+pushed through the interval (16 cell-sorted chunks), particles split at
+thresholds x1.2 per level from 2 p0, and the momentum histogram is taken every
+10 frames (the reference's production script, scripts/run_total_benchmark_suite_
+local.sh:132-134). The timed window starts at frame 160 (--c2-start-frame) with
+the 160 x 25 600 particles of the earlier frames stored, in their injected state
+(uniform in the domain at the window's start time, p = p0), so a short run
+measures the loop at a late-run particle count. --parker-push-only pushes 4 194 304 particles injected once instead.
 
 Workload `ensemble`: 64 independent runs, each on its own seeded 512 x 256
 island-chain field variant (guide field Bz ~ U[0, 0.5], inflow ~ U[0.05, 0.15],
@@ -350,7 +358,6 @@ def run_generic(args):
     steps_total = n * k * args.steps
     value = steps_total / elapsed
     secs_per_solver_step = elapsed / (k * args.steps)
-    achieved = n * per_step / secs_per_solver_step / 1e9
     workload = f"generic_{args.solver}_{args.nx}x{args.ny}_island_{n}"
     out = {
         "metric": f"particle-steps/s, generic {args.solver} solver (global dt), FP64",
@@ -369,21 +376,12 @@ def run_generic(args):
         "config": {"workload": workload, "solver": args.solver, "field": f"{args.nx}x{args.ny} uniform, periodic",
                    "particles": n, "solver_steps_per_bench_step": k, "failures": int(fails),
                    "active_after": active, "cell_sorted": True},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(workload),
-            "kernel": ("focus" if focus else "generic") + "_dt_kernel + " + ("focus" if focus else "generic")
-                      + "_step_kernel",
-            "timing": "host clock per synchronous solver step (both launches and the dt read-back)",
-            "ms_per_solver_step": secs_per_solver_step * 1e3,
-            "algorithmic_bytes_per_solver_step": n * per_step,
-            "bytes_per_particle_step": per_step,
-        },
+        "roofline": roofline_block(workload, ("focus" if focus else "generic") + "_dt_kernel + "
+                                   + ("focus" if focus else "generic") + "_step_kernel", secs_per_solver_step,
+                                   k * args.steps, n * per_step),
     }
+    out["roofline"]["timing"] = "host clock per synchronous solver step (both launches and the dt read-back)"
+    out["roofline"]["bytes_per_particle_step"] = per_step
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -470,7 +468,6 @@ def run_ensemble(args):
     value = steps / elapsed
     launches = args.steps * frames
     avg_launch_s = kms / 1e3 / launches
-    achieved = bytes_alg / launches / avg_launch_s / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -500,18 +497,8 @@ def run_ensemble(args):
             "mlp_training": {k: report[k] for k in ("steps", "train_ms", "val_rms_rel_d")},
             "mlp_activation": args.mlp_activation,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(f"ensemble_{len(specs)}_runs_512x256_fp32_{total}"),
-            "kernel": "ensemble_kernel",
-            "kernel_ms_avg": avg_launch_s * 1e3,
-            "launches": int(launches),
-            "algorithmic_bytes_per_launch": bytes_alg / launches,
-        },
+        "roofline": roofline_block(f"ensemble_{len(specs)}_runs_512x256_fp32_{total}", "ensemble_kernel",
+                                   avg_launch_s, launches, bytes_alg / launches),
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -521,7 +508,7 @@ def run_ensemble(args):
             s = W.c5_settings(spec)
             model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
             parts.append(cpu_baseline(spec["xe"], spec["ye"], spec["planes"], spec["planes"], model, s,
-                                      W.c5_particles(spec, s, frame=0), target_seconds=6.0))
+                                      W.c5_particles(spec, s, frame=0), target_seconds=6.0)[0])
         share = [per_kind["parker"] / max(steps, 1), per_kind["focused"] / max(steps, 1)]
         rate = 1.0 / sum(w / b["value"] for w, b in zip(share, parts))  # the measured mix of sub-steps
         out["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": parts[0]["cores"],
@@ -685,6 +672,133 @@ def run_push(args):
     print(json.dumps(out), flush=True)
 
 
+C2_PER_FRAME = 1600 * 16  # particles per frame times rank scale
+C2_CAPACITY = 16_000_000
+
+
+def c2_stored_particles(n, s, frame, seed=160):
+    """The stored particles at the start of `frame`: n particles in the state the
+    driver injects them in (src/main.cpp:995-1048), uniform in the domain, p = p0,
+    weight 1, at the frame's start time."""
+    rng = np.random.default_rng(seed)
+    parts = kpt.empty_particles(n)
+    parts["x"][:] = rng.uniform(s.x_lower, s.x_upper, n)
+    parts["y"][:] = rng.uniform(s.y_lower, s.y_upper, n)
+    parts["p"][:] = s.p0
+    parts["weight"][:] = 1.0
+    parts["t"][:] = frame * s.mhd_dt
+    parts["step_dt"][:] = s.dt_min
+    parts["id"][:] = np.arange(n, dtype=np.uint64)
+    parts["e0"][:] = np.sqrt(1.0 + s.p0 * s.p0) - 1.0
+    return parts
+
+
+def run_c2(args):
+    """The production Parker loop (C2): a window of frames of the frame loop
+    starting from a late-run particle count (see the module docstring)."""
+    prec = args.precision or "fp64"
+    fp32 = prec == "fp32"
+    f0 = args.c2_start_frame
+    n0 = f0 * C2_PER_FRAME
+    stretched = args.grid == "stretched"
+    y_edges = W.stretched_edges(args.ny, ratio=args.stretch_ratio) if stretched else None
+    nframes = args.warmup + args.steps + 1
+    fields = [W.island_field(args.nx, args.ny, seed=7, y_edges=y_edges, frame=f0 + k) for k in range(nframes)]
+    xe, ye = fields[0][0], fields[0][1]
+    h = kpt.default_host_settings()
+    h.split_ratio, h.pmin_split_over_p0 = 1.2, 2.0
+    s = kpt.make_device_settings(h, xe, ye)
+    s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
+    s.push_chunks = args.chunks
+    log(f"bench: C2 loop {args.nx}x{args.ny} {args.grid}, frames {f0}..{f0 + nframes - 2} from {n0} stored "
+        f"particles, {C2_PER_FRAME} injected per frame, {prec}")
+    with kpt.Context(C2_CAPACITY, seed=114514) as ctx:
+        ctx.field_upload(kpt.SLOT_LOWER, kpt.MODEL_PARKER16, *fields[0])
+        ctx.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, *fields[1])
+        ctx.upload(c2_stored_particles(n0, s, f0))
+        start = ctx.download()
+        totals = {"sub_steps": 0, "failures": 0, "split": 0, "skipped": 0, "injected": 0, "histograms": 0}
+        kernel_ms, launches, active_in = 0.0, 0, []
+        t_begin = None
+        for k in range(nframes - 1):
+            f = f0 + k
+            if k == args.warmup:
+                t_begin = time.perf_counter()
+            if k > 0:
+                ctx.field_advance()
+                ctx.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, *fields[k + 1])
+            t0 = f * s.mhd_dt
+            inj = ctx.inject(s, C2_PER_FRAME, t0, f)
+            st = ctx.push_interval(s, t0, f)
+            split, skipped = ctx.split(s)
+            if (f + 1) % 10 == 0:
+                ctx.histogram_logp(128, h.pmin, h.pmax)
+            log(f"  frame {f}: {st.particle_steps} sub-steps, {st.kernel_ms:.1f} ms push, stored {ctx.count}, "
+                f"split {split}")
+            if k >= args.warmup:
+                totals["sub_steps"] += st.particle_steps
+                totals["failures"] += st.failures
+                totals["split"] += split
+                totals["skipped"] += skipped
+                totals["injected"] += inj
+                totals["histograms"] += int((f + 1) % 10 == 0)
+                kernel_ms += st.kernel_ms
+                launches += st.launches
+                active_in.append(st.active_in)
+        elapsed = time.perf_counter() - t_begin
+        stored = ctx.count
+    value = totals["sub_steps"] / elapsed
+    launch_s = kernel_ms / 1e3 / launches
+    per_step = step_bytes(kpt.MODEL_PARKER16, fp32, stretched=stretched)
+    launch_bytes = totals["sub_steps"] / launches * per_step + float(np.mean(active_in)) * \
+        state_bytes(kpt.MODEL_PARKER16)
+    grid_tag = "_stretched" if stretched else ""
+    workload = f"c2_parker_loop_{prec}_{args.nx}x{args.ny}{grid_tag}_island_from_{n0}"
+    out = {
+        "metric": f"particle-steps/s, C2 production Parker loop, {prec.upper()}",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if fp32 else "f64",
+        "data": "synthetic (seeded drifting island-chain frame series; uniform injection)",
+        "config": {
+            "workload": workload,
+            "field": f"{args.nx}x{args.ny}, " + (f"y faces tanh-stretched toward y = 0 (spacing ratio "
+                                                 f"{args.stretch_ratio:g})" if stretched else "uniform grid")
+                     + ", 2 time-blended frames, the upper frame uploaded every step",
+            "frames": f"{f0 + args.warmup}..{f0 + nframes - 2} timed (one bench step = one frame: field advance, "
+                      f"injection, push, split, histogram every 10 frames)",
+            "stored_at_start": n0,
+            "stored_at_end": int(stored),
+            "injected_per_frame": C2_PER_FRAME,
+            "capacity": C2_CAPACITY,
+            "split_ratio": h.split_ratio,
+            "pmin_split_over_p0": h.pmin_split_over_p0,
+            "push_chunks": args.chunks,
+            **{k: int(v) for k, v in totals.items()},
+        },
+        "roofline": roofline_block(workload, "push_kernel", launch_s, launches, launch_bytes),
+    }
+    out["roofline"]["bytes_per_step"] = per_step
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        # one interval of the stored particles, from the window's first frame pair
+        sample = {k: v.copy() for k, v in start.items()}
+        sample["t"][:] = 0.0
+        xe1, ye1, p0 = fields[0]
+        _, _, p1 = fields[1]
+        out["cpu_baseline"], smp, want = cpu_baseline(xe1, ye1, p0, p1, kpt.MODEL_PARKER16, s, sample, seed=7)
+        log("accuracy of the measured path on the baseline's sample ...")
+        out["accuracy"] = accuracy_gate(xe1, ye1, p0, p1, kpt.MODEL_PARKER16, s, None, smp, want, 7)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true",
@@ -696,6 +810,10 @@ def main():
     ap.add_argument("--solver", choices=["parker", "focused"], default="parker",
                     help="generic workload: ParkerSolver or FocusTransportSolver")
     ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
+    ap.add_argument("--c2-start-frame", type=int, default=160,
+                    help="parker: first frame of the timed window (stored particles = frames x 25 600)")
+    ap.add_argument("--parker-push-only", action="store_true",
+                    help="parker: push 4 194 304 particles injected once, without the frame loop")
     ap.add_argument("--sort-every", type=int, default=1,
                     help="ensemble: intervals between batched cell sorts of every run")
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's size")
@@ -739,6 +857,9 @@ def main():
         return
     if args.workload == "generic":
         run_generic(args)
+        return
+    if args.workload == "parker" and not args.parker_push_only:
+        run_c2(args)
         return
     run_push(args)
 

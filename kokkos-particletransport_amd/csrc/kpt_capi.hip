@@ -512,6 +512,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     kpt::ParticleArrays& P = ctx->P;
     if ((e = kpt::alloc_arrays(P, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = dev_alloc(&ctx->d_args, 1)) != hipSuccess) return bail(e, "hipMalloc");
     ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
     if ((e = dev_alloc(&ctx->split_scratch, ctx->split_scratch_cap)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->hist_partials, kpt::hist_partial_elems(4096))) != hipSuccess) return bail(e, "hipMalloc");
@@ -542,7 +543,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
     if (S.stream) (void)hipStreamSynchronize(S.stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     kpt::free_arrays(ctx->P);
-    for (void* p : {static_cast<void*>(ctx->counters),
+    for (void* p : {static_cast<void*>(ctx->counters), static_cast<void*>(ctx->d_args),
                     static_cast<void*>(ctx->split_scratch), static_cast<void*>(ctx->hist_partials),
                     static_cast<void*>(ctx->hist_out), ctx->summary_partials,
                     static_cast<void*>(ctx->summary_out), static_cast<void*>(ctx->noise)})
@@ -1007,6 +1008,10 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
         a.first_chunk = k == 0 ? 1 : 0;
         a.t_stop = k == chunks - 1 ? a.t_end : t0 + s->mhd_dt * static_cast<double>(k + 1) / chunks;
         KPT_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        // the arguments in device memory for the push's near-null sub-steps; `a`
+        // stays unchanged until the stream is synchronised below
+        a.self = ctx->d_args;
+        KPT_HIP(ctx, hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, ctx->stream));
         KPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         KPT_HIP(ctx, kpt::launch_push(a, lo.model, focused, fp32, s->closure, ctx->stream));
         KPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));

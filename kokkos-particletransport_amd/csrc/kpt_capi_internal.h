@@ -94,6 +94,7 @@ struct kpt_ctx {
     uint64_t noise_particles = 0;
     uint32_t noise_steps = 0;
     unsigned long long* counters = nullptr;  // 8 x u64
+    kpt::PushArgs* d_args = nullptr;         // device copy of the current push launch's arguments
     uint32_t* split_scratch = nullptr;
     size_t split_scratch_cap = 0;
     double* hist_partials = nullptr;

@@ -129,6 +129,7 @@ kpt_status ens_sync(kpt_ensemble* e) {
         }
         if (R.kind == kpt::kEnsFocusedMlp32) e->has_mlp = true;
         a.counters = e->d_counters + 4 * static_cast<size_t>(r);
+        a.self = &e->d_runs[r].a;  // the run table is the arguments' device copy
         R.pad = 0;
     }
     // tiles of 256 particles; groups: one queue, or one per XCD with the runs

@@ -210,6 +210,9 @@ struct PushArgs {
     int32_t mlp_on;
     int32_t mlp_act;
     unsigned long long* counters;  // [0] particle steps, [1] failures, [2] active at entry
+    // These arguments in device memory: the FP32 push's rare near-null sub-step
+    // (a non-inlined call) reads the FP64 frames and constants through it.
+    const PushArgs* self;
 };
 
 // One run of a batched ensemble: its full push arguments (particles point into

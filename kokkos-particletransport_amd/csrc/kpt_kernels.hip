@@ -501,6 +501,24 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
     }
 }
 
+// The sub-step of a near-null lane of the FP32 push in FP64: the FP64 frames
+// sampled four components at a time, the rates in FP64 arithmetic, narrowed to
+// float. Not inlined: its 21-double sample would otherwise set the register
+// budget of the whole FP32 sub-step loop, which spilled to scratch on every
+// sub-step for a branch a few lanes take; a call keeps those registers and the
+// spills around it on the rare path.
+template <int MODEL, bool FOCUSED, bool QLT>
+__device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, bool blend, double uw, int64_t n00,
+                                                         int nxp, double wx, double wy, double p, double mu,
+                                                         StepTerms<float>* out) {
+    double cd[MODEL];
+    sample_blend_fp64_grouped<MODEL>(a->field_lo64, a->field_hi64, blend, uw, n00, nxp, a->plane, wx, wy, cd);
+    StepTerms<double> Td;
+    const bool ok = eval_terms<MODEL, FOCUSED, QLT, double>(cd, p, mu, a->s, a->k64, Td);
+    narrow_terms(Td, *out);
+    return ok;
+}
+
 // ---------------------------------------------------------------------------
 // The push: one lane advances one particle through the whole interval
 // (move_parker_particles_one_interval, src/main.cpp:1113-1325, and the shared
@@ -609,12 +627,10 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
                 if constexpr (FP32) {
                     if (!(c[MODEL == 16 ? 5 : 6] >= a.k32.null_b)) {
                         // near a null: the FP64 frames and FP64 arithmetic for this sub-step
-                        double cd[CP];
-                        sample_blend_fp64_grouped<CP>(a.field_lo64, a.field_hi64, blend, uw, n00, nxp, a.plane,
-                                                      wx, wy, cd);
-                        StepTerms<double> Td;
-                        ok = eval_terms<MODEL, FOCUSED, QLT, double>(cd, p, pitch_mu, s, a.k64, Td);
-                        narrow_terms(Td, T);
+                        StepTerms<float> Tn;
+                        ok = near_null_terms<MODEL, FOCUSED, QLT>(a.self, blend, uw, n00, nxp, wx, wy, p, pitch_mu,
+                                                                  &Tn);
+                        T = Tn;
                     } else {
                         ok = eval_terms<MODEL, FOCUSED, QLT, float>(c, p, pitch_mu, s, a.k32, T);
                     }
@@ -914,11 +930,9 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
                                             blend, uw, n00, nxp, a.plane, wx, wy, c);
                 bool ok;
                 if (!(c[6] >= K.null_b)) {
-                    double cd[CP];
-                    sample_blend_fp64_grouped<CP>(a.field_lo64, a.field_hi64, blend, uw, n00, nxp, a.plane, wx, wy, cd);
-                    StepTerms<double> Td;
-                    ok = eval_terms<21, true, true, double>(cd, p, pitch_mu, s, a.k64, Td);
-                    narrow_terms(Td, T);
+                    StepTerms<float> Tn;
+                    ok = near_null_terms<21, true, true>(a.self, blend, uw, n00, nxp, wx, wy, p, pitch_mu, &Tn);
+                    T = Tn;
                 } else {
                     ok = eval_terms<21, true, true, float>(c, p, pitch_mu, s, K, T);
                 }
