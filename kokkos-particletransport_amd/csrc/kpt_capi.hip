@@ -1999,8 +1999,6 @@ kpt_status kpt_generic_parker_build(kpt_ctx* ctx, const kpt_generic_parker_param
     if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
     FieldSlot& f = ctx->slot[slot];
     if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
-    if (f.g.face[0] || f.g.face[1])
-        return fail(ctx, KPT_ERR_ARG, "the generic solvers run on a uniform grid; this frame has stretched faces.");
     (void)hipSetDevice(ctx->device);
     const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_PARKER_COMPONENTS;
     if (elems > f.gen_cap) {
@@ -2308,8 +2306,6 @@ kpt_status kpt_generic_focus_build(kpt_ctx* ctx, const kpt_generic_focus_params*
     if (slot != KPT_SLOT_LOWER && slot != KPT_SLOT_UPPER) return fail(ctx, KPT_ERR_ARG, "invalid field slot.");
     FieldSlot& f = ctx->slot[slot];
     if (!f.valid) return fail(ctx, KPT_ERR_STATE, "field slot is empty.");
-    if (f.g.face[0] || f.g.face[1])
-        return fail(ctx, KPT_ERR_ARG, "the generic solvers run on a uniform grid; this frame has stretched faces.");
     (void)hipSetDevice(ctx->device);
     const size_t elems = kpt::coefficient_plane(f.g.nx, f.g.ny) * KPT_GENERIC_FOCUS_COMPONENTS;
     if (elems > f.foc_cap) {

@@ -149,6 +149,7 @@ struct FocTerms {
     double p, mu, gamma, speed, kperp, prate;
     double bmag, divv, bbg, bdv, foc, s2, lc;
     double D, mu_rate;
+    double h[2];  // width of the particle's cell along each axis
 };
 
 __device__ __forceinline__ double floor_positive(double v, double f) { return f <= 0.0 ? v : (v < f ? f : v); }
@@ -196,7 +197,7 @@ __device__ __forceinline__ bool focus_terms(const double* __restrict__ f, uint64
     const kpt_generic_parker_params& M = P.model;
     int i0, j0;
     double wx, wy;
-    if (!axis_stencil_cc(x, g, 0, g.nx, i0, wx) || !axis_stencil_cc(y, g, 1, g.ny, j0, wy)) return false;
+    if (!cell_stencil(x, g, 0, g.nx, i0, wx, T.h[0]) || !cell_stencil(y, g, 1, g.ny, j0, wy, T.h[1])) return false;
     const int nxp = g.nx + 2;
     const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
     const double w00 = (1.0 - wx) * (1.0 - wy), w10 = wx * (1.0 - wy), w01 = (1.0 - wx) * wy, w11 = wx * wy;
@@ -247,11 +248,11 @@ __device__ __forceinline__ bool focus_terms(const double* __restrict__ f, uint64
 
 // local_stable_time_step (include/FocusSolver.hpp:1149-1232): diffusion,
 // advection, momentum and the three pitch-angle bounds (relaxed by 10).
-__device__ __forceinline__ double focus_local_dt(const GridDesc& g, const FocTerms& T) {
+__device__ __forceinline__ double focus_local_dt(const FocTerms& T) {
     const double large = 1.0e300;
     double ddt = large, adt = large;
     for (int d = 0; d < 2; ++d) {
-        const double dx = fabs(g.step[d]);
+        const double dx = T.h[d];
         double di = large, ai = large;
         if (T.keff[d] > 0.0) {
             const double lim = 0.5 * dx;
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(kBlk) focus_dt_kernel(ParticleArrays P, uint64
         if (P.status[i] != 0) continue;
         FocTerms T;
         if (focus_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], P.mu[i], T)) {
-            const double l = focus_local_dt(g, T);
+            const double l = focus_local_dt(T);
             m = l < m ? l : m;
         }
     }

@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(kBlk) generic_pack_kernel(const double* __rest
 
 struct GenTerms {
     double u[2], b[3], kpar, kperp, keff[2], divv, p, prate;
+    double h[2];  // width of the particle's cell along each axis
 };
 
 // evaluate_terms_from_packed (ParkerSolver.hpp:771-875) at a wrapped position.
@@ -78,7 +79,7 @@ __device__ __forceinline__ bool generic_terms(const double* __restrict__ f, uint
                                               GenTerms& T) {
     int i0, j0;
     double wx, wy;
-    if (!axis_stencil_cc(x, g, 0, g.nx, i0, wx) || !axis_stencil_cc(y, g, 1, g.ny, j0, wy)) return false;
+    if (!cell_stencil(x, g, 0, g.nx, i0, wx, T.h[0]) || !cell_stencil(y, g, 1, g.ny, j0, wy, T.h[1])) return false;
     const int nxp = g.nx + 2;
     const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
     const double w00 = (1.0 - wx) * (1.0 - wy), w10 = wx * (1.0 - wy), w01 = (1.0 - wx) * wy, w11 = wx * wy;
@@ -117,12 +118,13 @@ __device__ __forceinline__ bool generic_terms(const double* __restrict__ f, uint
     return true;
 }
 
-// local_stable_time_step (ParkerSolver.hpp:956-1007).
-__device__ __forceinline__ double generic_local_dt(const GridDesc& g, const GenTerms& T) {
+// local_stable_time_step (ParkerSolver.hpp:956-1007) with the width of the
+// particle's cell.
+__device__ __forceinline__ double generic_local_dt(const GenTerms& T) {
     const double large = 1.0e300;
     double ddt = large, adt = large;
     for (int d = 0; d < 2; ++d) {
-        const double dx = fabs(g.step[d]);
+        const double dx = T.h[d];
         double di = large, ai = large;
         if (T.keff[d] > 0.0) {
             const double lim = 0.5 * dx;
@@ -150,7 +152,7 @@ __global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint
         if (P.status[i] != 0) continue;
         GenTerms T;
         if (generic_terms(f, plane, g, p, wrap_axis(g, 0, P.x[i]), wrap_axis(g, 1, P.y[i]), P.p[i], T)) {
-            const double l = generic_local_dt(g, T);
+            const double l = generic_local_dt(T);
             m = l < m ? l : m;
         }
     }

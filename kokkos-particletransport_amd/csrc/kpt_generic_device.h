@@ -20,8 +20,14 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     return w < 0 ? w + n : w;
 }
 
-// Cell-centre coordinate (AnalyticCoordinateGrid, include/AnalyticCoordinateGrid.hpp:159-176).
+// Cell-centre coordinate (AnalyticCoordinateGrid, include/AnalyticCoordinateGrid.hpp:159-176;
+// on a stretched axis StoredCoordinateGrid's (e_k + e_{k+1}) / 2 of the periodic faces,
+// include/StoredCoordinateGrid.hpp:411-425), k in [-2, n].
 __device__ __forceinline__ double cell_coord(const GridDesc& g, int d, int k) {
+    if (g.face[d] != nullptr) {
+        const int n = d == 0 ? g.nx : g.ny;
+        return 0.5 * (stretched_node(g, d, n, k) + stretched_node(g, d, n, k + 1));
+    }
     return 0.5 * ((g.start[d] + k * g.step[d]) + (g.start[d] + (k + 1.0) * g.step[d]));
 }
 

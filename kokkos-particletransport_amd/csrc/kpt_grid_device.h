@@ -86,9 +86,38 @@ __device__ __forceinline__ double stretched_node(const GridDesc& g, int d, int n
     return g.face[d][m] + static_cast<double>(period) * g.width[d];
 }
 
-// The cell-centred stencil of the generic solvers (no interpolation shift).
-__device__ __forceinline__ bool axis_stencil_cc(double q, const GridDesc& g, int d, int n, int& i0, double& w) {
-    return axis_stencil(q, g, d, n, i0, w);
+// The cell-centred stencil of the generic solvers (no interpolation shift), with
+// `h` the width of the cell containing q (the physical_cell_width of their dt
+// bound). Uniform axis: the analytic inverse map. Stretched axis: cell centres
+// c_k = (e_k + e_{k+1}) / 2 of StoredCoordinateGrid (include/
+// StoredCoordinateGrid.hpp:411-425); q lies in face interval m (one bin record),
+// and the centre interval is [c_m, c_{m+1}) or [c_{m-1}, c_m) by the side of c_m,
+// as the reference's bisection over cell coordinates finds it (:128-173).
+__device__ __forceinline__ bool cell_stencil(double q, const GridDesc& g, int d, int n, int& i0, double& w,
+                                             double& h) {
+    if (g.face[d] == nullptr) {
+        h = fabs(g.step[d]);
+        return axis_stencil(q, g, d, n, i0, w);
+    }
+    int m;
+    double wf;
+    if (!node_stencil(q, g, d, n, m, wf, h)) return false;
+    const double qw = grid_wrap(q, g.phys_lo[d], g.phys_hi[d], g.width[d], g.inv_width[d]);
+    const double el = stretched_node(g, d, n, m), eu = stretched_node(g, d, n, m + 1);
+    const double cm = 0.5 * (el + eu);
+    double clo, chi;
+    if (qw >= cm) {
+        i0 = m;
+        clo = cm;
+        chi = 0.5 * (eu + stretched_node(g, d, n, m + 2));
+    } else {
+        i0 = m - 1;
+        clo = 0.5 * (stretched_node(g, d, n, m - 1) + el);
+        chi = cm;
+    }
+    double ww = (qw - clo) / (chi - clo);
+    w = ww < 0.0 ? 0.0 : (ww > 1.0 ? 1.0 : ww);
+    return true;
 }
 
 }  // namespace kpt

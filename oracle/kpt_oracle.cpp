@@ -93,6 +93,9 @@ struct OGrid {
     // value k sits at face k, extended periodically
     bool stretched[2] = {false, false};
     std::vector<double> faces[2];
+    // the generic solvers' convention: value k sits at cell centre k, the mean of
+    // faces k and k + 1 (StoredCoordinateGrid, include/StoredCoordinateGrid.hpp:411-425)
+    bool cell_centred = false;
 
     double node(int d, int k) const {
         const int m = ((k % n[d]) + n[d]) % n[d];
@@ -102,6 +105,10 @@ struct OGrid {
     // AnalyticCoordinateGrid face / cell coordinate (:159-176).
     double face(int d, double i) const { return start[d] + i * step[d]; }
     double cell(int d, double i) const { return 0.5 * (face(d, i) + face(d, i + 1.0)); }
+    // Cell-centre coordinate of stored cell k (any k; ghost cells by periodic faces).
+    double cell_at(int d, int k) const {
+        return stretched[d] ? 0.5 * (node(d, k) + node(d, k + 1)) : cell(d, static_cast<double>(k));
+    }
     // Cell-centred inverse map, uniform spacing (:205-210).
     double cell_grid_index(int d, double x) const { return (x - start[d]) / step[d] - 0.5; }
     int lower_ghosted(int) const { return -kGhost; }
@@ -173,6 +180,16 @@ void stretched_interval(const OGrid& g, int d, double q, int& k, double& w, doub
     h = e[k + 1] - e[k];
     w = (q - e[k]) / h;
     w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+}
+
+// physical_cell_width at a coordinate: the uniform spacing, or on a stretched axis
+// the width of the face interval holding the wrapped coordinate.
+double cell_width(const OGrid& g, int d, double q) {
+    if (!g.stretched[d]) return std::fabs(g.step[d]);
+    int k;
+    double w, h;
+    stretched_interval(g, d, g.wrap(d, q), k, w, h);
+    return h;
 }
 
 // Field::fill_ghost_cells, periodic in both axes (Field.hpp:274-316).
@@ -328,6 +345,20 @@ void sample(const OField& f, const double q_in[2], double* result) {
     int lo[2], hi[2];
     double w[2];
     for (int d = 0; d < 2; ++d) {
+        if (f.g.stretched[d] && f.g.cell_centred) {
+            // bisection over the cell coordinates (StoredCoordinateGrid::grid_index,
+            // include/StoredCoordinateGrid.hpp:128-173): q lies in face interval m,
+            // hence in [c_m, c_{m+1}) or [c_{m-1}, c_m)
+            int m;
+            double wf, h;
+            stretched_interval(f.g, d, q[d], m, wf, h);
+            const int k = q[d] >= f.g.cell_at(d, m) ? m : m - 1;
+            const double clo = f.g.cell_at(d, k), chi = f.g.cell_at(d, k + 1);
+            lo[d] = k;
+            hi[d] = k + 1;
+            w[d] = std::min(1.0, std::max(0.0, (q[d] - clo) / (chi - clo)));
+            continue;
+        }
         if (f.g.stretched[d]) {
             double h;
             stretched_interval(f.g, d, q[d], lo[d], w[d], h);
@@ -1050,7 +1081,7 @@ double gen_derivative(const OGrid& g, int i, int j, int dim, F f) {
     lo[dim] -= 1;
     up[dim] += 1;
     const int c = dim == 0 ? i : j;
-    return centered_three_point(g.cell(dim, c - 1), g.cell(dim, c), g.cell(dim, c + 1), f(lo[0], lo[1]),
+    return centered_three_point(g.cell_at(dim, c - 1), g.cell_at(dim, c), g.cell_at(dim, c + 1), f(lo[0], lo[1]),
                                 f(i, j), f(up[0], up[1]));
 }
 
@@ -1189,6 +1220,7 @@ struct GenTerms {
     bool ok = false;
     double u[3] = {0, 0, 0}, b[3] = {0, 0, 0}, kpar = 0, kperp = 0, keff[2] = {0, 0}, divv = 0, p = 0,
            prate = 0;
+    double h[2] = {0, 0};  // width of the particle's cell along each axis
 };
 
 // evaluate_terms_from_packed (ParkerSolver.hpp:771-875).
@@ -1196,6 +1228,7 @@ GenTerms gen_terms(const OField& F, const kpt_generic_parker_params& P, const do
     GenTerms T;
     double v[KPT_GENERIC_PARKER_COMPONENTS];
     sample(F, pos, v);
+    for (int d = 0; d < 2; ++d) T.h[d] = cell_width(F.g, d, pos[d]);
     const double pm = std::fabs(p_in);
     const double nm = pm / (P.rest_mass * P.speed_of_light);
     const double gamma = std::sqrt(1.0 + nm * nm);
@@ -1228,12 +1261,13 @@ GenTerms gen_terms(const OField& F, const kpt_generic_parker_params& P, const do
     return T;
 }
 
-// local_stable_time_step (ParkerSolver.hpp:956-1007).
-double gen_local_dt(const OGrid& g, const GenTerms& T) {
+// local_stable_time_step (ParkerSolver.hpp:956-1007) with the width of the
+// particle's cell.
+double gen_local_dt(const GenTerms& T) {
     const double large = 1.0e300;
     double ddt = large, adt = large;
     for (int d = 0; d < 2; ++d) {
-        const double dx = std::fabs(g.step[d]);
+        const double dx = T.h[d];
         double di = large, ai = large;
         if (T.keff[d] > 0.0) {
             const double lim = 0.5 * dx;
@@ -1321,6 +1355,7 @@ void* kpto_generic_create(int32_t nx, int32_t ny, const double* xe, const double
     OField B, V;
     for (OField* f : {&B, &V}) {
         make_grid(f->g, nx, ny, xe, ye);
+        f->g.cell_centred = true;
         *f = gen_alloc(f->g, 3);
     }
     for (int j = 0; j < ny; ++j)
@@ -1355,7 +1390,7 @@ double kpto_generic_time_step(const void* field, const kpt_generic_parker_params
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
         const GenTerms T = gen_terms(F, *p, pos, P->p[i]);
         if (!T.ok) continue;
-        const double d = gen_local_dt(F.g, T);
+        const double d = gen_local_dt(T);
         m = m < d ? m : d;
     }
     if (!std::isfinite(m) || m >= 0.5e300) return 0.0;
@@ -1679,6 +1714,7 @@ struct FocTerms {
     double p = 0, mu = 0, gamma = 0, speed = 0, kpar = 0, kperp = 0, prate = 0;
     double bmag = 0, divv = 0, bbg = 0, bdv = 0, foc = 0, s2 = 0, lc = 0;
     double D = 0, dD = 0, mu_rate = 0;
+    double h[2] = {0, 0};  // width of the particle's cell along each axis
 };
 
 // evaluate_pitch_angle_at_mu (FocusSolver.hpp:665-694) with
@@ -1710,6 +1746,7 @@ FocTerms foc_terms(const OField& F, const kpt_generic_focus_params& P, const dou
     FocTerms T;
     double v[kFC];
     sample(F, pos, v);
+    for (int d = 0; d < 2; ++d) T.h[d] = cell_width(F.g, d, pos[d]);
     T.p = std::fabs(p_in);
     T.mu = reflect_mu(mu_in, P.maximum_pitch_angle_cosine);
     const double nm = T.p / (M.rest_mass * M.speed_of_light);
@@ -1752,11 +1789,11 @@ FocTerms foc_terms(const OField& F, const kpt_generic_focus_params& P, const dou
 }
 
 // local_stable_time_step (FocusSolver.hpp:1149-1232).
-double foc_local_dt(const OGrid& g, const FocTerms& T) {
+double foc_local_dt(const FocTerms& T) {
     const double large = 1.0e300;
     double ddt = large, adt = large;
     for (int d = 0; d < 2; ++d) {
-        const double dx = std::fabs(g.step[d]);
+        const double dx = T.h[d];
         double di = large, ai = large;
         if (T.keff[d] > 0.0) {
             const double lim = 0.5 * dx;
@@ -1820,6 +1857,7 @@ void* kpto_generic_focus_create(int32_t nx, int32_t ny, const double* xe, const 
     OField B, V, T;
     for (OField* f : {&B, &V, &T}) {
         make_grid(f->g, nx, ny, xe, ye);
+        f->g.cell_centred = true;
         *f = gen_alloc(f->g, 3);
     }
     const size_t plane = static_cast<size_t>(nx) * ny;
@@ -1852,7 +1890,7 @@ double kpto_generic_focus_time_step(const void* field, const kpt_generic_focus_p
         const double pos[2] = {F.g.wrap(0, P->x[i]), F.g.wrap(1, P->y[i])};
         const FocTerms T = foc_terms(F, *p, pos, P->p[i], P->mu[i]);
         if (!T.ok) continue;
-        const double d = foc_local_dt(F.g, T);
+        const double d = foc_local_dt(T);
         m = m < d ? m : d;
     }
     if (!std::isfinite(m) || m >= 0.5e300) return 0.0;
@@ -1867,7 +1905,7 @@ int kpto_generic_focus_terms(const void* field, const kpt_generic_focus_params* 
     const double pos[2] = {F.g.wrap(0, x), F.g.wrap(1, y)};
     const FocTerms T = foc_terms(F, *p, pos, pm, mu);
     if (!T.ok) return 0;
-    const double v[14] = {foc_local_dt(F.g, T), T.u[0], T.u[1], T.prate, T.mu_rate, T.D, T.dD,
+    const double v[14] = {foc_local_dt(T), T.u[0], T.u[1], T.prate, T.mu_rate, T.D, T.dD,
                           T.kpar, T.kperp, T.keff[0], T.keff[1], T.b[0], T.b[1], T.b[2]};
     for (int k = 0; k < 14; ++k) out[k] = v[k];
     return 1;

@@ -1,7 +1,7 @@
 """GPU parity of the generic focused-transport solver (include/FocusSolver.hpp)
 against the oracle: the packed 27-component basis at every stored node (with
 and without dV/dt), the global time step and ten injected-noise steps of x, y,
-p and mu to 1e-12; with Philox, the perpendicular spread against 2 kappa_perp t;
+p and mu to 1e-12, on a uniform grid and on a grid stretched along both axes; with Philox, the perpendicular spread against 2 kappa_perp t;
 zero-momentum particles marked and counted; the reference's abort cases
 rejected at build time."""
 import numpy as np
@@ -14,8 +14,14 @@ import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _case(model, n=4096, seed=3):
-    xe, ye, planes = W.island_field(64, 32, seed=seed)
+def _case(model, n=4096, seed=3, stretched=False):
+    if stretched:
+        # both axes stretched: x by 4 toward x = 0, y by 10 toward the sheet
+        ye = W.stretched_edges(32, ratio=10.0)
+        _, ye, planes = W.island_field(64, 32, seed=seed, y_edges=ye)
+        xe = W.stretched_edges(64, -1.0, 1.0, ratio=4.0)
+    else:
+        xe, ye, planes = W.island_field(64, 32, seed=seed)
     planes[2] = planes[2] + 0.2  # guide field: |B| > 0 everywhere
     p = kpt.generic_focus_default_params()
     p.model.perpendicular_model = model
@@ -41,9 +47,10 @@ def _close(got, want, scale_axis=(0, 1)):
     return err.max(), np.unravel_index(err.argmax(), err.shape)
 
 
-@pytest.mark.parametrize("model,with_dvdt", [(0, False), (1, True)])
-def test_focus_basis_and_steps_match_oracle(model, with_dvdt):
-    xe, ye, planes, p, parts, dvdt = _case(model)
+@pytest.mark.parametrize("model,with_dvdt,stretched", [(0, False, False), (1, True, False), (0, True, True),
+                                                        (1, False, True)])
+def test_focus_basis_and_steps_match_oracle(model, with_dvdt, stretched):
+    xe, ye, planes, p, parts, dvdt = _case(model, stretched=stretched)
     dv = dvdt if with_dvdt else None
     n = len(parts["x"])
     of = O.GenericFocusField(xe, ye, planes, p, dvdt=dv)

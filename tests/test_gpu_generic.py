@@ -1,6 +1,8 @@
 """GPU parity of the generic Parker solver (include/ParkerSolver.hpp) against the
 oracle: the packed 18-component coefficient basis at every stored node, the
-global time step, and ten injected-noise steps to 1e-12; with Philox, the
+global time step, and ten injected-noise steps to 1e-12, on a uniform grid and
+on a grid stretched along both axes (cell centres midway between stored faces,
+the cell width of the particle's cell in the time-step bound); with Philox, the
 diffusive spread against 2 kappa t."""
 import numpy as np
 import pytest
@@ -12,8 +14,14 @@ import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _case(model, n=4096, seed=3):
-    xe, ye, planes = W.island_field(64, 32, seed=seed)
+def _case(model, n=4096, seed=3, stretched=False):
+    if stretched:
+        # both axes stretched: x by 4 toward x = 0, y by 10 toward the sheet
+        ye = W.stretched_edges(32, ratio=10.0)
+        _, ye, planes = W.island_field(64, 32, seed=seed, y_edges=ye)
+        xe = W.stretched_edges(64, -1.0, 1.0, ratio=4.0)
+    else:
+        xe, ye, planes = W.island_field(64, 32, seed=seed)
     planes[2] = planes[2] + 0.2  # guide field: |B| > 0 everywhere
     p = kpt.generic_default_params()
     p.perpendicular_model = model
@@ -30,9 +38,9 @@ def _case(model, n=4096, seed=3):
     return xe, ye, planes, p, parts
 
 
-@pytest.mark.parametrize("model", [0, 1])
-def test_generic_basis_and_steps_match_oracle(model):
-    xe, ye, planes, p, parts = _case(model)
+@pytest.mark.parametrize("model,stretched", [(0, False), (1, False), (0, True), (1, True)])
+def test_generic_basis_and_steps_match_oracle(model, stretched):
+    xe, ye, planes, p, parts = _case(model, stretched=stretched)
     n = len(parts["x"])
     of = O.GenericField(xe, ye, planes, p)
     want_c = of.coefficients()

@@ -238,3 +238,45 @@ def test_zero_momentum_fails(oracle_mod):
     parts["id"][:] = [0, 1]
     dt, nf = f.step(parts, 0, seed=3)
     assert nf == 1 and parts["status"].tolist() == [2, 0] and dt > 0
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_stretched_grid_sampling_and_local_dt(oracle_mod, model):
+    """Both axes stretched (spacing ratios 6 and 10), B = b0 x-hat and V_x linear
+    in the cell-centre y coordinate: between the first and last cell centres the
+    sampled V_x is that line (linear interpolation between the face midpoints), so
+    u_x = v mu + V_x(y) exactly, and the local time step takes the face-to-face
+    widths of the particle's own cell."""
+    import workloads as W
+    xe, ye = W.stretched_edges(8, X0, X1, ratio=6.0), W.stretched_edges(NY, Y0, Y1, ratio=10.0)
+    nx, ny = len(xe) - 1, len(ye) - 1
+    yc = 0.5 * (ye[:-1] + ye[1:])
+    one = np.ones((ny, nx))
+    b0 = 1.3
+    vx = (0.05 + 0.02 * yc)[:, None] * one
+    p = _params(perpendicular_model=model, perpendicular_parallel_ratio=0.2 if model else 0.0, sigma2=0.2,
+                lc=0.03)
+    f = O.GenericFocusField(xe, ye, [b0 * one, 0 * one, 0 * one, vx, 0 * one, 0 * one], p)
+    rng = np.random.default_rng(9)
+    widths = set()
+    for x, y, pm, mu in zip(rng.uniform(X0, X1, 30), rng.uniform(yc[0], yc[-1], 30), np.exp(rng.uniform(-2, 1, 30)),
+                            rng.uniform(-0.95, 0.95, 30)):
+        t = f.terms(x, y, pm, mu)
+        g = np.sqrt(1 + pm * pm)
+        v = pm / g
+        ux = v * mu + 0.05 + 0.02 * y
+        assert t["u_x"] == pytest.approx(ux, rel=1e-13, abs=1e-16)
+        i = np.searchsorted(xe, x, side="right") - 1
+        j = np.searchsorted(ye, y, side="right") - 1
+        hx, hy = xe[i + 1] - xe[i], ye[j + 1] - ye[j]
+        D, dD = O.qlt_scattering(mu, b0 / g, v, 0.2, 0.03)
+        cands = [(0.5 * hy) ** 2 / (2 * t["kperp"]), hx / abs(ux)]
+        if dD != 0:
+            cands.append(10 * 0.1 / abs(dD))
+        if D > 0:
+            cands.append(10 * 0.1 * 0.1 / (2 * D))
+        if D > 0 and dD != 0:
+            cands.append(10 * 2 * D / dD ** 2)
+        assert t["dt_local"] == pytest.approx(min(cands), rel=1e-13), (x, y)
+        widths.add(round(hy, 12))
+    assert len(widths) > 6

@@ -9,6 +9,7 @@ import pytest
 
 import kpt
 import kpt_oracle as O
+import workloads as W
 from helpers import uniform_edges
 
 NX, NY = 32, 16
@@ -219,3 +220,86 @@ def test_step_momentum_and_philox_normals(oracle_mod):
     g = dy / np.sqrt(2 * kperp * dt)
     assert abs(np.mean(g)) < 4 / np.sqrt(n)
     assert abs(np.var(g) - 1.0) < 0.05
+
+
+def _stretched_edges():
+    """Faces refined toward the middle of each axis (spacing ratio 6 in x, 10 in y)."""
+    return W.stretched_edges(8, X0, X1, ratio=6.0), W.stretched_edges(NY, Y0, Y1, ratio=10.0)
+
+
+def test_stretched_time_step_uses_the_particles_face_width(oracle_mod):
+    """Stretched grid, uniform B = x-hat and V = v0 x-hat: the terms do not depend
+    on position, so dt = 0.3 min((hx/2)^2 / (2 kappa_par), (hy/2)^2 / (2 kappa_perp),
+    hx / v0) with (hx, hy) the face-to-face widths of the particle's own cell
+    (StoredCoordinateGrid::physical_cell_width), particle by particle."""
+    xe, ye = _stretched_edges()
+    nx, ny = len(xe) - 1, len(ye) - 1
+    one = np.ones((ny, nx))
+    p = O.generic_default_params()
+    p.perpendicular_model, p.perpendicular_parallel_ratio = 1, 0.05
+    f = O.GenericField(xe, ye, [one, 0 * one, 0 * one, 0.3 * one, 0 * one, 0 * one], p)
+    kpar = f.coefficients()[2, 2, 15] * np.sqrt(1 + 2.0 ** 2) ** (1 / 3)
+    rng = np.random.default_rng(5)
+    widths, binding = set(), 0
+    for x, y in zip(rng.uniform(X0, X1, 40), rng.uniform(Y0, Y1, 40)):
+        parts = kpt.empty_particles(1)
+        parts["x"][:], parts["y"][:], parts["p"][:] = x, y, 2.0
+        i = np.searchsorted(xe, x, side="right") - 1
+        j = np.searchsorted(ye, y, side="right") - 1
+        hx, hy = xe[i + 1] - xe[i], ye[j + 1] - ye[j]
+        cands = [(0.5 * hx) ** 2 / (2 * kpar), (0.5 * hy) ** 2 / (2 * 0.05 * kpar), hx / 0.3]
+        assert f.time_step(parts) == pytest.approx(0.3 * min(cands), rel=1e-13), (x, y)
+        widths.add(round(hy, 12))
+        binding += int(np.argmin(cands) == 1)
+    assert len(widths) >= 6 and binding > 10  # the y width binds in cells of many widths (pairs by symmetry)
+
+
+def test_stretched_derivatives_at_cell_centres(oracle_mod):
+    """Cell-centred values on a stretched grid sit at the face midpoints: for V_y
+    linear in the cell-centre coordinate, div V (three-point formula at unequal
+    spacing, exact for linear data) equals the slope in every interior row."""
+    xe, ye = _stretched_edges()
+    nx, ny = len(xe) - 1, len(ye) - 1
+    yc = 0.5 * (ye[:-1] + ye[1:])
+    one = np.ones((ny, nx))
+    vy = (0.2 + 0.7 * yc)[:, None] * one
+    p = O.generic_default_params()
+    p.minimum_solar_wind_speed = 1e-3
+    f = O.GenericField(xe, ye, [one, 0 * one, 0.5 * one, 0.3 * one, vy, 0 * one], p)
+    div = f.coefficients()[1:-1, 1:-1, 17]
+    np.testing.assert_allclose(div[1:-1], 0.7, rtol=1e-12)
+    assert np.abs(div[0] - 0.7).max() > 0.1  # the periodic wrap breaks linearity at the ends
+
+
+def test_nearly_uniform_faces_take_the_stretched_path_to_the_same_result(oracle_mod):
+    """Faces moved by 1e-9 (far beyond the uniform-spacing test's 1024 ulp) switch
+    the oracle to its stored-face path; the basis, the time step and three
+    injected-noise steps then agree with the uniform grid's to 1e-6."""
+    _, _, X, Y = _grid()
+    xe, ye = uniform_edges(NX, X0, X1), uniform_edges(NY, Y0, Y1)
+    xs, ys = xe.copy(), ye.copy()
+    xs[1:-1] += 1e-9 * np.sin(np.arange(1, NX))
+    ys[1:-1] += 1e-9 * np.cos(np.arange(1, NY))
+    planes = [1.0 + 0.2 * np.sin(K * X), 0.3 * np.cos(L * Y), 0 * X + 0.4, 0.3 + 0.1 * np.sin(K * X + L * Y),
+              0.05 * np.cos(K * X), 0 * X]
+    p = O.generic_default_params()
+    p.minimum_solar_wind_speed = 1e-3
+    a, b = O.GenericField(xe, ye, planes, p), O.GenericField(xs, ys, planes, p)
+    ca, cb = a.coefficients(), b.coefficients()
+    scale = np.abs(ca).max(axis=(0, 1)) + 1e-300
+    assert (np.abs(ca - cb) / scale).max() < 1e-6
+    rng = np.random.default_rng(2)
+    pa = kpt.empty_particles(500)
+    pa["x"][:], pa["y"][:] = rng.uniform(X0, X1, 500), rng.uniform(Y0, Y1, 500)
+    pa["p"][:], pa["id"][:] = np.exp(rng.uniform(-1, 2, 500)), np.arange(500)
+    pb = {k: v.copy() for k, v in pa.items()}
+    assert b.time_step(pb) == pytest.approx(a.time_step(pa), rel=1e-6)
+    noise = rng.standard_normal((500, 3, 4))
+    for step in range(3):
+        da, _ = a.step(pa, step, noise=noise)
+        db, _ = b.step(pb, step, noise=noise)
+        assert db == pytest.approx(da, rel=1e-6)
+    for k, w in (("x", X1 - X0), ("y", Y1 - Y0)):
+        d = np.abs(pa[k] - pb[k])
+        assert (np.minimum(d, w - d) / w).max() < 1e-6, k
+    np.testing.assert_allclose(pb["p"], pa["p"], rtol=1e-6)
