@@ -358,6 +358,30 @@ kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inje
 kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double t0,
                              uint32_t frame, kpt_push_stats* stats);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). kpt_push_interval with its per-chunk launch arguments and
+ * counters in a file-scope device array of the library, KPT_NOALLOC_SCRATCH_BYTES
+ * on each GPU the library runs on, held until the process ends; the call makes
+ * no device allocation (the particles, frames and noise table are the ones
+ * already held by the context). Differences from kpt_push_interval:
+ *  - at most KPT_NOALLOC_MAX_CHUNKS push chunks (KPT_ERR_ARG beyond);
+ *  - no cell re-sort between chunks: the particle order is kept, and every
+ *    particle ends where kpt_push_interval moves it;
+ *  - an FP32 time-blended push reads the two frames directly instead of the
+ *    interleaved copy kpt_push_interval builds, so its blend weights round in
+ *    another order (FP32-level differences; FP64, and FP32 without blending,
+ *    are bit-identical).
+ * The call returns when the push has finished; `stats` may be NULL (its
+ * kernel_ms also covers one small staging kernel per chunk). Every call
+ * on one GPU shares that GPU's array and overwrites what the last call left
+ * there: calls that overlap on one GPU (other contexts, other host threads)
+ * race. */
+#define KPT_NOALLOC_MAX_CHUNKS 16
+#define KPT_NOALLOC_SCRATCH_BYTES 24576
+kpt_status kpt_push_interval_noalloc(kpt_ctx* ctx, const kpt_device_settings* s, double t0,
+                                     uint32_t frame, kpt_push_stats* stats);
+
 /* split_particles (src/main.cpp:1760-1839). Children are appended in parent slot
  * order (one admissible schedule of the reference's atomic counter). */
 kpt_status kpt_split(kpt_ctx* ctx, const kpt_device_settings* s, uint64_t* split,

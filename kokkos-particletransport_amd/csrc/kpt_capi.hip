@@ -513,6 +513,9 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     if ((e = kpt::alloc_arrays(P, capacity)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->counters, 8)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->d_args, 1)) != hipSuccess) return bail(e, "hipMalloc");
+    // the no-allocation push's scratch address on this device; a failure here is
+    // reported by that entry point, not by every context
+    (void)noalloc_setup(device);
     ctx->split_scratch_cap = kpt::split_scratch_elems(capacity);
     if ((e = dev_alloc(&ctx->split_scratch, ctx->split_scratch_cap)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = dev_alloc(&ctx->hist_partials, kpt::hist_partial_elems(4096))) != hipSuccess) return bail(e, "hipMalloc");
@@ -920,8 +923,12 @@ kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inje
     return KPT_OK;
 }
 
-kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame,
-                             kpt_push_stats* stats) {
+// The push's launch arguments for the interval [t0, t0 + mhd_dt) of `frame`
+// (per-chunk fields t_stop, first_chunk, counters and self are the caller's).
+// `paired`: an FP32 time-blended push reads the frames from the interleaved
+// pair buffer, built (and grown) here; otherwise the kernel reads both frames.
+extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame, bool paired,
+                               kpt::PushArgs& a, bool& focused) {
     if (!ctx) return KPT_ERR_ARG;
     kpt_status st = check_settings(ctx, s);
     if (st != KPT_OK) return st;
@@ -940,7 +947,7 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
         if (static_cast<uint64_t>(s->max_particle_steps_per_interval) > ctx->noise_steps)
             return fail(ctx, KPT_ERR_STATE, "max_particle_steps_per_interval exceeds the noise table's steps.");
     }
-    kpt::PushArgs a{};
+    a = kpt::PushArgs{};
     a.count = ctx->count;
     const bool fp32 = s->precision == KPT_PREC_FP32;
     a.field_lo = fp32 ? static_cast<const void*>(lo.c32) : static_cast<const void*>(lo.c64);
@@ -949,7 +956,7 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.field_lo64 = lo.c64;
     a.field_hi64 = s->time_interpolation != 0 ? hi.c64 : lo.c64;
     a.field_pair = nullptr;
-    if (fp32 && s->time_interpolation != 0) {
+    if (paired && fp32 && s->time_interpolation != 0) {
         const size_t elems = kpt::coefficient_plane(lo.g.nx, lo.g.ny) * static_cast<size_t>(lo.model);
         if (2 * elems > ctx->pair32_cap) {
             dev_free(ctx->pair32);
@@ -971,12 +978,13 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
     a.s = *s;
     a.t0 = t0;
     a.t_end = t0 + s->mhd_dt;
+    a.t_stop = a.t_end;
+    a.first_chunk = 1;
     a.key[0] = static_cast<uint32_t>(ctx->seed);
     a.key[1] = static_cast<uint32_t>(ctx->seed >> 32);
     a.frame = frame;
     a.noise = ctx->noise;
     a.noise_steps = ctx->noise_steps;
-    a.counters = ctx->counters;
     a.qlt_norm = s->qlt_spectral_index * std::sin(M_PI / s->qlt_spectral_index) / M_PI;
     a.k64 = kpt::make_step_const<double>(*s, a.qlt_norm);
     a.k32 = kpt::make_step_const<float>(*s, a.qlt_norm);
@@ -993,8 +1001,20 @@ kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double 
         a.mlp_on = (s->qlt_spectral_index >= h.gamma_lo && s->qlt_spectral_index <= h.gamma_hi &&
                     s->qlt_h0 == h.h0) ? 1 : 0;
     }
+    focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
+    return KPT_OK;
+}
+
+kpt_status kpt_push_interval(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame,
+                             kpt_push_stats* stats) {
+    kpt::PushArgs a;
+    bool focused = false;
+    kpt_status st = push_args(ctx, s, t0, frame, true, a, focused);
+    if (st != KPT_OK) return st;
+    const FieldSlot& lo = ctx->slot[KPT_SLOT_LOWER];
+    const bool fp32 = s->precision == KPT_PREC_FP32;
+    a.counters = ctx->counters;
     (void)hipSetDevice(ctx->device);
-    const bool focused = lo.model == KPT_MODEL_FOCUSED21 && s->focused_transport != 0;
     const int chunks = s->push_chunks > 1 ? s->push_chunks : 1;
     const bool resort = chunks > 1 && ctx->noise == nullptr;
     unsigned long long total[3] = {0, 0, 0};

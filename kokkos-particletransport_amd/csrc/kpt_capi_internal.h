@@ -174,6 +174,12 @@ kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr);
 kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
                        const double* ye, const double* const planes[6]);
 bool faces_uniform(const double* edges, size_t count);
+kpt_status push_args(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame, bool paired,
+                     kpt::PushArgs& a, bool& focused);
+
+// Defined in kpt_noalloc.hip: looks up, once per device and with that device
+// current, the address of the no-allocation push's device scratch array.
+hipError_t noalloc_setup(int device);
 
 }  // namespace kpt_capi
 

@@ -26,6 +26,8 @@ PREC_FP32 = 1
 CLOSURE_REDUCED = 0
 CLOSURE_QLT = 1
 CLOSURE_MLP = 2
+NOALLOC_MAX_CHUNKS = 16          # KPT_NOALLOC_MAX_CHUNKS
+NOALLOC_SCRATCH_BYTES = 24576    # KPT_NOALLOC_SCRATCH_BYTES
 
 PARTICLE_FIELDS = (
     ("x", np.float64), ("y", np.float64), ("p", np.float64), ("t", np.float64),
@@ -341,6 +343,8 @@ EXPORTS = {
                              C.c_uint64, C.c_double, C.c_uint32, C.POINTER(C.c_uint64)]),
     "kpt_push_interval": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.c_double,
                                     C.c_uint32, C.POINTER(PushStats)]),
+    "kpt_push_interval_noalloc": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.c_double,
+                                            C.c_uint32, C.POINTER(PushStats)]),
     "kpt_split": (C.c_int, [C.c_void_p, C.POINTER(DeviceSettings), C.POINTER(C.c_uint64),
                             C.POINTER(C.c_uint64)]),
     "kpt_histogram_logp": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double,
@@ -727,6 +731,14 @@ class Context:
     def push_interval(self, s: DeviceSettings, t0, frame=0) -> PushStats:
         st = PushStats()
         self._check(self.lib.kpt_push_interval(self.h, C.byref(s), t0, frame, C.byref(st)))
+        return st
+
+    def push_interval_noalloc(self, s: DeviceSettings, t0, frame=0, stats=True) -> PushStats | None:
+        """kpt_push_interval_noalloc: the interval push with no device allocation in
+        the call (stats=False passes no stats record and returns None)."""
+        st = PushStats() if stats else None
+        self._check(self.lib.kpt_push_interval_noalloc(self.h, C.byref(s), t0, frame,
+                                                       C.byref(st) if stats else None))
         return st
 
     def split(self, s: DeviceSettings):

@@ -54,3 +54,17 @@ def test_create_without_gpu_fails_cleanly(kpt_mod):
         return
     ctx.close()
     pytest.skip("a GPU is present; covered by the gpu tests")
+
+
+def test_noalloc_limits_match_the_header():
+    """The no-allocation push's published limits: the binding's constants are the
+    header's, and the scratch array fits the chunk records of the launch
+    arguments the kernels read."""
+    text = open(os.path.join(REPO, "include", "kpt.h")).read()
+    consts = dict(re.findall(r"^#define (KPT_NOALLOC_\w+) (\d+)$", text, re.M))
+    assert int(consts["KPT_NOALLOC_MAX_CHUNKS"]) == kpt.NOALLOC_MAX_CHUNKS
+    assert int(consts["KPT_NOALLOC_SCRATCH_BYTES"]) == kpt.NOALLOC_SCRATCH_BYTES
+    src = open(os.path.join(REPO, "kokkos-particletransport_amd", "csrc", "kpt_noalloc.hip")).read()
+    assert re.search(r"^static __device__ __align__\(256\) unsigned char scratch\[KPT_NOALLOC_SCRATCH_BYTES\];",
+                     src, re.M)
+    assert "hipMalloc" not in re.sub(r"//.*", "", src) and "hipFree" not in re.sub(r"//.*", "", src)
