@@ -608,10 +608,28 @@ def run_push(args):
     per_step = step_bytes(model, fp32, stretched=stretched)
     launch_bytes = steps / main_leg["launches"] * per_step + main_leg["active_in"] * state_bytes(model)
     grid_tag = "_stretched" if stretched else ""
-    act_tag = ("_" + args.mlp_activation.replace("-", "")) if learned else ""
-    workload = (f"focused_qlt_{args.closure}{act_tag}_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}" if focused
-                else f"parker_{prec}_{args.nx}x{args.ny}{grid_tag}_island_{n}")
+
+    def name(closure, act, p):
+        act_tag = ("_" + act.replace("-", "")) if closure == "mlp" else ""
+        return (f"focused_qlt_{closure}{act_tag}_{p}_{args.nx}x{args.ny}{grid_tag}_island_{n}" if focused
+                else f"parker_{p}_{args.nx}x{args.ny}{grid_tag}_island_{n}")
+
+    workload = name(args.closure, args.mlp_activation, prec)
     kernel = "push_kernel_mlp" if learned else "push_kernel"
+    # each comparison leg's roofline under its own workload name
+    for key, leg in legs.items():
+        if leg is main_leg:
+            continue
+        if key.startswith("mlp_"):
+            lw, lk, lp = name("mlp", key[4:], "fp32"), "push_kernel_mlp", True
+        elif key == "fp32_exact_qlt":
+            lw, lk, lp = name("qlt", "", "fp32"), "push_kernel", True
+        else:
+            lw, lk, lp = name("qlt", "", "fp64"), "push_kernel", False
+        lb = leg["sub_steps"] / leg["launches"] * step_bytes(model, lp, stretched=stretched) + \
+            leg["active_in"] * state_bytes(model)
+        leg["workload"] = lw
+        leg["roofline"] = roofline_block(lw, lk, leg["kernel_ms_avg"] / 1e3, leg["launches"], lb)
     cfg = {
         "workload": workload,
         "field": (f"{args.nx}x{args.ny}, " + (f"y faces tanh-stretched toward y = 0 (spacing ratio {args.stretch_ratio:g})"
