@@ -957,17 +957,18 @@ extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_setti
     a.field_hi64 = s->time_interpolation != 0 ? hi.c64 : lo.c64;
     a.field_pair = nullptr;
     if (paired && fp32 && s->time_interpolation != 0) {
-        const size_t elems = kpt::coefficient_plane(lo.g.nx, lo.g.ny) * static_cast<size_t>(lo.model);
-        if (2 * elems > ctx->pair32_cap) {
+        const uint64_t plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
+        const size_t elems = kpt::pair_elems(plane, lo.model);
+        if (elems > ctx->pair32_cap) {
             dev_free(ctx->pair32);
             ctx->pair32 = nullptr;
             ctx->pair32_cap = 0;
             ctx->pair_versions[0] = ctx->pair_versions[1] = 0;
-            KPT_HIP(ctx, dev_alloc(&ctx->pair32, 2 * elems));
-            ctx->pair32_cap = 2 * elems;
+            KPT_HIP(ctx, dev_alloc(&ctx->pair32, elems));
+            ctx->pair32_cap = elems;
         }
         if (ctx->pair_versions[0] != lo.version || ctx->pair_versions[1] != hi.version) {
-            KPT_HIP(ctx, kpt::launch_pair(lo.c32, hi.c32, elems, ctx->pair32, ctx->stream));
+            KPT_HIP(ctx, kpt::launch_pair(lo.c32, hi.c32, plane, lo.model, ctx->pair32, ctx->stream));
             ctx->pair_versions[0] = lo.version;
             ctx->pair_versions[1] = hi.version;
         }

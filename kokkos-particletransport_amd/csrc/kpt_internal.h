@@ -21,6 +21,10 @@ inline uint64_t coefficient_plane(int nx, int ny) {
     return static_cast<uint64_t>(nx + 2) * static_cast<uint64_t>(ny + 2);
 }
 
+// Floats per node record of the FP32 frame pair: both frames' components,
+// rounded up to whole 16-byte loads.
+__host__ __device__ constexpr int pair_stride(int model) { return (2 * model + 3) / 4 * 4; }
+
 // One bin of a stretched axis' lookup table, read with one 32-byte load: the
 // face interval k that contains the bin's lower end (less a guard band) and the
 // faces e_k, e_{k+1}, e_{k+2} (the last two repeat e_n past the end). The host
@@ -246,8 +250,11 @@ hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const do
                                const kpt_generic_parker_params& p, double dt, uint32_t step, const uint32_t key[2],
                                const double* noise, uint32_t noise_steps, int deterministic, unsigned long long* failures,
                                hipStream_t stream);
-// out[2 e + 0] = lo[e], out[2 e + 1] = hi[e] for e < n
-hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream);
+// The FP32 frame pair of a time-blended push: floats of the buffer for `plane`
+// nodes of `model` components, and its build from the two component-planar
+// frames (layout in kpt_kernels.hip, sample_pair32).
+uint64_t pair_elems(uint64_t plane, int model);
+hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream);
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);

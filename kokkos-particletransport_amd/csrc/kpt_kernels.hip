@@ -456,6 +456,19 @@ __device__ __forceinline__ void sample_blend(const T* __restrict__ flo, const T*
     }
 }
 
+// Records of the node-major pair whose loads are issued together (0: all): the
+// exact-closure and Parker pushes, and the learned-closure push. Measured on the
+// 1024 x 512 stretched island field, 8 M focused particles, per launch: FP32
+// exact 68.6 ms unphased at 4 waves, 44.7 ms at one record per phase (4 waves),
+// 42.3 ms at 3 waves; learned closure (SiLU / hard-swish) 75.1 / 57.9 ms
+// unphased, 74.6 / 57.3 ms at 3 records, 74.4 / 56.1 ms at 2.
+#ifndef KPT_PAIR_RECORDS_EXACT
+#define KPT_PAIR_RECORDS_EXACT 1
+#endif
+#ifndef KPT_PAIR_RECORDS_MLP
+#define KPT_PAIR_RECORDS_MLP 2
+#endif
+#if KPT_PAIR_PLANAR
 // FP32 time-blended sample from the frame-interleaved pair buffer
 // pair[2 (k plane + node) + f], f = 0 lower, 1 upper: one row of the stencil,
 // (lo_i, hi_i, lo_i+1, hi_i+1), is one 16-byte load; the bilinear and time
@@ -476,6 +489,53 @@ __device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, in
                b11 * r[3];
     }
 }
+#else
+// FP32 time-blended sample from the node-major pair buffer: node n's record of
+// pair_stride(C) floats starts at pair + n pair_stride(C) (16-byte aligned) and
+// holds (lower, upper) of component k at slots 2k, 2k + 1. A stencil row is two
+// adjacent records, read as 16-byte loads at immediate offsets from one address;
+// the bilinear and time weights are folded into eight products per component.
+template <int C, int RPP = 0>
+__device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, int64_t n00, int nxp,
+                                              uint64_t plane, double uw, double wx, double wy, float* c) {
+    (void)plane;
+    constexpr int V = pair_stride(C) / 4;  // float4 per record
+    const double lw = 1.0 - uw;
+    const float a00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * lw), b00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * uw);
+    const float a10 = static_cast<float>(wx * (1.0 - wy) * lw), b10 = static_cast<float>(wx * (1.0 - wy) * uw);
+    const float a01 = static_cast<float>((1.0 - wx) * wy * lw), b01 = static_cast<float>((1.0 - wx) * wy * uw);
+    const float a11 = static_cast<float>(wx * wy * lw), b11 = static_cast<float>(wx * wy * uw);
+#if KPT_PAIR_NARROW
+    // 8-byte loads: (lower, upper) of one component at one node
+    const float2* __restrict__ q2 = reinterpret_cast<const float2*>(pair) + n00 * (2 * V);
+    const float2* __restrict__ r2 = q2 + static_cast<int64_t>(nxp) * (2 * V);
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const float2 u0 = q2[k], u1 = q2[2 * V + k], w0 = r2[k], w1 = r2[2 * V + k];
+        c[k] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y + a11 * w1.x + b11 * w1.y;
+    }
+    return;
+#endif
+    const float4* __restrict__ q = reinterpret_cast<const float4*>(pair) + n00 * V;
+    const float4* __restrict__ r = q + static_cast<int64_t>(nxp) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        // RPP > 0: the loads of RPP records at a time. Without the barrier the
+        // scheduler clusters all 4 V loads of the stencil (same base, immediate
+        // offsets) ahead of their products, which spilled to scratch in the
+        // 3- and 4-wave FP32 exact push.
+        if (RPP > 0 && v > 0 && v % RPP == 0) __builtin_amdgcn_sched_barrier(0);
+        const float4 u0 = q[v], u1 = q[V + v], w0 = r[v], w1 = r[V + v];
+        const int k = 2 * v;
+        if (k < C)
+            c[k] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y + a11 * w1.x +
+                   b11 * w1.y;
+        if (k + 1 < C)
+            c[k + 1] = a00 * u0.z + b00 * u0.w + a10 * u1.z + b10 * u1.w + a01 * w0.z + b01 * w0.w + a11 * w1.z +
+                       b11 * w1.w;
+    }
+}
+#endif
 
 // The same sample from the FP64 frames for the rare near-null lanes of the FP32
 // push: four components at a time, so this branch does not set the register
@@ -506,8 +566,10 @@ __device__ __forceinline__ void sample_blend_fp64_grouped(const double* __restri
 // float. Not inlined: its 21-double sample would otherwise set the register
 // budget of the whole FP32 sub-step loop, which spilled to scratch on every
 // sub-step for a branch a few lanes take; a call keeps those registers and the
-// spills around it on the rare path.
-template <int MODEL, bool FOCUSED, bool QLT>
+// spills around it on the rare path. CALLER: one instance per calling kernel, so
+// that each is compiled for its caller's register budget (a shared instance
+// takes the smallest, which cost the other caller 10% per launch).
+template <int MODEL, bool FOCUSED, bool QLT, int CALLER = 0>
 __device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, bool blend, double uw, int64_t n00,
                                                          int nxp, double wx, double wy, double p, double mu,
                                                          StepTerms<float>* out) {
@@ -535,6 +597,8 @@ __device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, boo
 //   FP64 focused: 3 waves +63%, 4 waves +106%;   FP64 Parker: 3 waves -1%, 4 +47%;
 //   FP32 focused: 3 waves -20%, 4 waves -29%, 5 waves +1%;
 //   FP32 Parker: 3 waves -20%, 4 waves -16%.
+// With the node-major FP32 pair (one record per load phase), FP32 focused: 3
+// waves 42.3 ms, 4 waves 44.7 ms per launch.
 #ifndef KPT_PUSH_WAVES_FP64
 #define KPT_PUSH_WAVES_FP64 2
 #endif
@@ -542,7 +606,7 @@ __device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, boo
 #define KPT_PUSH_WAVES_FP32_PARKER 3
 #endif
 #ifndef KPT_PUSH_WAVES_FP32_FOCUSED
-#define KPT_PUSH_WAVES_FP32_FOCUSED 4
+#define KPT_PUSH_WAVES_FP32_FOCUSED 3
 #endif
 // One lane of the push: particle `idx` of the set in `a` (lanes past a.count
 // or with status != 0 only join the wave reduction of the counters). Every lane
@@ -618,7 +682,7 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
                 FT c[CP];
                 if constexpr (FP32) {
                     if (a.field_pair != nullptr)
-                        sample_pair32<CP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                        sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
                     else
                         sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
                 } else {
@@ -924,14 +988,14 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
                 }
                 float c[CP];
                 if (a.field_pair != nullptr)
-                    sample_pair32<CP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                    sample_pair32<CP, KPT_PAIR_RECORDS_MLP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
                 else
                     sample_blend<CP, float>(static_cast<const float*>(a.field_lo), static_cast<const float*>(a.field_hi),
                                             blend, uw, n00, nxp, a.plane, wx, wy, c);
                 bool ok;
                 if (!(c[6] >= K.null_b)) {
                     StepTerms<float> Tn;
-                    ok = near_null_terms<21, true, true>(a.self, blend, uw, n00, nxp, wx, wy, p, pitch_mu, &Tn);
+                    ok = near_null_terms<21, true, true, 1>(a.self, blend, uw, n00, nxp, wx, wy, p, pitch_mu, &Tn);
                     T = Tn;
                 } else {
                     ok = eval_terms<21, true, true, float>(c, p, pitch_mu, s, K, T);
@@ -1127,6 +1191,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
     push_lane_mlp<STR, ACT, PASSES>(a, im, b40, b41, push_first_particle(run));
 }
+
+#if !KPT_NO_BUDGET_KERNEL
+// Never launched. It calls the learned-closure push's near-null instance from a
+// 4-wave kernel, so the compiler builds that instance within 128 VGPRs: the
+// registers a call clobbers are the ones its caller spills around it, and a
+// callee built for the learned-closure kernel's own 168 cost that kernel 11% per
+// launch in spills.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+    near_null_budget_kernel(const PushArgs* a, StepTerms<float>* out) {
+    (void)near_null_terms<21, true, true, 1>(a, true, 0.5, 0, a->g.nx + 2, 0.5, 0.5, 1.0, 0.5, out);
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // Batched ensemble: one persistent launch advances many independent runs.
@@ -1539,16 +1615,37 @@ hipError_t launch_pack(const double* raw, const GridDesc& g, int model, double* 
 }
 
 // Frame-interleaved FP32 pair of the two coefficient frames of an interval.
+#if KPT_PAIR_PLANAR
 __global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ lo, const float* __restrict__ hi,
-                                                      uint64_t n, float2* __restrict__ out) {
+                                                      uint64_t plane, int model, float* __restrict__ out) {
     const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    if (e < n) out[e] = make_float2(lo[e], hi[e]);
+    if (e < plane * model) reinterpret_cast<float2*>(out)[e] = make_float2(lo[e], hi[e]);
 }
 
-hipError_t launch_pair(const float* lo, const float* hi, uint64_t n, float* out, hipStream_t stream) {
+uint64_t pair_elems(uint64_t plane, int model) { return 2 * plane * static_cast<uint64_t>(model); }
+#else
+// Node-major record: slot j < 2 model of node n holds component j / 2 of the
+// lower (j even) or upper (j odd) frame; the slots up to the record stride are 0.
+__global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ lo, const float* __restrict__ hi,
+                                                      uint64_t plane, int model, float* __restrict__ out) {
+    const int stride = pair_stride(model);
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    if (e >= plane * stride) return;
+    const uint64_t n = e / stride;
+    const int j = static_cast<int>(e - n * stride);
+    float v = 0.0f;
+    if (j < 2 * model) v = ((j & 1) ? hi : lo)[static_cast<uint64_t>(j >> 1) * plane + n];
+    out[e] = v;
+}
+
+uint64_t pair_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(pair_stride(model)); }
+#endif
+
+hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream) {
+    const uint64_t n = pair_elems(plane, model);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(pair_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
-                       lo, hi, n, reinterpret_cast<float2*>(out));
+                       lo, hi, plane, model, out);
     return hipGetLastError();
 }
 
