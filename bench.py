@@ -574,6 +574,8 @@ def run_push(args):
             o = kpt.mlp_default_train_opts()
             o.steps = args.mlp_steps
             o.hull.activation = kpt.MLP_ACTIVATIONS[act]
+            if args.mlp_mu_lo > 0:
+                o.hull.mu_lo = args.mlp_mu_lo
             t_train = time.perf_counter()
             reports[act] = ctx.mlp_train(o)
             reports[act]["wall_s"] = time.perf_counter() - t_train
@@ -845,6 +847,8 @@ def main():
                     help="default: fp32 for focused, fp64 for parker")
     ap.add_argument("--closure", choices=["qlt", "mlp"], default="mlp")
     ap.add_argument("--mlp-steps", type=int, default=8000, help="Adam steps of the network training")
+    ap.add_argument("--mlp-mu-lo", type=float, default=0.0,
+                    help="lower |mu| of the network's training hull (0: the library default)")
     ap.add_argument("--mlp-activation", choices=sorted(kpt.MLP_ACTIVATIONS), default="silu",
                     help="hidden-layer activation of the measured learned closure (the other one is "
                          "pushed as a further leg)")

@@ -826,7 +826,7 @@ enum {
 /* The trained network's description: its input hull, output scaling and
  * activation. */
 typedef struct kpt_mlp_hull {
-    double mu_lo;                 /* |mu| >= mu_lo (and < 1)            default 0.02  */
+    double mu_lo;                 /* |mu| >= mu_lo (and < 1)            default 1e-3  */
     double log_xi_lo, log_xi_hi;  /* ln xi range                        ln 1e-7, 0    */
     double gamma_lo, gamma_hi;    /* gamma_k range                      1.5, 2        */
     double h0;                    /* resonance regularisation trained   1e-6          */

@@ -1705,7 +1705,7 @@ kpt_status kpt_mlp_default_train_opts(kpt_mlp_train_opts* o) {
     o->batch = 4096;
     o->lr = 3.0e-3;
     o->seed = 0x4D4C50ull;
-    o->hull.mu_lo = 0.02;
+    o->hull.mu_lo = 1.0e-3;
     o->hull.log_xi_lo = std::log(1.0e-7);
     o->hull.log_xi_hi = 0.0;
     o->hull.gamma_lo = 1.5;
