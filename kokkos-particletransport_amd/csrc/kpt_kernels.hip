@@ -581,6 +581,27 @@ __device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, boo
     return ok;
 }
 
+// Push arguments in the constant address space: the kernel's own argument block
+// (kernel_args) or a run of the batched ensemble (as_args), which no kernel
+// writes while a push reads it. Fields read through such a pointer load with
+// scalar loads. args_reload hides the pointer's value from the compiler, so
+// the fields a sub-step uses are loaded inside that sub-step rather than kept
+// live across the loop: the push uses more uniform values than a wave has
+// SGPRs, and values kept live were spilled to VGPR lanes and read back with
+// v_readlane (VALU instructions) on every sub-step.
+typedef const __attribute__((address_space(4))) PushArgs* ArgsK;
+
+// The push arguments are the first explicit kernel argument (offset 0 of the
+// kernel argument segment).
+__device__ __forceinline__ ArgsK kernel_args() {
+    return (ArgsK)(__builtin_amdgcn_kernarg_segment_ptr());
+}
+__device__ __forceinline__ ArgsK as_args(const PushArgs& a) { return (ArgsK)(&a); }
+__device__ __forceinline__ const PushArgs& args_reload(ArgsK p) {
+    asm volatile("" : "+s"(p));
+    return *(const PushArgs*)(p);
+}
+
 // ---------------------------------------------------------------------------
 // The push: one lane advances one particle through the whole interval
 // (move_parker_particles_one_interval, src/main.cpp:1113-1325, and the shared
@@ -614,7 +635,8 @@ __device__ __attribute__((noinline)) bool near_null_terms(const PushArgs* a, boo
 // STR: the frame has a stretched axis (node_stencil); the uniform kernels keep
 // the plain stencil.
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR = false>
-__device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) {
+__device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
+    const PushArgs& a = *(const PushArgs*)(ap);
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr bool FP32 = sizeof(FT) == 4;
     constexpr int CP = MODEL;  // components read per corner
@@ -646,6 +668,9 @@ __device__ __forceinline__ void push_lane(const PushArgs& a, const int64_t idx) 
         const double t_stop = a.t_stop;
 
         while (t < t_stop && step_guard < s.max_particle_steps_per_interval && status == 0) {
+            // The uniform arguments of this sub-step, read again from constant memory.
+            const PushArgs& a = args_reload(ap);
+            const kpt_device_settings& s = a.s;
             ++step_guard;
             x = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
             y = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
@@ -858,7 +883,8 @@ __global__ void __launch_bounds__(kBlock)
                                                       : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED
                                                                  : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
-    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(a, push_first_particle(run));
+    (void)a;
+    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(kernel_args(), push_first_particle(run));
 }
 
 // Closure from the network outputs at pitch cosine mu (|mu| in the hull), with
@@ -920,11 +946,10 @@ __device__ __forceinline__ void closure_at(bool in_hull, float y0, float y1, flo
 // LDS as `im`, output biases b40, b41). Every lane of a wavefront calls it
 // together: the sub-step loop and the network evaluations are wave-uniform.
 template <bool STR = false, int ACT = kActSilu, int PASSES = 1>
-__device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& im, const float b40, const float b41,
+__device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const float b40, const float b41,
                                               const int64_t idx) {
     static_assert(PASSES == 1 || PASSES == 3, "one or three network evaluations per sub-step");
-    const MlpScale ms = a.mlp_scale;
-    const StepConst<float>& K = a.k32;
+    const PushArgs& a = *(const PushArgs*)(ap);
     const kpt_device_settings& s = a.s;
     constexpr int CP = 21;
     const bool use_net = a.mlp_on != 0;
@@ -953,6 +978,11 @@ __device__ __forceinline__ void push_lane_mlp(const PushArgs& a, const MlpLds& i
     for (;;) {
         bool run = live && status == 0 && t < t_stop && step_guard < s.max_particle_steps_per_interval;
         if (!__any(run)) break;
+        // The uniform arguments of this sub-step, read again from constant memory.
+        const PushArgs& a = args_reload(ap);
+        const kpt_device_settings& s = a.s;
+        const MlpScale& ms = a.mlp_scale;
+        const StepConst<float>& K = a.k32;
 
         // phase 1: sample, rates, features at the initial mu
         StepTerms<float> T{};
@@ -1189,7 +1219,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     float b40, b41;
     const MlpLds im = mlp_stage(a.mlp_image, mlp_smem, b40, b41);
-    push_lane_mlp<STR, ACT, PASSES>(a, im, b40, b41, push_first_particle(run));
+    push_lane_mlp<STR, ACT, PASSES>(kernel_args(), im, b40, b41, push_first_particle(run));
 }
 
 #if !KPT_NO_BUDGET_KERNEL
@@ -1243,13 +1273,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
         const int64_t idx = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(t.y)) + threadIdx.x;
         const EnsembleRun& R = runs[r];
         if (R.kind == kEnsParker32)
-            push_lane<16, false, float, KPT_CLOSURE_REDUCED>(R.a, idx);
+            push_lane<16, false, float, KPT_CLOSURE_REDUCED>(as_args(R.a), idx);
         else if (R.kind == kEnsFocusedQlt32)
-            push_lane<21, true, float, KPT_CLOSURE_QLT>(R.a, idx);
+            push_lane<21, true, float, KPT_CLOSURE_QLT>(as_args(R.a), idx);
         else if (R.a.mlp_act == kActHardSwish)
-            push_lane_mlp<false, kActHardSwish, 1>(R.a, im, b40, b41, idx);
+            push_lane_mlp<false, kActHardSwish, 1>(as_args(R.a), im, b40, b41, idx);
         else
-            push_lane_mlp<false, kActSilu, 1>(R.a, im, b40, b41, idx);
+            push_lane_mlp<false, kActSilu, 1>(as_args(R.a), im, b40, b41, idx);
     }
 }
 
