@@ -21,9 +21,15 @@ inline uint64_t coefficient_plane(int nx, int ny) {
     return static_cast<uint64_t>(nx + 2) * static_cast<uint64_t>(ny + 2);
 }
 
-// Floats per node record of the FP32 frame pair: both frames' components,
-// rounded up to whole 16-byte loads.
-__host__ __device__ constexpr int pair_stride(int model) { return (2 * model + 3) / 4 * 4; }
+// Components the FP32 frame pair stores per node: every packed component but
+// the focused model's Vz (component 2), which no rate of the FP32 push reads
+// (the focused terms take only the gradients of Vz).
+__host__ __device__ constexpr int pair_components(int model) { return model == 21 ? 20 : model; }
+// Packed component held in stored slot s of the pair.
+__host__ __device__ constexpr int pair_component(int model, int s) { return model == 21 && s >= 2 ? s + 1 : s; }
+// Floats per node record of the FP32 frame pair: both frames' stored
+// components, rounded up to whole 16-byte loads.
+__host__ __device__ constexpr int pair_stride(int model) { return (2 * pair_components(model) + 3) / 4 * 4; }
 
 // One bin of a stretched axis' lookup table, read with one 32-byte load: the
 // face interval k that contains the bin's lower end (less a guard band) and the
@@ -154,6 +160,8 @@ struct StepConst {
     RT qlt_lc, qlt_gk, qlt_h0;
     RT qlt_pref;           // (pi / 4) sigma2 gamma_k sin(pi / gamma_k) / pi
     RT null_b;             // fp64_null_b
+    double inv_mhd_dt;     // 1 / mhd_dt and 1 / p0, correctly rounded (the float
+    double inv_p0;         // push multiplies by them instead of dividing)
 };
 
 template <typename RT>
@@ -182,6 +190,8 @@ inline StepConst<RT> make_step_const(const kpt_device_settings& s, double qlt_no
     k.qlt_h0 = static_cast<RT>(s.qlt_h0);
     k.qlt_pref = static_cast<RT>(0.25 * 3.14159265358979323846 * s.qlt_sigma2 * qlt_norm);
     k.null_b = static_cast<RT>(s.fp64_null_b);
+    k.inv_mhd_dt = 1.0 / s.mhd_dt;
+    k.inv_p0 = 1.0 / s.p0;
     return k;
 }
 

@@ -229,12 +229,12 @@ __device__ __forceinline__ void qlt_scattering(RT mu, RT xi, RT xi_g, RT pref, R
     dD = D * (rdiv(RT(-2) * mu, one_minus_mu2) + shape);
 }
 
-// p / p0: double division in the FP64 path, a multiply by the hoisted 1 / p0 in
-// the FP32 path.
+// p / p0: double division in the FP64 path, a multiply by 1 / p0 (formed on
+// the host) in the FP32 path.
 template <typename RT>
-__device__ __forceinline__ RT p_ratio(double p, double p0) {
+__device__ __forceinline__ RT p_ratio(double p, double p0, double inv_p0) {
     if constexpr (sizeof(RT) == 4)
-        return static_cast<RT>(p * (1.0 / p0));
+        return static_cast<RT>(p * inv_p0);
     else
         return p / p0;
 }
@@ -293,7 +293,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
 
     RT knorm = RT(1);
     if (s.magnetic_dependency == 1) knorm *= rpow(b, k.exp_b_kappa);
-    if (s.momentum_dependency == 1) knorm *= rpow(p_ratio<RT>(p_d, s.p0), k.pindex);
+    if (s.momentum_dependency == 1) knorm *= rpow(p_ratio<RT>(p_d, s.p0, k.inv_p0), k.pindex);
     const RT kpara = k.kpara0 * knorm;
     const RT kperp = kpara * k.kperp_ratio;
     if (!rfinite(kpara) || kpara <= RT(0) || !rfinite(kperp) || kperp < RT(0)) return false;
@@ -380,7 +380,7 @@ __device__ __forceinline__ bool eval_terms(const RT* c, double p_d, double mu_d,
             }
             RT duu_norm = RT(1);
             if (s.magnetic_dependency == 1) duu_norm *= rpow(b, k.exp_b_duu);
-            if (s.momentum_dependency == 1) duu_norm *= rpow(p_ratio<RT>(p_d, s.p0), k.exp_mu1);
+            if (s.momentum_dependency == 1) duu_norm *= rpow(p_ratio<RT>(p_d, s.p0, k.inv_p0), k.exp_mu1);
             duu *= duu_norm;
             duu_du *= duu_norm;
         }
@@ -468,54 +468,24 @@ __device__ __forceinline__ void sample_blend(const T* __restrict__ flo, const T*
 #ifndef KPT_PAIR_RECORDS_MLP
 #define KPT_PAIR_RECORDS_MLP 2
 #endif
-#if KPT_PAIR_PLANAR
-// FP32 time-blended sample from the frame-interleaved pair buffer
-// pair[2 (k plane + node) + f], f = 0 lower, 1 upper: one row of the stencil,
-// (lo_i, hi_i, lo_i+1, hi_i+1), is one 16-byte load; the bilinear and time
-// weights are folded into eight products.
-template <int C>
-__device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, int64_t n00, int nxp,
-                                              uint64_t plane, double uw, double wx, double wy, float* c) {
-    const double lw = 1.0 - uw;
-    const float a00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * lw), b00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * uw);
-    const float a10 = static_cast<float>(wx * (1.0 - wy) * lw), b10 = static_cast<float>(wx * (1.0 - wy) * uw);
-    const float a01 = static_cast<float>((1.0 - wx) * wy * lw), b01 = static_cast<float>((1.0 - wx) * wy * uw);
-    const float a11 = static_cast<float>(wx * wy * lw), b11 = static_cast<float>(wx * wy * uw);
-#pragma unroll
-    for (int k = 0; k < C; ++k) {
-        const float* q = pair + 2 * (k * plane + n00);
-        const float* r = q + 2 * nxp;
-        c[k] = a00 * q[0] + b00 * q[1] + a10 * q[2] + b10 * q[3] + a01 * r[0] + b01 * r[1] + a11 * r[2] +
-               b11 * r[3];
-    }
-}
-#else
 // FP32 time-blended sample from the node-major pair buffer: node n's record of
 // pair_stride(C) floats starts at pair + n pair_stride(C) (16-byte aligned) and
-// holds (lower, upper) of component k at slots 2k, 2k + 1. A stencil row is two
-// adjacent records, read as 16-byte loads at immediate offsets from one address;
-// the bilinear and time weights are folded into eight products per component.
+// holds (lower, upper) of stored slot s at floats 2s, 2s + 1 (slot s is packed
+// component pair_component(C, s); components the pair does not store are left
+// unset). A stencil row is two adjacent records, read as 16-byte loads at
+// immediate offsets from one address; the bilinear and time weights are folded
+// into eight products per component.
 template <int C, int RPP = 0>
-__device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, int64_t n00, int nxp,
-                                              uint64_t plane, double uw, double wx, double wy, float* c) {
-    (void)plane;
+__device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, int64_t n00, int nxp, double uw,
+                                              double wx, double wy, float* c) {
     constexpr int V = pair_stride(C) / 4;  // float4 per record
-    const double lw = 1.0 - uw;
-    const float a00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * lw), b00 = static_cast<float>((1.0 - wx) * (1.0 - wy) * uw);
-    const float a10 = static_cast<float>(wx * (1.0 - wy) * lw), b10 = static_cast<float>(wx * (1.0 - wy) * uw);
-    const float a01 = static_cast<float>((1.0 - wx) * wy * lw), b01 = static_cast<float>((1.0 - wx) * wy * uw);
-    const float a11 = static_cast<float>(wx * wy * lw), b11 = static_cast<float>(wx * wy * uw);
-#if KPT_PAIR_NARROW
-    // 8-byte loads: (lower, upper) of one component at one node
-    const float2* __restrict__ q2 = reinterpret_cast<const float2*>(pair) + n00 * (2 * V);
-    const float2* __restrict__ r2 = q2 + static_cast<int64_t>(nxp) * (2 * V);
-#pragma unroll
-    for (int k = 0; k < C; ++k) {
-        const float2 u0 = q2[k], u1 = q2[2 * V + k], w0 = r2[k], w1 = r2[2 * V + k];
-        c[k] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y + a11 * w1.x + b11 * w1.y;
-    }
-    return;
-#endif
+    constexpr int S = pair_components(C);
+    // bilinear and time weights in float (the weights lie in [0, 1])
+    const float fx = static_cast<float>(wx), fy = static_cast<float>(wy), fu = static_cast<float>(uw);
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gu = 1.0f - fu;
+    const float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+    const float a00 = w00 * gu, b00 = w00 * fu, a10 = w10 * gu, b10 = w10 * fu;
+    const float a01 = w01 * gu, b01 = w01 * fu, a11 = w11 * gu, b11 = w11 * fu;
     const float4* __restrict__ q = reinterpret_cast<const float4*>(pair) + n00 * V;
     const float4* __restrict__ r = q + static_cast<int64_t>(nxp) * V;
 #pragma unroll
@@ -526,16 +496,16 @@ __device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, in
         // 3- and 4-wave FP32 exact push.
         if (RPP > 0 && v > 0 && v % RPP == 0) __builtin_amdgcn_sched_barrier(0);
         const float4 u0 = q[v], u1 = q[V + v], w0 = r[v], w1 = r[V + v];
-        const int k = 2 * v;
-        if (k < C)
-            c[k] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y + a11 * w1.x +
-                   b11 * w1.y;
-        if (k + 1 < C)
-            c[k + 1] = a00 * u0.z + b00 * u0.w + a10 * u1.z + b10 * u1.w + a01 * w0.z + b01 * w0.w + a11 * w1.z +
-                       b11 * w1.w;
+        const int s0 = 2 * v;
+        if (s0 < S)
+            c[pair_component(C, s0)] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y +
+                                       a11 * w1.x + b11 * w1.y;
+        if (s0 + 1 < S)
+            c[pair_component(C, s0 + 1)] = a00 * u0.z + b00 * u0.w + a10 * u1.z + b10 * u1.w + a01 * w0.z +
+                                           b01 * w0.w + a11 * w1.z + b11 * w1.w;
     }
+    if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
 }
-#endif
 
 // The same sample from the FP64 frames for the rare near-null lanes of the FP32
 // push: four components at a time, so this branch does not set the register
@@ -697,7 +667,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
             const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
             double uw = 0.0;
             if (blend) {
-                const double rt = FP32 ? (t - interval_start) * (1.0 / s.mhd_dt) : (t - interval_start) / s.mhd_dt;
+                const double rt = FP32 ? (t - interval_start) * a.k32.inv_mhd_dt : (t - interval_start) / s.mhd_dt;
                 uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
             }
 
@@ -707,7 +677,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
                 FT c[CP];
                 if constexpr (FP32) {
                     if (a.field_pair != nullptr)
-                        sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                        sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, uw, wx, wy, c);
                     else
                         sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
                 } else {
@@ -806,7 +776,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
 #pragma unroll 1
                     for (int it = 0; it < 3; ++it) {
                         if (it > 0) {
-                            const FT m = static_cast<FT>(reflect_mu(trial, s.mu_max));
+                            const FT m = static_cast<FT>(trial);
                             FT dD;
                             qlt_scattering<FT>(m, T.q_xi, T.q_xig, T.q_pref, step_const<FT>(a).qlt_gk,
                                                step_const<FT>(a).qlt_h0, D, dD);
@@ -1013,12 +983,12 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                 const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
                 double uw = 0.0;
                 if (blend) {
-                    const double rt = (t - interval_start) * (1.0 / s.mhd_dt);
+                    const double rt = (t - interval_start) * K.inv_mhd_dt;
                     uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
                 }
                 float c[CP];
                 if (a.field_pair != nullptr)
-                    sample_pair32<CP, KPT_PAIR_RECORDS_MLP>(a.field_pair, n00, nxp, a.plane, uw, wx, wy, c);
+                    sample_pair32<CP, KPT_PAIR_RECORDS_MLP>(a.field_pair, n00, nxp, uw, wx, wy, c);
                 else
                     sample_blend<CP, float>(static_cast<const float*>(a.field_lo), static_cast<const float*>(a.field_hi),
                                             blend, uw, n00, nxp, a.plane, wx, wy, c);
@@ -1112,7 +1082,7 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                     trial = reflect_mu(mu0 + static_cast<double>(dtr * T.dmu_dt) + static_cast<double>(scale * r3),
                                        s.mu_max);
                     if constexpr (PASSES == 3) {
-                        const float m1 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                        const float m1 = static_cast<float>(trial);
                         float l2m;
                         h1 = use_net && mlp_features(ms, fabsf(m1), lxi, K.qlt_gk, f0, f1, f2, l2m);
                     }
@@ -1126,7 +1096,7 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                 const float l2r0 = mlp_log2_r(y0, ms);
 #pragma unroll 1
                 for (int it = 1; it < 3; ++it) {
-                    const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    const float m = static_cast<float>(trial);
                     float D, dD;
                     if (h0 && mlp_mu_in_hull(ms, fabsf(m)))
                         mlp_closure_extend(l2r0, y1, l2mu0, m, T.q_pref, D, dD);
@@ -1149,7 +1119,7 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                 bool hn = false;
                 f0 = f1 = f2 = 0.0f;
                 if (run) {
-                    const float m = static_cast<float>(reflect_mu(trial, s.mu_max));
+                    const float m = static_cast<float>(trial);
                     float D, dD;
                     closure_at(h1, y0, y1, m, T, K, ms, D, dD);
                     const float rate = 0.5f * (1.0f - m * m) *
@@ -1160,7 +1130,7 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                     trial = reflect_mu(mu0 + static_cast<double>(dtr * rate) + static_cast<double>(scale * r3),
                                        s.mu_max);
                     if (it == 1) {
-                        const float m2 = static_cast<float>(reflect_mu(trial, s.mu_max));
+                        const float m2 = static_cast<float>(trial);
                         float l2m;
                         hn = use_net && mlp_features(ms, fabsf(m2), lxi, K.qlt_gk, f0, f1, f2, l2m);
                     }
@@ -1645,17 +1615,9 @@ hipError_t launch_pack(const double* raw, const GridDesc& g, int model, double* 
 }
 
 // Frame-interleaved FP32 pair of the two coefficient frames of an interval.
-#if KPT_PAIR_PLANAR
-__global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ lo, const float* __restrict__ hi,
-                                                      uint64_t plane, int model, float* __restrict__ out) {
-    const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-    if (e < plane * model) reinterpret_cast<float2*>(out)[e] = make_float2(lo[e], hi[e]);
-}
-
-uint64_t pair_elems(uint64_t plane, int model) { return 2 * plane * static_cast<uint64_t>(model); }
-#else
-// Node-major record: slot j < 2 model of node n holds component j / 2 of the
-// lower (j even) or upper (j odd) frame; the slots up to the record stride are 0.
+// Node-major record: float 2s + f of node n holds stored slot s (packed
+// component pair_component(model, s)) of the lower (f = 0) or upper (f = 1)
+// frame; the floats up to the record stride are 0.
 __global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ lo, const float* __restrict__ hi,
                                                       uint64_t plane, int model, float* __restrict__ out) {
     const int stride = pair_stride(model);
@@ -1664,12 +1626,12 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ 
     const uint64_t n = e / stride;
     const int j = static_cast<int>(e - n * stride);
     float v = 0.0f;
-    if (j < 2 * model) v = ((j & 1) ? hi : lo)[static_cast<uint64_t>(j >> 1) * plane + n];
+    if (j < 2 * pair_components(model))
+        v = ((j & 1) ? hi : lo)[static_cast<uint64_t>(pair_component(model, j >> 1)) * plane + n];
     out[e] = v;
 }
 
 uint64_t pair_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(pair_stride(model)); }
-#endif
 
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream) {
     const uint64_t n = pair_elems(plane, model);
