@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <limits>
@@ -500,6 +501,7 @@ kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t see
     ctx->device = device;
     ctx->capacity = capacity;
     ctx->seed = seed;
+    if (const char* st = std::getenv("KPT_PUSH_STAGE")) ctx->push_stage = std::strcmp(st, "0") != 0;
     std::snprintf(ctx->device_name, sizeof(ctx->device_name), "%s", prop.gcnArchName);
     auto bail = [&](hipError_t err, const char* what) {
         g_create_error = std::string(what) + ": " + hipGetErrorString(err);
@@ -974,6 +976,7 @@ extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_setti
         }
         a.field_pair = ctx->pair32;
     }
+    a.stage = ctx->push_stage ? 1 : 0;
     a.plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
     a.g = lo.g;
     a.s = *s;

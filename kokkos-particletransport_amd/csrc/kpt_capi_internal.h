@@ -86,6 +86,9 @@ struct kpt_ctx {
     // mlp_version counts installs (ensembles rebuild their run tables on a change)
     bool mlp_valid = false;
     uint64_t mlp_version = 0;
+    // FP32 pushes stage their stencil records in LDS (environment KPT_PUSH_STAGE=0
+    // at context creation turns it off, for A/B measurements and tests)
+    bool push_stage = true;
     kpt_mlp_hull mlp_hull{};
     float* mlp_params = nullptr;
     void* mlp_image = nullptr;

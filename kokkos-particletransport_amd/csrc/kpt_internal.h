@@ -201,6 +201,7 @@ struct PushArgs {
     const void* field_lo;  // double or float, component-planar (see coefficient_plane)
     const void* field_hi;
     const float* field_pair;   // FP32 time-blended push: lower/upper interleaved, or nullptr
+    int32_t stage;             // 1: FP32 pushes stage field_pair records in LDS
     const double* field_lo64;  // FP64 frames, read by FP32 lanes below fp64_null_b
     const double* field_hi64;
     uint64_t plane;

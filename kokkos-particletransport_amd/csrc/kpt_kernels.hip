@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -507,6 +508,108 @@ __device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, in
     if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
 }
 
+// The same time-blended sample from records staged in LDS: q points at the
+// lower-left corner's record, the upper row's records start `row` records on.
+template <int C>
+__device__ __forceinline__ void sample_staged(const float4* q, int row, double uw, double wx, double wy, float* c) {
+    constexpr int V = pair_stride(C) / 4;
+    constexpr int S = pair_components(C);
+    const float fx = static_cast<float>(wx), fy = static_cast<float>(wy), fu = static_cast<float>(uw);
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gu = 1.0f - fu;
+    const float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+    const float a00 = w00 * gu, b00 = w00 * fu, a10 = w10 * gu, b10 = w10 * fu;
+    const float a01 = w01 * gu, b01 = w01 * fu, a11 = w11 * gu, b11 = w11 * fu;
+    const float4* r = q + row * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const float4 u0 = q[v], u1 = q[V + v], w0 = r[v], w1 = r[V + v];
+        const int s0 = 2 * v;
+        if (s0 < S)
+            c[pair_component(C, s0)] = a00 * u0.x + b00 * u0.y + a10 * u1.x + b10 * u1.y + a01 * w0.x + b01 * w0.y +
+                                       a11 * w1.x + b11 * w1.y;
+        if (s0 + 1 < S)
+            c[pair_component(C, s0 + 1)] = a00 * u0.z + b00 * u0.w + a10 * u1.z + b10 * u1.w + a01 * w0.z +
+                                           b01 * w0.w + a11 * w1.z + b11 * w1.w;
+    }
+    if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront staging of the FP32 frame pair in LDS. The vector memory path (the
+// texture address and data units) bounds the FP32 push when every lane loads
+// its four stencil records from the L1/L2 caches on every sub-step, although
+// the cell-sorted lanes of a wavefront share most of those nodes. Instead each
+// wavefront copies the records of the node box that covers its lanes' stencils
+// into its own LDS region (contiguous rows of the node-major pair, so the copy
+// is coalesced), and its lanes sample from LDS. Particles move a fraction of a
+// cell per sub-step, so the box, widened by a margin, serves many sub-steps;
+// it is rebuilt when a lane's stencil leaves it. A box that needs more than
+// CAPN records (a dispersed wavefront) is not staged: that sub-step samples
+// from the pair in global memory.
+// ---------------------------------------------------------------------------
+// LDS per wavefront for staged records (0: no staging). At 3 waves per SIMD a
+// CU holds 12 wavefronts: 12 x 12 800 B of its 160 KiB; 80 focused records of
+// 160 B or 100 Parker records of 128 B.
+#ifndef KPT_STAGE_BYTES
+#define KPT_STAGE_BYTES 12800
+#endif
+#ifndef KPT_STAGE_MARGIN
+#define KPT_STAGE_MARGIN 2
+#endif
+struct StageBox {
+    int i_lo, j_lo, w, h;  // staged nodes (i_lo + u, j_lo + v), u < w, v < h; w = 0: none
+};
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Keeps `box` covering the stencils (i0 .. i0 + 1, j0 .. j0 + 1) of the lanes
+// with `run` set, restaging when one leaves it. Every lane of the wavefront
+// calls it together; `lds` is the wavefront's region of CAPN records. Node
+// indices lie in [-1, nx] x [-1, ny] (record (j + 1) (nx + 2) + i + 1).
+template <int C, int CAPN>
+__device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const float* __restrict__ pair, int nx,
+                                             int ny, bool run, int i0, int j0) {
+    constexpr int V = pair_stride(C) / 4;
+    const bool out = run && !(i0 >= box.i_lo && i0 + 1 < box.i_lo + box.w && j0 >= box.j_lo &&
+                              j0 + 1 < box.j_lo + box.h);
+    if (!__any(out)) return;
+    const int imin = wave_min_i32(run ? i0 : INT_MAX), imax = wave_max_i32(run ? i0 + 1 : INT_MIN);
+    const int jmin = wave_min_i32(run ? j0 : INT_MAX), jmax = wave_max_i32(run ? j0 + 1 : INT_MIN);
+    int i_lo = 0, j_lo = 0, w = 0, h = 0;
+    for (int m = KPT_STAGE_MARGIN; m >= 0; --m) {
+        i_lo = max(imin - m, -1);
+        j_lo = max(jmin - m, -1);
+        w = min(imax + m, nx) - i_lo + 1;
+        h = min(jmax + m, ny) - j_lo + 1;
+        if (w * h <= CAPN) break;
+    }
+    if (w * h > CAPN) {
+        box.w = box.h = 0;
+        return;
+    }
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int nxp = nx + 2;
+    const float4* src = reinterpret_cast<const float4*>(pair);
+    for (int r = 0; r < h; ++r) {
+        const float4* row = src + (static_cast<int64_t>(j_lo + r + 1) * nxp + (i_lo + 1)) * V;
+        float4* dst = lds + r * w * V;
+        for (int e = lane; e < w * V; e += 64) dst[e] = row[e];
+    }
+    box.i_lo = i_lo;
+    box.j_lo = j_lo;
+    box.w = w;
+    box.h = h;
+}
+
 // The same sample from the FP64 frames for the rare near-null lanes of the FP32
 // push: four components at a time, so this branch does not set the register
 // budget of the whole kernel.
@@ -604,50 +707,71 @@ __device__ __forceinline__ const PushArgs& args_reload(ArgsK p) {
 // of a wavefront calls it together.
 // STR: the frame has a stretched axis (node_stencil); the uniform kernels keep
 // the plain stencil.
-template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR = false>
-__device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
+template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR = false, int STAGE = 0>
+__device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* stage_lds = nullptr) {
     const PushArgs& a = *(const PushArgs*)(ap);
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr bool FP32 = sizeof(FT) == 4;
     constexpr int CP = MODEL;  // components read per corner
+    static_assert(STAGE == 0 || FP32, "LDS staging serves the FP32 frame pair");
     uint64_t my_steps = 0;
     uint32_t my_fail = 0, my_active = 0;
 
-    if (idx < static_cast<int64_t>(a.count) && a.P.status[idx] == 0) {
+    const bool live = idx < static_cast<int64_t>(a.count) && a.P.status[idx] == 0;
+    const kpt_device_settings& s = a.s;
+    const FT* __restrict__ flo = static_cast<const FT*>(a.field_lo);
+    const FT* __restrict__ fhi = static_cast<const FT*>(a.field_hi);
+    const int nxp = a.g.nx + 2;
+    const double interval_start = a.t0;
+    const double interval_end = a.t_end;
+    const bool blend = s.time_interpolation != 0;
+    double x = 0.0, y = 0.0, p = 1.0, t = interval_end, dt = 0.0, diagnostic_dt = 0.0, pitch_mu = 0.0;
+    uint64_t id = 0;
+    // Sub-steps already taken in this interval (earlier chunks): the step guard
+    // and the Philox sub-step counter continue from there.
+    int step_guard = 0, status = 0;
+    if (live) {
         my_active = 1;
-        const kpt_device_settings& s = a.s;
-        const FT* __restrict__ flo = static_cast<const FT*>(a.field_lo);
-        const FT* __restrict__ fhi = static_cast<const FT*>(a.field_hi);
-        const int nxp = a.g.nx + 2;
-        const double interval_start = a.t0;
-        const double interval_end = a.t_end;
-        const bool blend = s.time_interpolation != 0;
-        double x = a.P.x[idx], y = a.P.y[idx], p = a.P.p[idx], t = a.P.t[idx];
-        double dt = a.P.step_dt[idx];
-        double diagnostic_dt = dt;
-        double pitch_mu = 0.0;
+        x = a.P.x[idx];
+        y = a.P.y[idx];
+        p = a.P.p[idx];
+        t = a.P.t[idx];
+        dt = a.P.step_dt[idx];
+        diagnostic_dt = dt;
         if (MODEL == 21) {
             const double m = a.P.mu[idx];
             pitch_mu = QLT ? reflect_mu(m, s.mu_max) : (m > s.mu_max ? s.mu_max : (m < -s.mu_max ? -s.mu_max : m));
         }
-        const uint64_t id = a.P.id[idx];
-        // Sub-steps already taken in this interval (earlier chunks): the step
-        // guard and the Philox sub-step counter continue from there.
-        int step_guard = a.first_chunk ? 0 : static_cast<int>(a.P.substep[idx]);
-        int status = 0;
-        const double t_stop = a.t_stop;
+        id = a.P.id[idx];
+        step_guard = a.first_chunk ? 0 : static_cast<int>(a.P.substep[idx]);
+    }
+    const double t_stop = a.t_stop;
+    // STAGE: the wavefront's staged node box (sub-steps run wave-uniformly, all
+    // lanes take part in restaging); without it each lane leaves the loop when
+    // its particle is done.
+    StageBox box{0, 0, 0, 0};
+    const bool stage = STAGE != 0 && a.field_pair != nullptr && a.stage != 0;
 
-        while (t < t_stop && step_guard < s.max_particle_steps_per_interval && status == 0) {
-            // The uniform arguments of this sub-step, read again from constant memory.
-            const PushArgs& a = args_reload(ap);
-            const kpt_device_settings& s = a.s;
+    for (;;) {
+        bool run = live && status == 0 && t < t_stop && step_guard < s.max_particle_steps_per_interval;
+        if constexpr (STAGE != 0) {
+            if (!__any(run)) break;
+        } else {
+            if (!run) break;
+        }
+        // The uniform arguments of this sub-step, read again from constant memory.
+        const PushArgs& a = args_reload(ap);
+        const kpt_device_settings& s = a.s;
+
+        // phase 1: wrap and stencil
+        int i0 = 0, j0 = 0;
+        double wx = 0.0, wy = 0.0, uw = 0.0;
+        FT half_dx = step_const<FT>(a).half_dx, half_dy = step_const<FT>(a).half_dy;
+        do {
+            if (!run) break;
             ++step_guard;
             x = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
             y = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
-
-            int i0, j0;
-            double wx, wy;
-            FT half_dx = step_const<FT>(a).half_dx, half_dy = step_const<FT>(a).half_dy;
             if constexpr (STR) {
                 // half cell widths of the dt limiter: the local face interval on a
                 // stretched axis
@@ -655,6 +779,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
                 if (!node_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx, hx) ||
                     !node_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy, hy)) {
                     status = 2;
+                    run = false;
                     break;
                 }
                 if (a.g.face[0]) half_dx = static_cast<FT>(0.5 * hx);
@@ -662,24 +787,42 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
             } else if (!axis_stencil(x + s.interpolation_shift_x, a.g, 0, a.g.nx, i0, wx) ||
                        !axis_stencil(y + s.interpolation_shift_y, a.g, 1, a.g.ny, j0, wy)) {
                 status = 2;
+                run = false;
                 break;
             }
-            const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
-            double uw = 0.0;
             if (blend) {
                 const double rt = FP32 ? (t - interval_start) * a.k32.inv_mhd_dt : (t - interval_start) / s.mhd_dt;
                 uw = rt < 0.0 ? 0.0 : (rt > 1.0 ? 1.0 : rt);
             }
+        } while (false);
+        if constexpr (STAGE != 0) {
+            if (stage) stage_update<CP, STAGE>(box, stage_lds, a.field_pair, a.g.nx, a.g.ny, run, i0, j0);
+        }
 
+        // phase 2: sample, rates, closure, step (a failure ends the particle's
+        // interval with status 2)
+        do {
+            if (!run) break;
+            const int64_t n00 = static_cast<int64_t>(j0 + 1) * nxp + (i0 + 1);
             StepTerms<FT> T;
             bool ok;
             {
                 FT c[CP];
                 if constexpr (FP32) {
-                    if (a.field_pair != nullptr)
+                    if constexpr (STAGE != 0) {
+                        if (box.w > 0) {
+                            const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
+                            sample_staged<CP>(stage_lds + local * (pair_stride(CP) / 4), box.w, uw, wx, wy, c);
+                        } else if (a.field_pair != nullptr) {
+                            sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, uw, wx, wy, c);
+                        } else {
+                            sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                        }
+                    } else if (a.field_pair != nullptr) {
                         sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, uw, wx, wy, c);
-                    else
+                    } else {
                         sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                    }
                 } else {
                     sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
                 }
@@ -811,9 +954,11 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx) {
                 status = 2;
                 break;
             }
-        }
-        if (step_guard >= s.max_particle_steps_per_interval && t < interval_end) status = 2;
+        } while (false);
+    }
 
+    if (live) {
+        if (step_guard >= s.max_particle_steps_per_interval && t < interval_end) status = 2;
         a.P.x[idx] = wrap_periodic(x, s.x_lower, s.x_upper, s.x_width);
         a.P.y[idx] = wrap_periodic(y, s.y_lower, s.y_upper, s.y_width);
         a.P.p[idx] = p;
@@ -854,7 +999,16 @@ __global__ void __launch_bounds__(kBlock)
                                                                  : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
     (void)a;
-    push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(kernel_args(), push_first_particle(run));
+    if constexpr (sizeof(FT) == 4 && KPT_STAGE_BYTES > 0) {
+        constexpr int V = pair_stride(MODEL) / 4;
+        constexpr int CAPN = KPT_STAGE_BYTES / (V * 16);
+        __shared__ float4 stage_mem[kBlock / 64 * CAPN * V];
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        push_lane<MODEL, FOCUSED, FT, CLOSURE, STR, CAPN>(kernel_args(), push_first_particle(run),
+                                                          stage_mem + wave * (CAPN * V));
+    } else {
+        push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(kernel_args(), push_first_particle(run));
+    }
 }
 
 // Closure from the network outputs at pitch cosine mu (|mu| in the hull), with

@@ -1,0 +1,101 @@
+"""LDS staging of the FP32 frame pair: the FP32 pushes sample the stencil records a
+wavefront stages in LDS (its node box, rebuilt when a lane leaves it) or, for a
+wavefront whose lanes are too dispersed for the box, the pair in global memory.
+The staged records are copies and the sample's arithmetic is the same, so a push
+with staging must equal the push without it (KPT_PUSH_STAGE=0 at context
+creation) bit for bit, on every particle field and counter.
+
+Cases: the FP32 focused push with the exact slab-QLT closure and with the
+reduced closure, and the FP32 Parker push (adaptive dt), on the tanh-stretched
+and the uniform island field; cell-sorted chunks (compact wavefronts: the staged
+path) and one unsorted chunk of randomly placed particles (dispersed wavefronts:
+mostly the global fallback)."""
+import os
+
+import numpy as np
+import pytest
+
+import kpt
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+N = 32768
+
+
+def _field(grid):
+    if grid == "stretched":
+        ye = W.stretched_edges(256, ratio=10.0)
+        xe, ye, f0 = W.island_field(512, 256, seed=7, y_edges=ye)
+        _, _, f1 = W.island_field(512, 256, seed=8, y_edges=ye)
+    else:
+        xe, ye, f0 = W.island_field(512, 256, seed=7)
+        _, _, f1 = W.island_field(512, 256, seed=8)
+    return xe, ye, f0, f1
+
+
+def _push(field, model, s, start, stage):
+    xe, ye, f0, f1 = field
+    old = os.environ.get("KPT_PUSH_STAGE")
+    os.environ["KPT_PUSH_STAGE"] = "1" if stage else "0"
+    try:
+        with kpt.Context(len(start["x"]), seed=21) as c:
+            c.field_upload(kpt.SLOT_LOWER, model, xe, ye, f0)
+            c.field_upload(kpt.SLOT_UPPER, model, xe, ye, f1)
+            c.upload(start)
+            st = c.push_interval(s, 0.0, 0)
+            got = c.download()
+    finally:
+        if old is None:
+            del os.environ["KPT_PUSH_STAGE"]
+        else:
+            os.environ["KPT_PUSH_STAGE"] = old
+    return got, st
+
+
+def _equal(field, model, s, start):
+    a, sa = _push(field, model, s, start, True)
+    b, sb = _push(field, model, s, start, False)
+    assert sa.particle_steps == sb.particle_steps and sa.failures == sb.failures
+    assert sa.particle_steps > 0
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    return sa
+
+
+@pytest.mark.parametrize("grid", ["stretched", "uniform"])
+@pytest.mark.parametrize("closure", [kpt.CLOSURE_QLT, kpt.CLOSURE_REDUCED], ids=["qlt", "reduced"])
+@pytest.mark.parametrize("chunks", [8, 1], ids=["sorted", "unsorted"])
+def test_focused_fp32_staged_equals_global(grid, closure, chunks):
+    field = _field(grid)
+    xe, ye = field[0], field[1]
+    s = W.c3_device_settings(xe, ye, closure=closure, precision=kpt.PREC_FP32)
+    steps = 200
+    s.mhd_dt = steps * s.dt_min
+    s.max_particle_steps_per_interval = steps + 2
+    s.push_chunks = chunks
+    start = W.c3_particles(N, s, seed=4)
+    st = _equal(field, 21, s, start)
+    assert st.failures == 0
+
+
+@pytest.mark.parametrize("grid", ["stretched", "uniform"])
+@pytest.mark.parametrize("chunks", [8, 1], ids=["sorted", "unsorted"])
+def test_parker_fp32_staged_equals_global(grid, chunks):
+    field = _field(grid)
+    xe, ye = field[0], field[1]
+    h = kpt.default_host_settings()
+    h.split_particles = 0
+    h.mhd_dt = 0.01
+    s = kpt.make_device_settings(h, xe, ye)
+    s.precision = kpt.PREC_FP32
+    s.push_chunks = chunks
+    rng = np.random.default_rng(5)
+    start = kpt.empty_particles(N)
+    start["x"][:] = rng.uniform(s.x_lower, s.x_upper, N)
+    start["y"][:] = rng.uniform(s.y_lower, s.y_upper, N)
+    start["p"][:] = np.exp(rng.uniform(0.0, np.log(2.0), N))
+    start["weight"][:] = 1.0
+    start["id"][:] = np.arange(N, dtype=np.uint64)
+    st = _equal(field, 16, s, start)
+    assert st.failures == 0
