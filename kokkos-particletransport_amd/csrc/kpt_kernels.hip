@@ -539,13 +539,18 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
 // texture address and data units) bounds the FP32 push when every lane loads
 // its four stencil records from the L1/L2 caches on every sub-step, although
 // the cell-sorted lanes of a wavefront share most of those nodes. Instead each
-// wavefront copies the records of the node box that covers its lanes' stencils
-// into its own LDS region (contiguous rows of the node-major pair, so the copy
-// is coalesced), and its lanes sample from LDS. Particles move a fraction of a
-// cell per sub-step, so the box, widened by a margin, serves many sub-steps;
-// it is rebuilt when a lane's stencil leaves it. A box that needs more than
-// CAPN records (a dispersed wavefront) is not staged: that sub-step samples
-// from the pair in global memory.
+// wavefront copies the records of a node box around its lanes' stencils into
+// its own LDS region (contiguous rows of the node-major pair, so the copy is
+// coalesced); lanes whose stencil lies in the box sample from LDS, the others
+// from the pair in global memory. Particles move a fraction of a cell per
+// sub-step, so a box serves many sub-steps:
+//  - when the bounding box of the lanes' stencils, widened by a margin where
+//    it fits, holds at most CAPN records, the box covers every lane and is
+//    rebuilt as soon as a lane leaves it;
+//  - otherwise (a dispersed wavefront) the box is the CAPN-record window with
+//    the bounding box's aspect, centred on the lanes' mean node; it is rebuilt
+//    only when KPT_STAGE_OUTLIERS more lanes are outside than right after the
+//    last rebuild.
 // ---------------------------------------------------------------------------
 // LDS per wavefront for staged records (0: no staging). At 3 waves per SIMD a
 // CU holds 12 wavefronts: 12 x 12 800 B of its 160 KiB; 80 focused records of
@@ -556,8 +561,15 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
 #ifndef KPT_STAGE_MARGIN
 #define KPT_STAGE_MARGIN 2
 #endif
+#ifndef KPT_STAGE_OUTLIERS
+#define KPT_STAGE_OUTLIERS 8
+#endif
 struct StageBox {
     int i_lo, j_lo, w, h;  // staged nodes (i_lo + u, j_lo + v), u < w, v < h; w = 0: none
+    int out0;              // lanes outside the box after its build (-1: none; 0: the box covers all)
+    __device__ __forceinline__ bool holds(int i0, int j0) const {
+        return i0 >= i_lo && i0 + 1 < i_lo + w && j0 >= j_lo && j0 + 1 < j_lo + h;
+    }
 };
 
 __device__ __forceinline__ int wave_min_i32(int v) {
@@ -570,18 +582,24 @@ __device__ __forceinline__ int wave_max_i32(int v) {
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return __builtin_amdgcn_readfirstlane(v);
 }
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_count(bool b) { return __builtin_popcountll(__ballot(b)); }
 
-// Keeps `box` covering the stencils (i0 .. i0 + 1, j0 .. j0 + 1) of the lanes
-// with `run` set, restaging when one leaves it. Every lane of the wavefront
-// calls it together; `lds` is the wavefront's region of CAPN records. Node
-// indices lie in [-1, nx] x [-1, ny] (record (j + 1) (nx + 2) + i + 1).
+// Keeps `box` around the stencils (i0 .. i0 + 1, j0 .. j0 + 1) of the lanes with
+// `run` set (rules above). Every lane of the wavefront calls it together; `lds`
+// is the wavefront's region of CAPN records. Node indices lie in [-1, nx] x
+// [-1, ny] (record (j + 1) (nx + 2) + i + 1).
 template <int C, int CAPN>
 __device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const float* __restrict__ pair, int nx,
                                              int ny, bool run, int i0, int j0) {
     constexpr int V = pair_stride(C) / 4;
-    const bool out = run && !(i0 >= box.i_lo && i0 + 1 < box.i_lo + box.w && j0 >= box.j_lo &&
-                              j0 + 1 < box.j_lo + box.h);
-    if (!__any(out)) return;
+    static_assert(CAPN >= 4, "a staged box holds at least one cell's four nodes");
+    const int n_out = wave_count(run && !box.holds(i0, j0));
+    if (n_out == 0 || (box.out0 > 0 && n_out <= box.out0 + KPT_STAGE_OUTLIERS)) return;
     const int imin = wave_min_i32(run ? i0 : INT_MAX), imax = wave_max_i32(run ? i0 + 1 : INT_MIN);
     const int jmin = wave_min_i32(run ? j0 : INT_MAX), jmax = wave_max_i32(run ? j0 + 1 : INT_MIN);
     int i_lo = 0, j_lo = 0, w = 0, h = 0;
@@ -592,9 +610,19 @@ __device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const f
         h = min(jmax + m, ny) - j_lo + 1;
         if (w * h <= CAPN) break;
     }
-    if (w * h > CAPN) {
-        box.w = box.h = 0;
-        return;
+    bool full = w * h <= CAPN;
+    if (!full) {
+        // the CAPN-record window with the bounding box's aspect around the mean node
+        const int W = imax - imin + 1, H = jmax - jmin + 1;  // both >= 2
+        int ww = static_cast<int>(sqrtf(static_cast<float>(CAPN) * static_cast<float>(W) / static_cast<float>(H)));
+        ww = max(2, min(ww, min(W, CAPN / 2)));
+        const int hh = max(2, min(H, CAPN / ww));
+        const int nrun = wave_count(run);
+        const int ci = wave_sum_i32(run ? i0 : 0) / nrun, cj = wave_sum_i32(run ? j0 : 0) / nrun;
+        i_lo = max(imin, min(ci - (ww - 1) / 2, imax - ww + 1));
+        j_lo = max(jmin, min(cj - (hh - 1) / 2, jmax - hh + 1));
+        w = ww;
+        h = hh;
     }
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int nxp = nx + 2;
@@ -608,6 +636,7 @@ __device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const f
     box.j_lo = j_lo;
     box.w = w;
     box.h = h;
+    box.out0 = full ? 0 : wave_count(run && !box.holds(i0, j0));
 }
 
 // The same sample from the FP64 frames for the rare near-null lanes of the FP32
@@ -749,7 +778,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
     // STAGE: the wavefront's staged node box (sub-steps run wave-uniformly, all
     // lanes take part in restaging); without it each lane leaves the loop when
     // its particle is done.
-    StageBox box{0, 0, 0, 0};
+    StageBox box{0, 0, 0, 0, -1};
     const bool stage = STAGE != 0 && a.field_pair != nullptr && a.stage != 0;
 
     for (;;) {
@@ -810,7 +839,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
                 FT c[CP];
                 if constexpr (FP32) {
                     if constexpr (STAGE != 0) {
-                        if (box.w > 0) {
+                        if (stage && box.holds(i0, j0)) {
                             const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
                             sample_staged<CP>(stage_lds + local * (pair_stride(CP) / 4), box.w, uw, wx, wy, c);
                         } else if (a.field_pair != nullptr) {
