@@ -552,17 +552,17 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
 //    only when KPT_STAGE_OUTLIERS more lanes are outside than right after the
 //    last rebuild.
 // ---------------------------------------------------------------------------
-// LDS per wavefront for staged records (0: no staging). At 3 waves per SIMD a
-// CU holds 12 wavefronts: 12 x 12 800 B of its 160 KiB; 80 focused records of
-// 160 B or 100 Parker records of 128 B.
-#ifndef KPT_STAGE_BYTES
-#define KPT_STAGE_BYTES 12800
+// LDS for staged records per CU (0: no staging), shared by the wavefronts the
+// kernel's occupancy puts on a CU: at 4 waves per SIMD, 16 wavefronts of
+// 9 600 B (60 focused records of 160 B); at 3, 12 of 12 800 B.
+#ifndef KPT_STAGE_CU_BYTES
+#define KPT_STAGE_CU_BYTES 153600
 #endif
 #ifndef KPT_STAGE_MARGIN
 #define KPT_STAGE_MARGIN 2
 #endif
 #ifndef KPT_STAGE_OUTLIERS
-#define KPT_STAGE_OUTLIERS 8
+#define KPT_STAGE_OUTLIERS 16
 #endif
 struct StageBox {
     int i_lo, j_lo, w, h;  // staged nodes (i_lo + u, j_lo + v), u < w, v < h; w = 0: none
@@ -721,15 +721,18 @@ __device__ __forceinline__ const PushArgs& args_reload(ArgsK p) {
 //   FP32 focused: 3 waves -20%, 4 waves -29%, 5 waves +1%;
 //   FP32 Parker: 3 waves -20%, 4 waves -16%.
 // With the node-major FP32 pair (one record per load phase), FP32 focused: 3
-// waves 42.3 ms, 4 waves 44.7 ms per launch.
+// waves 42.3 ms, 4 waves 44.7 ms per launch. With records staged in LDS (8 M
+// focused particles, exact closure): 3 waves 28.3 ms, 4 waves 25.3 ms, 5 waves
+// 28.1 ms per launch; FP32 Parker (4 M particles, adaptive dt): 3 waves 23.5 ms
+// (44.3 ms unstaged), 4 waves 21.7 ms, 5 waves 26.9 ms.
 #ifndef KPT_PUSH_WAVES_FP64
 #define KPT_PUSH_WAVES_FP64 2
 #endif
 #ifndef KPT_PUSH_WAVES_FP32_PARKER
-#define KPT_PUSH_WAVES_FP32_PARKER 3
+#define KPT_PUSH_WAVES_FP32_PARKER 4
 #endif
 #ifndef KPT_PUSH_WAVES_FP32_FOCUSED
-#define KPT_PUSH_WAVES_FP32_FOCUSED 3
+#define KPT_PUSH_WAVES_FP32_FOCUSED 4
 #endif
 // One lane of the push: particle `idx` of the set in `a` (lanes past a.count
 // or with status != 0 only join the wave reduction of the counters). Every lane
@@ -1028,9 +1031,10 @@ __global__ void __launch_bounds__(kBlock)
                                                                  : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
     (void)a;
-    if constexpr (sizeof(FT) == 4 && KPT_STAGE_BYTES > 0) {
+    if constexpr (sizeof(FT) == 4 && KPT_STAGE_CU_BYTES > 0) {
         constexpr int V = pair_stride(MODEL) / 4;
-        constexpr int CAPN = KPT_STAGE_BYTES / (V * 16);
+        constexpr int waves = FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER;
+        constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (V * 16);
         __shared__ float4 stage_mem[kBlock / 64 * CAPN * V];
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         push_lane<MODEL, FOCUSED, FT, CLOSURE, STR, CAPN>(kernel_args(), push_first_particle(run),
