@@ -566,6 +566,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         }
     }
     dev_free(ctx->pair32);
+    dev_free(ctx->pair64);
     dev_free(ctx->mlp_params);
     dev_free(ctx->mlp_image);
     if (ctx->p2_allocated) kpt::free_arrays(ctx->P2);
@@ -927,8 +928,9 @@ kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inje
 
 // The push's launch arguments for the interval [t0, t0 + mhd_dt) of `frame`
 // (per-chunk fields t_stop, first_chunk, counters and self are the caller's).
-// `paired`: an FP32 time-blended push reads the frames from the interleaved
-// pair buffer, built (and grown) here; otherwise the kernel reads both frames.
+// `paired`: a time-blended push reads the frames from the node-major pair
+// buffer of its precision, built (and grown) here; otherwise the kernel reads
+// both component-planar frames.
 extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame, bool paired,
                                kpt::PushArgs& a, bool& focused) {
     if (!ctx) return KPT_ERR_ARG;
@@ -975,6 +977,25 @@ extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_setti
             ctx->pair_versions[1] = hi.version;
         }
         a.field_pair = ctx->pair32;
+    }
+    a.field_pair64 = nullptr;
+    if (paired && !fp32 && s->time_interpolation != 0) {
+        const uint64_t plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);
+        const size_t elems = kpt::pair64_elems(plane, lo.model);
+        if (elems > ctx->pair64_cap) {
+            dev_free(ctx->pair64);
+            ctx->pair64 = nullptr;
+            ctx->pair64_cap = 0;
+            ctx->pair64_versions[0] = ctx->pair64_versions[1] = 0;
+            KPT_HIP(ctx, dev_alloc(&ctx->pair64, elems));
+            ctx->pair64_cap = elems;
+        }
+        if (ctx->pair64_versions[0] != lo.version || ctx->pair64_versions[1] != hi.version) {
+            KPT_HIP(ctx, kpt::launch_pair64(lo.c64, hi.c64, plane, lo.model, ctx->pair64, ctx->stream));
+            ctx->pair64_versions[0] = lo.version;
+            ctx->pair64_versions[1] = hi.version;
+        }
+        a.field_pair64 = ctx->pair64;
     }
     a.stage = ctx->push_stage ? 1 : 0;
     a.plane = kpt::coefficient_plane(lo.g.nx, lo.g.ny);

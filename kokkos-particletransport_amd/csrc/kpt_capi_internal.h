@@ -82,6 +82,9 @@ struct kpt_ctx {
     float* pair32 = nullptr;
     size_t pair32_cap = 0;
     uint64_t pair_versions[2] = {0, 0};
+    double* pair64 = nullptr;
+    size_t pair64_cap = 0;
+    uint64_t pair64_versions[2] = {0, 0};
     // learned closure: FP32 parameters (device and host copies), hull, MFMA image;
     // mlp_version counts installs (ensembles rebuild their run tables on a change)
     bool mlp_valid = false;

@@ -30,6 +30,9 @@ __host__ __device__ constexpr int pair_component(int model, int s) { return mode
 // Floats per node record of the FP32 frame pair: both frames' stored
 // components, rounded up to whole 16-byte loads.
 __host__ __device__ constexpr int pair_stride(int model) { return (2 * pair_components(model) + 3) / 4 * 4; }
+// Doubles per node record of the FP64 frame pair: (lower, upper) of every
+// packed component, k at doubles 2k, 2k + 1 (one 16-byte load per component).
+__host__ __device__ constexpr int pair64_stride(int model) { return 2 * model; }
 
 // One bin of a stretched axis' lookup table, read with one 32-byte load: the
 // face interval k that contains the bin's lower end (less a guard band) and the
@@ -201,6 +204,7 @@ struct PushArgs {
     const void* field_lo;  // double or float, component-planar (see coefficient_plane)
     const void* field_hi;
     const float* field_pair;   // FP32 time-blended push: lower/upper interleaved, or nullptr
+    const double* field_pair64;  // FP64 time-blended push: node-major (lower, upper) records, or nullptr
     int32_t stage;             // 1: FP32 pushes stage field_pair records in LDS
     const double* field_lo64;  // FP64 frames, read by FP32 lanes below fp64_null_b
     const double* field_hi64;
@@ -266,6 +270,10 @@ hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const do
 // frames (layout in kpt_kernels.hip, sample_pair32).
 uint64_t pair_elems(uint64_t plane, int model);
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream);
+// The FP64 frame pair (node-major records of pair64_stride(model) doubles).
+uint64_t pair64_elems(uint64_t plane, int model);
+hipError_t launch_pair64(const double* lo, const double* hi, uint64_t plane, int model, double* out,
+                         hipStream_t stream);
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);

@@ -534,9 +534,32 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
     if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
 }
 
+// FP64 time-blended sample from node-major (lower, upper) records: the same
+// weights and the same sum per frame as sample_blend over the component-planar
+// frames, so the result is the same bit for bit. q points at the lower-left
+// corner's record, the upper row's records start `row` records on (nx + 2 in
+// the pair in global memory, the box width in a staged box).
+template <int C>
+__device__ __forceinline__ void sample_pair64(const double2* q, int64_t row, double uw, double wx, double wy,
+                                              double* c) {
+    const double w00 = (1.0 - wx) * (1.0 - wy);
+    const double w10 = wx * (1.0 - wy);
+    const double w01 = (1.0 - wx) * wy;
+    const double w11 = wx * wy;
+    const double lw = 1.0 - uw;
+    const double2* r = q + row * C;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const double2 u0 = q[k], u1 = q[C + k], v0 = r[k], v1 = r[C + k];
+        const double lo = w00 * u0.x + w10 * u1.x + w01 * v0.x + w11 * v1.x;
+        const double hi = w00 * u0.y + w10 * u1.y + w01 * v0.y + w11 * v1.y;
+        c[k] = lw * lo + uw * hi;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Wavefront staging of the FP32 frame pair in LDS. The vector memory path (the
-// texture address and data units) bounds the FP32 push when every lane loads
+// Wavefront staging of the frame pair in LDS. The vector memory path (the
+// texture address and data units) bounds the push when every lane loads
 // its four stencil records from the L1/L2 caches on every sub-step, although
 // the cell-sorted lanes of a wavefront share most of those nodes. Instead each
 // wavefront copies the records of a node box around its lanes' stencils into
@@ -591,12 +614,12 @@ __device__ __forceinline__ int wave_count(bool b) { return __builtin_popcountll(
 
 // Keeps `box` around the stencils (i0 .. i0 + 1, j0 .. j0 + 1) of the lanes with
 // `run` set (rules above). Every lane of the wavefront calls it together; `lds`
-// is the wavefront's region of CAPN records. Node indices lie in [-1, nx] x
-// [-1, ny] (record (j + 1) (nx + 2) + i + 1).
-template <int C, int CAPN>
-__device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const float* __restrict__ pair, int nx,
+// is the wavefront's region of CAPN records of V 16-byte units (the FP32 or
+// FP64 pair's records). Node indices lie in [-1, nx] x [-1, ny] (record
+// (j + 1) (nx + 2) + i + 1).
+template <int V, int CAPN>
+__device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const void* __restrict__ pair, int nx,
                                              int ny, bool run, int i0, int j0) {
-    constexpr int V = pair_stride(C) / 4;
     static_assert(CAPN >= 4, "a staged box holds at least one cell's four nodes");
     const int n_out = wave_count(run && !box.holds(i0, j0));
     if (n_out == 0 || (box.out0 > 0 && n_out <= box.out0 + KPT_STAGE_OUTLIERS)) return;
@@ -626,7 +649,7 @@ __device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const f
     }
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int nxp = nx + 2;
-    const float4* src = reinterpret_cast<const float4*>(pair);
+    const float4* src = static_cast<const float4*>(pair);
     for (int r = 0; r < h; ++r) {
         const float4* row = src + (static_cast<int64_t>(j_lo + r + 1) * nxp + (i_lo + 1)) * V;
         float4* dst = lds + r * w * V;
@@ -745,7 +768,6 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
     constexpr bool QLT = FOCUSED && CLOSURE != KPT_CLOSURE_REDUCED;
     constexpr bool FP32 = sizeof(FT) == 4;
     constexpr int CP = MODEL;  // components read per corner
-    static_assert(STAGE == 0 || FP32, "LDS staging serves the FP32 frame pair");
     uint64_t my_steps = 0;
     uint32_t my_fail = 0, my_active = 0;
 
@@ -782,7 +804,9 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
     // lanes take part in restaging); without it each lane leaves the loop when
     // its particle is done.
     StageBox box{0, 0, 0, 0, -1};
-    const bool stage = STAGE != 0 && a.field_pair != nullptr && a.stage != 0;
+    const void* pair = FP32 ? static_cast<const void*>(a.field_pair) : static_cast<const void*>(a.field_pair64);
+    constexpr int PV = FP32 ? pair_stride(CP) / 4 : CP;  // 16-byte units per pair record
+    const bool stage = STAGE != 0 && pair != nullptr && a.stage != 0;
 
     for (;;) {
         bool run = live && status == 0 && t < t_stop && step_guard < s.max_particle_steps_per_interval;
@@ -828,7 +852,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
             }
         } while (false);
         if constexpr (STAGE != 0) {
-            if (stage) stage_update<CP, STAGE>(box, stage_lds, a.field_pair, a.g.nx, a.g.ny, run, i0, j0);
+            if (stage) stage_update<PV, STAGE>(box, stage_lds, pair, a.g.nx, a.g.ny, run, i0, j0);
         }
 
         // phase 2: sample, rates, closure, step (a failure ends the particle's
@@ -844,7 +868,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
                     if constexpr (STAGE != 0) {
                         if (stage && box.holds(i0, j0)) {
                             const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
-                            sample_staged<CP>(stage_lds + local * (pair_stride(CP) / 4), box.w, uw, wx, wy, c);
+                            sample_staged<CP>(stage_lds + local * PV, box.w, uw, wx, wy, c);
                         } else if (a.field_pair != nullptr) {
                             sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, uw, wx, wy, c);
                         } else {
@@ -856,7 +880,16 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
                         sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
                     }
                 } else {
-                    sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                    if (STAGE != 0 && stage && box.holds(i0, j0)) {
+                        const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
+                        sample_pair64<CP>(reinterpret_cast<const double2*>(stage_lds) + local * CP, box.w, uw, wx,
+                                          wy, c);
+                    } else if (a.field_pair64 != nullptr) {
+                        sample_pair64<CP>(reinterpret_cast<const double2*>(a.field_pair64) + n00 * CP, nxp, uw, wx,
+                                          wy, c);
+                    } else {
+                        sample_blend<CP, FT>(flo, fhi, blend, uw, n00, nxp, a.plane, wx, wy, c);
+                    }
                 }
                 if constexpr (FP32) {
                     if (!(c[MODEL == 16 ? 5 : 6] >= a.k32.null_b)) {
@@ -1031,9 +1064,10 @@ __global__ void __launch_bounds__(kBlock)
                                                                  : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
     (void)a;
-    if constexpr (sizeof(FT) == 4 && KPT_STAGE_CU_BYTES > 0) {
-        constexpr int V = pair_stride(MODEL) / 4;
-        constexpr int waves = FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER;
+    if constexpr (KPT_STAGE_CU_BYTES > 0) {
+        constexpr int V = sizeof(FT) == 4 ? pair_stride(MODEL) / 4 : MODEL;  // 16-byte units per record
+        constexpr int waves = sizeof(FT) == 8 ? KPT_PUSH_WAVES_FP64
+                                              : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER);
         constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (V * 16);
         __shared__ float4 stage_mem[kBlock / 64 * CAPN * V];
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1819,6 +1853,29 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ 
 }
 
 uint64_t pair_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(pair_stride(model)); }
+
+// FP64 pair: double 2k + f of node n's record holds component k of the lower
+// (f = 0) or upper (f = 1) frame.
+__global__ void __launch_bounds__(kBlock) pair64_kernel(const double* __restrict__ lo, const double* __restrict__ hi,
+                                                        uint64_t plane, int model, double* __restrict__ out) {
+    const int stride = pair64_stride(model);
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    if (e >= plane * stride) return;
+    const uint64_t n = e / stride;
+    const int j = static_cast<int>(e - n * stride);
+    out[e] = ((j & 1) ? hi : lo)[static_cast<uint64_t>(j >> 1) * plane + n];
+}
+
+uint64_t pair64_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(pair64_stride(model)); }
+
+hipError_t launch_pair64(const double* lo, const double* hi, uint64_t plane, int model, double* out,
+                         hipStream_t stream) {
+    const uint64_t n = pair64_elems(plane, model);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pair64_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                       lo, hi, plane, model, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream) {
     const uint64_t n = pair_elems(plane, model);

@@ -1,15 +1,17 @@
-"""LDS staging of the FP32 frame pair: the FP32 pushes sample the stencil records a
-wavefront stages in LDS (its node box, rebuilt when a lane leaves it) or, for a
-wavefront whose lanes are too dispersed for the box, the pair in global memory.
-The staged records are copies and the sample's arithmetic is the same, so a push
-with staging must equal the push without it (KPT_PUSH_STAGE=0 at context
-creation) bit for bit, on every particle field and counter.
+"""LDS staging of the frame pair: the FP32 and FP64 pushes sample the stencil
+records a wavefront stages in LDS (a node box around its lanes' stencils) or, for
+lanes outside the box, the node-major pair in global memory. The staged records
+are copies and the sample's arithmetic is the same, so a push with staging must
+equal the push without it (KPT_PUSH_STAGE=0 at context creation) bit for bit, on
+every particle field and counter. (The FP64 pushes' agreement with the oracle,
+1e-12 with injected noise, is tested in test_gpu_parity.py and
+test_gpu_stretched.py through the same staged path.)
 
-Cases: the FP32 focused push with the exact slab-QLT closure and with the
-reduced closure, and the FP32 Parker push (adaptive dt), on the tanh-stretched
-and the uniform island field; cell-sorted chunks (compact wavefronts: the staged
-path) and one unsorted chunk of randomly placed particles (dispersed wavefronts:
-mostly the global fallback)."""
+Cases: the focused push with the exact slab-QLT closure and with the reduced
+closure, and the Parker push (adaptive dt), in FP32 and FP64, on the
+tanh-stretched and the uniform island field; cell-sorted chunks (compact
+wavefronts: most lanes staged) and one unsorted chunk of randomly placed
+particles (dispersed wavefronts: most lanes read global memory)."""
 import os
 
 import numpy as np
@@ -63,13 +65,17 @@ def _equal(field, model, s, start):
     return sa
 
 
+PRECISIONS = pytest.mark.parametrize("prec", [kpt.PREC_FP32, kpt.PREC_FP64], ids=["fp32", "fp64"])
+
+
+@PRECISIONS
 @pytest.mark.parametrize("grid", ["stretched", "uniform"])
 @pytest.mark.parametrize("closure", [kpt.CLOSURE_QLT, kpt.CLOSURE_REDUCED], ids=["qlt", "reduced"])
 @pytest.mark.parametrize("chunks", [8, 1], ids=["sorted", "unsorted"])
-def test_focused_fp32_staged_equals_global(grid, closure, chunks):
+def test_focused_staged_equals_global(prec, grid, closure, chunks):
     field = _field(grid)
     xe, ye = field[0], field[1]
-    s = W.c3_device_settings(xe, ye, closure=closure, precision=kpt.PREC_FP32)
+    s = W.c3_device_settings(xe, ye, closure=closure, precision=prec)
     steps = 200
     s.mhd_dt = steps * s.dt_min
     s.max_particle_steps_per_interval = steps + 2
@@ -79,16 +85,17 @@ def test_focused_fp32_staged_equals_global(grid, closure, chunks):
     assert st.failures == 0
 
 
+@PRECISIONS
 @pytest.mark.parametrize("grid", ["stretched", "uniform"])
 @pytest.mark.parametrize("chunks", [8, 1], ids=["sorted", "unsorted"])
-def test_parker_fp32_staged_equals_global(grid, chunks):
+def test_parker_staged_equals_global(prec, grid, chunks):
     field = _field(grid)
     xe, ye = field[0], field[1]
     h = kpt.default_host_settings()
     h.split_particles = 0
     h.mhd_dt = 0.01
     s = kpt.make_device_settings(h, xe, ye)
-    s.precision = kpt.PREC_FP32
+    s.precision = prec
     s.push_chunks = chunks
     rng = np.random.default_rng(5)
     start = kpt.empty_particles(N)
