@@ -137,6 +137,7 @@ def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0, seed=7):
 # wave64 VALU instruction holds its SIMD for 4 cycles; unpacked FP32 and FP64 FMA
 # 78.6 TFLOP/s (packed FP32 is off in this build); dense BF16 MFMA 2.5 PFLOP/s.
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+TA_BUSY_PEAK = 256 * 2.4e9  # one texture address unit per CU at the 2.4 GHz engine clock
 FP32_PEAK = 78.6e12
 FP64_PEAK = 78.6e12
 BF16_MFMA_PEAK = 2.5e15
@@ -176,6 +177,17 @@ def roofline_block(workload, kernel, launch_s, launches, alg_bytes):
         v = c["SQ_INSTS_VALU"] / launch_s
         res["valu_issue"] = {"achieved": v / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
                              "frac": v / VALU_ISSUE_PEAK}
+    if "SQ_ACTIVE_INST_VALU" in c:
+        # VALU pipe occupancy: issue slots of 4 cycles per wave64 instruction, 8 for
+        # a transcendental
+        v = c["SQ_ACTIVE_INST_VALU"] / launch_s
+        res["valu_busy"] = {"achieved": v / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-slots/s",
+                            "frac": v / VALU_ISSUE_PEAK}
+    if "TA_TA_BUSY_sum" in c:
+        # vector memory address unit (one per CU) busy cycles: the gather path
+        v = c["TA_TA_BUSY_sum"] / launch_s
+        res["vmem_ta"] = {"achieved": v / 1e9, "peak": TA_BUSY_PEAK / 1e9, "unit": "G busy cycles/s",
+                          "frac": v / TA_BUSY_PEAK}
     if "FLOP_F32" in c:
         v = c["FLOP_F32"] / launch_s
         res["fp32"] = {"achieved": v / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": v / FP32_PEAK}

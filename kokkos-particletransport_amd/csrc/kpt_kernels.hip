@@ -508,11 +508,19 @@ __device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, in
     if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
 }
 
+// Staged records are stage_stride(V) 16-byte units apart in LDS (V units of
+// data). The FP64 Parker record (16 units, 256 B) starts every node on the same
+// LDS banks; one unit of padding spreads them (FP64 Parker push 62.3 -> 52.9 ms
+// per launch). Padding the other records measured no gain (FP32 Parker, 8
+// units: 21.8 -> 23.0 ms from the smaller box; FP32 and FP64 focused, 10 and 21
+// units: within 1%).
+__host__ __device__ constexpr int stage_stride(int v) { return v % 16 == 0 ? v + 1 : v; }
+
 // The same time-blended sample from records staged in LDS: q points at the
 // lower-left corner's record, the upper row's records start `row` records on.
 template <int C>
 __device__ __forceinline__ void sample_staged(const float4* q, int row, double uw, double wx, double wy, float* c) {
-    constexpr int V = pair_stride(C) / 4;
+    constexpr int V = stage_stride(pair_stride(C) / 4);  // record stride in LDS
     constexpr int S = pair_components(C);
     const float fx = static_cast<float>(wx), fy = static_cast<float>(wy), fu = static_cast<float>(uw);
     const float gx = 1.0f - fx, gy = 1.0f - fy, gu = 1.0f - fu;
@@ -521,7 +529,7 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
     const float a01 = w01 * gu, b01 = w01 * fu, a11 = w11 * gu, b11 = w11 * fu;
     const float4* r = q + row * V;
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
+    for (int v = 0; v < pair_stride(C) / 4; ++v) {
         const float4 u0 = q[v], u1 = q[V + v], w0 = r[v], w1 = r[V + v];
         const int s0 = 2 * v;
         if (s0 < S)
@@ -537,9 +545,10 @@ __device__ __forceinline__ void sample_staged(const float4* q, int row, double u
 // FP64 time-blended sample from node-major (lower, upper) records: the same
 // weights and the same sum per frame as sample_blend over the component-planar
 // frames, so the result is the same bit for bit. q points at the lower-left
-// corner's record, the upper row's records start `row` records on (nx + 2 in
-// the pair in global memory, the box width in a staged box).
-template <int C>
+// corner's record; records are SV double2 apart (C in the pair in global
+// memory, stage_stride(C) in a staged box) and the upper row's records start
+// `row` records on (nx + 2 in global memory, the box width in a staged box).
+template <int C, int SV = C>
 __device__ __forceinline__ void sample_pair64(const double2* q, int64_t row, double uw, double wx, double wy,
                                               double* c) {
     const double w00 = (1.0 - wx) * (1.0 - wy);
@@ -547,10 +556,10 @@ __device__ __forceinline__ void sample_pair64(const double2* q, int64_t row, dou
     const double w01 = (1.0 - wx) * wy;
     const double w11 = wx * wy;
     const double lw = 1.0 - uw;
-    const double2* r = q + row * C;
+    const double2* r = q + row * SV;
 #pragma unroll
     for (int k = 0; k < C; ++k) {
-        const double2 u0 = q[k], u1 = q[C + k], v0 = r[k], v1 = r[C + k];
+        const double2 u0 = q[k], u1 = q[SV + k], v0 = r[k], v1 = r[SV + k];
         const double lo = w00 * u0.x + w10 * u1.x + w01 * v0.x + w11 * v1.x;
         const double hi = w00 * u0.y + w10 * u1.y + w01 * v0.y + w11 * v1.y;
         c[k] = lw * lo + uw * hi;
@@ -577,7 +586,8 @@ __device__ __forceinline__ void sample_pair64(const double2* q, int64_t row, dou
 // ---------------------------------------------------------------------------
 // LDS for staged records per CU (0: no staging), shared by the wavefronts the
 // kernel's occupancy puts on a CU: at 4 waves per SIMD, 16 wavefronts of
-// 9 600 B (60 focused records of 160 B); at 3, 12 of 12 800 B.
+// 9 600 B (60 FP32 focused records of 160 B); at 2 (FP64), 8 of 19 200 B (57
+// focused records of 336 B, 70 Parker records at their LDS stride of 272 B).
 #ifndef KPT_STAGE_CU_BYTES
 #define KPT_STAGE_CU_BYTES 153600
 #endif
@@ -652,8 +662,11 @@ __device__ __forceinline__ void stage_update(StageBox& box, float4* lds, const v
     const float4* src = static_cast<const float4*>(pair);
     for (int r = 0; r < h; ++r) {
         const float4* row = src + (static_cast<int64_t>(j_lo + r + 1) * nxp + (i_lo + 1)) * V;
-        float4* dst = lds + r * w * V;
-        for (int e = lane; e < w * V; e += 64) dst[e] = row[e];
+        float4* dst = lds + r * w * stage_stride(V);
+        for (int e = lane; e < w * V; e += 64) {
+            const int node = e / V;
+            dst[node * stage_stride(V) + (e - node * V)] = row[e];
+        }
     }
     box.i_lo = i_lo;
     box.j_lo = j_lo;
@@ -868,7 +881,7 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
                     if constexpr (STAGE != 0) {
                         if (stage && box.holds(i0, j0)) {
                             const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
-                            sample_staged<CP>(stage_lds + local * PV, box.w, uw, wx, wy, c);
+                            sample_staged<CP>(stage_lds + local * stage_stride(PV), box.w, uw, wx, wy, c);
                         } else if (a.field_pair != nullptr) {
                             sample_pair32<CP, KPT_PAIR_RECORDS_EXACT>(a.field_pair, n00, nxp, uw, wx, wy, c);
                         } else {
@@ -882,8 +895,9 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
                 } else {
                     if (STAGE != 0 && stage && box.holds(i0, j0)) {
                         const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
-                        sample_pair64<CP>(reinterpret_cast<const double2*>(stage_lds) + local * CP, box.w, uw, wx,
-                                          wy, c);
+                        sample_pair64<CP, stage_stride(CP)>(
+                            reinterpret_cast<const double2*>(stage_lds) + local * stage_stride(CP), box.w, uw, wx,
+                            wy, c);
                     } else if (a.field_pair64 != nullptr) {
                         sample_pair64<CP>(reinterpret_cast<const double2*>(a.field_pair64) + n00 * CP, nxp, uw, wx,
                                           wy, c);
@@ -1068,11 +1082,12 @@ __global__ void __launch_bounds__(kBlock)
         constexpr int V = sizeof(FT) == 4 ? pair_stride(MODEL) / 4 : MODEL;  // 16-byte units per record
         constexpr int waves = sizeof(FT) == 8 ? KPT_PUSH_WAVES_FP64
                                               : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER);
-        constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (V * 16);
-        __shared__ float4 stage_mem[kBlock / 64 * CAPN * V];
+        constexpr int SV = stage_stride(V);
+        constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (SV * 16);
+        __shared__ float4 stage_mem[kBlock / 64 * CAPN * SV];
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         push_lane<MODEL, FOCUSED, FT, CLOSURE, STR, CAPN>(kernel_args(), push_first_particle(run),
-                                                          stage_mem + wave * (CAPN * V));
+                                                          stage_mem + wave * (CAPN * SV));
     } else {
         push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(kernel_args(), push_first_particle(run));
     }
