@@ -250,7 +250,10 @@ typedef struct kpt_ctx kpt_ctx;
 
 /* Create a context on HIP device `device` with particle capacity `capacity` and
  * RNG seed `seed` (the reference's RandomManager(seed), include/RandomManager.hpp:28;
- * ReconnectionParticleStorage(cap), src/main.cpp:318-332). */
+ * ReconnectionParticleStorage(cap), src/main.cpp:318-332).
+ * The environment variable KPT_PUSH_STAGE=0, read here, turns off the staging of
+ * stencil records in LDS by the context's pushes (results are the same bit for
+ * bit; the switch is for A/B measurements). */
 kpt_status kpt_create(kpt_ctx** out, int device, uint64_t capacity, uint64_t seed);
 kpt_status kpt_destroy(kpt_ctx* ctx);
 const char* kpt_last_error(const kpt_ctx* ctx);

@@ -126,6 +126,15 @@ def test_training_reaches_exact_qlt(trained):
     err = h["out_mean"] + h["out_scale"] * y[:, 0] - ln_r
     assert np.sqrt(np.mean(err ** 2)) < 0.05
     assert np.sqrt(np.mean((y[:, 1] - y1) ** 2)) < 0.05
+    # what the training gains: the same network before training (its initial
+    # weights) misses ln R and its slope by an order of magnitude more
+    with kpt.Context(16) as c:
+        c.mlp_set(_random_net(1), hull)
+        y0 = c.mlp_infer(x, mfma=True)
+    err0 = h["out_mean"] + h["out_scale"] * y0[:, 0] - ln_r
+    print(f"rms error in ln R: trained {np.sqrt(np.mean(err ** 2)):.4f}, untrained {np.sqrt(np.mean(err0 ** 2)):.4f}")
+    assert np.sqrt(np.mean(err0 ** 2)) > 10 * np.sqrt(np.mean(err ** 2))
+    assert np.sqrt(np.mean((y0[:, 1] - y1) ** 2)) > 10 * np.sqrt(np.mean((y[:, 1] - y1) ** 2))
 
 
 def _qlt_case(n=32768, steps=400, seed=3):
@@ -203,7 +212,11 @@ def test_mlp_push_matches_oracle_and_reports_gain(trained):
     for k in ("mu", "p", "y"):
         assert stats.ks_2samp(want[k], got[k]).pvalue > 0.01, k
 
-    # A/B: the same FP32 push with the exact formula and with the network
+    # A/B: the same FP32 push with the exact formula and with the network. On
+    # MI355X the network (192 activations and 44 MFMA per 64 lanes per sub-step)
+    # costs more VALU issue than the closed form it replaces (two transcendentals
+    # and about 20 VALU per evaluation): the closure gains accuracy-preserving
+    # generality, not speed, and this A/B pins that.
     big = W.c3_particles(262144, s, seed=8)
     rates = {}
     for name, closure in (("exact", kpt.CLOSURE_QLT), ("mlp", kpt.CLOSURE_MLP)):
@@ -214,4 +227,4 @@ def test_mlp_push_matches_oracle_and_reports_gain(trained):
         rates[name] = st.particle_steps / (st.kernel_ms * 1e-3)
     print(f"FP32 push, exact QLT {rates['exact']:.3e} steps/s, learned closure {rates['mlp']:.3e} "
           f"steps/s, ratio {rates['mlp'] / rates['exact']:.3f}")
-    assert rates["mlp"] > 0 and rates["exact"] > 0
+    assert 0 < rates["mlp"] < rates["exact"]
