@@ -100,7 +100,9 @@ kpt_status ens_sync(kpt_ensemble* e) {
         a.P = arrays_at(e->P, e->off[r]);
         a.count = e->count[r];
         a.field_lo = a.field_hi = f.c32;
-        a.field_pair = nullptr;
+        a.field_pair = nullptr;  // planar frames: a persistent workgroup moves between runs
+        a.field_pair64 = nullptr;
+        a.stage = 0;
         a.field_lo64 = a.field_hi64 = f.c64;
         a.plane = kpt::coefficient_plane(f.g.nx, f.g.ny);
         a.g = f.g;
