@@ -760,7 +760,13 @@ __device__ __forceinline__ const PushArgs& args_reload(ArgsK p) {
 // waves 42.3 ms, 4 waves 44.7 ms per launch. With records staged in LDS (8 M
 // focused particles, exact closure): 3 waves 28.3 ms, 4 waves 25.3 ms, 5 waves
 // 28.1 ms per launch; FP32 Parker (4 M particles, adaptive dt): 3 waves 23.5 ms
-// (44.3 ms unstaged), 4 waves 21.7 ms, 5 waves 26.9 ms.
+// (44.3 ms unstaged), 4 waves 21.7 ms, 5 waves 26.9 ms. FP64 staged: focused 2
+// waves 79.6 ms, 3 waves 149.6 ms; Parker 2 waves 52.8 ms, 3 waves 84.5 ms (62.0
+// ms with its sample's loads issued two components at a time, 29 -> 12 VGPRs
+// spilled).
+#ifndef KPT_PUSH_WAVES_FP64_PARKER
+#define KPT_PUSH_WAVES_FP64_PARKER 2
+#endif
 #ifndef KPT_PUSH_WAVES_FP64
 #define KPT_PUSH_WAVES_FP64 2
 #endif
@@ -1073,14 +1079,14 @@ __device__ __forceinline__ int64_t push_first_particle(uint32_t run) {
 
 template <int MODEL, bool FOCUSED, typename FT, int CLOSURE, bool STR>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 8 ? KPT_PUSH_WAVES_FP64
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 8 ? (FOCUSED ? KPT_PUSH_WAVES_FP64 : KPT_PUSH_WAVES_FP64_PARKER)
                                                       : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED
                                                                  : KPT_PUSH_WAVES_FP32_PARKER))))
     push_kernel(PushArgs a, uint32_t run) {
     (void)a;
     if constexpr (KPT_STAGE_CU_BYTES > 0) {
         constexpr int V = sizeof(FT) == 4 ? pair_stride(MODEL) / 4 : MODEL;  // 16-byte units per record
-        constexpr int waves = sizeof(FT) == 8 ? KPT_PUSH_WAVES_FP64
+        constexpr int waves = sizeof(FT) == 8 ? (FOCUSED ? KPT_PUSH_WAVES_FP64 : KPT_PUSH_WAVES_FP64_PARKER)
                                               : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER);
         constexpr int SV = stage_stride(V);
         constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (SV * 16);
