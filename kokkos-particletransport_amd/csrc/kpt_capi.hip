@@ -558,6 +558,7 @@ kpt_status kpt_destroy(kpt_ctx* ctx) {
         dev_free(f.raw);
         dev_free(f.c64);
         dev_free(f.c32);
+        dev_free(f.rec32);
         dev_free(f.gen);
         dev_free(f.foc);
         for (int d = 0; d < 2; ++d) {
@@ -926,6 +927,22 @@ kpt_status kpt_inject(kpt_ctx* ctx, const kpt_device_settings* s, const kpt_inje
     return KPT_OK;
 }
 
+extern "C++" kpt_status kpt_capi::ensure_rec32(kpt_ctx* ctx, FieldSlot& f) {
+    if (f.rec32 != nullptr && f.rec32_version == f.version) return KPT_OK;
+    const uint64_t plane = kpt::coefficient_plane(f.g.nx, f.g.ny);
+    const size_t elems = kpt::rec_elems(plane, f.model);
+    if (elems > f.rec32_cap) {
+        dev_free(f.rec32);
+        f.rec32 = nullptr;
+        f.rec32_cap = 0;
+        KPT_HIP(ctx, dev_alloc(&f.rec32, elems));
+        f.rec32_cap = elems;
+    }
+    KPT_HIP(ctx, kpt::launch_rec(f.c32, plane, f.model, f.rec32, ctx->stream));
+    f.rec32_version = f.version;
+    return KPT_OK;
+}
+
 // The push's launch arguments for the interval [t0, t0 + mhd_dt) of `frame`
 // (per-chunk fields t_stop, first_chunk, counters and self are the caller's).
 // `paired`: a time-blended push reads the frames from the node-major pair
@@ -977,6 +994,12 @@ extern "C++" kpt_status kpt_capi::push_args(kpt_ctx* ctx, const kpt_device_setti
             ctx->pair_versions[1] = hi.version;
         }
         a.field_pair = ctx->pair32;
+    }
+    a.field_rec = nullptr;
+    if (paired && fp32 && s->time_interpolation == 0) {
+        st = ensure_rec32(ctx, ctx->slot[KPT_SLOT_LOWER]);
+        if (st != KPT_OK) return st;
+        a.field_rec = lo.rec32;
     }
     a.field_pair64 = nullptr;
     if (paired && !fp32 && s->time_interpolation != 0) {

@@ -22,6 +22,11 @@ struct FieldSlot {
     double* raw = nullptr;  // 6 planes of nx*ny (Bx, By, Bz, Vx, Vy, Vz)
     double* c64 = nullptr;
     float* c32 = nullptr;
+    // node-major records of the FP32 frame (static-field FP32 pushes), built
+    // from c32 when a push first needs them after an upload
+    float* rec32 = nullptr;
+    size_t rec32_cap = 0;
+    uint64_t rec32_version = 0;
     size_t nodes_cap = 0;
     size_t raw_cap = 0;
     uint64_t version = 0;  // changes with every upload into the slot
@@ -180,6 +185,8 @@ kpt_status check_properties(kpt_ctx* ctx, const kpt_particle_properties* pr);
 kpt_status upload_into(kpt_ctx* ctx, FieldSlot& f, int32_t model, int32_t nx, int32_t ny, const double* xe,
                        const double* ye, const double* const planes[6]);
 bool faces_uniform(const double* edges, size_t count);
+// The slot's node-major FP32 records (rec32), rebuilt when the slot changed.
+kpt_status ensure_rec32(kpt_ctx* ctx, FieldSlot& f);
 kpt_status push_args(kpt_ctx* ctx, const kpt_device_settings* s, double t0, uint32_t frame, bool paired,
                      kpt::PushArgs& a, bool& focused);
 

@@ -100,9 +100,14 @@ kpt_status ens_sync(kpt_ensemble* e) {
         a.P = arrays_at(e->P, e->off[r]);
         a.count = e->count[r];
         a.field_lo = a.field_hi = f.c32;
-        a.field_pair = nullptr;  // planar frames: a persistent workgroup moves between runs
+        a.field_pair = nullptr;  // one frame per run: node-major records, staged in LDS per wavefront
         a.field_pair64 = nullptr;
-        a.stage = 0;
+        {
+            const kpt_status rs = ensure_rec32(ctx, e->field[r]);
+            if (rs != KPT_OK) return rs;
+        }
+        a.field_rec = f.rec32;
+        a.stage = ctx->push_stage ? 1 : 0;
         a.field_lo64 = a.field_hi64 = f.c64;
         a.plane = kpt::coefficient_plane(f.g.nx, f.g.ny);
         a.g = f.g;
@@ -261,6 +266,7 @@ kpt_status kpt_ensemble_destroy(kpt_ensemble* e) {
         dev_free(f.raw);
         dev_free(f.c64);
         dev_free(f.c32);
+        dev_free(f.rec32);
         dev_free(f.gen);
         dev_free(f.foc);
     }

@@ -30,6 +30,10 @@ __host__ __device__ constexpr int pair_component(int model, int s) { return mode
 // Floats per node record of the FP32 frame pair: both frames' stored
 // components, rounded up to whole 16-byte loads.
 __host__ __device__ constexpr int pair_stride(int model) { return (2 * pair_components(model) + 3) / 4 * 4; }
+// Floats per node record of a single FP32 frame (static-field pushes and the
+// batched ensemble): the stored components of the pair, one frame, rounded up
+// to whole 16-byte loads.
+__host__ __device__ constexpr int rec_stride(int model) { return (pair_components(model) + 3) / 4 * 4; }
 // Doubles per node record of the FP64 frame pair: (lower, upper) of every
 // packed component, k at doubles 2k, 2k + 1 (one 16-byte load per component).
 __host__ __device__ constexpr int pair64_stride(int model) { return 2 * model; }
@@ -205,6 +209,7 @@ struct PushArgs {
     const void* field_hi;
     const float* field_pair;   // FP32 time-blended push: lower/upper interleaved, or nullptr
     const double* field_pair64;  // FP64 time-blended push: node-major (lower, upper) records, or nullptr
+    const float* field_rec;      // FP32 push of one frame: node-major records, or nullptr
     int32_t stage;             // 1: FP32 pushes stage field_pair records in LDS
     const double* field_lo64;  // FP64 frames, read by FP32 lanes below fp64_null_b
     const double* field_hi64;
@@ -270,6 +275,9 @@ hipError_t launch_generic_step(const ParticleArrays& P, uint64_t count, const do
 // frames (layout in kpt_kernels.hip, sample_pair32).
 uint64_t pair_elems(uint64_t plane, int model);
 hipError_t launch_pair(const float* lo, const float* hi, uint64_t plane, int model, float* out, hipStream_t stream);
+// Node-major records of one FP32 frame (rec_stride(model) floats per node).
+uint64_t rec_elems(uint64_t plane, int model);
+hipError_t launch_rec(const float* c32, uint64_t plane, int model, float* out, hipStream_t stream);
 // The FP64 frame pair (node-major records of pair64_stride(model) doubles).
 uint64_t pair64_elems(uint64_t plane, int model);
 hipError_t launch_pair64(const double* lo, const double* hi, uint64_t plane, int model, double* out,

@@ -508,6 +508,31 @@ __device__ __forceinline__ void sample_pair32(const float* __restrict__ pair, in
     if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
 }
 
+// Bilinear sample of one FP32 frame from node-major records (rec_stride(C)
+// floats of stored slots): the weights and the sum of sample_corners over the
+// component-planar frame, so the result is the same bit for bit. q points at the
+// lower-left corner's record; records are SV 16-byte units apart and the upper
+// row's records start `row` records on.
+template <int C, int SV>
+__device__ __forceinline__ void sample_rec32(const float4* q, int64_t row, double wx, double wy, float* c) {
+    constexpr int S = pair_components(C);
+    const float w00 = static_cast<float>((1.0 - wx) * (1.0 - wy));
+    const float w10 = static_cast<float>(wx * (1.0 - wy));
+    const float w01 = static_cast<float>((1.0 - wx) * wy);
+    const float w11 = static_cast<float>(wx * wy);
+    const float4* r = q + row * SV;
+#pragma unroll
+    for (int v = 0; v < rec_stride(C) / 4; ++v) {
+        const float4 u0 = q[v], u1 = q[SV + v], w0 = r[v], w1 = r[SV + v];
+        const float a0[4] = {u0.x, u0.y, u0.z, u0.w}, a1[4] = {u1.x, u1.y, u1.z, u1.w};
+        const float b0[4] = {w0.x, w0.y, w0.z, w0.w}, b1[4] = {w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * v + i < S) c[pair_component(C, 4 * v + i)] = w00 * a0[i] + w10 * a1[i] + w01 * b0[i] + w11 * b1[i];
+    }
+    if constexpr (C == 21) c[2] = 0.0f;  // Vz: not stored
+}
+
 // Staged records are stage_stride(V) 16-byte units apart in LDS (V units of
 // data). The FP64 Parker record (16 units, 256 B) starts every node on the same
 // LDS banks; one unit of padding spreads them (FP64 Parker push 62.3 -> 52.9 ms
@@ -823,8 +848,15 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
     // lanes take part in restaging); without it each lane leaves the loop when
     // its particle is done.
     StageBox box{0, 0, 0, 0, -1};
-    const void* pair = FP32 ? static_cast<const void*>(a.field_pair) : static_cast<const void*>(a.field_pair64);
+    // The node-major records the push samples: the frame pair of its precision, or
+    // (FP32, one frame) the frame's own records. STAGE: LDS bytes of the
+    // wavefront's region (0: no staging).
+    const bool recs = FP32 && a.field_pair == nullptr && a.field_rec != nullptr;
+    const void* pair = FP32 ? (recs ? static_cast<const void*>(a.field_rec) : static_cast<const void*>(a.field_pair))
+                            : static_cast<const void*>(a.field_pair64);
     constexpr int PV = FP32 ? pair_stride(CP) / 4 : CP;  // 16-byte units per pair record
+    constexpr int RV = rec_stride(CP) / 4;               // and per single-frame record
+    constexpr int CAP_P = STAGE / (stage_stride(PV) * 16), CAP_R = STAGE / (stage_stride(RV) * 16);
     const bool stage = STAGE != 0 && pair != nullptr && a.stage != 0;
 
     for (;;) {
@@ -871,7 +903,12 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
             }
         } while (false);
         if constexpr (STAGE != 0) {
-            if (stage) stage_update<PV, STAGE>(box, stage_lds, pair, a.g.nx, a.g.ny, run, i0, j0);
+            if (stage) {
+                if (recs)
+                    stage_update<RV, CAP_R>(box, stage_lds, pair, a.g.nx, a.g.ny, run, i0, j0);
+                else
+                    stage_update<PV, CAP_P>(box, stage_lds, pair, a.g.nx, a.g.ny, run, i0, j0);
+            }
         }
 
         // phase 2: sample, rates, closure, step (a failure ends the particle's
@@ -884,7 +921,16 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
             {
                 FT c[CP];
                 if constexpr (FP32) {
-                    if constexpr (STAGE != 0) {
+                    if (recs) {
+                        if (STAGE != 0 && stage && box.holds(i0, j0)) {
+                            const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
+                            sample_rec32<CP, stage_stride(RV)>(stage_lds + local * stage_stride(RV), box.w, wx, wy,
+                                                               c);
+                        } else {
+                            sample_rec32<CP, RV>(reinterpret_cast<const float4*>(a.field_rec) + n00 * RV, nxp, wx,
+                                                 wy, c);
+                        }
+                    } else if constexpr (STAGE != 0) {
                         if (stage && box.holds(i0, j0)) {
                             const int local = (j0 - box.j_lo) * box.w + (i0 - box.i_lo);
                             sample_staged<CP>(stage_lds + local * stage_stride(PV), box.w, uw, wx, wy, c);
@@ -1085,15 +1131,13 @@ __global__ void __launch_bounds__(kBlock)
     push_kernel(PushArgs a, uint32_t run) {
     (void)a;
     if constexpr (KPT_STAGE_CU_BYTES > 0) {
-        constexpr int V = sizeof(FT) == 4 ? pair_stride(MODEL) / 4 : MODEL;  // 16-byte units per record
         constexpr int waves = sizeof(FT) == 8 ? (FOCUSED ? KPT_PUSH_WAVES_FP64 : KPT_PUSH_WAVES_FP64_PARKER)
                                               : (FOCUSED ? KPT_PUSH_WAVES_FP32_FOCUSED : KPT_PUSH_WAVES_FP32_PARKER);
-        constexpr int SV = stage_stride(V);
-        constexpr int CAPN = KPT_STAGE_CU_BYTES / (4 * waves) / (SV * 16);
-        __shared__ float4 stage_mem[kBlock / 64 * CAPN * SV];
+        constexpr int BYTES = KPT_STAGE_CU_BYTES / (4 * waves) / 16 * 16;  // per wavefront
+        __shared__ float4 stage_mem[kBlock / 64 * (BYTES / 16)];
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        push_lane<MODEL, FOCUSED, FT, CLOSURE, STR, CAPN>(kernel_args(), push_first_particle(run),
-                                                          stage_mem + wave * (CAPN * SV));
+        push_lane<MODEL, FOCUSED, FT, CLOSURE, STR, BYTES>(kernel_args(), push_first_particle(run),
+                                                           stage_mem + wave * (BYTES / 16));
     } else {
         push_lane<MODEL, FOCUSED, FT, CLOSURE, STR>(kernel_args(), push_first_particle(run));
     }
@@ -1458,12 +1502,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
 #ifndef KPT_ENSEMBLE_WAVES
 #define KPT_ENSEMBLE_WAVES KPT_PUSH_WAVES_MLP
 #endif
+// LDS per wavefront for staged records of the Parker and exact-closure runs (0:
+// none). 3 840 B is what 3 workgroups per CU leave beside their network images
+// (60 Parker or 48 focused single-frame records); on the 64-run C5 workload it
+// measured 1.86e10 particle-steps/s against 2.23e10 without staging (the runs
+// read their node-major single-frame records from L1/L2, and each lane leaves
+// the sub-step loop when its particle is done).
+#ifndef KPT_ENSEMBLE_STAGE_BYTES
+#define KPT_ENSEMBLE_STAGE_BYTES 0
+#endif
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT_ENSEMBLE_WAVES)))
     ensemble_kernel(const EnsembleRun* __restrict__ runs, const uint2* __restrict__ tiles,
                     const uint32_t* __restrict__ group_start, int groups, unsigned int* __restrict__ queues,
                     const void* __restrict__ mlp_image, int has_mlp) {
     __shared__ uint4 mlp_smem[kMlpLdsBytes / 16];
     __shared__ uint32_t s_tile;
+    constexpr int SB = KPT_ENSEMBLE_STAGE_BYTES;
+    __shared__ float4 ens_stage[kBlock / 64 * (SB > 0 ? SB / 16 : 1)];
+    float4* const my_stage = ens_stage + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * (SB / 16);
     float b40 = 0.0f, b41 = 0.0f;
     MlpLds im{};
     if (has_mlp) im = mlp_stage(mlp_image, mlp_smem, b40, b41);
@@ -1485,9 +1541,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
         const int64_t idx = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(t.y)) + threadIdx.x;
         const EnsembleRun& R = runs[r];
         if (R.kind == kEnsParker32)
-            push_lane<16, false, float, KPT_CLOSURE_REDUCED>(as_args(R.a), idx);
+            push_lane<16, false, float, KPT_CLOSURE_REDUCED, false, SB>(as_args(R.a), idx, my_stage);
         else if (R.kind == kEnsFocusedQlt32)
-            push_lane<21, true, float, KPT_CLOSURE_QLT>(as_args(R.a), idx);
+            push_lane<21, true, float, KPT_CLOSURE_QLT, false, SB>(as_args(R.a), idx, my_stage);
         else if (R.a.mlp_act == kActHardSwish)
             push_lane_mlp<false, kActHardSwish, 1>(as_args(R.a), im, b40, b41, idx);
         else
@@ -1874,6 +1930,28 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(const float* __restrict__ 
 }
 
 uint64_t pair_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(pair_stride(model)); }
+
+// Single-frame FP32 records: float s of node n's record holds stored slot s
+// (packed component pair_component(model, s)); the floats up to the stride are 0.
+__global__ void __launch_bounds__(kBlock) rec_kernel(const float* __restrict__ c32, uint64_t plane, int model,
+                                                     float* __restrict__ out) {
+    const int stride = rec_stride(model);
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    if (e >= plane * stride) return;
+    const uint64_t n = e / stride;
+    const int j = static_cast<int>(e - n * stride);
+    out[e] = j < pair_components(model) ? c32[static_cast<uint64_t>(pair_component(model, j)) * plane + n] : 0.0f;
+}
+
+uint64_t rec_elems(uint64_t plane, int model) { return plane * static_cast<uint64_t>(rec_stride(model)); }
+
+hipError_t launch_rec(const float* c32, uint64_t plane, int model, float* out, hipStream_t stream) {
+    const uint64_t n = rec_elems(plane, model);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rec_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                       c32, plane, model, out);
+    return hipGetLastError();
+}
 
 // FP64 pair: double 2k + f of node n's record holds component k of the lower
 // (f = 0) or upper (f = 1) frame.
