@@ -1275,6 +1275,10 @@ __device__ __forceinline__ void push_lane_mlp(ArgsK ap, const MlpLds& im, const 
                 float c[CP];
                 if (a.field_pair != nullptr)
                     sample_pair32<CP, KPT_PAIR_RECORDS_MLP>(a.field_pair, n00, nxp, uw, wx, wy, c);
+                else if (a.field_rec != nullptr)
+                    sample_rec32<CP, rec_stride(CP) / 4>(reinterpret_cast<const float4*>(a.field_rec) +
+                                                             n00 * (rec_stride(CP) / 4),
+                                                         nxp, wx, wy, c);
                 else
                     sample_blend<CP, float>(static_cast<const float*>(a.field_lo), static_cast<const float*>(a.field_hi),
                                             blend, uw, n00, nxp, a.plane, wx, wy, c);
