@@ -844,13 +844,12 @@ __device__ __forceinline__ void push_lane(ArgsK ap, const int64_t idx, float4* s
         step_guard = a.first_chunk ? 0 : static_cast<int>(a.P.substep[idx]);
     }
     const double t_stop = a.t_stop;
-    // STAGE: the wavefront's staged node box (sub-steps run wave-uniformly, all
-    // lanes take part in restaging); without it each lane leaves the loop when
-    // its particle is done.
+    // STAGE > 0: the LDS bytes of the wavefront's region for its staged node box;
+    // sub-steps then run wave-uniformly (all lanes take part in restaging).
+    // STAGE = 0: no staging, and each lane leaves the loop when its particle is done.
     StageBox box{0, 0, 0, 0, -1};
     // The node-major records the push samples: the frame pair of its precision, or
-    // (FP32, one frame) the frame's own records. STAGE: LDS bytes of the
-    // wavefront's region (0: no staging).
+    // (FP32, one frame) the frame's own records.
     const bool recs = FP32 && a.field_pair == nullptr && a.field_rec != nullptr;
     const void* pair = FP32 ? (recs ? static_cast<const void*>(a.field_rec) : static_cast<const void*>(a.field_pair))
                             : static_cast<const void*>(a.field_pair64);
