@@ -87,6 +87,20 @@ def state_bytes(model):
     return 88 if model == kpt.MODEL_PARKER16 else 104
 
 
+def host_cpu():
+    """This host's CPU model and logical CPU count (the CPU baseline's machine)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0, seed=7):
     """Time the oracle (FP64 restatement of the reference push) on a bounded
     sample of the bench's own particles, using this host's cores. Returns the
@@ -127,6 +141,7 @@ def cpu_baseline(xe, ye, f0, f1, model, s, start, target_seconds=10.0, seed=7):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
+        "host": host_cpu(),
         "sample": f"{m} of the bench particles, one 0.1 interval ({steps} sub-steps, {dt:.1f} s), "
                   f"FP64 restatement of {what}, OpenMP",
     }
@@ -411,6 +426,7 @@ def run_generic(args):
             s_cpu += 1
             done += m
         out["cpu_baseline"] = {"value": done / t_cpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "host": host_cpu(),
                                "sample": f"{m} of the bench particles, {s_cpu} global-dt steps ({t_cpu:.1f} s), "
                                          "FP64 restatement of the solver's compute_adaptive_time_step and "
                                          "advance_stochastic, OpenMP over particles"}
@@ -524,6 +540,7 @@ def run_ensemble(args):
         share = [per_kind["parker"] / max(steps, 1), per_kind["focused"] / max(steps, 1)]
         rate = 1.0 / sum(w / b["value"] for w, b in zip(share, parts))  # the measured mix of sub-steps
         out["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": parts[0]["cores"],
+                               "host": host_cpu(),
                                "kind": "port",
                                "sample": f"the largest Parker run and the largest focused run (exact QLT), combined "
                                          f"with the measured shares {share[0]:.3f} / {share[1]:.3f} of sub-steps: "
