@@ -7,8 +7,9 @@
 // momentum_histogram_XXXXX.csv, particles_XXXXX.bin) and the same per-frame
 // order: [load upper frame] -> inject -> move -> split -> [summary row] ->
 // [histogram] -> [snapshot] -> [walltime check]. The push runs on the GPU
-// through kpt_push_interval; the options after --no-time-interp in --help
-// select this build's extensions (precision, chunked push, pitch-angle closure).
+// through kpt_push_interval; --precision, --push-chunks, the closure options,
+// --resumable-snapshots, --allow-stretched-fields, --synchronous-field-reads and
+// --device select this program's extensions.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -181,57 +182,72 @@ int snapshot_frame(const std::string& path) {
     return std::stoi(digits);
 }
 
+// --help: the option names are the reference driver's command line
+// (src/main.cpp:2444-2611); the grouping and descriptions are this program's.
+struct HelpEntry {
+    const char* option;
+    const char* text;
+};
+
 void print_help() {
-    std::cout
-        << "Usage: kpt_run [options]\n"
-        << "  --profile NAME                particle-64000(default), smoke, fortran-rank,\n"
-        << "                                fortran-global, or full\n"
-        << "  --transport MODEL             parker(default) or focused\n"
-        << "  --focused-transport           shorthand for --transport focused\n"
-        << "  --parker-transport            shorthand for --transport parker\n"
-        << "  --restart-particle-snapshot PATH\n"
-        << "                                load particles_XXXXX.bin and continue from frame XXXXX\n"
-        << "  --overwrite-output            allow a fresh non-restart run to replace existing outputs\n"
-        << "  --field-dir PATH              override profile field directory\n"
-        << "  --output-dir PATH             override profile output directory\n"
-        << "  --start-frame N               default 0\n"
-        << "  --end-frame N                 reads through this frame\n"
-        << "  --frames N                    sets end-frame = start-frame + N\n"
-        << "  --particles-per-frame N       override profile particle injection count\n"
-        << "  --rank-scale N                multiply particles per frame\n"
-        << "  --capacity N                  override profile particle capacity\n"
-        << "  --seed N                      default 114514\n"
-        << "  --dt-out X                    default 0.1\n"
-        << "  --diagnostic-interval N       profile default is usually 1\n"
-        << "  --histogram-interval N        0 writes final histogram only\n"
-        << "  --particle-snapshot-interval N\n"
-        << "                                0 disables particle binary snapshots\n"
-        << "  --max-steps-per-interval N    default 200000\n"
-        << "  --walltime-hours X            stop after X hours, disabled at 0\n"
-        << "  --walltime-seconds X          stop after X seconds, disabled at 0\n"
-        << "  --walltime-reserve-minutes X  subtract X minutes from the stop limit\n"
-        << "  --walltime-reserve-seconds X  subtract X seconds from the stop limit\n"
-        << "  --particle-v0 X               focused initial speed at p0\n"
-        << "  --duu0 X                      focused D_mumu normalization\n"
-        << "  --split-ratio X               split threshold ratio between split levels\n"
-        << "  --pmin-split-over-p0 X        first split threshold divided by p0\n"
-        << "  --no-split                    disable Fortran-style particle splitting\n"
-        << "  --no-time-interp              use the lower MHD frame only\n"
-        << "  --precision fp64|fp32         push arithmetic (default fp64)\n"
-        << "  --push-chunks N               cell-sorted time chunks per interval (default 8)\n"
-        << "  --closure reduced|qlt|mlp     focused pitch-angle closure (default reduced)\n"
-        << "  --qlt-sigma2 X --qlt-correlation-length X --qlt-spectral-index X\n"
-        << "                                slab-QLT turbulence (default 0.1, 0.02, 5/3)\n"
-        << "  --mlp-train-steps N           Adam steps of the learned closure (default 8000)\n"
-        << "  --mlp-activation silu|hard-swish\n"
-        << "                                hidden-layer activation of the learned closure (default hard-swish)\n"
-        << "  --resumable-snapshots         snapshots carry particle ids and split baselines,\n"
-        << "                                so a restart continues the run bit for bit\n"
-        << "  --allow-stretched-fields      accept non-uniform field faces (value k at face k,\n"
-        << "                                three-point derivatives, local cell widths in dt)\n"
-        << "  --synchronous-field-reads     read each frame between intervals instead of\n"
-        << "                                staging the next one during the push\n"
-        << "  --device N                    GPU ordinal (default 0)\n";
+    static const struct {
+        const char* title;
+        std::vector<HelpEntry> entries;
+    } groups[] = {
+        {"Run and transport",
+         {{"--profile NAME", "preset: particle-64000 (default), smoke, fortran-rank, fortran-global, full"},
+          {"--transport parker|focused", "transport equation (default parker)"},
+          {"--parker-transport", "same as --transport parker"},
+          {"--focused-transport", "same as --transport focused"},
+          {"--device N", "GPU ordinal (default 0)"}}},
+        {"Frames and files",
+         {{"--field-dir PATH", "compact-field frames (fieldXXXXX.bin) to read"},
+          {"--output-dir PATH", "where CSVs, run.log and snapshots go"},
+          {"--start-frame N", "first frame (default 0)"},
+          {"--end-frame N", "last frame read"},
+          {"--frames N", "end frame = start frame + N"},
+          {"--dt-out X", "time between frames (default 0.1)"},
+          {"--overwrite-output", "a fresh run may replace existing outputs"},
+          {"--restart-particle-snapshot PATH", "continue from particles_XXXXX.bin at frame XXXXX"},
+          {"--resumable-snapshots", "snapshots keep ids and split baselines (bitwise restarts)"},
+          {"--allow-stretched-fields", "accept non-uniform faces (local widths in dt)"},
+          {"--synchronous-field-reads", "read each frame between intervals, not during the push"}}},
+        {"Particles",
+         {{"--particles-per-frame N", "injected per frame (profile value otherwise)"},
+          {"--rank-scale N", "multiplies the per-frame injection"},
+          {"--capacity N", "particle storage (profile value otherwise)"},
+          {"--seed N", "random seed (default 114514)"},
+          {"--particle-v0 X", "focused transport: speed at p0"},
+          {"--duu0 X", "focused transport: D_mumu scale"},
+          {"--no-split", "no momentum splitting"},
+          {"--split-ratio X", "momentum ratio between split levels"},
+          {"--pmin-split-over-p0 X", "first split momentum, in units of p0"}}},
+        {"Push",
+         {{"--max-steps-per-interval N", "sub-step cap per frame interval (default 200000)"},
+          {"--no-time-interp", "sample the lower frame only"},
+          {"--precision fp64|fp32", "arithmetic of the push (default fp64)"},
+          {"--push-chunks N", "cell sorts per interval (default 8)"},
+          {"--closure reduced|qlt|mlp", "focused pitch-angle closure (default reduced)"},
+          {"--qlt-sigma2 X", "slab turbulence level (default 0.1)"},
+          {"--qlt-correlation-length X", "slab correlation length (default 0.02)"},
+          {"--qlt-spectral-index X", "slab spectral index (default 5/3)"},
+          {"--mlp-train-steps N", "training steps of the learned closure (default 8000)"},
+          {"--mlp-activation silu|hard-swish", "hidden activation of the learned closure (default hard-swish)"}}},
+        {"Diagnostics and limits",
+         {{"--diagnostic-interval N", "frames between summary rows"},
+          {"--histogram-interval N", "frames between momentum histograms (0: last frame only)"},
+          {"--particle-snapshot-interval N", "frames between snapshots (0: none)"},
+          {"--walltime-hours X", "stop after X hours (0: no limit)"},
+          {"--walltime-seconds X", "stop after X seconds (0: no limit)"},
+          {"--walltime-reserve-minutes X", "stop X minutes before the limit"},
+          {"--walltime-reserve-seconds X", "stop X seconds before the limit"}}},
+    };
+    std::cout << "kpt_run: reconnection calibration loop on the GPU push\n";
+    for (const auto& g : groups) {
+        std::cout << "\n" << g.title << ":\n";
+        for (const HelpEntry& e : g.entries)
+            std::cout << "  " << std::left << std::setw(36) << e.option << e.text << "\n";
+    }
 }
 
 // parse_settings (src/main.cpp:2444-2611)
