@@ -5,7 +5,9 @@ empty (1, 63, 65 and 129 particles). The FP64 pushes of ragged counts match the
 oracle to 1e-12 with injected noise; the FP32 pushes, whose loop runs
 wave-uniformly when the stencil records are staged in LDS (lanes past the count
 take part in the staging but not in the push), equal the unstaged push bit for
-bit."""
+bit; the learned-closure push, whose MFMAs take dummy features from empty
+lanes, gives each particle of a ragged set exactly its result in full
+wavefronts."""
 import os
 
 import numpy as np
@@ -144,3 +146,53 @@ def test_ragged_fp32_staged_equals_unstaged(n, model, focused, closure):
     assert len(a["x"]) == n
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.fixture(scope="module")
+def small_net():
+    """A briefly trained closure network per activation (its accuracy is not
+    under test here, only that a lane's result is independent of its wavefront)."""
+    nets = {}
+    for act in (kpt.MLP_SILU, kpt.MLP_HARD_SWISH):
+        o = kpt.mlp_default_train_opts()
+        o.samples = 1 << 16
+        o.steps = 200
+        o.hull.activation = act
+        with kpt.Context(16) as c:
+            c.mlp_train(o)
+            nets[act] = c.mlp_get()
+    return nets
+
+
+@pytest.mark.parametrize("act", [kpt.MLP_SILU, kpt.MLP_HARD_SWISH], ids=["silu", "hard-swish"])
+@pytest.mark.parametrize("grid", ["stretched", "uniform"])
+@pytest.mark.parametrize("n", [1, 65])
+def test_ragged_mlp_push_equals_full_wavefronts(small_net, act, grid, n):
+    """The learned-closure push runs its sub-steps wave-uniformly (the MFMAs need
+    all 64 lanes; lanes without a particle evaluate dummy features). Each
+    particle's trajectory depends only on its own state and id, so a ragged set
+    must give every particle exactly what it gets inside a set of full
+    wavefronts (matched by id after the push's cell sorts)."""
+    ye = W.stretched_edges(32, ratio=10.0) if grid == "stretched" else None
+    xe, ye, f0 = W.island_field(64, 32, seed=3, y_edges=ye)
+    _, _, f1 = W.island_field(64, 32, seed=4, y_edges=ye)
+    s = W.c3_device_settings(xe, ye, closure=kpt.CLOSURE_MLP, precision=kpt.PREC_FP32)
+    s.mhd_dt = 100 * s.dt_min
+    full = W.c3_particles(192, s, seed=8)
+    sub = {k: v[:n].copy() for k, v in full.items()}
+
+    def run(parts):
+        with kpt.Context(len(parts["x"]), seed=13) as c:
+            c.mlp_set(*small_net[act])
+            c.field_upload(kpt.SLOT_LOWER, 21, xe, ye, f0)
+            c.field_upload(kpt.SLOT_UPPER, 21, xe, ye, f1)
+            c.upload(parts)
+            st = c.push_interval(s, 0.0, 0)
+            return c.download(), st
+
+    a, sa = run(sub)
+    b, sb = run(full)
+    assert sa.active_in == n and sb.active_in == 192 and sa.particle_steps > 0 and sa.failures == 0
+    oa, ob = np.argsort(a["id"]), np.argsort(b["id"])
+    for k in a:
+        np.testing.assert_array_equal(a[k][oa], b[k][ob][:n], err_msg=k)
