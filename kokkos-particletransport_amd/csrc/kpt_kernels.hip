@@ -1487,7 +1487,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KPT
 // registers a call clobbers are the ones its caller spills around it, and a
 // callee built for the learned-closure kernel's own 168 cost that kernel 11% per
 // launch in spills.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4), used))
     near_null_budget_kernel(const PushArgs* a, StepTerms<float>* out) {
     (void)near_null_terms<21, true, true, 1>(a, true, 0.5, 0, a->g.nx + 2, 0.5, 0.5, 1.0, 0.5, out);
 }
