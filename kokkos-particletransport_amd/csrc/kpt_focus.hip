@@ -29,6 +29,29 @@ namespace kpt {
 namespace {
 
 constexpr int kBlk = 256;
+
+// Waves per SIMD the time-step and step kernels are built for (0: the
+// compiler's choice). Left to itself the compiler builds the time-step kernel
+// with 262 VGPRs, one wave per SIMD; held to 2 waves it takes 256 and a 28-byte
+// spill, and the solver step (4 M particles, 1024 x 512 island field) went
+// from 18.8 to 16.7 ms. Holding either kernel to 3 or 4 waves spills 228-568
+// bytes per lane and measured 1.5-1.9x slower.
+#ifndef KPT_FOCUS_DT_WAVES
+#define KPT_FOCUS_DT_WAVES 2
+#endif
+#ifndef KPT_FOCUS_STEP_WAVES
+#define KPT_FOCUS_STEP_WAVES 0
+#endif
+#if KPT_FOCUS_DT_WAVES > 0
+#define KPT_FOCUS_DT_ATTR __attribute__((amdgpu_waves_per_eu(KPT_FOCUS_DT_WAVES, KPT_FOCUS_DT_WAVES)))
+#else
+#define KPT_FOCUS_DT_ATTR
+#endif
+#if KPT_FOCUS_STEP_WAVES > 0
+#define KPT_FOCUS_STEP_ATTR __attribute__((amdgpu_waves_per_eu(KPT_FOCUS_STEP_WAVES, KPT_FOCUS_STEP_WAVES)))
+#else
+#define KPT_FOCUS_STEP_ATTR
+#endif
 constexpr int kFC = KPT_GENERIC_FOCUS_COMPONENTS;
 
 // Quantities the focused basis takes from B, V and dV/dt at one physical point.
@@ -277,7 +300,7 @@ __device__ __forceinline__ double focus_local_dt(const FocTerms& T) {
     return b < e ? b : e;
 }
 
-__global__ void __launch_bounds__(kBlk) focus_dt_kernel(ParticleArrays P, uint64_t count,
+__global__ void __launch_bounds__(kBlk) KPT_FOCUS_DT_ATTR focus_dt_kernel(ParticleArrays P, uint64_t count,
                                                         const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                         kpt_generic_focus_params p,
                                                         unsigned long long* __restrict__ min_bits) {
@@ -294,7 +317,7 @@ __global__ void __launch_bounds__(kBlk) focus_dt_kernel(ParticleArrays P, uint64
     block_min_to(m, min_bits);
 }
 
-__global__ void __launch_bounds__(kBlk) focus_step_kernel(ParticleArrays P, uint64_t count,
+__global__ void __launch_bounds__(kBlk) KPT_FOCUS_STEP_ATTR focus_step_kernel(ParticleArrays P, uint64_t count,
                                                           const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                           kpt_generic_focus_params p, double dt, uint32_t step,
                                                           uint32_t k0, uint32_t k1, const double* __restrict__ noise,

@@ -26,6 +26,27 @@ namespace kpt {
 namespace {
 
 constexpr int kBlk = 256;
+
+// Waves per SIMD the time-step and step kernels are built for (0: the
+// compiler's choice: 211 and 164 VGPRs, 2 and 3 waves). Held to 3 or 4 waves
+// they spill 148-300 bytes per lane and the solver step (4 M particles,
+// 1024 x 512 island field) measured 1.3-2.3x slower.
+#ifndef KPT_GENERIC_DT_WAVES
+#define KPT_GENERIC_DT_WAVES 0
+#endif
+#ifndef KPT_GENERIC_STEP_WAVES
+#define KPT_GENERIC_STEP_WAVES 0
+#endif
+#if KPT_GENERIC_DT_WAVES > 0
+#define KPT_GENERIC_DT_ATTR __attribute__((amdgpu_waves_per_eu(KPT_GENERIC_DT_WAVES, KPT_GENERIC_DT_WAVES)))
+#else
+#define KPT_GENERIC_DT_ATTR
+#endif
+#if KPT_GENERIC_STEP_WAVES > 0
+#define KPT_GENERIC_STEP_ATTR __attribute__((amdgpu_waves_per_eu(KPT_GENERIC_STEP_WAVES, KPT_GENERIC_STEP_WAVES)))
+#else
+#define KPT_GENERIC_STEP_ATTR
+#endif
 constexpr int kGC = KPT_GENERIC_PARKER_COMPONENTS;
 
 __global__ void __launch_bounds__(kBlk) generic_pack_kernel(const double* __restrict__ raw, GridDesc g,
@@ -142,7 +163,7 @@ __device__ __forceinline__ double generic_local_dt(const GenTerms& T) {
     return m < mdt ? m : mdt;
 }
 
-__global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint64_t count,
+__global__ void __launch_bounds__(kBlk) KPT_GENERIC_DT_ATTR generic_dt_kernel(ParticleArrays P, uint64_t count,
                                                           const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                           kpt_generic_parker_params p,
                                                           unsigned long long* __restrict__ min_bits) {
@@ -159,7 +180,7 @@ __global__ void __launch_bounds__(kBlk) generic_dt_kernel(ParticleArrays P, uint
     block_min_to(m, min_bits);
 }
 
-__global__ void __launch_bounds__(kBlk) generic_step_kernel(ParticleArrays P, uint64_t count,
+__global__ void __launch_bounds__(kBlk) KPT_GENERIC_STEP_ATTR generic_step_kernel(ParticleArrays P, uint64_t count,
                                                             const double* __restrict__ f, uint64_t plane, GridDesc g,
                                                             kpt_generic_parker_params p, double dt, uint32_t step,
                                                             uint32_t k0, uint32_t k1, const double* __restrict__ noise,
