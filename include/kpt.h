@@ -812,8 +812,8 @@ kpt_status kpt_generator_calls(const kpt_ctx* ctx, uint32_t* calls);
  *   D = (pi/4) Omega sigma2 C_gamma (1 - mu^2) R(|mu|, xi, gamma_k),
  *   R = xi mu_h^(gamma_k - 1) / (mu_h^gamma_k + xi^gamma_k),
  * network inputs (ln|mu|, ln xi, gamma_k), outputs (ln R - out_mean) / out_scale
- * and |mu| d ln R / d|mu|; 3 -> 64 -> 64 -> 64 -> 2 with hard-swish (default) or
- * SiLU hidden units (kpt_mlp_hull.activation). It is trained on
+ * and |mu| d ln R / d|mu|; 3 -> 64 -> 64 -> 64 -> 2 with SiLU (default) or
+ * hard-swish hidden units (kpt_mlp_hull.activation). It is trained on
  * the GPU (kpt_mlp_train) on exact-QLT samples and evaluated inside the FP32
  * focused push with BF16 MFMA and FP32 accumulation; samples outside the hull
  * (and every sample when the push's gamma_k or h0 differ from the hull's) take
@@ -834,7 +834,7 @@ typedef struct kpt_mlp_hull {
     double gamma_lo, gamma_hi;    /* gamma_k range                      1.5, 2        */
     double h0;                    /* resonance regularisation trained   1e-6          */
     double out_mean, out_scale;   /* ln R = out_mean + out_scale * y0 (set by training) */
-    int32_t activation;           /* KPT_MLP_SILU or KPT_MLP_HARD_SWISH (default)      */
+    int32_t activation;           /* KPT_MLP_SILU (default) or KPT_MLP_HARD_SWISH      */
     int32_t reserved;             /* 0                                                  */
 } kpt_mlp_hull;
 

@@ -61,7 +61,7 @@ struct RunSettings {
     double qlt_correlation_length{0.02};
     double qlt_spectral_index{5.0 / 3.0};
     uint32_t mlp_train_steps{8000};
-    int32_t mlp_activation{KPT_MLP_HARD_SWISH};
+    int32_t mlp_activation{KPT_MLP_SILU};
     int device{0};
     bool resumable_snapshots{false};
     bool allow_stretched_fields{false};
@@ -232,7 +232,7 @@ void print_help() {
           {"--qlt-correlation-length X", "slab correlation length (default 0.02)"},
           {"--qlt-spectral-index X", "slab spectral index (default 5/3)"},
           {"--mlp-train-steps N", "training steps of the learned closure (default 8000)"},
-          {"--mlp-activation silu|hard-swish", "hidden activation of the learned closure (default hard-swish)"}}},
+          {"--mlp-activation silu|hard-swish", "hidden activation of the learned closure (default silu)"}}},
         {"Diagnostics and limits",
          {{"--diagnostic-interval N", "frames between summary rows"},
           {"--histogram-interval N", "frames between momentum histograms (0: last frame only)"},

@@ -1760,7 +1760,7 @@ kpt_status kpt_mlp_default_train_opts(kpt_mlp_train_opts* o) {
     o->hull.h0 = 1.0e-6;
     o->hull.out_mean = 0.0;
     o->hull.out_scale = 1.0;
-    o->hull.activation = KPT_MLP_HARD_SWISH;
+    o->hull.activation = KPT_MLP_SILU;
     return KPT_OK;
 }
 
