@@ -16,14 +16,14 @@ the GPU before the timed region) or with the exact formula (--closure qlt). The
 same particles are then pushed by the FP32 exact-QLT push (when the measured
 path is the learned one) and by the plain FP64 exact-QLT push, for the ratios.
 
-Workload `parker`: the production Parker loop (C2) on the same field and grid
+Workload `parker`: the production Parker loop on the same field and grid
 as a drifting frame series, the driver's default physics with adaptive dt, FP64
 or FP32. One bench step is one frame of the reference's loop: the upper frame is
 uploaded, 1600 x 16 = 25 600 particles are injected, every stored particle is
 pushed through the interval (16 cell-sorted chunks), particles split at
 thresholds x1.2 per level from 2 p0, and the momentum histogram is taken every
 10 frames (the reference's production script, scripts/run_total_benchmark_suite_
-local.sh:132-134). The timed window starts at frame 160 (--c2-start-frame) with
+local.sh:132-134). The timed window starts at frame 160 (--loop-start-frame) with
 the 160 x 25 600 particles of the earlier frames stored, in their injected state
 (uniform in the domain at the window's start time, p = p0), so a short run
 measures the loop at a late-run particle count. --parker-push-only pushes 4 194 304 particles injected once instead.
@@ -438,11 +438,11 @@ def run_generic(args):
 
 def run_ensemble(args):
     """The batched-ensemble workload (see the module docstring)."""
-    specs = W.c5_runs(runs=args.runs or W.C5["runs"])
-    frames = W.C5["frames"]
-    settings = [W.c5_settings(spec) for spec in specs]
+    specs = W.ensemble_runs(runs=args.runs or W.ENSEMBLE_64X512X256["runs"])
+    frames = W.ENSEMBLE_64X512X256["frames"]
+    settings = [W.ensemble_run_settings(spec) for spec in specs]
     total = sum(spec["count"] for spec in specs)
-    log(f"bench: ensemble of {len(specs)} runs, {total} particles, {frames} x {W.C5['steps_per_interval']} "
+    log(f"bench: ensemble of {len(specs)} runs, {total} particles, {frames} x {W.ENSEMBLE_64X512X256['steps_per_interval']} "
         f"sub-steps per step")
     with kpt.Context(16, seed=114514) as ctx:
         log("training the D_mumu network on the GPU ...")
@@ -455,7 +455,7 @@ def run_ensemble(args):
                 model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
                 e.field_upload(r, model, spec["xe"], spec["ye"], spec["planes"])
                 e.set_run(r, s, spec["seed"])
-                e.upload(r, W.c5_particles(spec, s, frame=0))
+                e.upload(r, W.ensemble_run_particles(spec, s, frame=0))
             e.sort()
             chosen, table = e.autotune(0, window=0.2)
             log(f"  autotune: chosen {chosen}; tried {table}")
@@ -500,7 +500,7 @@ def run_ensemble(args):
     launches = args.steps * frames
     avg_launch_s = kms / 1e3 / launches
     out = {
-        "metric": METRIC,
+        "metric": "particle-steps/s, batched ensemble of independent runs (FP32, Parker and learned-closure focused)",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -511,12 +511,12 @@ def run_ensemble(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (64 seeded island-chain field variants; C1 Parker and C3 focused particle sets)",
+        "data": "synthetic (64 seeded island-chain field variants; Harris-sheet Parker and slab-QLT focused particle sets)",
         "config": {
             "workload": f"ensemble_{len(specs)}_runs_512x256_fp32_{total}",
             "runs": len(specs),
             "particles": total,
-            "sub_steps_per_step": frames * W.C5["steps_per_interval"],
+            "sub_steps_per_step": frames * W.ENSEMBLE_64X512X256["steps_per_interval"],
             "sub_steps": int(steps),
             "parker_sub_steps": int(per_kind["parker"]),
             "focused_mlp_sub_steps": int(per_kind["focused"]),
@@ -536,10 +536,10 @@ def run_ensemble(args):
         parts = []
         for focused in (False, True):
             spec = max((sp for sp in specs if sp["focused"] == focused), key=lambda sp: sp["count"])
-            s = W.c5_settings(spec)
+            s = W.ensemble_run_settings(spec)
             model = kpt.MODEL_FOCUSED21 if focused else kpt.MODEL_PARKER16
             parts.append(cpu_baseline(spec["xe"], spec["ye"], spec["planes"], spec["planes"], model, s,
-                                      W.c5_particles(spec, s, frame=0), target_seconds=6.0)[0])
+                                      W.ensemble_run_particles(spec, s, frame=0), target_seconds=6.0)[0])
         share = [per_kind["parker"] / max(steps, 1), per_kind["focused"] / max(steps, 1)]
         rate = 1.0 / sum(w / b["value"] for w, b in zip(share, parts))  # the measured mix of sub-steps
         out["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": parts[0]["cores"],
@@ -572,12 +572,12 @@ def run_push(args):
     _, _, f1 = W.island_field(args.nx, args.ny, seed=8, y_edges=y_edges)
     learned = focused and args.closure == "mlp"
     if focused:
-        n = args.particles or W.C4["particles"]
+        n = args.particles or W.FOCUSED_QLT_1024X512_8M["particles"]
         closure = kpt.CLOSURE_MLP if learned else kpt.CLOSURE_QLT
-        s = W.c3_device_settings(xe, ye, closure=closure,
+        s = W.focused_qlt_device_settings(xe, ye, closure=closure,
                                  precision=kpt.PREC_FP32 if fp32 else kpt.PREC_FP64)
         s.mlp_passes = args.mlp_passes
-        start = W.c3_particles(n, s, seed=11)
+        start = W.focused_qlt_particles(n, s, seed=11)
     else:
         n = args.particles or 4 * 1024 * 1024
         h = kpt.default_host_settings()
@@ -724,11 +724,11 @@ def run_push(args):
     print(json.dumps(out), flush=True)
 
 
-C2_PER_FRAME = 1600 * 16  # particles per frame times rank scale
-C2_CAPACITY = 16_000_000
+PARKER_LOOP_PER_FRAME = 1600 * 16  # particles per frame times rank scale
+PARKER_LOOP_CAPACITY = 16_000_000
 
 
-def c2_stored_particles(n, s, frame, seed=160):
+def parker_loop_stored_particles(n, s, frame, seed=160):
     """The stored particles at the start of `frame`: n particles in the state the
     driver injects them in (src/main.cpp:995-1048), uniform in the domain, p = p0,
     weight 1, at the frame's start time."""
@@ -745,13 +745,13 @@ def c2_stored_particles(n, s, frame, seed=160):
     return parts
 
 
-def run_c2(args):
-    """The production Parker loop (C2): a window of frames of the frame loop
+def run_parker_loop(args):
+    """The production Parker loop: a window of frames of the frame loop
     starting from a late-run particle count (see the module docstring)."""
     prec = args.precision or "fp64"
     fp32 = prec == "fp32"
-    f0 = args.c2_start_frame
-    n0 = f0 * C2_PER_FRAME
+    f0 = args.loop_start_frame
+    n0 = f0 * PARKER_LOOP_PER_FRAME
     stretched = args.grid == "stretched"
     y_edges = W.stretched_edges(args.ny, ratio=args.stretch_ratio) if stretched else None
     nframes = args.warmup + args.steps + 1
@@ -762,12 +762,12 @@ def run_c2(args):
     s = kpt.make_device_settings(h, xe, ye)
     s.precision = kpt.PREC_FP32 if fp32 else kpt.PREC_FP64
     s.push_chunks = args.chunks
-    log(f"bench: C2 loop {args.nx}x{args.ny} {args.grid}, frames {f0}..{f0 + nframes - 2} from {n0} stored "
-        f"particles, {C2_PER_FRAME} injected per frame, {prec}")
-    with kpt.Context(C2_CAPACITY, seed=114514) as ctx:
+    log(f"bench: Parker loop {args.nx}x{args.ny} {args.grid}, frames {f0}..{f0 + nframes - 2} from {n0} stored "
+        f"particles, {PARKER_LOOP_PER_FRAME} injected per frame, {prec}")
+    with kpt.Context(PARKER_LOOP_CAPACITY, seed=114514) as ctx:
         ctx.field_upload(kpt.SLOT_LOWER, kpt.MODEL_PARKER16, *fields[0])
         ctx.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, *fields[1])
-        ctx.upload(c2_stored_particles(n0, s, f0))
+        ctx.upload(parker_loop_stored_particles(n0, s, f0))
         start = ctx.download()
         totals = {"sub_steps": 0, "failures": 0, "split": 0, "skipped": 0, "injected": 0, "histograms": 0}
         kernel_ms, launches, active_in = 0.0, 0, []
@@ -780,7 +780,7 @@ def run_c2(args):
                 ctx.field_advance()
                 ctx.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, *fields[k + 1])
             t0 = f * s.mhd_dt
-            inj = ctx.inject(s, C2_PER_FRAME, t0, f)
+            inj = ctx.inject(s, PARKER_LOOP_PER_FRAME, t0, f)
             st = ctx.push_interval(s, t0, f)
             split, skipped = ctx.split(s)
             if (f + 1) % 10 == 0:
@@ -805,9 +805,9 @@ def run_c2(args):
     launch_bytes = totals["sub_steps"] / launches * per_step + float(np.mean(active_in)) * \
         state_bytes(kpt.MODEL_PARKER16)
     grid_tag = "_stretched" if stretched else ""
-    workload = f"c2_parker_loop_{prec}_{args.nx}x{args.ny}{grid_tag}_island_from_{n0}"
+    workload = f"parker_loop_{prec}_{args.nx}x{args.ny}{grid_tag}_island_from_{n0}"
     out = {
-        "metric": f"particle-steps/s, C2 production Parker loop, {prec.upper()}",
+        "metric": f"particle-steps/s, production Parker loop, {prec.upper()}",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -828,8 +828,8 @@ def run_c2(args):
                       f"injection, push, split, histogram every 10 frames)",
             "stored_at_start": n0,
             "stored_at_end": int(stored),
-            "injected_per_frame": C2_PER_FRAME,
-            "capacity": C2_CAPACITY,
+            "injected_per_frame": PARKER_LOOP_PER_FRAME,
+            "capacity": PARKER_LOOP_CAPACITY,
             "split_ratio": h.split_ratio,
             "pmin_split_over_p0": h.pmin_split_over_p0,
             "push_chunks": args.chunks,
@@ -862,7 +862,7 @@ def main():
     ap.add_argument("--solver", choices=["parker", "focused"], default="parker",
                     help="generic workload: ParkerSolver or FocusTransportSolver")
     ap.add_argument("--runs", type=int, default=0, help="ensemble: runs (0: 64)")
-    ap.add_argument("--c2-start-frame", type=int, default=160,
+    ap.add_argument("--loop-start-frame", type=int, default=160,
                     help="parker: first frame of the timed window (stored particles = frames x 25 600)")
     ap.add_argument("--parker-push-only", action="store_true",
                     help="parker: push 4 194 304 particles injected once, without the frame loop")
@@ -913,7 +913,7 @@ def main():
         run_generic(args)
         return
     if args.workload == "parker" and not args.parker_push_only:
-        run_c2(args)
+        run_parker_loop(args)
         return
     run_push(args)
 

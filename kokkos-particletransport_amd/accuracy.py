@@ -1,6 +1,6 @@
 """Two-sample statistics of pushed particle sets (numpy and scipy only).
 
-An FP32 push is judged against the FP64 oracle statistically: the two runs start
+An FP32 push is compared with the FP64 oracle statistically: the two runs start
 from one particle set with independent noise, and their final distributions and
 per-particle displacements over the interval are compared with two-sample
 chi-square and Kolmogorov-Smirnov tests. Both runs share the start set, so their

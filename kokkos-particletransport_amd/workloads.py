@@ -76,7 +76,7 @@ def island_field(nx=1024, ny=512, x_range=(-1.0, 1.0), y_range=(-0.5, 0.5), widt
     return xe, ye, [Bx, By, Bz, Vx, Vy, Vz]
 
 
-def c1_host_settings() -> "kpt.HostSettings":
+def parker_harris_host_settings() -> "kpt.HostSettings":
     """Parker transport, plain FP64, small: kappa_par = 0.01 p^(4/3), kperp/kpar = 0.05,
     fixed dt = 1e-4 over one 0.1 interval (1000 steps), no splitting."""
     h = kpt.default_host_settings()
@@ -96,29 +96,30 @@ def c1_host_settings() -> "kpt.HostSettings":
     return h
 
 
-C1 = dict(nx=256, ny=128, particles=65536, region_y=0.1, hist_bins=64, hist_pmin=0.1,
+PARKER_HARRIS_256X128 = dict(nx=256, ny=128, particles=65536, region_y=0.1, hist_bins=64, hist_pmin=0.1,
           hist_pmax=1.0e3, xy_bins=(64, 32))
 
 
-def c1_inject_opts() -> "kpt.InjectOpts":
+def parker_harris_inject_opts() -> "kpt.InjectOpts":
     o = kpt.InjectOpts()
     o.use_region = 1
     o.fixed_time = 1
     o.region_x_lo, o.region_x_hi = -1.0, 1.0
-    o.region_y_lo, o.region_y_hi = -C1["region_y"], C1["region_y"]
+    o.region_y_lo, o.region_y_hi = -PARKER_HARRIS_256X128["region_y"], PARKER_HARRIS_256X128["region_y"]
     o.p_init = 1.0
     return o
 
 
-def c3_host_settings() -> "kpt.HostSettings":
+def focused_qlt_host_settings() -> "kpt.HostSettings":
     """Focused transport on the 1024 x 512 island field: reflecting |mu| <= 1,
     fixed dt = 5e-5 over one 0.1 interval (2000 sub-steps), no splitting.
     The focused equation carries parallel transport explicitly (v mu b plus
     pitch-angle scattering), so only kappa_perp enters; it is held constant at
     kpara0 * kperp/kpar = 0.01 * 0.05 = 5e-4: the field-line random walk that
     sets kappa_perp scales with v, which is within 30% of c over p in [1, 1e3]
-    (the C1 Parker scaling p^(4/3) would give kappa_perp = 5 at p = 1e3)."""
-    h = c1_host_settings()
+    (the small Harris-sheet Parker scaling p^(4/3) would give kappa_perp = 5 at
+    p = 1e3)."""
+    h = parker_harris_host_settings()
     h.focused_transport = 1
     h.momentum_dependency = 0
     h.dt_min_rel = 5.0e-4
@@ -129,29 +130,29 @@ def c3_host_settings() -> "kpt.HostSettings":
     return h
 
 
-C3 = dict(nx=1024, ny=512, particles=4 * 1024 * 1024, p_range=(1.0, 1.0e3), qlt_sigma2=0.1,
+FOCUSED_QLT_1024X512 = dict(nx=1024, ny=512, particles=4 * 1024 * 1024, p_range=(1.0, 1.0e3), qlt_sigma2=0.1,
           qlt_correlation_length=0.02, qlt_spectral_index=5.0 / 3.0, qlt_h0=1.0e-6,
           hist_bins=64, mu_bins=32)
-C4 = dict(C3, particles=8 * 1024 * 1024)
+FOCUSED_QLT_1024X512_8M = dict(FOCUSED_QLT_1024X512, particles=8 * 1024 * 1024)
 
 
-def c3_device_settings(xe, ye, closure=None, precision=None) -> "kpt.DeviceSettings":
+def focused_qlt_device_settings(xe, ye, closure=None, precision=None) -> "kpt.DeviceSettings":
     """Slab-QLT scattering: gamma_k = 5/3, sigma2 = 0.1, Lc = 0.02, h0 = 1e-6
     (PitchAngleScatteringParameters, include/LegencyModel.hpp:196-198)."""
-    s = kpt.make_device_settings(c3_host_settings(), xe, ye)
+    s = kpt.make_device_settings(focused_qlt_host_settings(), xe, ye)
     s.closure = kpt.CLOSURE_QLT if closure is None else closure
     s.precision = kpt.PREC_FP64 if precision is None else precision
-    s.qlt_sigma2 = C3["qlt_sigma2"]
-    s.qlt_correlation_length = C3["qlt_correlation_length"]
-    s.qlt_spectral_index = C3["qlt_spectral_index"]
-    s.qlt_h0 = C3["qlt_h0"]
+    s.qlt_sigma2 = FOCUSED_QLT_1024X512["qlt_sigma2"]
+    s.qlt_correlation_length = FOCUSED_QLT_1024X512["qlt_correlation_length"]
+    s.qlt_spectral_index = FOCUSED_QLT_1024X512["qlt_spectral_index"]
+    s.qlt_h0 = FOCUSED_QLT_1024X512["qlt_h0"]
     return s
 
 
-def c3_particles(n, s, seed=11, p_range=None) -> dict:
+def focused_qlt_particles(n, s, seed=11, p_range=None) -> dict:
     """(x, y) uniform on the domain, mu ~ U[-1, 1], p log-uniform on p_range,
     t = 0, weight 1, ids 0..n-1."""
-    lo, hi = p_range or C3["p_range"]
+    lo, hi = p_range or FOCUSED_QLT_1024X512["p_range"]
     rng = np.random.default_rng(seed)
     parts = kpt.empty_particles(n)
     parts["x"][:] = rng.uniform(s.x_lower, s.x_upper, n)
@@ -167,25 +168,25 @@ def c3_particles(n, s, seed=11, p_range=None) -> dict:
     return parts
 
 
-C5 = dict(runs=64, nx=512, ny=256, count_range=(1.0e3, 1.0e6), bz_range=(0.0, 0.5), ratio_range=(0.01, 0.2),
+ENSEMBLE_64X512X256 = dict(runs=64, nx=512, ny=256, count_range=(1.0e3, 1.0e6), bz_range=(0.0, 0.5), ratio_range=(0.01, 0.2),
           inflow_range=(0.05, 0.15), steps_per_interval=50, frames=10)
 
 
-def c5_runs(seed=2024, runs=None, nx=None, ny=None, count_range=None) -> list:
+def ensemble_runs(seed=2024, runs=None, nx=None, ny=None, count_range=None) -> list:
     """The batched-ensemble workload: per run a seeded island-chain field variant
     (guide field Bz ~ U[0, 0.5], inflow ~ U[0.05, 0.15]), kperp/kpar ~ U[0.01, 0.2],
     a particle count log-uniform on count_range; even runs Parker, odd runs
     focused transport with the learned closure."""
-    runs = C5["runs"] if runs is None else runs
-    nx = C5["nx"] if nx is None else nx
-    ny = C5["ny"] if ny is None else ny
-    lo, hi = C5["count_range"] if count_range is None else count_range
+    runs = ENSEMBLE_64X512X256["runs"] if runs is None else runs
+    nx = ENSEMBLE_64X512X256["nx"] if nx is None else nx
+    ny = ENSEMBLE_64X512X256["ny"] if ny is None else ny
+    lo, hi = ENSEMBLE_64X512X256["count_range"] if count_range is None else count_range
     rng = np.random.default_rng(seed)
     out = []
     for r in range(runs):
-        bz = rng.uniform(*C5["bz_range"])
-        ratio = rng.uniform(*C5["ratio_range"])
-        inflow = rng.uniform(*C5["inflow_range"])
+        bz = rng.uniform(*ENSEMBLE_64X512X256["bz_range"])
+        ratio = rng.uniform(*ENSEMBLE_64X512X256["ratio_range"])
+        inflow = rng.uniform(*ENSEMBLE_64X512X256["inflow_range"])
         count = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
         xe, ye, planes = island_field(nx, ny, bz=bz, inflow=inflow, seed=seed * 1000 + r)
         out.append(dict(run=r, bz=bz, kperp_over_kpara=ratio, inflow=inflow, count=count, focused=r % 2 == 1,
@@ -193,17 +194,17 @@ def c5_runs(seed=2024, runs=None, nx=None, ny=None, count_range=None) -> list:
     return out
 
 
-def c5_settings(spec, steps_per_interval=None, closure=None) -> "kpt.DeviceSettings":
-    """FP32 settings of one ensemble run: the C1 Parker physics or the C3 focused
+def ensemble_run_settings(spec, steps_per_interval=None, closure=None) -> "kpt.DeviceSettings":
+    """FP32 settings of one ensemble run: the small Harris-sheet Parker physics or the focused
     QLT physics with the run's kperp/kpar, fixed dt = mhd_dt / steps_per_interval,
     one frame (no time interpolation)."""
-    n = C5["steps_per_interval"] if steps_per_interval is None else steps_per_interval
-    h = c3_host_settings() if spec["focused"] else c1_host_settings()
+    n = ENSEMBLE_64X512X256["steps_per_interval"] if steps_per_interval is None else steps_per_interval
+    h = focused_qlt_host_settings() if spec["focused"] else parker_harris_host_settings()
     h.kperp_over_kpara = spec["kperp_over_kpara"]
     h.dt_min_rel = h.dt_max_rel = 1.0 / n
     h.time_interpolation = 0
     if spec["focused"]:
-        s = c3_device_settings(spec["xe"], spec["ye"], closure=kpt.CLOSURE_MLP if closure is None else closure,
+        s = focused_qlt_device_settings(spec["xe"], spec["ye"], closure=kpt.CLOSURE_MLP if closure is None else closure,
                                precision=kpt.PREC_FP32)
         s2 = kpt.make_device_settings(h, spec["xe"], spec["ye"])
         for k in ("kperp_over_kpara", "dt_min", "dt_max", "time_interpolation", "kpara0"):
@@ -215,12 +216,12 @@ def c5_settings(spec, steps_per_interval=None, closure=None) -> "kpt.DeviceSetti
     return s
 
 
-def c5_particles(spec, s, frame=0) -> dict:
+def ensemble_run_particles(spec, s, frame=0) -> dict:
     """The run's particles at the start of interval `frame`: Parker runs at p = 1
-    in |y| < 0.1, focused runs with the C3 distribution."""
+    in |y| < 0.1, focused runs with the focused slab-QLT distribution."""
     n = spec["count"]
     if spec["focused"]:
-        parts = c3_particles(n, s, seed=spec["seed"])
+        parts = focused_qlt_particles(n, s, seed=spec["seed"])
     else:
         rng = np.random.default_rng(spec["seed"])
         parts = kpt.empty_particles(n)

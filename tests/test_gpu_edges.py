@@ -176,9 +176,9 @@ def test_ragged_mlp_push_equals_full_wavefronts(small_net, act, grid, n):
     ye = W.stretched_edges(32, ratio=10.0) if grid == "stretched" else None
     xe, ye, f0 = W.island_field(64, 32, seed=3, y_edges=ye)
     _, _, f1 = W.island_field(64, 32, seed=4, y_edges=ye)
-    s = W.c3_device_settings(xe, ye, closure=kpt.CLOSURE_MLP, precision=kpt.PREC_FP32)
+    s = W.focused_qlt_device_settings(xe, ye, closure=kpt.CLOSURE_MLP, precision=kpt.PREC_FP32)
     s.mhd_dt = 100 * s.dt_min
-    full = W.c3_particles(192, s, seed=8)
+    full = W.focused_qlt_particles(192, s, seed=8)
     sub = {k: v[:n].copy() for k, v in full.items()}
 
     def run(parts):

@@ -32,7 +32,7 @@ def _params():
 
 
 def _specs(nx=48, ny=24):
-    specs = W.c5_runs(seed=9, runs=len(COUNTS), nx=nx, ny=ny)
+    specs = W.ensemble_runs(seed=9, runs=len(COUNTS), nx=nx, ny=ny)
     for spec, n, kind in zip(specs, COUNTS, KINDS):
         spec["count"] = n
         spec["focused"] = kind != "parker"
@@ -42,7 +42,7 @@ def _specs(nx=48, ny=24):
 
 def _settings(spec):
     closure = {"parker": None, "qlt": kpt.CLOSURE_QLT, "mlp": kpt.CLOSURE_MLP}[spec["kind"]]
-    return W.c5_settings(spec, steps_per_interval=20, closure=closure)
+    return W.ensemble_run_settings(spec, steps_per_interval=20, closure=closure)
 
 
 def _build(c, specs):
@@ -52,13 +52,13 @@ def _build(c, specs):
         model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
         e.field_upload(r, model, spec["xe"], spec["ye"], spec["planes"])
         e.set_run(r, s, spec["seed"])
-        e.upload(r, W.c5_particles(spec, s, frame=FRAME))
+        e.upload(r, W.ensemble_run_particles(spec, s, frame=FRAME))
     return e
 
 
 def _alone(spec, act=kpt.MLP_SILU):
     s = _settings(spec)
-    parts = W.c5_particles(spec, s, frame=FRAME)
+    parts = W.ensemble_run_particles(spec, s, frame=FRAME)
     model = kpt.MODEL_FOCUSED21 if spec["focused"] else kpt.MODEL_PARKER16
     with kpt.Context(max(spec["count"], 1), seed=spec["seed"]) as c:
         if spec["kind"] == "mlp":

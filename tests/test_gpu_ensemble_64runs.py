@@ -1,6 +1,6 @@
-"""The batched ensemble at the C5 workload's shape, against the FP64 oracle.
+"""The batched ensemble at the 64-run workload's shape, against the FP64 oracle.
 
-C5: 64 independent runs, each on its own seeded 512 x 256 island-chain field
+Workload: 64 independent runs, each on its own seeded 512 x 256 island-chain field
 variant (guide field, inflow and kperp/kpar drawn per run), particle counts
 log-uniform on [1e3, 1e6], even runs Parker and odd runs focused transport with
 the learned closure, FP32, a 500-sub-step frame loop (10 intervals of 50 fixed
@@ -8,11 +8,11 @@ steps) with a cell sort of every run before each interval and the interval
 replayed from a hipGraph, as the bench runs it.
 
 1. Each run kind (Parker FP32, focused FP32 with the exact slab-QLT closure,
-   focused FP32 with a network trained on the GPU) on C5 field variants through
+   focused FP32 with a network trained on the GPU) on such field variants through
    that loop, against FP64 oracle runs of the same particles: with the run's own
    seed the median particle follows the oracle's trajectory; with another seed
    the displacement statistics match (KS and chi-square, family-wise level 0.01).
-2. The full C5 shape: every run's sub-step count equals its particle count times
+2. The full 64-run shape: every run's sub-step count equals its particle count times
    the sub-steps one particle takes over the loop (the interval arithmetic
    replayed in double), no failures, every particle at the loop's end time,
    each run's ids a permutation of its start ids, and each run's momentum
@@ -27,8 +27,8 @@ import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-FRAMES = W.C5["frames"]
-STEPS = W.C5["steps_per_interval"]
+FRAMES = W.ENSEMBLE_64X512X256["frames"]
+STEPS = W.ENSEMBLE_64X512X256["steps_per_interval"]
 
 
 @pytest.fixture(scope="module")
@@ -70,15 +70,15 @@ def _oracle(spec, s, start, seed, frames=FRAMES):
     return parts, steps
 
 
-def test_c5_run_kinds_match_oracle(network):
-    specs = W.c5_runs(seed=31, runs=3)
+def test_ensemble_run_kinds_match_oracle(network):
+    specs = W.ensemble_runs(seed=31, runs=3)
     kinds = ["parker", "qlt", "mlp"]
     for spec, kind in zip(specs, kinds):
         spec["count"] = 20000
         spec["focused"] = kind != "parker"
-    settings = [W.c5_settings(spec, closure={"parker": None, "qlt": kpt.CLOSURE_QLT, "mlp": kpt.CLOSURE_MLP}[kind])
+    settings = [W.ensemble_run_settings(spec, closure={"parker": None, "qlt": kpt.CLOSURE_QLT, "mlp": kpt.CLOSURE_MLP}[kind])
                 for spec, kind in zip(specs, kinds)]
-    starts = [W.c5_particles(spec, s, frame=0) for spec, s in zip(specs, settings)]
+    starts = [W.ensemble_run_particles(spec, s, frame=0) for spec, s in zip(specs, settings)]
     with kpt.Context(16, seed=1) as c:
         c.mlp_set(*network)
         with kpt.Ensemble(c, [spec["count"] for spec in specs]) as e:
@@ -120,10 +120,10 @@ def _sub_steps_per_particle(t_start, mhd_dt, dt, frames):
     return n, t
 
 
-def test_c5_full_shape_properties(network):
-    specs = W.c5_runs()
-    settings = [W.c5_settings(spec) for spec in specs]
-    starts = [W.c5_particles(spec, s, frame=0) for spec, s in zip(specs, settings)]
+def test_ensemble_64_runs_properties(network):
+    specs = W.ensemble_runs()
+    settings = [W.ensemble_run_settings(spec) for spec in specs]
+    starts = [W.ensemble_run_particles(spec, s, frame=0) for spec, s in zip(specs, settings)]
     total = sum(spec["count"] for spec in specs)
     assert len(specs) == 64 and total > 1_000_000
     with kpt.Context(16, seed=2) as c:

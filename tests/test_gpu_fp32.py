@@ -43,15 +43,15 @@ def _pathwise(a, b, s, keys=("x", "y", "p", "mu")):
     return out
 
 
-def test_fp32_c1_spectrum_and_density_match_oracle():
-    """Parker C1 (256 x 128 Harris sheet, 65 536 particles, 1000 fixed steps):
-    FP32 push against the FP64 oracle. Independent seeds: 64 x 32 spatial density
+def test_fp32_parker_harris_spectrum_and_density_match_oracle():
+    """Parker transport on the small Harris sheet (256 x 128, 65 536 particles,
+    1000 fixed steps): FP32 push against the FP64 oracle. Independent seeds: 64 x 32 spatial density
     and the 64-bin log-p spectrum chi-square p > 0.01, KS on y and p > 0.01.
     Same seed: the per-particle error stays small (median / 99th percentile)."""
-    cfg = W.C1
+    cfg = W.PARKER_HARRIS_256X128
     xe, ye, planes = W.harris_field(cfg["nx"], cfg["ny"])
-    s = kpt.make_device_settings(W.c1_host_settings(), xe, ye)
-    o = W.c1_inject_opts()
+    s = kpt.make_device_settings(W.parker_harris_host_settings(), xe, ye)
+    o = W.parker_harris_inject_opts()
     n = cfg["particles"]
     start = kpt.empty_particles(n)
     O.inject(s, o, n, 0.0, 0, 114514, 0, start, 0, n)
@@ -81,9 +81,9 @@ def test_fp32_c1_spectrum_and_density_match_oracle():
 def _qlt_case(n=32768, steps=400, seed=3):
     xe, ye, f0 = W.island_field(64, 32, seed=seed)
     _, _, f1 = W.island_field(64, 32, seed=seed + 1)
-    s = W.c3_device_settings(xe, ye)
+    s = W.focused_qlt_device_settings(xe, ye)
     s.mhd_dt = steps * s.dt_min
-    parts = W.c3_particles(n, s, seed=seed)
+    parts = W.focused_qlt_particles(n, s, seed=seed)
     return xe, ye, f0, f1, s, parts
 
 
@@ -131,7 +131,7 @@ def test_fp32_near_null_lanes_take_fp64_steps():
     with fp64_null_b above every |B| each sub-step's coefficient arithmetic is
     FP64 and only the float increments differ (1e-6)."""
     xe, ye, planes = W.harris_field(64, 32, bz=0.0)
-    h = W.c1_host_settings()
+    h = W.parker_harris_host_settings()
     h.mhd_dt = 10 * 1e-4
     h.dt_min_rel = h.dt_max_rel = 0.1  # dt = 1e-4, 10 sub-steps
     s = kpt.make_device_settings(h, xe, ye)

@@ -140,9 +140,9 @@ def test_training_reaches_exact_qlt(trained):
 def _qlt_case(n=32768, steps=400, seed=3):
     xe, ye, f0 = W.island_field(64, 32, seed=seed)
     _, _, f1 = W.island_field(64, 32, seed=seed + 1)
-    s = W.c3_device_settings(xe, ye, precision=kpt.PREC_FP32)
+    s = W.focused_qlt_device_settings(xe, ye, precision=kpt.PREC_FP32)
     s.mhd_dt = steps * s.dt_min
-    parts = W.c3_particles(n, s, seed=seed)
+    parts = W.focused_qlt_particles(n, s, seed=seed)
     return xe, ye, f0, f1, s, parts
 
 
@@ -217,7 +217,7 @@ def test_mlp_push_matches_oracle_and_reports_gain(trained):
     # costs more VALU issue than the closed form it replaces (two transcendentals
     # and about 20 VALU per evaluation): the closure gains accuracy-preserving
     # generality, not speed, and this A/B pins that.
-    big = W.c3_particles(262144, s, seed=8)
+    big = W.focused_qlt_particles(262144, s, seed=8)
     rates = {}
     for name, closure in (("exact", kpt.CLOSURE_QLT), ("mlp", kpt.CLOSURE_MLP)):
         s2 = s.copy()

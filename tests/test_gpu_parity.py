@@ -82,8 +82,8 @@ def _noise_case(model, focused, time_interp, physics, steps, n=4096, seed=5):
     xe, ye, planes = _harris()
     upper = _perturbed(planes, rng, 0.02)
     h = kpt.default_host_settings()
-    if physics == "c1":
-        h = W.c1_host_settings()
+    if physics == "harris":
+        h = W.parker_harris_host_settings()
     h.focused_transport = int(focused)
     h.time_interpolation = int(time_interp)
     h.max_particle_steps_per_interval = steps
@@ -103,12 +103,13 @@ def _noise_case(model, focused, time_interp, physics, steps, n=4096, seed=5):
 
 @pytest.mark.parametrize("model,focused", [(16, False), (21, False), (21, True)])
 @pytest.mark.parametrize("time_interp", [1, 0])
-@pytest.mark.parametrize("physics", ["reference", "c1"])
+@pytest.mark.parametrize("physics", ["reference", "harris"])
 @pytest.mark.parametrize("steps", [1, 10])
 def test_push_injected_noise_matches_oracle(model, focused, time_interp, physics, steps):
-    if focused and physics == "c1" and steps > 1:
-        # C1 forces dt = 1e-4 with the driver's duu0 = 5578: the pitch-angle map is
-        # then expanding (|d mu_{n+1} / d mu_n| >> 1) and amplifies 1e-16 rounding past
+    if focused and physics == "harris" and steps > 1:
+        # The small Harris-sheet settings force dt = 1e-4 with the driver's duu0 =
+        # 5578: the pitch-angle map is then expanding (|d mu_{n+1} / d mu_n| >> 1)
+        # and amplifies 1e-16 rounding past
         # 1e-12 within 10 steps, so the sub-steps are checked one at a time from a
         # common state instead.
         _stepwise_matches_oracle(model, focused, time_interp, physics, steps)
@@ -204,14 +205,14 @@ def test_push_philox_trajectories_match_oracle():
     assert close.mean() > 0.99, close.mean()
 
 
-def test_c1_statistics_against_oracle():
+def test_parker_harris_statistics_against_oracle():
     """Parker, plain FP64, small: the full 256 x 128 / 65 536-particle / 1000-step case.
     Same seed -> per-particle agreement; independent seeds -> the spatial density
     (64 x 32) and momentum spectrum agree within Monte Carlo error."""
-    cfg = W.C1
+    cfg = W.PARKER_HARRIS_256X128
     xe, ye, planes = W.harris_field(cfg["nx"], cfg["ny"])
-    s = kpt.make_device_settings(W.c1_host_settings(), xe, ye)
-    o = W.c1_inject_opts()
+    s = kpt.make_device_settings(W.parker_harris_host_settings(), xe, ye)
+    o = W.parker_harris_inject_opts()
     n = cfg["particles"]
     lo = O.Field(xe, ye, planes, 16)
     results = {}

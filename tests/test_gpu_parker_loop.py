@@ -1,15 +1,15 @@
-"""The production Parker loop (C2) through the kpt_run driver against an
+"""The production Parker loop through the kpt_run driver against an
 oracle-driven loop.
 
-C2: Parker transport on the 1024 x 512 island-chain field with y faces
+Workload: Parker transport on the 1024 x 512 island-chain field with y faces
 tanh-stretched toward the sheet (spacing ratio 10), time-blended frames of a
 drifting series, 1600 particles per frame times rank scale 16 (25 600) injected
 per frame, splitting at thresholds x1.2 per level, FP64. kpt_run runs the
 reference's frame order (inject, push, split, diagnostics) for four frames with
 staged field reads and 8 cell-sorted push chunks; the oracle runs the same order
 (inject, push_interval, split) with the same Philox seed, frame by frame on the
-same compact-field files. The first split threshold is p0 here (the C2 bench
-uses 2 p0): with the reference's default physics p grows by at most ~20% per
+same compact-field files. The first split threshold is p0 here (the Parker-loop
+bench uses 2 p0): with the reference's default physics p grows by at most ~20% per
 interval, so four frames reach split levels 1-4 only from p0.
 
 Compared: every particle by id (resumable snapshots carry ids): more than 99%
@@ -48,7 +48,7 @@ def _fields(d):
         kpt.compact_field_write(os.path.join(d, f"field{f:05d}.bin"), xe, ye2, planes)
 
 
-def test_c2_loop_matches_oracle(tmp_path):
+def test_parker_loop_matches_oracle(tmp_path):
     fields, out = tmp_path / "fields", tmp_path / "out"
     fields.mkdir()
     _fields(str(fields))

@@ -75,12 +75,12 @@ PRECISIONS = pytest.mark.parametrize("prec", [kpt.PREC_FP32, kpt.PREC_FP64], ids
 def test_focused_staged_equals_global(prec, grid, closure, chunks):
     field = _field(grid)
     xe, ye = field[0], field[1]
-    s = W.c3_device_settings(xe, ye, closure=closure, precision=prec)
+    s = W.focused_qlt_device_settings(xe, ye, closure=closure, precision=prec)
     steps = 200
     s.mhd_dt = steps * s.dt_min
     s.max_particle_steps_per_interval = steps + 2
     s.push_chunks = chunks
-    start = W.c3_particles(N, s, seed=4)
+    start = W.focused_qlt_particles(N, s, seed=4)
     st = _equal(field, 21, s, start)
     assert st.failures == 0
 

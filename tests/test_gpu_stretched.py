@@ -86,7 +86,7 @@ def test_stretched_push_injected_noise_matches_oracle(model, focused, closure, s
 
 def test_stretched_fp32_follows_fp64():
     xe, ye, planes, upper = _field(96, 80)
-    h = W.c1_host_settings()
+    h = W.parker_harris_host_settings()
     s = kpt.make_device_settings(h, xe, ye)
     n = 20000
     rng = np.random.default_rng(4)

@@ -52,10 +52,10 @@ def field():
 
 @pytest.fixture(scope="module")
 def focused_oracle(field):
-    """C4 physics on the bench field: the start set and the FP64 oracle's interval."""
+    """The learned-closure workload's physics on the bench field: the start set and the FP64 oracle's interval."""
     xe, ye, f0, f1 = field
-    s = W.c3_device_settings(xe, ye)
-    start = W.c3_particles(N_FOCUSED, s, seed=11)
+    s = W.focused_qlt_device_settings(xe, ye)
+    start = W.focused_qlt_particles(N_FOCUSED, s, seed=11)
     want = {k: v.copy() for k, v in start.items()}
     steps, fails = O.push_interval(O.Field(xe, ye, f0, 21), O.Field(xe, ye, f1, 21), s, 0.0, 0, ORACLE_SEED, want)
     assert fails == 0
@@ -145,11 +145,11 @@ def test_one_network_evaluation_follows_three(field, network):
     the closure that evaluates the network at every pass, on one injected noise
     table: the pitch-angle increments agree to 1e-3 of their rms (median)."""
     xe, ye, f0, f1 = field
-    s = W.c3_device_settings(xe, ye, closure=kpt.CLOSURE_MLP, precision=kpt.PREC_FP32)
+    s = W.focused_qlt_device_settings(xe, ye, closure=kpt.CLOSURE_MLP, precision=kpt.PREC_FP32)
     n, steps = 16384, 8
     s.mhd_dt = steps * s.dt_min
     s.max_particle_steps_per_interval = steps + 1
-    start = W.c3_particles(n, s, seed=3)
+    start = W.focused_qlt_particles(n, s, seed=3)
     noise = np.random.default_rng(4).uniform(-np.sqrt(3), np.sqrt(3), (n, steps + 1, 4))
     out = {}
     for passes in (1, 3):

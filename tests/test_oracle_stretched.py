@@ -76,7 +76,7 @@ def test_nearly_uniform_stretched_grid_reproduces_uniform_push(oracle_mod, adapt
     xe, ye_u, planes = W.harris_field(nx, ny)
     ye_s = ye_u.copy()
     ye_s[1:-1] += 1e-9 * np.sin(np.arange(1, ny))  # breaks uniformity, moves faces by 1e-9
-    h = W.c1_host_settings()
+    h = W.parker_harris_host_settings()
     if adaptive:
         h.dt_min_rel, h.dt_max_rel = 1e-5, 1e-2
     s_u = kpt.make_device_settings(h, xe, ye_u)

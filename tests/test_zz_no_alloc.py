@@ -35,7 +35,7 @@ def frames():
 
 
 def _settings(xe, ye, precision, blend, chunks, null_b=None):
-    s = W.c3_device_settings(xe, ye, precision=precision)
+    s = W.focused_qlt_device_settings(xe, ye, precision=precision)
     s.mhd_dt = 100 * s.dt_min  # about 100 fixed sub-steps
     s.time_interpolation = int(blend)
     s.push_chunks = chunks
@@ -83,9 +83,9 @@ def _same(got, want, by_id):
 def test_noalloc_push_matches_push_interval_on_two_calls(frames):
     xe, ye, _, _ = frames
     s1 = _settings(xe, ye, kpt.PREC_FP64, blend=True, chunks=4)
-    p1 = W.c3_particles(200_000, s1, seed=5)
+    p1 = W.focused_qlt_particles(200_000, s1, seed=5)
     s2 = _settings(xe, ye, kpt.PREC_FP32, blend=False, chunks=1, null_b=1.0e30)
-    p2 = W.c3_particles(30_000, s2, seed=6)
+    p2 = W.focused_qlt_particles(30_000, s2, seed=6)
     want1 = _reference(frames, s1, p1)
     want2 = _reference(frames, s2, p2)
     assert want1[0][0] == 200_000 * _sub_steps(s1) and want2[0][0] == 30_000 * _sub_steps(s2)
@@ -104,7 +104,7 @@ def test_noalloc_push_matches_push_interval_on_two_calls(frames):
 def test_noalloc_push_without_stats_and_fp32_blended(frames):
     xe, ye, _, _ = frames
     s = _settings(xe, ye, kpt.PREC_FP32, blend=True, chunks=2)
-    parts = W.c3_particles(50_000, s, seed=8)
+    parts = W.focused_qlt_particles(50_000, s, seed=8)
     want_stats, want = _reference(frames, s, parts)
     with _context(frames, 50_000) as c:
         c.upload(parts)
@@ -123,7 +123,7 @@ def test_noalloc_push_without_stats_and_fp32_blended(frames):
 def test_noalloc_push_refuses_more_chunks_than_records(frames):
     xe, ye, _, _ = frames
     s = _settings(xe, ye, kpt.PREC_FP64, blend=False, chunks=kpt.NOALLOC_MAX_CHUNKS + 1)
-    parts = W.c3_particles(1000, s, seed=9)
+    parts = W.focused_qlt_particles(1000, s, seed=9)
     with _context(frames, 1000) as c:
         c.upload(parts)
         with pytest.raises(kpt.KptError, match="exceeds the 16 chunk records"):
