@@ -100,7 +100,9 @@ kpt_status ens_sync(kpt_ensemble* e) {
         a.P = arrays_at(e->P, e->off[r]);
         a.count = e->count[r];
         a.field_lo = a.field_hi = f.c32;
-        a.field_pair = nullptr;  // one frame per run: node-major records, staged in LDS per wavefront
+        // one frame per run: its node-major single-frame records (read through L1/L2;
+        // LDS staging of them is compiled out by default, KPT_ENSEMBLE_STAGE_BYTES)
+        a.field_pair = nullptr;
         a.field_pair64 = nullptr;
         {
             const kpt_status rs = ensure_rec32(ctx, e->field[r]);
