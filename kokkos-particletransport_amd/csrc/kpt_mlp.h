@@ -61,8 +61,15 @@ constexpr int kFragA2 = 2;
 constexpr int kFragA3 = 10;
 constexpr int kFragA4 = 18;
 constexpr int kFrags = 22;
-// Accumulator initialisers (biases of layers 2 and 3): 16 floats per lane per tile.
+// Accumulator initialisers (biases of layers 2 and 3): 16 floats per lane per tile,
+// stored as four quarters of 64 lanes x 16 bytes, so that a wavefront reads each
+// quarter as one 16-byte load per lane at consecutive addresses (a lane-major
+// 64-byte record per lane put every fourth lane on the same LDS banks).
 constexpr int kBiasTiles = 4;  // B2[t], B3[t]
+__host__ __device__ constexpr size_t mlp_bias_index(int tile, int lane, int j) {
+    return ((static_cast<size_t>(tile) * 4 + static_cast<size_t>(j / 4)) * 64 + static_cast<size_t>(lane)) * 4 +
+           static_cast<size_t>(j % 4);
+}
 constexpr size_t kMlpFragBytes = static_cast<size_t>(kFrags) * 64 * 16;
 constexpr size_t kMlpBiasBytes = static_cast<size_t>(kBiasTiles) * 64 * 16 * 4;
 constexpr size_t kMlpLdsBytes = kMlpFragBytes + kMlpBiasBytes;

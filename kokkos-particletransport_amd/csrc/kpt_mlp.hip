@@ -382,11 +382,12 @@ void mlp_build_image(const float* p, int act, void* image) {
                 put(kFragA4 + s, lane, i, row < kMlpOut ? p[kOffW4 + kMlpH * row + feat] * kOut : 0.0f);
             }
         }
+        // accumulator j of a lane sits in the lane's 16-byte slot of quarter j / 4
         for (int t = 0; t < 2; ++t)
             for (int j = 0; j < 16; ++j) {
                 const int m = mlp_acc_row(t, j, half);
-                bias[(static_cast<size_t>(0 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB2 + m] * kPre;
-                bias[(static_cast<size_t>(1 * 2 + t) * 64 + lane) * 16 + j] = p[kOffB3 + m] * kPre;
+                bias[mlp_bias_index(0 * 2 + t, lane, j)] = p[kOffB2 + m] * kPre;
+                bias[mlp_bias_index(1 * 2 + t, lane, j)] = p[kOffB3 + m] * kPre;
             }
     }
     float* b4 = reinterpret_cast<float*>(static_cast<char*>(image) + kMlpLdsBytes);

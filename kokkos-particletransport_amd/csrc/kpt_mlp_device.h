@@ -15,14 +15,28 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // kBiasTiles accumulator initialisers, each 64 lanes wide.
 struct MlpLds {
     const bf16x8* frag;
-    const f32x16* bias;
+    const float4* bias;  // tile t, quarter q, lane l at bias[(4 t + q) 64 + l]
 };
 
 __device__ __forceinline__ MlpLds mlp_lds_view(const void* lds) {
     MlpLds v;
     v.frag = static_cast<const bf16x8*>(lds);
-    v.bias = reinterpret_cast<const f32x16*>(static_cast<const char*>(lds) + kMlpFragBytes);
+    v.bias = reinterpret_cast<const float4*>(static_cast<const char*>(lds) + kMlpFragBytes);
     return v;
+}
+
+// One bias tile of a lane as its accumulator initialiser.
+__device__ __forceinline__ f32x16 mlp_bias_tile(const MlpLds& im, int tile, int lane) {
+    f32x16 r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = im.bias[(tile * 4 + q) * 64 + lane];
+        r[4 * q + 0] = v.x;
+        r[4 * q + 1] = v.y;
+        r[4 * q + 2] = v.z;
+        r[4 * q + 3] = v.w;
+    }
+    return r;
 }
 
 // Cooperative copy of the image (global -> LDS) by a whole block; the caller
@@ -77,8 +91,8 @@ __device__ __forceinline__ void mlp_block(const MlpLds& im, int lane, const bf16
 #pragma unroll
     for (int layer = 0; layer < 2; ++layer) {
         const int fa = layer == 0 ? kFragA2 : kFragA3;
-        a0 = im.bias[(2 * layer + 0) * 64 + lane];
-        a1 = im.bias[(2 * layer + 1) * 64 + lane];
+        a0 = mlp_bias_tile(im, 2 * layer + 0, lane);
+        a1 = mlp_bias_tile(im, 2 * layer + 1, lane);
         a0 = KPT_MFMA(im.frag[(fa + 0) * 64 + lane], h0, a0);
         a1 = KPT_MFMA(im.frag[(fa + 4) * 64 + lane], h0, a1);
         a0 = KPT_MFMA(im.frag[(fa + 1) * 64 + lane], h1, a0);
