@@ -100,3 +100,38 @@ def test_fp64_exact_push_4m(field):
     n = W.FOCUSED_QLT_1024X512["particles"]
     start, got, st = _run(field, n, s)
     _check(s, start, got, st, n)
+
+
+@pytest.mark.parametrize("prec", [kpt.PREC_FP64, kpt.PREC_FP32], ids=["fp64", "fp32"])
+def test_parker_push_4m(field, prec):
+    """The Parker push on 4 194 304 particles injected over the domain at the
+    interval start (the bench's push-alone workload), adaptive dt: no failures,
+    every particle at the interval end with at least one sub-step and at most
+    the step guard, p above the floor, positions in the domain, ids and
+    weights preserved."""
+    xe, ye, f0, f1 = field
+    h = kpt.default_host_settings()
+    h.split_particles = 0
+    s = kpt.make_device_settings(h, xe, ye)
+    s.precision = prec
+    s.push_chunks = 8
+    n = 4 * 1024 * 1024
+    with kpt.Context(n, seed=114514) as c:
+        c.field_upload(kpt.SLOT_LOWER, kpt.MODEL_PARKER16, xe, ye, f0)
+        c.field_upload(kpt.SLOT_UPPER, kpt.MODEL_PARKER16, xe, ye, f1)
+        o = kpt.InjectOpts()
+        o.fixed_time = 1
+        assert c.inject(s, n, 0.0, 0, o) == n
+        start = c.download()
+        st = c.push_interval(s, 0.0, 0)
+        got = c.download()
+    assert st.failures == 0 and st.active_in == n
+    assert n <= st.particle_steps <= n * s.max_particle_steps_per_interval
+    assert np.all(got["status"] == 0)
+    assert np.all(got["t"] >= s.mhd_dt) and np.all(got["t"] - s.mhd_dt <= 4e-16 * s.mhd_dt)
+    assert np.all(got["p"] >= 0.25 * s.p0) and np.all(np.isfinite(got["p"]))
+    assert np.all((got["x"] >= s.x_lower) & (got["x"] <= s.x_upper))
+    assert np.all((got["y"] >= s.y_lower) & (got["y"] <= s.y_upper))
+    order_got, order_start = np.argsort(got["id"]), np.argsort(start["id"])
+    np.testing.assert_array_equal(got["id"][order_got], start["id"][order_start])
+    np.testing.assert_array_equal(got["weight"][order_got], start["weight"][order_start])
