@@ -198,7 +198,9 @@ def roofline_block(workload, kernel, launch_s, launches, alg_bytes):
         # a transcendental
         v = c["SQ_ACTIVE_INST_VALU"] / launch_s
         res["valu_busy"] = {"achieved": v / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-slots/s",
-                            "frac": v / VALU_ISSUE_PEAK}
+                            "frac": v / VALU_ISSUE_PEAK,
+                            "note": "peak at the nominal 2.4 GHz engine clock: a fraction at or slightly above 1 "
+                                    "is a saturated VALU pipe"}
     if "TA_TA_BUSY_sum" in c:
         # vector memory address unit (one per CU) busy cycles: the gather path
         v = c["TA_TA_BUSY_sum"] / launch_s
