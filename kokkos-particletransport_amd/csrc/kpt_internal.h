@@ -285,6 +285,21 @@ hipError_t launch_pair64(const double* lo, const double* hi, uint64_t plane, int
 hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, uint64_t first_root,
                          const kpt_device_settings& s, const kpt_inject_opts& o, double t0,
                          uint32_t frame, const uint32_t key[2], hipStream_t stream);
+// Id of the child of a split whose parent (id pid) was at split level `level`.
+// Ids are root | path << 32: the root is the injection counter and path bit L
+// is set when an ancestor split at level L, so every particle of a tree has its
+// own id (and Philox stream) while the tree is at most 32 levels deep. Deeper
+// children take the splitmix64 finalizer (Steele, Lea and Flood 2014) of
+// pid + (level + 1) * golden gamma, a 64-bit mix under which distinct particles
+// collide with probability about 2^-64 per pair.
+__host__ __device__ constexpr uint64_t split_child_id(uint64_t pid, int level) {
+    if (level < 32) return (pid & 0xFFFFFFFFull) | (((pid >> 32) | (1ull << level)) << 32);
+    uint64_t z = pid + static_cast<uint64_t>(level + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 // Split thresholds pmin_split * split_ratio^level for level < kLevels.
 struct SplitTable {
     static constexpr int kLevels = 64;

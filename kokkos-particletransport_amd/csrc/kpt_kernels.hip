@@ -1720,9 +1720,7 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(ParticleArrays P, u
             P.status[child] = 0;
             P.split_level[i] = new_level;
             P.split_level[child] = new_level;
-            const uint64_t pid = P.id[i];
-            const uint64_t path = (pid >> 32) | (old_level < 32 ? (1ull << old_level) : 0ull);
-            P.id[child] = (pid & 0xFFFFFFFFull) | (path << 32);
+            P.id[child] = split_child_id(P.id[i], old_level);
             done = 1;
         }
     }

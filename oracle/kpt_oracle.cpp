@@ -936,9 +936,23 @@ uint64_t kpto_inject(const kpt_device_settings* s, const kpt_inject_opts* o, uin
     return n;
 }
 
+// Id of a split child whose parent (id pid) was at level L: (root | path << 32)
+// gets path | (1 << L) for L < 32; deeper children take the splitmix64 finalizer
+// (Steele, Lea and Flood 2014) of pid + (L + 1) * 0x9E3779B97F4A7C15.
+uint64_t kpto_split_child_id(uint64_t pid, int32_t level) {
+    if (level < 32) {
+        const uint64_t path = (pid >> 32) | (static_cast<uint64_t>(1) << level);
+        return (pid & 0xFFFFFFFFull) | (path << 32);
+    }
+    uint64_t z = pid + static_cast<uint64_t>(level + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 // split_particles (src/main.cpp:1760-1839), children appended in parent slot
-// order. The child of a parent at level L with id (root | path << 32) gets
-// path | (1 << L). Returns the new count; split / skipped through pointers.
+// order; child ids from kpto_split_child_id. Returns the new count; split /
+// skipped through pointers.
 uint64_t kpto_split(const kpt_device_settings* s, kpt_particles_host* P, uint64_t count,
                     uint64_t capacity, uint64_t* split_out, uint64_t* skipped_out) {
     uint64_t split = 0, skipped = 0;
@@ -973,12 +987,7 @@ uint64_t kpto_split(const kpt_device_settings* s, kpt_particles_host* P, uint64_
         P->status[child] = 0;
         P->split_level[parent] = new_level;
         P->split_level[child] = new_level;
-        if (P->id) {
-            const uint64_t pid = P->id[parent];
-            const uint64_t path =
-                (pid >> 32) | (old_level < 32 ? (static_cast<uint64_t>(1) << old_level) : 0);
-            P->id[child] = (pid & 0xFFFFFFFFull) | (path << 32);
-        }
+        if (P->id) P->id[child] = kpto_split_child_id(P->id[parent], old_level);
         ++split;
     }
     if (split_out) *split_out = split;

@@ -61,6 +61,7 @@ def load():
                                          P(kpt.ParticlesHost), C.c_uint64, C.c_uint64]),
             "kpto_split": (C.c_uint64, [P(kpt.DeviceSettings), P(kpt.ParticlesHost), C.c_uint64,
                                         C.c_uint64, P(C.c_uint64), P(C.c_uint64)]),
+            "kpto_split_child_id": (C.c_uint64, [C.c_uint64, C.c_int32]),
             "kpto_histogram_logp": (None, [P(kpt.ParticlesHost), C.c_uint64, C.c_int32, C.c_double,
                                            C.c_double, dp]),
             "kpto_histogram_xy": (None, [P(kpt.ParticlesHost), C.c_uint64, C.c_int32, C.c_int32,
@@ -201,6 +202,10 @@ def inject(s, opts, requested, t0, frame, seed, first_root, parts, count, capaci
     h = kpt.particles_struct(parts)
     return load().kpto_inject(C.byref(s), C.byref(opts) if opts else None, requested, t0, frame,
                               seed, first_root, C.byref(h), count, capacity)
+
+
+def split_child_id(pid, level):
+    return load().kpto_split_child_id(pid, level)
 
 
 def split(s, parts, count, capacity):

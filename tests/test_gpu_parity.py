@@ -366,6 +366,46 @@ def test_split_bitexact():
         np.testing.assert_array_equal(got[k], want[k][:new_n], err_msg=k)
 
 
+def test_split_deep_levels_bitexact():
+    """Parents at split levels 28-35 (ratio 1.01 keeps the thresholds below pmax):
+    children below level 32 set a path bit of the id, deeper ones take the
+    64-bit mix; every field, ids included, equals the oracle's bit for bit."""
+    h = kpt.default_host_settings()
+    h.p0 = 1.0
+    h.split_ratio = 1.01
+    h.pmax = 10.0
+    xe, ye = uniform_edges(8, -1, 1), uniform_edges(8, -1, 1)
+    s = kpt.make_device_settings(h, xe, ye)
+    rng = np.random.default_rng(12)
+    n, cap = 5000, 12000
+    parts = random_particles(n, s, rng, p_range=(2.0, 9.0))
+    parts["split_level"][:] = rng.integers(28, 36, n)
+    parts["weight"][:] = 0.5 ** parts["split_level"]
+    # ids as splits make them: below level 32 a root and path bits under the
+    # particle's level; from 32 on a 64-bit mix (random here)
+    lvl = parts["split_level"].astype(np.uint64)
+    root = rng.permutation(n).astype(np.uint64)
+    low = np.where(lvl < 32, (np.uint64(1) << np.minimum(lvl, np.uint64(31))) - np.uint64(1), np.uint64(0))
+    path = rng.integers(0, 2**32, n, dtype=np.int64).astype(np.uint64) & low
+    deep = rng.integers(2**40, 2**63, n, dtype=np.int64).astype(np.uint64)
+    parts["id"][:] = np.where(lvl < 32, root | (path << np.uint64(32)), deep)
+    assert len(np.unique(parts["id"])) == n
+    want = kpt.empty_particles(cap)
+    want["status"][:] = 2
+    for k in parts:
+        want[k][:n] = parts[k]
+    new_n, o_split, o_skip = O.split(s, want, n, cap)
+    with _ctx(cap) as c:
+        c.upload(parts)
+        split, skipped = c.split(s)
+        assert c.count == new_n
+        got = c.download()
+    assert (split, skipped) == (o_split, o_skip) and o_split > 0.5 * n
+    for k in ("x", "y", "p", "weight", "status", "split_level", "id"):
+        np.testing.assert_array_equal(got[k], want[k][:new_n], err_msg=k)
+    assert len(np.unique(got["id"])) == new_n
+
+
 def test_histograms_and_summary_match_oracle():
     h = kpt.default_host_settings()
     xe, ye = uniform_edges(8, -1, 1), uniform_edges(8, -0.5, 0.5)

@@ -453,6 +453,33 @@ def test_split_conserves_weight_and_orders_children(oracle_mod):
     assert len(np.unique(parts["id"][:new_n])) == new_n
 
 
+# SplitMix64 (Steele, Lea and Flood 2014) from state 0: its first outputs are the
+# finalizer of 1 and 2 golden gammas.
+SPLITMIX64_SEED0 = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+GOLDEN_GAMMA = 0x9E3779B97F4A7C15
+
+
+def test_split_child_id_deep_levels(oracle_mod):
+    """Below 32 levels a child's id sets the parent's path bit; from level 32 on it
+    is the splitmix64 finalizer of pid + (level + 1) * golden gamma, pinned by
+    the generator's published outputs from state 0."""
+    m64 = (1 << 64) - 1
+    for k, want in enumerate(SPLITMIX64_SEED0):
+        level = 32 + k
+        # pid + (level + 1) g = (k + 1) g  (mod 2^64)
+        pid = (-(level - k) * GOLDEN_GAMMA) & m64
+        assert O.split_child_id(pid, level) == want
+    root, path = 123456789, 0b1011
+    pid = root | (path << 32)
+    for level in (0, 5, 31):
+        assert O.split_child_id(pid, level) == root | ((path | (1 << level)) << 32)
+    # a tree split 40 levels deep: every particle keeps its own id
+    ids = [root]
+    for level in range(40):
+        ids += [O.split_child_id(i, level) for i in ids[-(2 ** min(level, 10)):]]
+    assert len(set(ids)) == len(ids)
+
+
 def test_split_capacity_clamp(oracle_mod):
     s = _split_settings()
     n, cap = 10, 13
