@@ -289,11 +289,12 @@ hipError_t launch_inject(const ParticleArrays& P, uint64_t first, uint64_t n, ui
 // Ids are root | path << 32: the root is the injection counter and path bit L
 // is set when an ancestor split at level L, so every particle of a tree has its
 // own id (and Philox stream) while the tree is at most 32 levels deep. Deeper
-// children take the splitmix64 finalizer (Steele, Lea and Flood 2014) of
-// pid + (level + 1) * golden gamma, a 64-bit mix under which distinct particles
-// collide with probability about 2^-64 per pair.
+// children (and those of a parent with a negative level) take the splitmix64
+// finalizer (Steele, Lea and Flood 2014) of pid + (level + 1) * golden gamma, a
+// 64-bit mix under which distinct particles collide with probability about
+// 2^-64 per pair.
 __host__ __device__ constexpr uint64_t split_child_id(uint64_t pid, int level) {
-    if (level < 32) return (pid & 0xFFFFFFFFull) | (((pid >> 32) | (1ull << level)) << 32);
+    if (level >= 0 && level < 32) return (pid & 0xFFFFFFFFull) | (((pid >> 32) | (1ull << level)) << 32);
     uint64_t z = pid + static_cast<uint64_t>(level + 1) * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;

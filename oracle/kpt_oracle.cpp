@@ -937,10 +937,10 @@ uint64_t kpto_inject(const kpt_device_settings* s, const kpt_inject_opts* o, uin
 }
 
 // Id of a split child whose parent (id pid) was at level L: (root | path << 32)
-// gets path | (1 << L) for L < 32; deeper children take the splitmix64 finalizer
+// gets path | (1 << L) for 0 <= L < 32; other levels take the splitmix64 finalizer
 // (Steele, Lea and Flood 2014) of pid + (L + 1) * 0x9E3779B97F4A7C15.
 uint64_t kpto_split_child_id(uint64_t pid, int32_t level) {
-    if (level < 32) {
+    if (level >= 0 && level < 32) {
         const uint64_t path = (pid >> 32) | (static_cast<uint64_t>(1) << level);
         return (pid & 0xFFFFFFFFull) | (path << 32);
     }
